@@ -1,0 +1,339 @@
+// Oracle solver for the two projection systems. TEST INFRASTRUCTURE ONLY.
+//
+// The reference solves these systems with TGMG, an external multigrid
+// library that is absent here and pinned to no version, so the oracle does
+// not restate a solver: it restates the OPERATORS exactly
+// (mgs_mac.hh:64-102 + mgs_mac.cc:43-49; mgs_fem.hh:86-136 +
+// mgs_fem.cc:139-162) and drives the residual ||b - A z||_2 / ||b||_2 down to
+// the requested tolerance with a plain geometric multigrid of its own
+// (colored Gauss-Seidel, rediscretized coarse levels). Tests pin the
+// solutions against dense direct solves (tests/test_oracle_mg.py).
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include "oracle.hh"
+
+namespace orc {
+
+namespace {
+
+// A level: nx x ny unknowns with an explicit 9-point stencil
+// c[k][p], k = 0..8 ordered (dl,dc,dr,cl,cc,cr,ul,uc,ur).
+struct level {
+    int nx, ny;
+    bool cell;               // cell-centred (MAC) or vertex (FEM)
+    std::vector<double> c[9];
+    std::vector<double> z, b, r;
+    // coefficients that define the operator and are averaged to coarsen
+    std::vector<double> fw, fs;  // MAC: face inverse densities (nx+1)*ny, nx*(ny+1)
+    std::vector<double> kc;      // FEM: cell inverse densities (nx-1)*(ny-1)
+    double kx, ky;               // MAC scalings 1/dx^2, 1/dy^2
+    double dxdy, dydx;           // FEM ratios
+    void alloc() {
+        const size_t N = (size_t)nx * ny;
+        for (auto &v : c) v.assign(N, 0.0);
+        z.assign(N, 0.0); b.assign(N, 0.0); r.assign(N, 0.0);
+    }
+};
+
+void mac_stencil(level &L) {
+    const int nx = L.nx, ny = L.ny;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int p = j * nx + i;
+            const double el = L.fw[j * (nx + 1) + i], er = L.fw[j * (nx + 1) + i + 1];
+            const double ed = L.fs[j * nx + i], eu = L.fs[(j + 1) * nx + i];
+            const double cl = i > 0 ? L.kx * el : 0, cr = i < nx - 1 ? L.kx * er : 0;
+            const double dc = j > 0 ? L.ky * ed : 0, uc = j < ny - 1 ? L.ky * eu : 0;
+            L.c[3][p] = cl; L.c[5][p] = cr; L.c[1][p] = dc; L.c[7][p] = uc;
+            L.c[4][p] = -L.kx * ((i > 0 ? el : 0) + (i < nx - 1 ? er : 0))
+                        - L.ky * ((j > 0 ? ed : 0) + (j < ny - 1 ? eu : 0));
+        }
+}
+
+void fem_stencil(level &L) {
+    const int nx = L.nx, ny = L.ny;
+    const double fm = 1. / 3. * (L.dxdy + L.dydx), fey = 1. / 3. * (-2 * L.dxdy + L.dydx);
+    const double hey = 0.5 * fey, fex = 1. / 3. * (-2 * L.dydx + L.dxdy), hex = 0.5 * fex;
+    const double fc = -1. / 6. * (L.dxdy + L.dydx);
+    auto K = [&](int ci, int cj) -> double {
+        return (ci >= 0 && ci < nx - 1 && cj >= 0 && cj < ny - 1) ? L.kc[cj * (nx - 1) + ci] : 0.0;
+    };
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int p = j * nx + i;
+            const double kdl = K(i - 1, j - 1), kdr = K(i, j - 1), kul = K(i - 1, j), kur = K(i, j);
+            L.c[0][p] = fc * kdl; L.c[2][p] = fc * kdr; L.c[6][p] = fc * kul; L.c[8][p] = fc * kur;
+            L.c[1][p] = hey * (kdl + kdr); L.c[7][p] = hey * (kul + kur);
+            L.c[3][p] = hex * (kdl + kul); L.c[5][p] = hex * (kdr + kur);
+            L.c[4][p] = fm * (kdl + kdr + kul + kur);
+        }
+}
+
+double stencil_at(const level &L, const std::vector<double> &x, int i, int j, bool offdiag_only) {
+    const int nx = L.nx, ny = L.ny, p = j * nx + i;
+    double s = 0;
+    for (int dj = -1; dj <= 1; dj++) {
+        const int jj = j + dj;
+        if (jj < 0 || jj >= ny) continue;
+        for (int di = -1; di <= 1; di++) {
+            const int ii = i + di;
+            if (ii < 0 || ii >= nx) continue;
+            if (offdiag_only && di == 0 && dj == 0) continue;
+            s += L.c[(dj + 1) * 3 + di + 1][p] * x[jj * nx + ii];
+        }
+    }
+    return s;
+}
+
+void apply(const level &L, const std::vector<double> &x, std::vector<double> &y) {
+#pragma omp parallel for
+    for (int j = 0; j < L.ny; j++)
+        for (int i = 0; i < L.nx; i++) y[j * L.nx + i] = stencil_at(L, x, i, j, false);
+}
+
+void residual(level &L) {
+#pragma omp parallel for
+    for (int j = 0; j < L.ny; j++)
+        for (int i = 0; i < L.nx; i++) {
+            const int p = j * L.nx + i;
+            L.r[p] = L.b[p] - stencil_at(L, L.z, i, j, false);
+        }
+}
+
+// Colored Gauss-Seidel: colors (i%2, j%2), a valid GS ordering for 9 points.
+void smooth(level &L, int sweeps, bool reverse) {
+    for (int s = 0; s < sweeps; s++)
+        for (int cc = 0; cc < 4; cc++) {
+            const int col = reverse ? 3 - cc : cc, ci = col & 1, cj = col >> 1;
+#pragma omp parallel for
+            for (int j = cj; j < L.ny; j += 2)
+                for (int i = ci; i < L.nx; i += 2) {
+                    const int p = j * L.nx + i;
+                    const double d = L.c[4][p];
+                    if (d != 0) L.z[p] = (L.b[p] - stencil_at(L, L.z, i, j, true)) / d;
+                }
+        }
+}
+
+double mean(const std::vector<double> &v) {
+    double s = 0;
+    for (double x : v) s += x;
+    return v.empty() ? 0 : s / v.size();
+}
+
+void restrict_to(const level &F, level &C) {
+    if (F.cell) {
+        for (int J = 0; J < C.ny; J++)
+            for (int I = 0; I < C.nx; I++) {
+                const int p = 2 * J * F.nx + 2 * I;
+                C.b[J * C.nx + I] = 0.25 * (F.r[p] + F.r[p + 1] + F.r[p + F.nx] + F.r[p + F.nx + 1]);
+            }
+    } else {
+        for (int J = 0; J < C.ny; J++)
+            for (int I = 0; I < C.nx; I++) {
+                double s = 0;
+                for (int dj = -1; dj <= 1; dj++) {
+                    const int j = 2 * J + dj;
+                    if (j < 0 || j >= F.ny) continue;
+                    for (int di = -1; di <= 1; di++) {
+                        const int i = 2 * I + di;
+                        if (i < 0 || i >= F.nx) continue;
+                        s += (di ? 0.5 : 1.0) * (dj ? 0.5 : 1.0) * F.r[j * F.nx + i];
+                    }
+                }
+                C.b[J * C.nx + I] = s;
+            }
+    }
+}
+
+void prolong_add(level &F, const level &C) {
+    if (F.cell) {
+        for (int j = 0; j < F.ny; j++)
+            for (int i = 0; i < F.nx; i++) {
+                const int I = i / 2, J = j / 2;
+                int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+                if (I2 < 0 || I2 >= C.nx) I2 = I;
+                if (J2 < 0 || J2 >= C.ny) J2 = J;
+                F.z[j * F.nx + i] += 0.5625 * C.z[J * C.nx + I] + 0.1875 * C.z[J * C.nx + I2]
+                                     + 0.1875 * C.z[J2 * C.nx + I] + 0.0625 * C.z[J2 * C.nx + I2];
+            }
+    } else {
+        for (int j = 0; j < F.ny; j++)
+            for (int i = 0; i < F.nx; i++) {
+                const int I = i / 2, J = j / 2;
+                const bool oi = i & 1, oj = j & 1;
+                double v;
+                auto e = [&](int a, int b) { return C.z[b * C.nx + a]; };
+                if (!oi && !oj) v = e(I, J);
+                else if (oi && !oj) v = 0.5 * (e(I, J) + e(I + 1, J));
+                else if (!oi && oj) v = 0.5 * (e(I, J) + e(I, J + 1));
+                else v = 0.25 * (e(I, J) + e(I + 1, J) + e(I, J + 1) + e(I + 1, J + 1));
+                F.z[j * F.nx + i] += v;
+            }
+    }
+}
+
+struct hierarchy {
+    std::vector<level> L;
+    void coarsen() {
+        while (true) {
+            const level &F = L.back();
+            level C;
+            C.cell = F.cell;
+            if (F.cell) {
+                if ((F.nx & 1) || (F.ny & 1) || F.nx < 4 || F.ny < 4) break;
+                C.nx = F.nx / 2; C.ny = F.ny / 2;
+                C.kx = F.kx * 0.25; C.ky = F.ky * 0.25;
+                C.fw.resize((C.nx + 1) * C.ny); C.fs.resize(C.nx * (C.ny + 1));
+                for (int J = 0; J < C.ny; J++)
+                    for (int I = 0; I <= C.nx; I++)
+                        C.fw[J * (C.nx + 1) + I] = 0.5 * (F.fw[2 * J * (F.nx + 1) + 2 * I]
+                                                          + F.fw[(2 * J + 1) * (F.nx + 1) + 2 * I]);
+                for (int J = 0; J <= C.ny; J++)
+                    for (int I = 0; I < C.nx; I++)
+                        C.fs[J * C.nx + I] = 0.5 * (F.fs[2 * J * F.nx + 2 * I] + F.fs[2 * J * F.nx + 2 * I + 1]);
+                C.alloc();
+                mac_stencil(C);
+            } else {
+                if (!(F.nx & 1) || !(F.ny & 1) || F.nx < 5 || F.ny < 5) break;
+                C.nx = (F.nx + 1) / 2; C.ny = (F.ny + 1) / 2;
+                C.dxdy = F.dxdy; C.dydx = F.dydx;
+                C.kc.resize((C.nx - 1) * (C.ny - 1));
+                for (int J = 0; J < C.ny - 1; J++)
+                    for (int I = 0; I < C.nx - 1; I++) {
+                        const int w = F.nx - 1, p = 2 * J * w + 2 * I;
+                        C.kc[J * (C.nx - 1) + I] = 0.25 * (F.kc[p] + F.kc[p + 1] + F.kc[p + w] + F.kc[p + w + 1]);
+                    }
+                C.alloc();
+                fem_stencil(C);
+            }
+            L.push_back(std::move(C));
+        }
+    }
+    void coarse_solve(level &C) {
+        const double mb = mean(C.b);
+        for (double &v : C.b) v -= mb;
+        std::fill(C.z.begin(), C.z.end(), 0.0);
+        double bn = 0;
+        for (double v : C.b) bn += v * v;
+        if (bn == 0) return;
+        for (int it = 0; it < 4000; it++) {
+            smooth(C, 1, false);
+            if ((it & 15) == 15) {
+                residual(C);
+                double rn = 0;
+                for (double v : C.r) rn += v * v;
+                if (rn <= 1e-30 * bn) break;
+            }
+        }
+        const double mz = mean(C.z);
+        for (double &v : C.z) v -= mz;
+    }
+    void vcycle(size_t l) {
+        level &F = L[l];
+        if (l + 1 == L.size()) { coarse_solve(F); return; }
+        smooth(F, 2, false);
+        residual(F);
+        level &C = L[l + 1];
+        restrict_to(F, C);
+        std::fill(C.z.begin(), C.z.end(), 0.0);
+        vcycle(l + 1);
+        prolong_add(F, C);
+        smooth(F, 2, true);
+    }
+    // Solve L[0] with z warm start; returns V-cycles or -1.
+    int solve(const double *b, double *z, double tol, double *relres) {
+        level &F = L[0];
+        const size_t N = (size_t)F.nx * F.ny;
+        F.b.assign(b, b + N);
+        const double mb = mean(F.b);
+        for (double &v : F.b) v -= mb;
+        F.z.assign(z, z + N);
+        double bn = 0;
+        for (double v : F.b) bn += v * v;
+        bn = std::sqrt(bn);
+        int cycles = 0;
+        double rr = 0;
+        while (true) {
+            residual(F);
+            double rn = 0;
+            for (double v : F.r) rn += v * v;
+            rr = bn > 0 ? std::sqrt(rn) / bn : 0;
+            if (rr <= tol || bn == 0) break;
+            if (cycles >= 200) { cycles = -1; break; }
+            vcycle(0);
+            cycles++;
+        }
+        std::copy(F.z.begin(), F.z.end(), z);
+        if (relres) *relres = rr;
+        return cycles;
+    }
+};
+
+void fill_mac_fine(const mac_system &s, level &F) {
+    const int m = s.m, n = s.n, mn = m * n;
+    F.cell = true; F.nx = m; F.ny = n; F.kx = s.xxsp; F.ky = s.yysp;
+    F.fw.assign((m + 1) * n, 0.0); F.fs.assign(m * (n + 1), 0.0);
+    // reference irho layout: [2ij] left edge, [2ij+1] down edge, then the
+    // right column (n) and top row (m) (fluid_2d.cc:1140-1160)
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            const int ij = j * m + i;
+            F.fw[j * (m + 1) + i] = s.irho ? s.irho[2 * ij] : 1.0;
+            F.fs[j * m + i] = s.irho ? s.irho[2 * ij + 1] : 1.0;
+        }
+    for (int j = 0; j < n; j++) F.fw[j * (m + 1) + m] = s.irho ? s.irho[2 * mn + j] : 1.0;
+    for (int i = 0; i < m; i++) F.fs[n * m + i] = s.irho ? s.irho[2 * mn + n + i] : 1.0;
+    F.alloc();
+    mac_stencil(F);
+}
+
+void fill_fem_fine(const fem_system &s, level &F) {
+    const int m = s.m, n = s.n;
+    F.cell = false; F.nx = m; F.ny = n; F.dxdy = s.dx / s.dy; F.dydx = s.dy / s.dx;
+    F.kc.assign((m - 1) * (n - 1), 1.0);
+    // reference corner layout: irho[j*m+i] = 1/rho of the cell below-left of
+    // node (i,j) (fluid_2d.cc:1163-1170)
+    if (s.irho)
+        for (int cj = 0; cj < n - 1; cj++)
+            for (int ci = 0; ci < m - 1; ci++) F.kc[cj * (m - 1) + ci] = s.irho[(cj + 1) * m + ci + 1];
+    F.alloc();
+    fem_stencil(F);
+}
+
+}  // namespace
+
+void mac_apply(const mac_system &s, const double *x, double *y) {
+    level F;
+    fill_mac_fine(s, F);
+    std::vector<double> xv(x, x + (size_t)s.m * s.n), yv(xv.size());
+    apply(F, xv, yv);
+    std::copy(yv.begin(), yv.end(), y);
+}
+
+void fem_apply(const fem_system &s, const double *x, double *y) {
+    level F;
+    fill_fem_fine(s, F);
+    std::vector<double> xv(x, x + (size_t)s.m * s.n), yv(xv.size());
+    apply(F, xv, yv);
+    std::copy(yv.begin(), yv.end(), y);
+}
+
+int mac_solve(const mac_system &s, const double *b, double *z, double tol, double *relres) {
+    hierarchy h;
+    h.L.resize(1);
+    fill_mac_fine(s, h.L[0]);
+    h.coarsen();
+    return h.solve(b, z, tol, relres);
+}
+
+int fem_solve(const fem_system &s, const double *b, double *z, double tol, double *relres) {
+    hierarchy h;
+    h.L.resize(1);
+    fill_fem_fine(s, h.L[0]);
+    h.coarsen();
+    return h.solve(b, z, tol, relres);
+}
+
+}  // namespace orc

@@ -1,0 +1,117 @@
+// C-ABI of the CPU oracle, for tests/ (ctypes) and bench.py's CPU baseline.
+// TEST INFRASTRUCTURE ONLY: the product never links or loads this library.
+#include <cstring>
+#include <vector>
+#include "oracle.hh"
+
+using namespace orc;
+
+extern "C" {
+
+// levelset::build_band (levelset.cc:199-335). order/norder: band traversal.
+int orc_build_band(int m, int n, double ax, double ay, double dx, double dy,
+                   double phi_down, double phi_up, double *phi, int *s,
+                   int *order, int *norder) {
+    ls_grid g(m, n, ax, ay, dx, dy, phi_down, phi_up);
+    std::vector<int> band;
+    build_band(g, phi, s, band);
+    if (order) {
+        std::memcpy(order, band.data(), sizeof(int) * band.size());
+        *norder = (int)band.size();
+    }
+    return 0;
+}
+
+// build_band then the two-field extension (traverse.hh:235-260).
+int orc_build_band_extrapolate(int m, int n, double ax, double ay, double dx,
+                               double dy, double phi_down, double phi_up,
+                               double *phi, int *s, double *fx, double *fy) {
+    ls_grid g(m, n, ax, ay, dx, dy, phi_down, phi_up);
+    std::vector<int> band;
+    build_band(g, phi, s, band);
+    return extrapolate2(g, phi, s, band, fx, fy) ? 0 : 1;
+}
+
+// Extension alone, given a band state (phi, s) and its traversal order.
+int orc_extrapolate(int m, int n, double ax, double ay, double dx, double dy,
+                    const double *phi, const int *s, const int *order,
+                    int norder, double *fx, double *fy) {
+    ls_grid g(m, n, ax, ay, dx, dy);
+    std::vector<int> band(order, order + norder);
+    return extrapolate2(g, phi, s, band, fx, fy) ? 0 : 1;
+}
+
+int orc_newton(int m, int n, double ax, double ay, double dx, double dy,
+               const double *phi, const int *s, int i, int ij, double *out4) {
+    ls_grid g(m, n, ax, ay, dx, dy);
+    double x0, y0, x, y;
+    bool ok = newton(g, phi, s, i, ij, x0, y0, x, y);
+    out4[0] = x0; out4[1] = y0; out4[2] = x; out4[3] = y;
+    return ok ? 1 : 0;
+}
+
+double orc_bicubic(int m, int n, double ax, double ay, double dx, double dy,
+                   const double *phi, double x, double y, double *grad2) {
+    ls_grid g(m, n, ax, ay, dx, dy);
+    return bicubic(g, phi, x, y, grad2[0], grad2[1]);
+}
+
+}
+
+extern "C" {
+
+// Projection operators and solves (mgs_mac.hh:64-102, mgs_fem.hh:86-136).
+// irho may be null for the constant-density systems.
+void orc_mac_apply(int m, int n, double xxsp, double yysp, const double *irho,
+                   const double *x, double *y) {
+    mac_system s{m, n, xxsp, yysp, irho};
+    mac_apply(s, x, y);
+}
+void orc_fem_apply(int m, int n, double dx, double dy, const double *irho,
+                   const double *x, double *y) {
+    fem_system s{m, n, dx, dy, irho};
+    fem_apply(s, x, y);
+}
+int orc_mac_solve(int m, int n, double xxsp, double yysp, const double *irho,
+                  const double *b, double *z, double tol, double *relres) {
+    mac_system s{m, n, xxsp, yysp, irho};
+    return mac_solve(s, b, z, tol, relres);
+}
+int orc_fem_solve(int m, int n, double dx, double dy, const double *irho,
+                  const double *b, double *z, double tol, double *relres) {
+    fem_system s{m, n, dx, dy, irho};
+    return fem_solve(s, b, z, tol, relres);
+}
+
+}
+
+extern "C" {
+
+// fluid_2d construction for a no-slip box (fluid_2d.cc:26-43, 101-122).
+// params: 9 doubles per object: 6 shape parameters, G, rhos, ev_trans_mult.
+void *orc_fluid_create(int m, int n, double ax, double bx, double ay, double by,
+                       double visc, double rhof, double tw, double pinw,
+                       double w_scale, double w_force, double ev_mult,
+                       double dt_pad, double dt_ev_pad, double mg_tol, int nobj,
+                       const int *kinds, const double *params) {
+    sim_spec sp{m, n, ax, bx, ay, by, visc, rhof, tw, pinw, w_scale, w_force,
+                ev_mult, dt_pad, dt_ev_pad, mg_tol};
+    std::vector<object_spec> os(nobj);
+    for (int k = 0; k < nobj; k++) {
+        os[k].kind = kinds[k];
+        for (int q = 0; q < 6; q++) os[k].p[q] = params[9 * k + q];
+        os[k].G = params[9 * k + 6];
+        os[k].rhos = params[9 * k + 7];
+        os[k].ev_trans_mult = params[9 * k + 8];
+    }
+    return fluid_create(sp, os);
+}
+void orc_fluid_destroy(void *h) { fluid_destroy((fluid *)h); }
+double orc_fluid_dt_reg(void *h) { return fluid_dt_reg((fluid *)h); }
+double orc_fluid_time(void *h) { return fluid_time((fluid *)h); }
+int orc_fluid_step(void *h, double dt) { return fluid_step((fluid *)h, dt); }
+int orc_fluid_get(void *h, const char *name, int k, double *out) { return fluid_get((fluid *)h, name, k, out); }
+int orc_fluid_set(void *h, const char *name, int k, const double *in) { return fluid_set((fluid *)h, name, k, in); }
+void orc_fluid_stats(void *h, double *out4) { fluid_solver_stats((fluid *)h, out4); }
+
+}

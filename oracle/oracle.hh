@@ -1,0 +1,115 @@
+// CPU oracle for the IncRMT per-timestep hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This is the project's own C++ restatement of the
+// reference algorithm (chr1shr/incrmt), used by tests/ as the checker, by
+// __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline leg
+// as the timed CPU baseline. The product (incrmt_amd/) never links, loads or
+// calls it, and must fail loudly when its HIP library is missing.
+//
+// Each function cites the reference file:line it follows. Parity pins (what
+// checks each part against something independent of this file) are listed in
+// the test files named beside each section.
+#ifndef INCRMT_ORACLE_HH
+#define INCRMT_ORACLE_HH
+
+#include <vector>
+#include <cstddef>
+
+namespace orc {
+
+// ---------------------------------------------------------------- level set
+// A level-set grid: m x n points at (ax + i dx, ay + j dy), band cut-offs.
+// Follows levelset::levelset (levelset/levelset.cc:13-26).
+struct ls_grid {
+    int m, n, mn;
+    double ax, ay, dx, dy, xsp, ysp;
+    double phi_down, phi_up;
+    ls_grid(int m_, int n_, double ax_, double ay_, double dx_, double dy_,
+            double pd = -1e30, double pu = 1e30);
+};
+
+// Status codes of the narrow band (levelset.cc:199-263, heap.cc:525-526):
+// 0 far inside, 1 inside trial, 2 inside accepted, 3 inside interface,
+// 4 outside interface, 5 outside accepted, 6 outside trial, 7 far outside.
+
+// Bicubic interpolation with gradient (levelset.cc:63-79, built/bicubic.cc).
+double bicubic(const ls_grid &g, const double *phi, double x, double y,
+               double &phix, double &phiy);
+// Chopp's modified Newton closest point (levelset.cc:107-127) with the
+// compiled reference's integer abs (8 fixed iterations, |d|<1 acceptance).
+bool newton(const ls_grid &g, const double *phi, const int *s, int i, int ij,
+            double &x0, double &y0, double &x, double &y);
+bool newton_failsafe(const ls_grid &g, const double *phi, const int *s, int i,
+                     int ij, double x0, double y0, double &x, double &y);
+
+// levelset::build_band (levelset.cc:199-335). phi in/out, s out; band out is
+// the traversal order used by extension (ascending phi, inside part first).
+void build_band(const ls_grid &g, double *phi, int *s, std::vector<int> &band);
+
+// levelset::extrapolate_fields<obj_field> (traverse.hh:169-260) for two fields.
+// Returns false if a least-squares fit stays singular up to radius 7.
+bool extrapolate2(const ls_grid &g, const double *phi, const int *s,
+                  const std::vector<int> &band, double *fx, double *fy);
+
+// --------------------------------------------------------------- multigrid
+// Projection systems (mgs_mac.hh:64-102, mgs_fem.hh:86-136), non-periodic,
+// no zero-pressure column. irho layouts are the reference's own
+// (fluid_2d.cc:1140-1170).
+struct mac_system {
+    int m, n;
+    double xxsp, yysp;
+    const double *irho;  // 2mn+n+m edge inverse densities, or null (const rho)
+};
+struct fem_system {
+    int m, n;            // nodes
+    double dx, dy;
+    const double *irho;  // m*n (+m padding row) corner inverse densities, or null
+};
+// y = A x (full operator, diagonal included)
+void mac_apply(const mac_system &s, const double *x, double *y);
+void fem_apply(const fem_system &s, const double *x, double *y);
+// Solve A z = b to ||b - A z||_2 <= tol ||b||_2 (b's mean is projected out
+// first). z is the warm start on entry. Returns V-cycles, or -1 on failure.
+int mac_solve(const mac_system &s, const double *b, double *z, double tol,
+              double *relres);
+int fem_solve(const fem_system &s, const double *b, double *z, double tol,
+              double *relres);
+
+// ------------------------------------------------------------ fluid step
+enum obj_kind { OBJ_SIMPLE_SPIN = 0, OBJ_CIRCLE = 1 };
+
+struct object_spec {
+    int kind;
+    // obj_simple_spin(r_scale,a1,a2,piv_r,th_max,omega) (object.hh:422-467)
+    // or obj_circle(cx,cy,cr,grav) (object.hh:128-138)
+    double p[6];
+    double G, rhos, ev_trans_mult;
+};
+
+struct sim_spec {
+    int m, n;
+    double ax, bx, ay, by;
+    double visc, rhof, tw, pinw;
+    double w_scale, w_force;
+    double ev_mult, dt_pad, dt_ev_pad;
+    double mg_tol;  // relative residual for both projections
+};
+
+class fluid;  // opaque
+
+fluid *fluid_create(const sim_spec &sp, const std::vector<object_spec> &objs);
+void fluid_destroy(fluid *f);
+double fluid_dt_reg(fluid *f);
+double fluid_time(fluid *f);
+// fluid_2d::step_forward (fluid_2d.cc:298-369). Returns 0 or an error code.
+int fluid_step(fluid *f, double dt);
+// Field access on the ghosted (m+4)x(n+4) grid, row-major, the reference's
+// own fbase/sbase indexing (fluid_2d.cc:37, obj_field.cc:18).
+// name: u v p rho  (fluid)   X Y nX nY phi s  (object k)
+int fluid_get(fluid *f, const char *name, int k, double *out);
+int fluid_set(fluid *f, const char *name, int k, const double *in);
+void fluid_solver_stats(fluid *f, double *out4);
+
+}  // namespace orc
+
+#endif

@@ -1,0 +1,164 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and to the reference's
+level-set library compiled from its sources (oracle/_ref/libreflevel.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libreflevel.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+_libs = {}
+
+
+def load(path):
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build())")
+        _libs[path] = ctypes.CDLL(path)
+    return _libs[path]
+
+
+def oracle():
+    return load(ORACLE_SO)
+
+
+def reference():
+    return load(REF_SO)
+
+
+def have_reference():
+    return os.path.exists(REF_SO)
+
+
+def _band_call(lib, prefix, m, n, ax, ay, dx, dy, pd, pu, phi):
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    s = np.zeros(m * n, dtype=np.int32)
+    order = np.zeros(m * n, dtype=np.int32)
+    norder = ctypes.c_int(0)
+    f = getattr(lib, prefix + "build_band")
+    f.argtypes = [ctypes.c_int] * 2 + [ctypes.c_double] * 6 + [_dp, _ip, _ip, _ip]
+    f(m, n, ax, ay, dx, dy, pd, pu, dptr(phi), iptr(s), iptr(order), ctypes.byref(norder))
+    return phi, s, order[: norder.value].copy()
+
+
+def build_band(which, m, n, ax, ay, dx, dy, pd, pu, phi):
+    lib = oracle() if which == "oracle" else reference()
+    prefix = "orc_" if which == "oracle" else "ref_"
+    return _band_call(lib, prefix, m, n, ax, ay, dx, dy, pd, pu, phi)
+
+
+def build_band_extrapolate(which, m, n, ax, ay, dx, dy, pd, pu, phi, fx, fy):
+    lib = oracle() if which == "oracle" else reference()
+    prefix = "orc_" if which == "oracle" else "ref_"
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    fx = np.ascontiguousarray(fx, dtype=np.float64).copy()
+    fy = np.ascontiguousarray(fy, dtype=np.float64).copy()
+    s = np.zeros(m * n, dtype=np.int32)
+    f = getattr(lib, prefix + "build_band_extrapolate")
+    f.argtypes = [ctypes.c_int] * 2 + [ctypes.c_double] * 6 + [_dp, _ip, _dp, _dp]
+    rc = f(m, n, ax, ay, dx, dy, pd, pu, dptr(phi), iptr(s), dptr(fx), dptr(fy))
+    return rc, phi, s, fx, fy
+
+
+# ------------------------------------------------------------------ fluid
+SPIN = 0
+CIRCLE = 1
+
+
+def simple_spin_spec(m, omega=1.0):
+    """ftest simple-spin (ftest.cc:116-143): G=48, visc=1e-2, rhos=3,
+    obj_simple_spin(0.8,0.5,0.125,0.2,pi,omega) in a no-slip [-1,1]^2 box."""
+    import math
+    return dict(m=m, n=m, ax=-1.0, bx=1.0, ay=-1.0, by=1.0, visc=1e-2, rhof=1.0,
+                tw=2.5, pinw=2.5, w_scale=3.0, w_force=20.0, ev_mult=0.8, dt_pad=0.4,
+                dt_ev_pad=0.8,
+                objects=[(SPIN, [0.8, 0.5, 0.125, 0.2, math.pi, omega], 48.0, 3.0, 1.0)])
+
+
+class OracleFluid:
+    def __init__(self, spec, mg_tol=1e-13):
+        lib = oracle()
+        self.lib = lib
+        self.m, self.n = spec["m"], spec["n"]
+        self.ml, self.nl = self.m + 4, self.n + 4
+        objs = spec["objects"]
+        kinds = np.array([o[0] for o in objs], dtype=np.int32)
+        params = np.array([list(o[1]) + [o[2], o[3], o[4]] for o in objs], dtype=np.float64).ravel()
+        D, I = ctypes.c_double, ctypes.c_int
+        lib.orc_fluid_create.restype = ctypes.c_void_p
+        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp]
+        lib.orc_fluid_destroy.argtypes = [ctypes.c_void_p]
+        lib.orc_fluid_step.argtypes = [ctypes.c_void_p, D]
+        lib.orc_fluid_dt_reg.argtypes = [ctypes.c_void_p]
+        lib.orc_fluid_dt_reg.restype = D
+        lib.orc_fluid_time.argtypes = [ctypes.c_void_p]
+        lib.orc_fluid_time.restype = D
+        lib.orc_fluid_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, I, _dp]
+        lib.orc_fluid_set.argtypes = [ctypes.c_void_p, ctypes.c_char_p, I, _dp]
+        lib.orc_fluid_stats.argtypes = [ctypes.c_void_p, _dp]
+        self.h = lib.orc_fluid_create(
+            spec["m"], spec["n"], spec["ax"], spec["bx"], spec["ay"], spec["by"], spec["visc"],
+            spec["rhof"], spec["tw"], spec["pinw"], spec["w_scale"], spec["w_force"],
+            spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"], mg_tol, len(objs),
+            iptr(kinds), dptr(params))
+        self.nobj = len(objs)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_fluid_destroy(self.h)
+            self.h = None
+
+    @property
+    def dt_reg(self):
+        return self.lib.orc_fluid_dt_reg(self.h)
+
+    @property
+    def time(self):
+        return self.lib.orc_fluid_time(self.h)
+
+    def step(self, dt):
+        rc = self.lib.orc_fluid_step(self.h, dt)
+        if rc:
+            raise RuntimeError(f"oracle step failed with code {rc}")
+
+    def get(self, name, k=0):
+        out = np.zeros(self.ml * self.nl)
+        if self.lib.orc_fluid_get(self.h, name.encode(), k, dptr(out)):
+            raise KeyError(name)
+        return out.reshape(self.nl, self.ml)
+
+    def set(self, name, arr, k=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if self.lib.orc_fluid_set(self.h, name.encode(), k, dptr(a)):
+            raise KeyError(name)
+
+    def stats(self):
+        out = np.zeros(4)
+        self.lib.orc_fluid_stats(self.h, dptr(out))
+        return dict(mac_cycles=int(out[0]), mac_res=out[1], fem_cycles=int(out[2]), fem_res=out[3])
+
+
+def timestep_select(interval, dt_reg):
+    """fluid_2d::timestep_select (fluid_2d.hh:173-177)."""
+    l = int(interval / dt_reg) + 1
+    return l, interval / l
