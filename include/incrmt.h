@@ -1,0 +1,132 @@
+/*
+ * incrmt.h -- C ABI of the MI355X IncRMT hot path (libincrmt_hip.so).
+ *
+ * The reference (chr1shr/incrmt) is a C++ library with no FFI of its own:
+ * its drivers call fluid_2d and the mgs_* projection solvers directly. These
+ * entry points are what a C-ABI binding of exactly those calls needs; each
+ * cites the reference interface it replaces. Device state is resident on the
+ * GPU between calls; host arrays cross only in the upload/download calls.
+ *
+ * Conventions: every function returns an int status (INCRMT_OK on success),
+ * never exits the process (the reference exits on solver failure: callers
+ * that want that behaviour test the status). Handles are single-threaded:
+ * one HIP stream per handle, any number of handles per process.
+ */
+#ifndef INCRMT_H
+#define INCRMT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define INCRMT_OK 0
+#define INCRMT_ERR_ARG 1           /* bad argument or call order */
+#define INCRMT_ERR_NOT_CONVERGED 2 /* "V-cycle failed to converge" (mgs_mac.cc:67-70, mgs_fem.cc:116-119) */
+#define INCRMT_ERR_NEWTON 3        /* "Newton iteration failed." (levelset.cc:275,282) */
+#define INCRMT_ERR_BAND 4          /* narrow band overflow (levelset.cc:509) */
+#define INCRMT_ERR_SINGULAR_FIT 5  /* singular extension fit (traverse.hh:211-222) */
+#define INCRMT_ERR_HIP 6           /* HIP runtime error; see incrmt_last_error() */
+#define INCRMT_ERR_UNSUPPORTED 7   /* option outside the implemented path */
+
+/* projection solver kinds (mgs_mac_* and mgs_fem_*) */
+#define INCRMT_MG_MAC 0
+#define INCRMT_MG_FEM 1
+
+/* solver arithmetic */
+#define INCRMT_PREC_FP64 0  /* FP64 V-cycles */
+#define INCRMT_PREC_MIXED 1 /* FP32 V-cycles inside FP64 iterative refinement */
+
+typedef struct incrmt_fluid incrmt_fluid;
+typedef struct incrmt_mg incrmt_mg;
+
+typedef struct incrmt_stats {
+    int mac_cycles, fem_cycles;       /* V-cycles of the last step's solves */
+    double mac_rel_res, fem_rel_res;  /* their final ||b-Az||/||b|| */
+    long total_mac_cycles, total_fem_cycles, steps;
+    int reinit_passes, extend_passes; /* fixed-point passes of the last step */
+    double smoother_ms;               /* summed fine-level smoother time (timing on) */
+    long smoother_launches;
+} incrmt_stats;
+
+/* Last error message of the calling thread. */
+const char *incrmt_last_error(void);
+/* Device name and HBM size of device 0, for reports. */
+int incrmt_device_info(char *name, int len, double *hbm_gib);
+
+/* ---- fluid_2d ------------------------------------------------------------
+ * fluid_2d::fluid_2d(m,n,sim_type&,visc,rhof,tw,pinw,ntrace,fflags,dir)
+ * (fluid_2d.cc:26-43) with sim_type(ax,bx,ay,by,x_prd,y_prd,w_scale,w_force)
+ * (sim_type.hh:63-71). Only non-periodic boxes are implemented. */
+int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
+                        double w_scale, double w_force, double visc, double rhof, double tw, double pinw,
+                        incrmt_fluid **out);
+/* fluid_2d::add_object(new obj_simple_spin(r_scale,a1,a2,piv_r,th_max,omega),
+ * mat_const(G,rhos,ev_trans_mult)) (fluid_2d.cc:74-90, object.hh:449-455). */
+int incrmt_fluid_add_spin(incrmt_fluid *f, double r_scale, double a1, double a2, double piv_r, double th_max,
+                          double omega, double G, double rhos, double ev_trans_mult);
+/* fluid_2d::add_object(new obj_circle(cx,cy,cr,grav), mat_const) (object.hh:128-138). */
+int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
+                            double ev_trans_mult);
+/* fluid_2d::initialize(ev_mult,dt_pad,dt_ev_pad) (fluid_2d.cc:101-122). */
+int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, double dt_ev_pad);
+/* Projection stopping rule: relative residual ||b-Az||/||b|| (the
+ * reference's TGMG rule is not available; see mgs_common.hh:37). */
+int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int precision);
+/* fluid_2d::step_forward(dt) (fluid_2d.cc:298-369). */
+int incrmt_fluid_step(incrmt_fluid *f, double dt);
+/* k calls of step_forward(dt), the inner loop of fluid_2d::solve (fluid_2d.cc:274). */
+int incrmt_fluid_steps(incrmt_fluid *f, double dt, int k);
+/* fluid_2d::dt_reg, fluid_2d::time, fluid_2d::timestep_select (fluid_2d.hh:173-177). */
+double incrmt_fluid_dt_reg(const incrmt_fluid *f);
+double incrmt_fluid_time(const incrmt_fluid *f);
+int incrmt_fluid_timestep_select(const incrmt_fluid *f, double interval, double *adt);
+/* State on the reference's ghosted (m+4)x(n+4) grid, row-major, fbase /
+ * sbase indexing (fluid_2d.cc:37, obj_field.cc:18). Fluid names: u v p rho
+ * s11 s12 s21 s22; solid names (object index obj): X Y nX nY phi s. */
+int incrmt_fluid_download(incrmt_fluid *f, const char *name, int obj, double *host);
+int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double *host);
+/* The current simulation time (fluid_2d::time) may be set for restarts. */
+int incrmt_fluid_set_time(incrmt_fluid *f, double t);
+int incrmt_fluid_sync(incrmt_fluid *f);
+int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
+/* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
+int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
+void incrmt_fluid_destroy(incrmt_fluid *f);
+
+/* ---- mgs_mac_* / mgs_fem_* ------------------------------------------------
+ * Standalone solver, the constructor mgs_*_const_rho(m,n,x_prd,y_prd,dx,dy,src)
+ * (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM). */
+int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out);
+/* Varying density: the reference's irho layouts (mgs_mac_varying_rho::irho,
+ * 2mn+n+m edge values, fluid_2d.cc:1140-1160; mgs_fem_varying_rho::irho, m*n
+ * corner values, fluid_2d.cc:1163-1170). NULL restores constant density. */
+int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
+/* mgs_base::solve (mgs_common.hh:72-77): solve A z = b, z warm-started from
+ * its input, to ||b-Az|| <= rel_tol ||b|| (b's mean is projected out first). */
+int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
+                    int *cycles, double *rel_res);
+/* y = A x with the solver's operator (mgs_*::mul_a + a_cc, full row). */
+int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y);
+/* Time `sweeps` fine-level smoother sweeps on the resident problem with HIP
+ * events; returns the mean milliseconds per smoother launch. */
+int incrmt_mg_time_smoother(incrmt_mg *h, int sweeps, int precision, double *ms_per_launch, int *launches);
+void incrmt_mg_destroy(incrmt_mg *h);
+
+/* ---- levelset -------------------------------------------------------------
+ * levelset(m,n,ax,ay,dx,dy,phi_down,phi_up) + levelset::build_band()
+ * (levelset.cc:13-26, 199-335) on host arrays: phi (m*n, row-major) in: the
+ * pinned field, out: the narrow band; s out: status codes 0..7. passes: the
+ * fixed-point passes the rebuild took. */
+int incrmt_levelset_build_band(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                               double phi_up, double *phi, int *s, int *passes);
+/* build_band followed by levelset::extrapolate_fields<E>() of two fields
+ * (traverse.hh:235-260), as obj_field::extrapolate does for (nX,nY). */
+int incrmt_levelset_extrapolate(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                                double phi_up, double *phi, int *s, double *fx, double *fy, int *passes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INCRMT_H */

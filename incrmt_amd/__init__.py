@@ -1,0 +1,300 @@
+"""MI355X IncRMT hot path: Python front end of libincrmt_hip.so.
+
+The classes mirror the reference's interfaces for the per-timestep path:
+
+* ``Fluid2D``   -- fluid_2d (fluid_2d.hh:31-272): add_object, initialize,
+  step_forward, timestep_select, solve; state access on the reference's
+  ghosted (m+4)x(n+4) grid.
+* ``MgSolver``  -- the mgs_mac_* / mgs_fem_* projection solvers
+  (mgs_mac.hh:41-102, mgs_fem.hh:61-136) with their standalone constructor.
+* ``build_band`` / ``extrapolate`` -- levelset::build_band and
+  levelset::extrapolate_fields (levelset.cc:199-335, traverse.hh:235-260).
+
+Everything runs in the HIP library; there is no CPU fallback. Importing this
+module does not touch the GPU, but any call fails with ``IncrmtError`` when
+the library has not been built.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libincrmt_hip.so")
+
+OK = 0
+ERR_ARG, ERR_NOT_CONVERGED, ERR_NEWTON, ERR_BAND, ERR_SINGULAR_FIT, ERR_HIP, ERR_UNSUPPORTED = range(1, 8)
+MG_MAC, MG_FEM = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_D, _I, _V = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+
+
+class IncrmtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"incrmt error {code}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("mac_cycles", _I), ("fem_cycles", _I), ("mac_rel_res", _D), ("fem_rel_res", _D),
+                ("total_mac_cycles", ctypes.c_long), ("total_fem_cycles", ctypes.c_long),
+                ("steps", ctypes.c_long), ("reinit_passes", _I), ("extend_passes", _I),
+                ("smoother_ms", _D), ("smoother_launches", ctypes.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every entry point in include/incrmt.h
+_PROTOS = [
+    ("incrmt_last_error", ctypes.c_char_p, []),
+    ("incrmt_device_info", _I, [ctypes.c_char_p, _I, _dp]),
+    ("incrmt_fluid_create", _I, [_I, _I, _D, _D, _D, _D, _I, _I, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_V)]),
+    ("incrmt_fluid_add_spin", _I, [_V, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    ("incrmt_fluid_add_circle", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
+    ("incrmt_fluid_initialize", _I, [_V, _D, _D, _D]),
+    ("incrmt_fluid_set_solver", _I, [_V, _D, _I, _I]),
+    ("incrmt_fluid_step", _I, [_V, _D]),
+    ("incrmt_fluid_steps", _I, [_V, _D, _I]),
+    ("incrmt_fluid_dt_reg", _D, [_V]),
+    ("incrmt_fluid_time", _D, [_V]),
+    ("incrmt_fluid_timestep_select", _I, [_V, _D, _dp]),
+    ("incrmt_fluid_download", _I, [_V, ctypes.c_char_p, _I, _dp]),
+    ("incrmt_fluid_upload", _I, [_V, ctypes.c_char_p, _I, _dp]),
+    ("incrmt_fluid_set_time", _I, [_V, _D]),
+    ("incrmt_fluid_sync", _I, [_V]),
+    ("incrmt_fluid_get_stats", _I, [_V, ctypes.POINTER(Stats)]),
+    ("incrmt_fluid_time_smoothers", _I, [_V, _I]),
+    ("incrmt_fluid_destroy", None, [_V]),
+    ("incrmt_mg_create", _I, [_I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_V)]),
+    ("incrmt_mg_set_irho", _I, [_V, _dp]),
+    ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
+    ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
+    ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
+    ("incrmt_mg_destroy", None, [_V]),
+    ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
+    ("incrmt_levelset_extrapolate", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _dp, _dp, _ip]),
+]
+SYMBOLS = [p[0] for p in _PROTOS]
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises IncrmtError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IncrmtError(ERR_HIP, f"{LIB_PATH} is missing: build it with "
+                              "`python __graft_entry__.py` or incrmt_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _PROTOS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise IncrmtError(rc, lib().incrmt_last_error().decode())
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    hbm = _D(0)
+    _check(lib().incrmt_device_info(name, 256, ctypes.byref(hbm)))
+    return name.value.decode(), hbm.value
+
+
+class Fluid2D:
+    """fluid_2d (fluid_2d.hh:31-272) on one GPU, for a non-periodic box."""
+
+    def __init__(self, m, n, visc, rhof, tw=2.5, pinw=2.5, ax=-1.0, bx=1.0, ay=-1.0, by=1.0,
+                 x_prd=False, y_prd=False, w_scale=3.0, w_force=20.0):
+        L = lib()
+        h = _V()
+        _check(L.incrmt_fluid_create(m, n, ax, bx, ay, by, int(x_prd), int(y_prd), w_scale, w_force,
+                                     visc, rhof, tw, pinw, ctypes.byref(h)))
+        self._h = h
+        self.m, self.n = m, n
+        self.ml, self.nl = m + 4, n + 4
+        self.nobj = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().incrmt_fluid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # obj_simple_spin / obj_circle with mat_const (object.hh:128-138, 422-467)
+    def add_spin(self, r_scale, a1, a2, piv_r, th_max, omega, G, rhos, ev_trans_mult=1.0):
+        _check(lib().incrmt_fluid_add_spin(self._h, r_scale, a1, a2, piv_r, th_max, omega, G, rhos, ev_trans_mult))
+        self.nobj += 1
+
+    def add_circle(self, cx, cy, cr, grav, G, rhos, ev_trans_mult=1.0):
+        _check(lib().incrmt_fluid_add_circle(self._h, cx, cy, cr, grav, G, rhos, ev_trans_mult))
+        self.nobj += 1
+
+    def initialize(self, ev_mult=0.8, dt_pad=0.4, dt_ev_pad=0.8):
+        _check(lib().incrmt_fluid_initialize(self._h, ev_mult, dt_pad, dt_ev_pad))
+
+    def set_solver(self, rel_tol=1e-8, max_cycles=100, precision=PREC_FP64):
+        _check(lib().incrmt_fluid_set_solver(self._h, rel_tol, max_cycles, precision))
+
+    def step_forward(self, dt):
+        _check(lib().incrmt_fluid_step(self._h, dt))
+
+    def steps(self, dt, k):
+        _check(lib().incrmt_fluid_steps(self._h, dt, k))
+
+    def sync(self):
+        _check(lib().incrmt_fluid_sync(self._h))
+
+    @property
+    def dt_reg(self):
+        return lib().incrmt_fluid_dt_reg(self._h)
+
+    @property
+    def time(self):
+        return lib().incrmt_fluid_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        _check(lib().incrmt_fluid_set_time(self._h, t))
+
+    def timestep_select(self, interval):
+        adt = _D(0)
+        l = lib().incrmt_fluid_timestep_select(self._h, interval, ctypes.byref(adt))
+        if l < 0:
+            raise IncrmtError(ERR_ARG, "timestep_select before initialize")
+        return l, adt.value
+
+    def solve(self, duration, frames, callback=None):
+        """fluid_2d::solve (fluid_2d.cc:260-294) without file output."""
+        l, adt = self.timestep_select(duration / frames)
+        for k in range(1, frames + 1):
+            self.steps(adt, l)
+            if callback is not None:
+                callback(k, self)
+
+    def get(self, name, k=0):
+        out = np.zeros(self.ml * self.nl)
+        _check(lib().incrmt_fluid_download(self._h, name.encode(), k, _d(out)))
+        return out.reshape(self.nl, self.ml)
+
+    def set(self, name, arr, k=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if a.size != self.ml * self.nl:
+            raise IncrmtError(ERR_ARG, "field has the wrong size")
+        _check(lib().incrmt_fluid_upload(self._h, name.encode(), k, _d(a)))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().incrmt_fluid_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def time_smoothers(self, enable=True):
+        _check(lib().incrmt_fluid_time_smoothers(self._h, int(enable)))
+
+
+def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64):
+    """`ftest simple-spin m` (ftest.cc:116-143): G=48, visc=1e-2, rhos=3, the
+    three-pronged rotor obj_simple_spin(0.8,0.5,0.125,0.2,pi,omega) in a
+    no-slip [-1,1]^2 box, initialized with ev_mult 0.8, dt_pad 0.4,
+    dt_ev_pad 0.8 (ftest.cc:100-103)."""
+    f = Fluid2D(m, m, visc=1e-2, rhof=1.0, tw=2.5, pinw=2.5)
+    f.add_spin(0.8, 0.5, 0.125, 0.2, math.pi, omega, G=48.0, rhos=3.0, ev_trans_mult=1.0)
+    f.set_solver(rel_tol=rel_tol, precision=precision)
+    f.initialize(0.8, 0.4, 0.8)
+    return f
+
+
+class MgSolver:
+    """mgs_mac_* / mgs_fem_* with the standalone constructor
+    (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM)."""
+
+    def __init__(self, kind, m, n, dx, dy, x_prd=False, y_prd=False):
+        h = _V()
+        _check(lib().incrmt_mg_create(kind, m, n, int(x_prd), int(y_prd), dx, dy, ctypes.byref(h)))
+        self._h = h
+        self.kind, self.m, self.n = kind, m, n
+        self.z = np.zeros(m * n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().incrmt_mg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_irho(self, irho):
+        if irho is None:
+            _check(lib().incrmt_mg_set_irho(self._h, None))
+        else:
+            a = np.ascontiguousarray(irho, dtype=np.float64)
+            self._irho = a
+            _check(lib().incrmt_mg_set_irho(self._h, _d(a)))
+
+    def solve(self, b, rel_tol=1e-10, max_cycles=200, precision=PREC_FP64, z=None):
+        """Solve A z = b; z warm-starts from the previous solution
+        (mgs_common.hh:48-50). Returns (z, cycles, relative residual)."""
+        bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        if z is not None:
+            self.z = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
+        c, r = _I(0), _D(0)
+        _check(lib().incrmt_mg_solve(self._h, _d(bb), _d(self.z), rel_tol, max_cycles, precision,
+                                     ctypes.byref(c), ctypes.byref(r)))
+        return self.z.copy(), c.value, r.value
+
+    def apply(self, x):
+        xx = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.zeros_like(xx)
+        _check(lib().incrmt_mg_apply(self._h, _d(xx), _d(y)))
+        return y
+
+    def time_smoother(self, sweeps, precision=PREC_FP64):
+        ms, n = _D(0), _I(0)
+        _check(lib().incrmt_mg_time_smoother(self._h, sweeps, precision, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def build_band(m, n, ax, ay, dx, dy, phi_down, phi_up, phi):
+    """levelset::build_band on an m x n grid. Returns (phi, status, passes)."""
+    p = np.ascontiguousarray(phi, dtype=np.float64).ravel().copy()
+    s = np.zeros(m * n, dtype=np.int32)
+    passes = _I(0)
+    _check(lib().incrmt_levelset_build_band(m, n, ax, ay, dx, dy, phi_down, phi_up, _d(p), _i(s),
+                                            ctypes.byref(passes)))
+    return p, s, passes.value
+
+
+def extrapolate(m, n, ax, ay, dx, dy, phi_down, phi_up, phi, fx, fy):
+    """build_band then levelset::extrapolate_fields of (fx, fy).
+    Returns (phi, status, fx, fy, extension passes)."""
+    p = np.ascontiguousarray(phi, dtype=np.float64).ravel().copy()
+    a = np.ascontiguousarray(fx, dtype=np.float64).ravel().copy()
+    b = np.ascontiguousarray(fy, dtype=np.float64).ravel().copy()
+    s = np.zeros(m * n, dtype=np.int32)
+    passes = _I(0)
+    _check(lib().incrmt_levelset_extrapolate(m, n, ax, ay, dx, dy, phi_down, phi_up, _d(p), _i(s), _d(a), _d(b),
+                                             ctypes.byref(passes)))
+    return p, s, a, b, passes.value

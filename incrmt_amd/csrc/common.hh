@@ -1,0 +1,91 @@
+// Shared definitions of the MI355X IncRMT hot path: device layout of the
+// ghosted grid, per-solid descriptors, and the pointwise formulas of the
+// reference step that several kernels share.
+//
+// Layout. Every field is its own FP64 array (structure of arrays) over the
+// reference's ghosted (m+4) x (n+4) grid (fluid_2d.cc:37: fm = fbase+2ml+2).
+// Rows are padded to a pitch P that is a multiple of 32 doubles (256 B) and
+// the array origin is shifted so that interior column i = 0 of every row is
+// 256 B aligned: each 64-lane wavefront that sweeps a row reads whole 512 B
+// segments.
+#ifndef INCRMT_COMMON_HH
+#define INCRMT_COMMON_HH
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+namespace incrmt {
+
+constexpr int kMaxObj = 64;
+constexpr int kOrigin = 32;  // doubles before the row of ghost column -2, +2
+
+struct Grid {
+    int m, n;    // cells
+    int ml, nl;  // ghosted extents m+4, n+4 (the level-set grid)
+    int P;       // row pitch (doubles)
+    long long N; // allocation length (doubles)
+    double ax, bx, ay, by, dx, dy, xsp, ysp, xxsp, yysp;
+    double lax, lay;  // origin of the level-set grid: ax-1.5dx, ay-1.5dy (obj_field.cc:18)
+    // cell (i,j), i in [-2,m+1], j in [-2,n+1]
+    __host__ __device__ inline int at(int i, int j) const { return kOrigin + (j + 2) * P + i; }
+    // level-set point (I,J) = cell (I-2,J-2)
+    __host__ __device__ inline int ls(int I, int J) const { return kOrigin - 2 + J * P + I; }
+};
+
+enum ObjKind : int { kSpin = 0, kCircle = 1 };
+
+// Constants of one solid (obj_field.hh:32-135, object.hh:128-138, 422-467).
+struct ObjConst {
+    int kind;
+    double eps, pinw, G, rhos, ev_trans_mult, ex_visc, tf1, tf2, tf3;
+    double c0, c1, c2, piv_r, th_max, omega, K_stiff, piv_thresh;  // simple spin
+    double cx, cy, cr, grav;                                       // circle
+    double phi_down, phi_up;                                       // band cut-offs
+};
+
+// Device arrays of one solid (all on the ghosted grid).
+struct ObjArrays {
+    double *X, *Y, *nX, *nY;
+    double *Xx, *Xy, *Yx, *Yy;
+    double *Xl, *Yl, *Xr, *Yr, *Xd, *Yd, *Xu, *Yu;
+    double *phi;
+    int *st;     // narrow-band status 0..7 (levelset.cc:199-263, heap.cc:525-526)
+};
+
+// Pointwise device helpers shared by the kernels ------------------------
+
+__device__ inline double trans_func_in(const ObjConst &o, double x) {
+    return x < -o.eps ? 1 : 0.5 - x * o.tf1 - o.tf2 * sin(o.tf3 * x);  // obj_field.hh:107-112
+}
+__device__ inline double trans_func(const ObjConst &o, double x) {
+    return x > o.eps ? 0 : trans_func_in(o, x);  // obj_field.hh:102-104
+}
+__device__ inline double tderiv_func_in(const ObjConst &o, double x) {
+    return x < -o.eps ? 0 : 0.5 * (1 + cos(M_PI * x / o.eps));  // obj_field.hh:117-120
+}
+__device__ inline double coll_func(const ObjConst &o, double x) {
+    return x > o.eps ? 0 : 0.5 - x * o.tf1;  // obj_field.hh:97-99
+}
+// object::phi for the supported solids (object.cc:22-24, 167-171)
+__host__ __device__ inline double obj_phi(const ObjConst &o, double X, double Y) {
+    if (o.kind == kSpin) {
+        double th = atan2(Y, X), c = cos(3 * th);
+        return sqrt(X * X + Y * Y) - o.c0 - c * (o.c1 + o.c2 * c);
+    }
+    return sqrt(X * X + Y * Y) - o.cr;
+}
+
+}  // namespace incrmt
+
+#define HIP_CHECK(x)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (x);                                              \
+        if (e_ != hipSuccess) return ::incrmt::hip_fail(e_, #x, __FILE__, __LINE__); \
+    } while (0)
+
+namespace incrmt {
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+}
+
+#endif

@@ -1,0 +1,743 @@
+// Host engine of the MI355X IncRMT hot path and its C ABI (include/incrmt.h).
+//
+// FluidEngine keeps the whole fluid_2d state resident on one GPU in the
+// structure-of-arrays layout of common.hh and runs fluid_2d::step_forward
+// (fluid_2d.cc:298-369) as a sequence of kernels on one HIP stream: the
+// stencil sweeps of step.hip, the multigrid solves of mg.hip and the level
+// set of levelset.hip. Scalars the reference computes on the host once per
+// step (the anchor rotation of pre_stress_setup, the time) are computed here
+// with the same host arithmetic and passed by value.
+#include <cmath>
+#include <cstring>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <limits>
+#include "step.hh"
+#include "mg.hh"
+#include "levelset.hh"
+#include "../../include/incrmt.h"
+
+namespace incrmt {
+
+static thread_local std::string g_last_error;
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    g_last_error = buf;
+    return INCRMT_ERR_HIP;
+}
+static int fail(int code, const char *msg) {
+    g_last_error = msg;
+    return code;
+}
+
+namespace {
+
+constexpr int BXe = 64, BYe = 4;
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Trapezoid-weighted mean of the nodal pressure (fluid_2d.cc:1332-1348), as
+// a deterministic two-pass reduction: weights 1/2 on the boundary rows and
+// columns, the sum divided by m*n.
+__global__ void k_pavg_part(int mf, int nf, const double *z, int zp, double *part) {
+    __shared__ double s[256];
+    double a = 0;
+    const int N = mf * nf;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int i = p % mf, j = p / mf;
+        double w = (i == 0 || i == mf - 1) ? 0.5 : 1.0;
+        if (j == 0 || j == nf - 1) w *= 0.5;
+        a += w * z[j * zp + i];
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void k_pavg_final(const double *part, int nb, double scale, double *out) {
+    __shared__ double s[256];
+    double a = 0;
+    for (int p = threadIdx.x; p < nb; p += blockDim.x) a += part[p];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0] * scale;
+}
+
+}  // namespace
+
+struct Solid {
+    ObjConst c;
+    ObjArrays a;
+    LsWork w;
+};
+
+struct FluidEngine {
+    Grid g{};
+    double visc = 0, rhof = 1, tw = 2.5, pinw = 2.5, eps = 0, w_scale = 3, w_force = 20;
+    double dt_reg = 0, time = 0;
+    bool const_rho = true, initialized = false;
+    std::vector<Solid> obj;
+    ObjConst *d_oc = nullptr;
+    ObjArrays *d_oa = nullptr;
+    Fields f{};
+    Phys ph{};
+    std::vector<void *> allocs;
+    double *d_xrow = nullptr, *d_pavg = nullptr, *d_part = nullptr;
+    Multigrid mac, fem;
+    hipStream_t s = nullptr;
+    double mg_tol = 1e-8;
+    int max_cycles = 100, precision = INCRMT_PREC_FP64;
+    incrmt_stats st{};
+
+    ~FluidEngine() {
+        for (auto &o : obj) ls_free(o.w);
+        for (void *p : allocs) (void)hipFree(p);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    int dalloc(double *&p) {
+        void *q;
+        HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(double)));
+        HIP_CHECK(hipMemset(q, 0, (size_t)g.N * sizeof(double)));
+        allocs.push_back(q);
+        p = (double *)q;
+        return 0;
+    }
+    Solids solids() const { return Solids{d_oc, d_oa, (int)obj.size()}; }
+
+    // fluid_2d::choose_ev_and_dt + obj_field::set_ex_visc (fluid_2d.cc:152-192, obj_field.cc:35-49)
+    void choose_ev_and_dt(double ev_mult, double dt_pad, double dt_ev_pad) {
+        double hmax = g.dx > g.dy ? g.dx : g.dy, hmin = g.dx > g.dy ? g.dy : g.dx, sws_max = 0, ex_visc_max = 0,
+               rhos_div_ev_min = std::numeric_limits<double>::max();
+        for (auto &o : obj) {
+            double tfac = ev_mult * hmax, sws = std::sqrt(o.c.G / o.c.rhos);
+            if (sws > sws_max) sws_max = sws;
+            o.c.ex_visc = tfac * o.c.rhos * sws;
+            if (o.c.ex_visc > ex_visc_max) ex_visc_max = o.c.ex_visc;
+            double r_d_ev = o.c.rhos / (o.c.ex_visc * (o.c.ev_trans_mult + 1));
+            if (r_d_ev < rhos_div_ev_min) rhos_div_ev_min = r_d_ev;
+        }
+        double dt0 = hmin / sws_max * dt_pad, dt1 = 0.5 * rhos_div_ev_min / (g.xxsp + g.yysp) * dt_ev_pad,
+               dt2 = 0.5 * rhof / (visc * (g.xxsp + g.yysp)) * dt_pad;
+        if (dt0 < dt1) dt_reg = dt0 < dt2 ? dt0 : dt2;
+        else dt_reg = dt1 < dt2 ? dt1 : dt2;
+    }
+
+    // obj_simple_spin::pre_stress_setup (object.cc:175-181), host arithmetic
+    void pre_stress_setup() {
+        for (size_t k = 0; k < obj.size(); k++) {
+            const ObjConst &c = obj[k].c;
+            if (c.kind != kSpin) continue;
+            double ang = time * c.omega, theta = ang > 2 * M_PI ? 0 : c.th_max * (1 - std::cos(ang));
+            ph.rot[2 * k] = std::cos(theta);
+            ph.rot[2 * k + 1] = std::sin(theta);
+        }
+    }
+    int compute_stress() {
+        pre_stress_setup();
+        return launch_stress(s, g, f, solids(), ph);
+    }
+    int set_rho() { return launch_set_rho(s, g, f, solids(), rhof); }
+
+    int initialize(double ev_mult, double dt_pad, double dt_ev_pad);
+    int step(double dt);
+    int pin_all(int *passes);
+};
+
+int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
+    if (obj.empty()) return fail(INCRMT_ERR_ARG, "no solid added");
+    if ((int)obj.size() > kMaxObj) return fail(INCRMT_ERR_ARG, "too many solids");
+    choose_ev_and_dt(ev_mult, dt_pad, dt_ev_pad);
+    const_rho = true;
+    for (auto &o : obj)
+        if (std::fabs(o.c.rhos - rhof) > 100. * std::numeric_limits<double>::epsilon()) const_rho = false;
+    for (auto &o : obj) {
+        // obj_simple_spin::start_extra (object.cc:152-165)
+        if (o.c.kind == kSpin) {
+            o.c.piv_thresh = o.c.piv_r + o.c.eps;
+            o.c.piv_thresh *= o.c.piv_thresh;
+            o.c.K_stiff = 0.1 / (dt_reg * dt_reg);
+        }
+    }
+    // fields
+    double **fl[] = {&f.u, &f.v, &f.p, &f.rho, &f.ux, &f.uy, &f.vx, &f.vy, &f.c0, &f.c1, &f.s11, &f.s12, &f.s21, &f.s22,
+                     &f.ulc, &f.vdc};
+    for (auto p : fl)
+        if (int rc = dalloc(*p)) return rc;
+    for (int k = 0; k < 8; k++) {
+        if (int rc = dalloc(f.eA[k])) return rc;
+        if (int rc = dalloc(f.eB[k])) return rc;
+    }
+    // x of each column as compute_ustar forms it by repeated addition
+    std::vector<double> xr(g.m);
+    double x = g.ax + 0.5 * g.dx;
+    for (int i = 0; i < g.m; i++, x += g.dx) xr[i] = x;
+    HIP_CHECK(hipMalloc((void **)&d_xrow, g.m * sizeof(double)));
+    allocs.push_back(d_xrow);
+    HIP_CHECK(hipMemcpy(d_xrow, xr.data(), g.m * sizeof(double), hipMemcpyHostToDevice));
+    ph.xrow = d_xrow;
+    HIP_CHECK(hipMalloc((void **)&d_part, 1024 * sizeof(double)));
+    allocs.push_back(d_part);
+    d_pavg = d_part + 1000;
+    ph.visc = visc; ph.rhof = rhof; ph.eps = eps;
+    ph.w = w_scale * eps; ph.winv = 1 / ph.w; ph.k_rep = w_force * ph.winv;
+    // solids: reference map and level set on the ghosted grid (obj_field::init,
+    // fluid_2d.cc:200-207), evaluated on the host as the reference does
+    std::vector<double> hX(g.N, 0.0), hY(g.N, 0.0), hphi(g.N, 0.0);
+    for (auto &o : obj) {
+        double **al[] = {&o.a.X, &o.a.Y, &o.a.nX, &o.a.nY, &o.a.Xx, &o.a.Xy, &o.a.Yx, &o.a.Yy, &o.a.Xl, &o.a.Yl,
+                         &o.a.Xr, &o.a.Yr, &o.a.Xd, &o.a.Yd, &o.a.Xu, &o.a.Yu, &o.a.phi};
+        for (auto p : al)
+            if (int rc = dalloc(*p)) return rc;
+        void *q;
+        HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(int)));
+        HIP_CHECK(hipMemset(q, 0, (size_t)g.N * sizeof(int)));
+        allocs.push_back(q);
+        o.a.st = (int *)q;
+        if (int rc = ls_alloc(g, o.w)) return rc;
+        for (int j = -2; j < g.n + 2; j++) {
+            double y = g.ay + g.dy * (0.5 + j);
+            for (int i = -2; i < g.m + 2; i++) {
+                const int ij = g.at(i, j);
+                double xx = g.ax + g.dx * (0.5 + i), X, Y;
+                if (o.c.kind == kCircle) { X = xx - o.c.cx; Y = y - o.c.cy; }
+                else { X = xx; Y = y; }
+                hX[ij] = X; hY[ij] = Y;
+                hphi[ij] = obj_phi(o.c, X, Y);
+            }
+        }
+        HIP_CHECK(hipMemcpy(o.a.X, hX.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(o.a.Y, hY.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(o.a.nX, hX.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(o.a.nY, hY.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(o.a.phi, hphi.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_CHECK(hipMalloc((void **)&d_oc, obj.size() * sizeof(ObjConst)));
+    HIP_CHECK(hipMalloc((void **)&d_oa, obj.size() * sizeof(ObjArrays)));
+    allocs.push_back(d_oc);
+    allocs.push_back(d_oa);
+    std::vector<ObjConst> hc;
+    std::vector<ObjArrays> ha;
+    for (auto &o : obj) { hc.push_back(o.c); ha.push_back(o.a); }
+    HIP_CHECK(hipMemcpy(d_oc, hc.data(), hc.size() * sizeof(ObjConst), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(d_oa, ha.data(), ha.size() * sizeof(ObjArrays), hipMemcpyHostToDevice));
+    // projection solvers (fluid_2d::setup_multigrid, fluid_2d.cc:126-142)
+    if (int rc = mac.init(kMAC, g.m, g.n, g.dx, g.dy, s)) return rc;
+    if (int rc = fem.init(kFEM, g.m + 1, g.n + 1, g.dx, g.dy, s)) return rc;
+    // fluid_2d::init_fields (fluid_2d.cc:195-254)
+    for (auto &o : obj) {
+        int passes;
+        if (int rc = ls_rebuild(s, g, o.c, o.a, o.w, &passes)) return rc;
+    }
+    if (int rc = launch_init_velocity(s, g, f, solids(), rhof)) return rc;
+    if (int rc = launch_set_boundaries(s, g, f)) return rc;
+    if (int rc = compute_stress()) return rc;
+    if (int rc = set_rho()) return rc;
+    HIP_CHECK(hipStreamSynchronize(s));
+    initialized = true;
+    return 0;
+}
+
+int FluidEngine::pin_all(int *passes) {
+    for (auto &o : obj) {
+        int p = 0;
+        if (int rc = ls_pin_rebuild(s, g, o.c, o.a, o.w, &p)) return rc;
+        if (passes) *passes = p;
+    }
+    if (!const_rho) return set_rho();
+    return 0;
+}
+
+int FluidEngine::step(double dt) {
+    if (!initialized) return fail(INCRMT_ERR_ARG, "step before initialize");
+    const double ntime = time + dt;
+    int rc;
+    const Solids o = solids();
+    if ((rc = launch_normal_derivs(s, g, f, o, dt))) return rc;
+    if ((rc = launch_godunov(s, g, f, o, true))) return rc;
+    if ((rc = launch_tang_half(s, g, f, o, ph, dt))) return rc;
+    if ((rc = launch_godunov(s, g, f, o, false))) return rc;
+    time += 0.5 * dt;
+    // MAC projection (fluid_2d.cc:1175-1236)
+    MgLevel &ml = mac.fine();
+    if ((rc = launch_mac_src(s, g, f, ml.b, ml.pitch))) return rc;
+    if (!const_rho && (rc = mac.set_density(g, f.rho))) return rc;
+    MgStats ms{};
+    rc = precision == INCRMT_PREC_MIXED ? mac.solve_mixed(mg_tol, max_cycles, &ms) : mac.solve(mg_tol, max_cycles, &ms);
+    if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in MAC problem");
+    if (rc) return rc;
+    st.mac_cycles = ms.cycles; st.mac_rel_res = ms.rel_res; st.total_mac_cycles += ms.cycles;
+    if ((rc = launch_mac_correct_predict(s, g, f, o, ml.z, ml.pitch, ml.fw, ml.fs, ml.cp, const_rho, dt))) return rc;
+    // update_reference_map (fluid_2d.cc:1242-1273)
+    for (auto &ob : obj) {
+        int p = 0;
+        if ((rc = ls_extend(s, g, ob.a, ob.w, &p))) return rc == 5 ? fail(INCRMT_ERR_SINGULAR_FIT, "singular extension fit") : rc;
+        st.extend_passes = p;
+    }
+    if ((rc = launch_half_timestep(s, g, o))) return rc;
+    int passes = 0;
+    if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
+    st.reinit_passes = passes;
+    // u,v are unchanged since the last set_boundaries, so its call here
+    // (fluid_2d.cc:340) would rewrite the same ghost values
+    if ((rc = compute_stress())) return rc;
+    if ((rc = launch_ustar(s, g, f, o, ph, dt))) return rc;
+    time = ntime;
+    // finite-element projection (fluid_2d.cc:1297-1313)
+    MgLevel &fl = fem.fine();
+    if ((rc = launch_fem_src(s, g, f, dt, fl.b, fl.pitch))) return rc;
+    if (!const_rho && (rc = fem.set_density(g, f.rho))) return rc;
+    rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
+    if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
+    if (rc) return rc;
+    st.fem_cycles = ms.cycles; st.fem_rel_res = ms.rel_res; st.total_fem_cycles += ms.cycles;
+    // copy_pressure + update_velocity (fluid_2d.cc:1317-1374)
+    k_pavg_part<<<256, 256, 0, s>>>(g.m + 1, g.n + 1, fl.z, fl.pitch, d_part);
+    HIP_CHECK(hipGetLastError());
+    k_pavg_final<<<1, 256, 0, s>>>(d_part, 256, 1. / (double(g.m) * g.n), d_pavg);
+    HIP_CHECK(hipGetLastError());
+    if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return rc;
+    if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
+    if ((rc = launch_set_boundaries(s, g, f))) return rc;
+    if ((rc = compute_stress())) return rc;
+    st.steps++;
+    return 0;
+}
+
+}  // namespace incrmt
+
+using namespace incrmt;
+
+struct incrmt_fluid {
+    FluidEngine e;
+};
+struct incrmt_mg {
+    Multigrid mg;
+    hipStream_t s = nullptr;
+    double *x = nullptr, *y = nullptr;
+    ~incrmt_mg() {
+        if (x) (void)hipFree(x);
+        if (y) (void)hipFree(y);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+// Copy between a ghosted (m+4)x(n+4) host array and a padded device array.
+int copy_field(FluidEngine &e, double *dev, double *host, bool down) {
+    const Grid &g = e.g;
+    double *d0 = dev + g.ls(0, 0);
+    if (down)
+        HIP_CHECK(hipMemcpy2DAsync(host, g.ml * sizeof(double), d0, g.P * sizeof(double), g.ml * sizeof(double), g.nl,
+                                   hipMemcpyDeviceToHost, e.s));
+    else
+        HIP_CHECK(hipMemcpy2DAsync(d0, g.P * sizeof(double), host, g.ml * sizeof(double), g.ml * sizeof(double), g.nl,
+                                   hipMemcpyHostToDevice, e.s));
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    return 0;
+}
+
+double *field_ptr(FluidEngine &e, const std::string &nm, int k, bool &is_int) {
+    is_int = false;
+    Fields &f = e.f;
+    if (nm == "u") return f.u;
+    if (nm == "v") return f.v;
+    if (nm == "p") return f.p;
+    if (nm == "rho") return f.rho;
+    if (nm == "s11") return f.s11;
+    if (nm == "s12") return f.s12;
+    if (nm == "s21") return f.s21;
+    if (nm == "s22") return f.s22;
+    if (k < 0 || k >= (int)e.obj.size()) return nullptr;
+    ObjArrays &a = e.obj[k].a;
+    if (nm == "X") return a.X;
+    if (nm == "Y") return a.Y;
+    if (nm == "nX") return a.nX;
+    if (nm == "nY") return a.nY;
+    if (nm == "phi") return a.phi;
+    if (nm == "s") { is_int = true; return (double *)a.st; }
+    return nullptr;
+}
+
+int add_object(incrmt_fluid *f, const ObjConst &c0) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "add_object after initialize");
+    Solid so{};
+    so.c = c0;
+    FluidEngine &e = f->e;
+    // obj_field constants (obj_field.cc:14-20)
+    so.c.eps = e.eps;
+    so.c.pinw = e.pinw * e.g.dx;
+    so.c.tf1 = 0.5 / e.eps;
+    so.c.tf2 = 0.5 / M_PI;
+    so.c.tf3 = M_PI / e.eps;
+    so.c.phi_down = -e.eps - 4.5 * e.g.dx;
+    so.c.phi_up = e.eps + 4.5 * e.g.dx;
+    e.obj.push_back(so);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *incrmt_last_error(void) { return g_last_error.c_str(); }
+
+int incrmt_device_info(char *name, int len, double *hbm_gib) {
+    hipDeviceProp_t p;
+    HIP_CHECK(hipGetDeviceProperties(&p, 0));
+    if (name && len > 0) {
+        strncpy(name, p.name, len - 1);
+        name[len - 1] = 0;
+    }
+    if (hbm_gib) *hbm_gib = p.totalGlobalMem / 1073741824.0;
+    return 0;
+}
+
+int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
+                        double w_scale, double w_force, double visc, double rhof, double tw, double pinw,
+                        incrmt_fluid **out) {
+    if (!out) return fail(INCRMT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (m < 8 || n < 8 || m > 65536 || n > 65536) return fail(INCRMT_ERR_ARG, "grid size out of range");
+    if (x_prd || y_prd) return fail(INCRMT_ERR_UNSUPPORTED, "periodic boundaries are not implemented");
+    if (!(bx > ax) || !(by > ay) || !(visc > 0) || !(rhof > 0) || !(tw > 0)) return fail(INCRMT_ERR_ARG, "bad geometry or material");
+    incrmt_fluid *f = new incrmt_fluid;
+    FluidEngine &e = f->e;
+    Grid &g = e.g;
+    g.m = m; g.n = n; g.ml = m + 4; g.nl = n + 4;
+    g.P = round_up(m + 4, 32);
+    g.N = (long long)kOrigin + (long long)(n + 4) * g.P + 64;
+    g.ax = ax; g.bx = bx; g.ay = ay; g.by = by;
+    g.dx = (bx - ax) / m; g.dy = (by - ay) / n;
+    g.xsp = 1 / g.dx; g.ysp = 1 / g.dy; g.xxsp = g.xsp * g.xsp; g.yysp = g.ysp * g.ysp;
+    g.lax = ax - 1.5 * g.dx; g.lay = ay - 1.5 * g.dy;
+    e.visc = visc; e.rhof = rhof; e.tw = tw; e.pinw = pinw; e.w_scale = w_scale; e.w_force = w_force;
+    e.eps = tw * (g.dx < g.dy ? g.dx : g.dy);
+    hipError_t er = hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking);
+    if (er != hipSuccess) {
+        delete f;
+        return hip_fail(er, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    *out = f;
+    return 0;
+}
+
+int incrmt_fluid_add_spin(incrmt_fluid *f, double r_scale, double a1, double a2, double piv_r, double th_max,
+                          double omega, double G, double rhos, double ev_trans_mult) {
+    ObjConst c{};
+    c.kind = kSpin;
+    const double scf = r_scale / (1 + a1 + a2);  // object.hh:449-455
+    c.c0 = scf * (1 - a2); c.c1 = scf * a1; c.c2 = 2 * scf * a2;
+    c.piv_r = piv_r; c.th_max = th_max; c.omega = omega;
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
+                            double ev_trans_mult) {
+    ObjConst c{};
+    c.kind = kCircle;
+    c.cx = cx; c.cy = cy; c.cr = cr; c.grav = grav;
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, double dt_ev_pad) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "initialize called twice");
+    return f->e.initialize(ev_mult, dt_pad, dt_ev_pad);
+}
+
+int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int precision) {
+    if (!f || !(rel_tol > 0) || max_cycles < 1 || (precision != INCRMT_PREC_FP64 && precision != INCRMT_PREC_MIXED))
+        return fail(INCRMT_ERR_ARG, "bad solver settings");
+    f->e.mg_tol = rel_tol;
+    f->e.max_cycles = max_cycles;
+    f->e.precision = precision;
+    return 0;
+}
+
+int incrmt_fluid_step(incrmt_fluid *f, double dt) {
+    if (!f || !(dt > 0)) return fail(INCRMT_ERR_ARG, "bad step");
+    return f->e.step(dt);
+}
+
+int incrmt_fluid_steps(incrmt_fluid *f, double dt, int k) {
+    if (!f || !(dt > 0) || k < 0) return fail(INCRMT_ERR_ARG, "bad step");
+    for (int q = 0; q < k; q++)
+        if (int rc = f->e.step(dt)) return rc;
+    return 0;
+}
+
+double incrmt_fluid_dt_reg(const incrmt_fluid *f) { return f ? f->e.dt_reg : 0; }
+double incrmt_fluid_time(const incrmt_fluid *f) { return f ? f->e.time : 0; }
+
+int incrmt_fluid_timestep_select(const incrmt_fluid *f, double interval, double *adt) {
+    if (!f || !adt || !(f->e.dt_reg > 0)) return -1;
+    int l = static_cast<int>(interval / f->e.dt_reg) + 1;  // fluid_2d.hh:173-177
+    *adt = interval / l;
+    return l;
+}
+
+int incrmt_fluid_download(incrmt_fluid *f, const char *name, int obj, double *host) {
+    if (!f || !name || !host || !f->e.initialized) return fail(INCRMT_ERR_ARG, "bad download");
+    bool is_int;
+    double *p = field_ptr(f->e, name, obj, is_int);
+    if (!p) return fail(INCRMT_ERR_ARG, "unknown field");
+    FluidEngine &e = f->e;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    if (!is_int) return copy_field(e, p, host, true);
+    std::vector<int> tmp(e.g.N);
+    HIP_CHECK(hipMemcpy(tmp.data(), p, e.g.N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int J = 0; J < e.g.nl; J++)
+        for (int I = 0; I < e.g.ml; I++) host[J * e.g.ml + I] = tmp[e.g.ls(I, J)];
+    return 0;
+}
+
+int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double *host) {
+    if (!f || !name || !host || !f->e.initialized) return fail(INCRMT_ERR_ARG, "bad upload");
+    bool is_int;
+    double *p = field_ptr(f->e, name, obj, is_int);
+    if (!p) return fail(INCRMT_ERR_ARG, "unknown field");
+    FluidEngine &e = f->e;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    if (!is_int) return copy_field(e, p, const_cast<double *>(host), false);
+    std::vector<int> tmp(e.g.N, 0);
+    HIP_CHECK(hipMemcpy(tmp.data(), p, e.g.N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int J = 0; J < e.g.nl; J++)
+        for (int I = 0; I < e.g.ml; I++) tmp[e.g.ls(I, J)] = (int)host[J * e.g.ml + I];
+    HIP_CHECK(hipMemcpy(p, tmp.data(), e.g.N * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int incrmt_fluid_set_time(incrmt_fluid *f, double t) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    f->e.time = t;
+    return 0;
+}
+
+int incrmt_fluid_sync(incrmt_fluid *f) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    HIP_CHECK(hipStreamSynchronize(f->e.s));
+    return 0;
+}
+
+int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out) {
+    if (!f || !out) return fail(INCRMT_ERR_ARG, "null argument");
+    *out = f->e.st;
+    out->smoother_ms = f->e.mac.smoother_ms + f->e.fem.smoother_ms;
+    out->smoother_launches = f->e.mac.smoother_launches + f->e.fem.smoother_launches;
+    return 0;
+}
+
+int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    f->e.mac.time_smoother = f->e.fem.time_smoother = enable != 0;
+    f->e.mac.smoother_ms = f->e.fem.smoother_ms = 0;
+    f->e.mac.smoother_launches = f->e.fem.smoother_launches = 0;
+    return 0;
+}
+
+void incrmt_fluid_destroy(incrmt_fluid *f) { delete f; }
+
+// ------------------------------------------------------------ standalone MG
+int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out) {
+    if (!out) return fail(INCRMT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
+    if (x_prd || y_prd) return fail(INCRMT_ERR_UNSUPPORTED, "periodic boundaries are not implemented");
+    if (m < 2 || n < 2 || !(dx > 0) || !(dy > 0)) return fail(INCRMT_ERR_ARG, "bad grid");
+    incrmt_mg *h = new incrmt_mg;
+    hipError_t er = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
+    if (er != hipSuccess) {
+        delete h;
+        return hip_fail(er, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, m, n, dx, dy, h->s);
+    if (!rc) {
+        const MgLevel &l = h->mg.fine();
+        const size_t len = (size_t)(l.ny + 2) * l.pitch + 64;
+        if (hipMalloc((void **)&h->x, len * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&h->y, len * sizeof(double)) != hipSuccess ||
+            hipMemset(h->x, 0, len * sizeof(double)) != hipSuccess ||
+            hipMemset(h->y, 0, len * sizeof(double)) != hipSuccess)
+            rc = fail(INCRMT_ERR_HIP, "allocation failed");
+    }
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int incrmt_mg_set_irho(incrmt_mg *h, const double *irho) {
+    if (!h) return fail(INCRMT_ERR_ARG, "null handle");
+    int rc = h->mg.set_irho_host(irho);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+static int mg_copy(incrmt_mg *h, double *dev, const double *host_in, double *host_out) {
+    const MgLevel &l = h->mg.fine();
+    if (host_in)
+        HIP_CHECK(hipMemcpy2DAsync(dev, l.pitch * sizeof(double), host_in, l.nx * sizeof(double), l.nx * sizeof(double),
+                                   l.ny, hipMemcpyHostToDevice, h->s));
+    else
+        HIP_CHECK(hipMemcpy2DAsync(host_out, l.nx * sizeof(double), dev, l.pitch * sizeof(double), l.nx * sizeof(double),
+                                   l.ny, hipMemcpyDeviceToHost, h->s));
+    HIP_CHECK(hipStreamSynchronize(h->s));
+    return 0;
+}
+
+int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
+                    int *cycles, double *rel_res) {
+    if (!h || !b || !z || !(rel_tol > 0) || max_cycles < 1) return fail(INCRMT_ERR_ARG, "bad solve arguments");
+    MgLevel &l = h->mg.fine();
+    if (int rc = mg_copy(h, l.b, b, nullptr)) return rc;
+    if (int rc = mg_copy(h, l.z, z, nullptr)) return rc;
+    MgStats st{};
+    int rc = precision == INCRMT_PREC_MIXED ? h->mg.solve_mixed(rel_tol, max_cycles, &st)
+                                            : h->mg.solve(rel_tol, max_cycles, &st);
+    if (rc && rc != 2) return rc;
+    if (int r2 = mg_copy(h, l.z, nullptr, z)) return r2;
+    if (cycles) *cycles = st.cycles;
+    if (rel_res) *rel_res = st.rel_res;
+    return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;
+}
+
+int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y) {
+    if (!h || !x || !y) return fail(INCRMT_ERR_ARG, "null argument");
+    const MgLevel &l = h->mg.fine();
+    double *xd = h->x + l.pitch + 1, *yd = h->y + l.pitch + 1;
+    if (int rc = mg_copy(h, xd, x, nullptr)) return rc;
+    if (int rc = h->mg.apply(xd, yd)) return rc;
+    return mg_copy(h, yd, nullptr, y);
+}
+
+int incrmt_mg_time_smoother(incrmt_mg *h, int sweeps, int precision, double *ms_per_launch, int *launches) {
+    if (!h || sweeps < 1) return fail(INCRMT_ERR_ARG, "bad arguments");
+    long n = 0;
+    int rc = h->mg.time_fine_smoother(sweeps, precision == INCRMT_PREC_MIXED, ms_per_launch, &n);
+    if (launches) *launches = (int)n;
+    return rc;
+}
+
+void incrmt_mg_destroy(incrmt_mg *h) { delete h; }
+
+// ------------------------------------------------------- standalone level set
+namespace {
+struct LsStandalone {
+    Grid g{};
+    ObjConst c{};
+    ObjArrays a{};
+    LsWork w{};
+    hipStream_t s = nullptr;
+    std::vector<void *> allocs;
+    ~LsStandalone() {
+        ls_free(w);
+        for (void *p : allocs) (void)hipFree(p);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    // an m x n level-set grid is the ghosted grid of (m-4) x (n-4) cells
+    int init(int m, int n, double ax, double ay, double dx, double dy, double pd, double pu) {
+        g.m = m - 4; g.n = n - 4; g.ml = m; g.nl = n;
+        g.P = round_up(m, 32);
+        g.N = (long long)kOrigin + (long long)n * g.P + 64;
+        g.dx = dx; g.dy = dy; g.ax = ax + 1.5 * dx; g.ay = ay + 1.5 * dy;
+        g.lax = ax; g.lay = ay;
+        g.bx = g.ax + g.m * dx; g.by = g.ay + g.n * dy;
+        g.xsp = 1 / dx; g.ysp = 1 / dy; g.xxsp = g.xsp * g.xsp; g.yysp = g.ysp * g.ysp;
+        c.phi_down = pd; c.phi_up = pu;
+        HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        double **al[] = {&a.phi, &a.nX, &a.nY};
+        for (auto p : al) {
+            void *q;
+            HIP_CHECK(hipMalloc(&q, g.N * sizeof(double)));
+            HIP_CHECK(hipMemset(q, 0, g.N * sizeof(double)));
+            allocs.push_back(q);
+            *p = (double *)q;
+        }
+        void *q;
+        HIP_CHECK(hipMalloc(&q, g.N * sizeof(int)));
+        HIP_CHECK(hipMemset(q, 0, g.N * sizeof(int)));
+        allocs.push_back(q);
+        a.st = (int *)q;
+        return ls_alloc(g, w);
+    }
+    int put(double *dev, const double *host) {
+        HIP_CHECK(hipMemcpy2D(dev + g.ls(0, 0), g.P * sizeof(double), host, g.ml * sizeof(double),
+                              g.ml * sizeof(double), g.nl, hipMemcpyHostToDevice));
+        return 0;
+    }
+    int get(const double *dev, double *host) {
+        HIP_CHECK(hipMemcpy2D(host, g.ml * sizeof(double), dev + g.ls(0, 0), g.P * sizeof(double),
+                              g.ml * sizeof(double), g.nl, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    int get_status(int *host) {
+        HIP_CHECK(hipMemcpy2D(host, g.ml * sizeof(int), a.st + g.ls(0, 0), g.P * sizeof(int), g.ml * sizeof(int),
+                              g.nl, hipMemcpyDeviceToHost));
+        return 0;
+    }
+};
+int ls_args_ok(int m, int n, double dx, double dy, double pd, double pu) {
+    if (m < 8 || n < 8 || !(dx > 0) || !(dy > 0) || !(pd < 0) || !(pu > 0))
+        return fail(INCRMT_ERR_ARG, "bad level-set arguments");
+    return 0;
+}
+}  // namespace
+
+int incrmt_levelset_build_band(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                               double phi_up, double *phi, int *s, int *passes) {
+    if (int rc = ls_args_ok(m, n, dx, dy, phi_down, phi_up)) return rc;
+    if (!phi || !s) return fail(INCRMT_ERR_ARG, "null array");
+    LsStandalone L;
+    if (int rc = L.init(m, n, ax, ay, dx, dy, phi_down, phi_up)) return rc;
+    if (int rc = L.put(L.a.phi, phi)) return rc;
+    int p = 0;
+    int rc = ls_rebuild(L.s, L.g, L.c, L.a, L.w, &p);
+    if (rc == 3) return fail(INCRMT_ERR_NEWTON, "Newton iteration failed.");
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(L.s));
+    if (passes) *passes = p;
+    if (int r2 = L.get(L.a.phi, phi)) return r2;
+    return L.get_status(s);
+}
+
+int incrmt_levelset_extrapolate(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                                double phi_up, double *phi, int *s, double *fx, double *fy, int *passes) {
+    if (int rc = ls_args_ok(m, n, dx, dy, phi_down, phi_up)) return rc;
+    if (!phi || !s || !fx || !fy) return fail(INCRMT_ERR_ARG, "null array");
+    LsStandalone L;
+    if (int rc = L.init(m, n, ax, ay, dx, dy, phi_down, phi_up)) return rc;
+    if (int rc = L.put(L.a.phi, phi)) return rc;
+    if (int rc = L.put(L.a.nX, fx)) return rc;
+    if (int rc = L.put(L.a.nY, fy)) return rc;
+    int p = 0;
+    int rc = ls_rebuild(L.s, L.g, L.c, L.a, L.w, &p);
+    if (rc == 3) return fail(INCRMT_ERR_NEWTON, "Newton iteration failed.");
+    if (rc) return rc;
+    rc = ls_extend(L.s, L.g, L.a, L.w, &p);
+    if (rc == 5) return fail(INCRMT_ERR_SINGULAR_FIT, "singular extension fit");
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(L.s));
+    if (passes) *passes = p;
+    if (int r2 = L.get(L.a.phi, phi)) return r2;
+    if (int r2 = L.get(L.a.nX, fx)) return r2;
+    if (int r2 = L.get(L.a.nY, fy)) return r2;
+    return L.get_status(s);
+}
+
+}  // extern "C"

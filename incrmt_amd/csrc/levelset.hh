@@ -1,0 +1,40 @@
+// Narrow-band level set of one solid on the GPU: pinning to the reference
+// map, band rebuild (reinitialisation) and reference-map extension.
+#ifndef INCRMT_LEVELSET_HH
+#define INCRMT_LEVELSET_HH
+
+#include "common.hh"
+
+namespace incrmt {
+
+// Counter slots in LsWork::cnt
+enum { kCntIface = 0, kCntCand, kCntPband, kCntErr, kCntIter };
+constexpr int kIterSlots = 64;
+enum { kErrNewton = 1, kErrSingular = 2 };
+
+struct LsWork {
+    double *va, *vb;      // fixed-point buffers (ghosted grid)
+    double *ex, *ey;      // extension buffers (ghosted grid), ping
+    double *fx, *fy;      // extension buffers, pong
+    int *iface, *cand, *pband;  // point lists
+    unsigned char *mark;  // dilation scratch
+    int *cnt;             // counters (kCntIter + kIterSlots ints)
+    int *hcnt;            // pinned host mirror
+};
+
+int ls_alloc(const Grid &g, LsWork &w);
+void ls_free(LsWork &w);
+
+// obj_field::pin + levelset::build_band (obj_field.cc:92-97,
+// levelset.cc:199-335). Returns 0, or an INCRMT_ERR_* code.
+int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                   int *fim_iters);
+// levelset::build_band alone on the current phi (initialisation).
+int ls_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, int *fim_iters);
+// obj_field::extrapolate (traverse.hh:169-260) over the band of the last
+// rebuild. Returns 0, or an INCRMT_ERR_* code.
+int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *ext_iters);
+
+}  // namespace incrmt
+
+#endif

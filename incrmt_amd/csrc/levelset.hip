@@ -1,0 +1,562 @@
+// GPU narrow-band level set (see levelset.hh).
+//
+// Rebuild. The reference rebuilds the band serially: a sign-change scan, an
+// exact closest-point distance for the two interface layers (Newton on a
+// bicubic), then a heap-ordered second-order fast march out to the band
+// limit (levelset.cc:199-335, heap.cc). Here the scan and the Newton solves
+// are one thread per point, and the march is replaced by a Jacobi fixed-point
+// iteration of the SAME local update (eikonal.hh) over the points near the
+// interface: a neighbour counts as known when its current value is upwind of
+// the point's own, which is exactly the set the heap has accepted when it
+// accepts the point. Where the heap's order is monotone and free of exact
+// ties the fixed point equals the heap's result bit for bit.
+//
+// Extension. The reference extends (nX,nY) outward in increasing-phi order,
+// each point by a weighted least-squares fit over the neighbours with
+// smaller phi (traverse.hh:169-260). The set a point reads depends on phi
+// alone, so a Jacobi iteration over the outside band reaches the sequential
+// result exactly once every dependency chain has been walked; it stops when
+// an iteration changes no value.
+#include <cstdio>
+#include "levelset.hh"
+#include "eikonal.hh"
+
+namespace incrmt {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+constexpr int kListBlocks = 1024, kListThreads = 256;
+constexpr int kDilate = 10;  // candidate radius (cells) around the interface
+
+inline dim3 grid2(int nx, int ny) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY); }
+
+__device__ inline void ls_coords(const Grid &g, int ij, int &I, int &J) {
+    const int r = ij - (kOrigin - 2);
+    J = r / g.P;
+    I = r - J * g.P;
+}
+
+// ---- bicubic with gradient on the ghosted grid (levelset.cc:63-79,
+// built/bicubic.cc); evaluation order as in the reference.
+struct xrow { double v, d; };
+__device__ inline xrow herm_mid(double pl, double f0, double f1, double pr, double e, double me) {
+    xrow r;
+    r.v = pl * e * me * me + f0 * me * me * (1 + 2 * e) + f1 * e * e * (3 - 2 * e) - pr * e * e * me;
+    r.d = pl * me * (1 - 3 * e) - f0 * 6 * e * me + f1 * 6 * e * me + pr * e * (3 * e - 2);
+    return r;
+}
+__device__ inline xrow herm_left(double f0, double f1, double pr, double e, double me) {
+    xrow r;
+    r.v = f0 * me * me + f1 * e * (2 - e) - pr * e * me;
+    r.d = -f0 * 2 * me + f1 * 2 * me + pr * (2 * e - 1);
+    return r;
+}
+__device__ inline xrow herm_right(double pl, double f0, double f1, double e, double me) {
+    xrow r;
+    r.v = pl * e * me + f0 * (1 - e * e) + f1 * e * e;
+    r.d = pl * (1 - 2 * e) - f0 * 2 * e + f1 * 2 * e;
+    return r;
+}
+
+struct LsGeom {
+    int m, n, P;
+    double ax, ay, dx, dy, xsp, ysp;
+};
+
+__device__ double bicubic(const LsGeom &g, const double *base, double x, double y, double &phix, double &phiy) {
+    const int m = g.m, n = g.n, P = g.P;
+    const double dxf = 0.5, dyf = 0.5, d2f = 0.25;
+    double e = (x - g.ax) * g.xsp, f = (y - g.ay) * g.ysp;
+    int i = int(e), j = int(f);
+    int xc, yc, ic, jc;
+    if (i < 1) { xc = 0; ic = 0; }
+    else if (i >= m - 2) { xc = 2; ic = m - 2; e -= double(m - 2); }
+    else { xc = 1; ic = i; e -= double(i); }
+    if (j < 1) { yc = 0; jc = 0; }
+    else if (j >= n - 2) { yc = 2; jc = n - 2; f -= double(n - 2); }
+    else { yc = 1; jc = j; f -= double(j); }
+    const double me = 1 - e, mf = 1 - f;
+    const double *p = base + (jc * P + ic);
+    auto row = [&](double l, double v0, double v1, double r) -> xrow {
+        return xc == 1 ? herm_mid(l, v0, v1, r, e, me) : xc == 0 ? herm_left(v0, v1, r, e, me) : herm_right(l, v0, v1, e, me);
+    };
+    double plbx = 0, pltx = 0, prbx = 0, prtx = 0;
+    if (xc != 0) { plbx = (p[1] - p[-1]) * dxf; pltx = (p[P + 1] - p[P - 1]) * dxf; }
+    if (xc != 2) { prbx = (p[2] - p[0]) * dxf; prtx = (p[P + 2] - p[P]) * dxf; }
+    xrow r1 = row(plbx, p[0], p[1], prbx);
+    xrow r2 = row(pltx, p[P], p[P + 1], prtx);
+    xrow r0 = {0, 0}, r3 = {0, 0};
+    if (yc != 0) {
+        double plby = (p[P] - p[-P]) * dyf, prby = (p[P + 1] - p[-P + 1]) * dyf, plbxy = 0, prbxy = 0;
+        if (xc != 0) plbxy = (p[P + 1] - p[-P + 1] - p[P - 1] + p[-P - 1]) * d2f;
+        if (xc != 2) prbxy = (p[P + 2] - p[-P + 2] - p[P] + p[-P]) * d2f;
+        r0 = row(plbxy, plby, prby, prbxy);
+    }
+    if (yc != 2) {
+        double plty = (p[2 * P] - p[0]) * dyf, prty = (p[2 * P + 1] - p[1]) * dyf, pltxy = 0, prtxy = 0;
+        if (xc != 0) pltxy = (p[2 * P + 1] - p[1] - p[2 * P - 1] + p[-1]) * d2f;
+        if (xc != 2) prtxy = (p[2 * P + 2] - p[2] - p[2 * P] + p[0]) * d2f;
+        r3 = row(pltxy, plty, prty, prtxy);
+    }
+    if (yc == 1) {
+        phix = g.xsp * (f * mf * mf * r0.d + mf * mf * (1 + 2 * f) * r1.d + f * f * (3 - 2 * f) * r2.d - f * f * mf * r3.d);
+        phiy = g.ysp * (mf * (1 - 3 * f) * r0.v - 6 * f * mf * r1.v + 6 * f * mf * r2.v + f * (3 * f - 2) * r3.v);
+        return f * mf * mf * r0.v + mf * mf * (1 + 2 * f) * r1.v + f * f * (3 - 2 * f) * r2.v - f * f * mf * r3.v;
+    } else if (yc == 0) {
+        phix = g.xsp * (mf * mf * r1.d + f * (2 - f) * r2.d - f * mf * r3.d);
+        phiy = g.ysp * (-2 * mf * r1.v + 2 * mf * r2.v + (2 * f - 1) * r3.v);
+        return mf * mf * r1.v + f * (2 - f) * r2.v - f * mf * r3.v;
+    }
+    phix = g.xsp * (f * mf * r0.d + (1 - f * f) * r1.d + f * f * r2.d);
+    phiy = g.ysp * ((1 - 2 * f) * r0.v - 2 * f * r1.v + 2 * f * r2.v);
+    return f * mf * r0.v + (1 - f * f) * r1.v + f * f * r2.v;
+}
+
+// The compiled reference truncates before abs (levelset.cc:124,126).
+__device__ inline double int_abs(double v) { return double(abs(int(v))); }
+
+// levelset::newton_failsafe (levelset.cc:137-195)
+__device__ bool newton_failsafe(const LsGeom &g, const double *phi, const int *st, int I, int J, int ij, double x0,
+                                double y0, double &x, double &y) {
+    const int ts = 7 - st[ij], P = g.P;
+    const double cphi = phi[ij];
+    bool hf = false, vf = false;
+    double fx = 0, fy = 0, t;
+    if (I > 0 && st[ij - 1] == ts) {
+        fx = cphi / (phi[ij - 1] - cphi);
+        if (I < g.m - 1 && st[ij + 1] == ts) {
+            t = cphi / (cphi - phi[ij + 1]);
+            if (t < -fx) fx = t;
+        }
+        hf = true;
+    } else if (I < g.m - 1 && st[ij + 1] == ts) {
+        fx = cphi / (cphi - phi[ij + 1]);
+        hf = true;
+    }
+    if (J > 0 && st[ij - P] == ts) {
+        fy = cphi / (phi[ij - P] - cphi);
+        if (J < g.n - 1 && st[ij + P] == ts) {
+            t = cphi / (cphi - phi[ij + P]);
+            if (t < -fy) fy = t;
+        }
+        vf = true;
+    } else if (J < g.n - 1 && st[ij + P] == ts) {
+        fy = cphi / (cphi - phi[ij + P]);
+        vf = true;
+    }
+    if (hf && vf) {
+        double yy = fy * g.dy, tx = fx * g.dx;
+        double cc = tx * yy / (tx * tx + yy * yy);
+        x = yy * cc + x0;
+        y = tx * cc + y0;
+        return true;
+    }
+    if (hf) { x = fx * g.dx + x0; y = y0; return true; }
+    if (vf) { x = x0; y = fy * g.dy + y0; return true; }
+    return false;
+}
+
+// levelset::newton (levelset.cc:107-127). b0 and st point at ghosted point
+// (0,0) and ij = J*P + I indexes both; y0 is formed from the reference's own
+// linear index J*m as (J*m)*(dy/m).
+__device__ bool newton(const LsGeom &g, const double *b0, const int *st, int I, int J, int ij, double &x0,
+                       double &y0, double &x, double &y) {
+    const double dyy = g.dy / double(g.m), step_limit = sqrt(g.dx * g.dx + g.dy * g.dy);
+    int iter = 0, cutoff = 0;
+    double ph, px, py;
+    x0 = x = g.ax + I * g.dx;
+    y0 = y = g.ay + double(J * g.m) * dyy;
+    ph = bicubic(g, b0, x, y, px, py);
+    do {
+        double delx = x0 - x, dely = y0 - y;
+        double m0 = delx * px + dely * py + ph;
+        double m1 = px * px + py * py;
+        iter++;
+        m0 /= m1;
+        if (iter > 100 || !isfinite(m0) || m0 > step_limit)
+            return newton_failsafe(g, b0, st, I, J, ij, x0, y0, x, y);
+        x += delx - m0 * px;
+        y += dely - m0 * py;
+        ph = bicubic(g, b0, x, y, px, py);
+        if (int_abs(ph) < 1e-10) cutoff++; else cutoff = 0;
+    } while (cutoff < 8);
+    if (int_abs(x0 - x) < 3 * g.dx && int_abs(y0 - y) < 3 * g.dy) return true;
+    return newton_failsafe(g, b0, st, I, J, ij, x0, y0, x, y);
+}
+
+LsGeom geom_of(const Grid &g) {
+    LsGeom l;
+    l.m = g.ml; l.n = g.nl; l.P = g.P;
+    l.ax = g.lax; l.ay = g.lay;
+    l.dx = g.dx; l.dy = g.dy; l.xsp = g.xsp; l.ysp = g.ysp;
+    return l;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_zero_counts(int *cnt, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) cnt[k] = 0;
+}
+
+// obj_field::pin (obj_field.cc:92-97) over the ghosted grid
+__global__ void k_pin(Grid g, ObjConst c, ObjArrays a) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J);
+    const double p = a.phi[ij];
+    if (p < c.pinw && p > -c.pinw) a.phi[ij] = obj_phi(c, a.X[ij], a.Y[ij]);
+}
+
+// Sign-change scan (levelset.cc:211-263): a point is on an interface layer
+// when a 4-neighbour has the other sign. Interface points are listed for
+// the Newton solves; the fixed-point buffers start at +-inf elsewhere.
+__global__ void k_scan(Grid g, ObjArrays a, LsWork w) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J), P = g.P;
+    const double *phi = a.phi;
+    const bool pos = phi[ij] > 0;
+    bool iface = false;
+    if (I > 0 && (phi[ij - 1] > 0) != pos) iface = true;
+    if (I < g.ml - 1 && (phi[ij + 1] > 0) != pos) iface = true;
+    if (J > 0 && (phi[ij - P] > 0) != pos) iface = true;
+    if (J < g.nl - 1 && (phi[ij + P] > 0) != pos) iface = true;
+    a.st[ij] = pos ? (iface ? 4 : 7) : (iface ? 3 : 0);
+    if (iface) {
+        const int k = atomicAdd(&w.cnt[kCntIface], 1);
+        w.iface[k] = ij;
+    } else {
+        const double inf = pos ? INFINITY : -INFINITY;
+        w.va[ij] = inf;
+        w.vb[ij] = inf;
+    }
+}
+
+// Exact closest-point distances of both interface layers, from the pinned
+// phi (levelset.cc:266-291). The results go to the fixed-point buffers; phi
+// itself keeps the pinned values until the band is finalised, since every
+// Newton solve reads the pinned field.
+__global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) {
+    const int n = w.cnt[kCntIface];
+    const double *b0 = a.phi + g.ls(0, 0);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int ij = w.iface[k];
+        int I, J;
+        ls_coords(g, ij, I, J);
+        double x0, y0, x, y;
+        if (!newton(lg, b0, a.st + g.ls(0, 0), I, J, ij - g.ls(0, 0), x0, y0, x, y)) {
+            atomicOr(&w.cnt[kCntErr], kErrNewton);
+            continue;
+        }
+        x -= x0; y -= y0;
+        const double d = sqrt(x * x + y * y);
+        const double v = a.st[ij] == 4 ? d : -d;
+        w.va[ij] = v;
+        w.vb[ij] = v;
+    }
+}
+
+// Candidate points: non-interface points within kDilate cells (max norm) of
+// an interface point; separable dilation, rows then columns.
+__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J);
+    bool hit = false;
+    for (int d = -kDilate; d <= kDilate && !hit; d++) {
+        const int I2 = I + d;
+        if (I2 < 0 || I2 >= g.ml) continue;
+        const int s = a.st[ij + d];
+        hit = s == 3 || s == 4;
+    }
+    w.mark[ij] = hit;
+}
+__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J), s = a.st[ij];
+    if (s == 3 || s == 4) return;
+    bool hit = false;
+    for (int d = -kDilate; d <= kDilate && !hit; d++) {
+        const int J2 = J + d;
+        if (J2 < 0 || J2 >= g.nl) continue;
+        hit = w.mark[ij + d * g.P] != 0;
+    }
+    if (hit) {
+        const int k = atomicAdd(&w.cnt[kCntCand], 1);
+        w.cand[k] = ij;
+    }
+}
+
+// Known-neighbour test of the heap march (heap.hh:59-83) at the fixed point:
+// the other side and the interface layer are always known; a same-side
+// point is known when it has been reached, lies within the band, and is
+// upwind of the point being updated.
+struct FimAcc {
+    const double *v;
+    const int *st;
+    double pval, limit;  // limit: the band cut-off of this side
+    bool pos;
+    __device__ bool in(int q) const {
+        const int s = st[q];
+        if (pos) return s <= 4 || (v[q] < pval && v[q] < limit);
+        return s >= 3 || (v[q] > pval && v[q] > limit);
+    }
+    __device__ double val(int q) const { return v[q]; }
+};
+
+__global__ void k_fim(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w, const double *src, double *dst,
+                      int slot) {
+    const int n = w.cnt[kCntCand], P = g.P;
+    int changed = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int ij = w.cand[k];
+        int I, J;
+        ls_coords(g, ij, I, J);
+        const bool pos = a.st[ij] == 7;
+        const double limit = pos ? c.phi_up : c.phi_down;
+        FimAcc acc{src, a.st, src[ij], limit, pos};
+        const bool any = (I > 0 && acc.in(ij - 1)) || (I < g.ml - 1 && acc.in(ij + 1)) ||
+                         (J > 0 && acc.in(ij - P)) || (J < g.nl - 1 && acc.in(ij + P));
+        double r;
+        if (!any) r = pos ? INFINITY : -INFINITY;
+        else if (pos) { Eik<true, FimAcc> e{acc, eg, g.ml, g.nl, P}; r = e.calc(I, J, ij); }
+        else { Eik<false, FimAcc> e{acc, eg, g.ml, g.nl, P}; r = e.calc(I, J, ij); }
+        dst[ij] = r;
+        if (__double_as_longlong(r) != __double_as_longlong(src[ij])) changed = 1;
+    }
+    if (__any(changed) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
+}
+
+// Final band (levelset.cc:300-335, 408-413): accepted points get their
+// fixed-point values and status 5/2, points the heap would have left in its
+// queue status 6/1, everything outside the band the cut-off value. Outside
+// band points are listed for the extension.
+__global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J), P = g.P, s = a.st[ij];
+    const double lim = c.phi_up, dlim = c.phi_down;
+    auto acc_pos = [&](int q) { const int t = a.st[q]; return t == 4 || (t == 7 && v[q] < lim); };
+    auto acc_neg = [&](int q) { const int t = a.st[q]; return t == 3 || (t == 0 && v[q] > dlim); };
+    int ns = s;
+    double phi;
+    if (s == 3 || s == 4) {
+        phi = v[ij];
+    } else if (s == 7) {
+        if (v[ij] < lim) { ns = 5; phi = v[ij]; }
+        else {
+            const bool tr = (I > 0 && acc_pos(ij - 1)) || (I < g.ml - 1 && acc_pos(ij + 1)) ||
+                            (J > 0 && acc_pos(ij - P)) || (J < g.nl - 1 && acc_pos(ij + P));
+            ns = tr ? 6 : 7;
+            phi = lim;
+        }
+    } else {
+        if (v[ij] > dlim) { ns = 2; phi = v[ij]; }
+        else {
+            const bool tr = (I > 0 && acc_neg(ij - 1)) || (I < g.ml - 1 && acc_neg(ij + 1)) ||
+                            (J > 0 && acc_neg(ij - P)) || (J < g.nl - 1 && acc_neg(ij + P));
+            ns = tr ? 1 : 0;
+            phi = dlim;
+        }
+    }
+    w.mark[ij] = (unsigned char)ns;  // statuses go to st after every thread read the old ones
+    a.phi[ij] = phi;
+    if (ns == 4 || ns == 5) {
+        const int k = atomicAdd(&w.cnt[kCntPband], 1);
+        w.pband[k] = ij;
+    }
+}
+__global__ void k_commit_status(Grid g, ObjArrays a, LsWork w) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= g.ml || J >= g.nl) return;
+    const int ij = g.ls(I, J);
+    a.st[ij] = w.mark[ij];
+}
+
+// One Jacobi pass of the extension (traverse.hh:169-231): outside band
+// points take a weighted least-squares linear fit of (nX,nY) over band
+// neighbours with smaller phi, in a (2r+1)^2 box, r = 3 growing to 7 while
+// the fit is singular. Outside points read the previous pass (sx,sy);
+// inside points read their own predictor values in nX,nY.
+__global__ void k_extend(Grid g, ObjArrays a, LsWork w, const double *sx, const double *sy, double *dx_,
+                         double *dy_, int slot) {
+    const int n = w.cnt[kCntPband], P = g.P, m = g.ml, nn = g.nl;
+    int changed = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int ij = w.pband[k];
+        int i, j;
+        ls_coords(g, ij, i, j);
+        const double pp = a.phi[ij];
+        double S, Sx, Sy, Sxx, Sxy, Syy, det, p0, p1, p2, p3, p4, p5;
+        int r = 3;
+        do {
+            const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
+                      jhi = j + r >= nn ? nn - 1 - j : r;
+            S = Sx = Sy = Sxx = Sxy = Syy = 0;
+            p0 = p1 = p2 = p3 = p4 = p5 = 0;
+            for (int dj = jlo; dj <= jhi; dj++)
+                for (int di = ilo; di <= ihi; di++) {
+                    const int q = ij + dj * P + di, s = a.st[q];
+                    if (s < 2 || s > 5) continue;
+                    double wgt = a.phi[q];
+                    if (!(wgt < pp)) continue;
+                    wgt = pp - wgt;
+                    S += wgt; Sx += di * wgt; Sy += dj * wgt;
+                    Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
+                    const double fa = s >= 4 ? sx[q] : a.nX[q], fb = s >= 4 ? sy[q] : a.nY[q];
+                    const double xx = di, yy = dj;
+                    p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                    p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                }
+            det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
+            r++;
+        } while (fabs(det) < 1e-10 && r <= 7);
+        if (fabs(det) < 1e-10) {
+            atomicOr(&w.cnt[kCntErr], kErrSingular);
+            continue;
+        }
+        det = 1 / det;
+        const double al = (Sxy * Sxy - Sxx * Syy) * det, be = (Sx * Syy - Sxy * Sy) * det,
+                     ga = (Sxx * Sy - Sxy * Sx) * det;
+        const double vx = p0 * al + p1 * be + p2 * ga, vy = p3 * al + p4 * be + p5 * ga;
+        dx_[ij] = vx;
+        dy_[ij] = vy;
+        if (__double_as_longlong(vx) != __double_as_longlong(sx[ij]) ||
+            __double_as_longlong(vy) != __double_as_longlong(sy[ij]))
+            changed = 1;
+    }
+    if (__any(changed) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
+}
+// Seed both extension buffers with the current nX,nY of the outside band,
+// or copy the converged buffer back into nX,nY.
+__global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double *fy, int to_obj) {
+    const int n = w.cnt[kCntPband];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int ij = w.pband[k];
+        if (to_obj) { a.nX[ij] = fx[ij]; a.nY[ij] = fy[ij]; }
+        else { w.ex[ij] = w.fx[ij] = a.nX[ij]; w.ey[ij] = w.fy[ij] = a.nY[ij]; }
+    }
+}
+
+#define KCHECK() HIP_CHECK(hipGetLastError())
+
+// Run a fixed-point iteration in batches of `batch` passes; after each batch
+// read the change count of its last pass. Returns passes, or -1 on no
+// convergence within max_passes, or an error (>0 codes negated below -1).
+template <class F>
+int iterate(hipStream_t s, LsWork &w, int batch, int max_passes, F pass, int *passes_out) {
+    int passes = 0;
+    while (passes < max_passes) {
+        k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntIter, kIterSlots);
+        KCHECK();
+        for (int b = 0; b < batch; b++) {
+            int rc = pass(passes, b);
+            if (rc) return rc;
+            passes++;
+        }
+        HIP_CHECK(hipMemcpyAsync(w.hcnt, w.cnt, (kCntIter + kIterSlots) * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (w.hcnt[kCntErr]) break;
+        if (w.hcnt[kCntIter + batch - 1] == 0) {
+            *passes_out = passes;
+            return 0;
+        }
+    }
+    *passes_out = passes;
+    return w.hcnt[kCntErr] ? 0 : 7;
+}
+
+}  // namespace
+
+int ls_alloc(const Grid &g, LsWork &w) {
+    const size_t N = (size_t)g.N;
+    double **d[] = {&w.va, &w.vb, &w.ex, &w.ey, &w.fx, &w.fy};
+    for (auto p : d) {
+        HIP_CHECK(hipMalloc((void **)p, N * sizeof(double)));
+        HIP_CHECK(hipMemset(*p, 0, N * sizeof(double)));
+    }
+    const size_t L = (size_t)g.ml * g.nl + 64;
+    HIP_CHECK(hipMalloc((void **)&w.iface, L * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&w.cand, L * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&w.pband, L * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&w.mark, N));
+    HIP_CHECK(hipMemset(w.mark, 0, N));
+    HIP_CHECK(hipMalloc((void **)&w.cnt, (kCntIter + kIterSlots) * sizeof(int)));
+    HIP_CHECK(hipHostMalloc((void **)&w.hcnt, (kCntIter + kIterSlots) * sizeof(int), hipHostMallocDefault));
+    return 0;
+}
+
+void ls_free(LsWork &w) {
+    void *p[] = {w.va, w.vb, w.ex, w.ey, w.fx, w.fy, w.iface, w.cand, w.pband, w.mark, w.cnt};
+    for (auto q : p)
+        if (q) (void)hipFree(q);
+    if (w.hcnt) (void)hipHostFree(w.hcnt);
+    w = LsWork{};
+}
+
+static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                          int *fim_iters) {
+    const dim3 gg = grid2(g.ml, g.nl), bb(BX, BY);
+    k_zero_counts<<<1, 64, 0, s>>>(w.cnt, kCntIter + kIterSlots);
+    KCHECK();
+    k_scan<<<gg, bb, 0, s>>>(g, a, w);
+    KCHECK();
+    k_newton<<<kListBlocks, kListThreads, 0, s>>>(g, geom_of(g), a, w);
+    KCHECK();
+    k_dilate_rows<<<gg, bb, 0, s>>>(g, a, w);
+    KCHECK();
+    k_dilate_cols<<<gg, bb, 0, s>>>(g, a, w);
+    KCHECK();
+    EikGeom eg;
+    eg.init(g.dx, g.dy);
+    double *bufs[2] = {w.va, w.vb};
+    int passes = 0;
+    int rc = iterate(s, w, 8, 4096, [&](int p, int slot) -> int {
+        k_fim<<<kListBlocks, kListThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot);
+        KCHECK();
+        return 0;
+    }, &passes);
+    if (rc) return rc;
+    if (w.hcnt[kCntErr] & kErrNewton) return 3;
+    if (fim_iters) *fim_iters = passes;
+    k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
+    KCHECK();
+    k_finalize<<<gg, bb, 0, s>>>(g, c, a, w, bufs[passes & 1]);
+    KCHECK();
+    k_commit_status<<<gg, bb, 0, s>>>(g, a, w);
+    KCHECK();
+    return 0;
+}
+
+int ls_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, int *fim_iters) {
+    return rebuild_common(s, g, c, a, w, fim_iters);
+}
+
+int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                   int *fim_iters) {
+    k_pin<<<grid2(g.ml, g.nl), dim3(BX, BY), 0, s>>>(g, c, a);
+    KCHECK();
+    return rebuild_common(s, g, c, a, w, fim_iters);
+}
+
+int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *ext_iters) {
+    k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(a, w, w.ex, w.ey, 0);
+    KCHECK();
+    double *bx[2] = {w.ex, w.fx}, *by[2] = {w.ey, w.fy};
+    int passes = 0;
+    int rc = iterate(s, w, 8, 4096, [&](int p, int slot) -> int {
+        k_extend<<<kListBlocks, kListThreads, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1],
+                                                      by[(p + 1) & 1], slot);
+        KCHECK();
+        return 0;
+    }, &passes);
+    if (rc) return rc;
+    if (w.hcnt[kCntErr] & kErrSingular) return 5;
+    if (ext_iters) *ext_iters = passes;
+    k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(a, w, bx[passes & 1], by[passes & 1], 1);
+    KCHECK();
+    return 0;
+}
+
+}  // namespace incrmt

@@ -1,0 +1,96 @@
+// Geometric multigrid for the two projection systems of the step:
+//   MAC: cell-centred 5-point  div((1/rho) grad q) (mgs_mac.hh:14-102)
+//   FEM: vertex-centred 9-point bilinear-FEM stiffness with cell 1/rho
+//        (mgs_fem.hh:15-136)
+// on non-periodic, pure-Neumann boxes. The reference's engine (TGMG) is an
+// external library absent from the reference tree; this solver honours its
+// contract (solve A z = b in place, z warm-started from the previous solve)
+// with its own algorithm: colored Gauss-Seidel V-cycles over rediscretized
+// coarse levels, stopped on the relative residual ||b - A z||_2 / ||b||_2.
+#ifndef INCRMT_MG_HH
+#define INCRMT_MG_HH
+
+#include <vector>
+#include "common.hh"
+
+namespace incrmt {
+
+enum MgKind { kMAC = 0, kFEM = 1 };
+
+// One level: nx x ny unknowns stored with a one-point zero ghost ring, so
+// stencils read neighbours without branches. Element (i,j) lives at
+// base[(j+1)*pitch + i+1]; the z/b/r pointers below point at (0,0).
+struct MgLevel {
+    int nx, ny, pitch;
+    double *z, *b, *r;
+    float *zf, *bf, *rf;  // single-precision copies for the mixed cycle
+    // MAC: face inverse densities, zero on walls: fw(i,j) face between cells
+    // (i-1,j),(i,j), i in [0,nx]; fs(i,j) face between (i,j-1),(i,j).
+    // FEM: kc(i,j) = 1/rho of cell (i,j), i in [0,nx-2], zero ring around.
+    double *fw, *fs, *kc;
+    float *fwf, *fsf, *kcf;
+    int cp;       // coefficient pitch
+    double kx, ky;  // MAC 1/dx^2, 1/dy^2 of this level
+};
+
+struct MgStats {
+    int cycles;
+    double rel_res, bnorm, rnorm;
+};
+
+class Multigrid {
+  public:
+    Multigrid() {}
+    ~Multigrid();
+    // Allocate a hierarchy for an m x n system (cells for MAC, nodes for FEM).
+    int init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s);
+    // Fine-level coefficients from the ghosted cell densities (fluid_2d.cc:
+    // 1140-1170 build the same values), then coarsen.
+    int set_density(const Grid &g, const double *rho);
+    // Fine-level coefficients from the reference's irho layouts on the host
+    // (mgs_mac_varying_rho::irho 2mn+n+m; mgs_fem_varying_rho::irho m*n), or
+    // constant density when irho is null.
+    int set_irho_host(const double *irho);
+    // Solve with the right-hand side in level 0's b and the warm start in z.
+    int solve(double tol, int max_cycles, MgStats *st);
+    // Mixed-precision solve: FP64 residual and correction accumulation,
+    // FP32 V-cycles (iterative refinement).
+    int solve_mixed(double tol, int max_cycles, MgStats *st);
+    // Launch `sweeps` fine-level smoothing sweeps (for timing the smoother).
+    int smooth_fine(int sweeps, bool fp32);
+    // y = A x on level 0; x and y are padded level arrays (pointers at (0,0)).
+    int apply(const double *x, double *y);
+    // Time `sweeps` fine-level sweeps with HIP events: mean ms per launch.
+    int time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches);
+    MgLevel &fine() { return L[0]; }
+    const std::vector<MgLevel> &levels() const { return L; }
+    MgKind kind() const { return kind_; }
+    int nu1 = 2, nu2 = 2;
+    // timing of fine-level smoothing launches (HIP events on the stream)
+    bool time_smoother = false;
+    double smoother_ms = 0;
+    long smoother_launches = 0;
+    double *d_red = nullptr;   // reduction partials
+    double *h_red = nullptr;   // pinned host mirror
+  private:
+    int vcycle(size_t l);
+    int vcycle_f(size_t l);
+    int coarsen_coefs();
+    int reduce_norms(double *out3);  // ||b||^2, ||r||^2, sum b of level 0
+    int smooth(size_t l, int sweeps, bool reverse);
+    int smooth_f(size_t l, int sweeps, bool reverse);
+    MgKind kind_ = kMAC;
+    std::vector<MgLevel> L;
+    std::vector<void *> allocs;
+    hipStream_t s_ = nullptr;
+    double dx_ = 1, dy_ = 1;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_;
+    size_t ev_used_ = 0;
+    int record_begin();
+    int record_end();
+    int collect_events();
+};
+
+}  // namespace incrmt
+
+#endif

@@ -1,0 +1,676 @@
+// Geometric multigrid for the MAC and FEM projection systems (see mg.hh).
+//
+// Design for gfx950: every level is a padded 2D array with a zero ghost
+// ring, so the smoothers, residuals and transfers are branch-free 5/9-point
+// stencils over rows of 64 consecutive doubles per wavefront. Fine-level
+// smoothing is the HBM-bound hot loop: per sweep it reads z, b and the
+// level's coefficients and writes z once. Reductions (norms, means) are
+// deterministic two-pass block reductions, so a solve is reproducible
+// launch to launch.
+#include <cstring>
+#include <cstdio>
+#include "mg.hh"
+
+namespace incrmt {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+constexpr int kRedBlocks = 512;   // partial sums per reduction
+constexpr int kRedThreads = 256;
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+inline dim3 grid2(int nx, int ny) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY); }
+
+// ------------------------------------------------------------ operators
+// MAC: A z = kx (fw_l (z_l - z) + fw_r (z_r - z)) + ky (fs_d (z_d - z) + fs_u (z_u - z))
+template <class T>
+__device__ inline T mac_offdiag(const T *z, const T *fw, const T *fs, int q, int c, int zp, int cp, T kx, T ky,
+                                T &diag) {
+    const T wl = fw[c], wr = fw[c + 1], sd = fs[c], su = fs[c + cp];
+    diag = -(kx * (wl + wr) + ky * (sd + su));
+    return kx * (wl * z[q - 1] + wr * z[q + 1]) + ky * (sd * z[q - zp] + su * z[q + zp]);
+}
+
+// FEM: bilinear-element stiffness with cell weights kc (mgs_fem.hh:86-136)
+struct FemC {
+    double fm, fc, hex, hey;
+};
+template <class T>
+__device__ inline T fem_offdiag(const T *z, const T *kc, int q, int c, int zp, int cp, const FemC &k, T &diag) {
+    // cells around node (i,j): dl=(i-1,j-1) dr=(i,j-1) ul=(i-1,j) ur=(i,j)
+    const T kdl = kc[c - cp - 1], kdr = kc[c - cp], kul = kc[c - 1], kur = kc[c];
+    diag = T(k.fm) * (kdl + kdr + kul + kur);
+    return T(k.fc) * (kdl * z[q - zp - 1] + kdr * z[q - zp + 1] + kul * z[q + zp - 1] + kur * z[q + zp + 1])
+           + T(k.hey) * ((kdl + kdr) * z[q - zp] + (kul + kur) * z[q + zp])
+           + T(k.hex) * ((kdl + kul) * z[q - 1] + (kdr + kur) * z[q + 1]);
+}
+
+struct LevelArgs {
+    int nx, ny, zp, cp;
+    double kx, ky;
+};
+
+// --------------------------------------------------------------- smoothers
+// Red-black Gauss-Seidel for the 5-point MAC operator: thread t of row j
+// updates i = 2t + ((j + color) & 1).
+template <class T>
+__global__ void k_mac_smooth(LevelArgs a, T *z, const T *b, const T *fw, const T *fs, int color) {
+    const int t = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int i = 2 * t + ((j + color) & 1);
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    T diag;
+    const T off = mac_offdiag<T>(z, fw, fs, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
+    if (diag != T(0)) z[q] = (b[q] - off) / diag;
+}
+
+// Four-colour Gauss-Seidel for the 9-point FEM operator (colours (i%2,j%2)
+// are a valid Gauss-Seidel ordering for a 9-point stencil).
+template <class T>
+__global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc, int color) {
+    const int i = 2 * (blockIdx.x * BX + threadIdx.x) + (color & 1);
+    const int j = 2 * (blockIdx.y * BY + threadIdx.y) + (color >> 1);
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    T diag;
+    const T off = fem_offdiag<T>(z, kc, q, c, a.zp, a.cp, k, diag);
+    if (diag != T(0)) z[q] = (b[q] - off) / diag;
+}
+
+// ------------------------------------------------- residual and transfers
+// MAC: fused residual of the four children and averaging restriction.
+template <class T>
+__global__ void k_mac_restrict(LevelArgs f, LevelArgs cl, const T *z, const T *b, const T *fw, const T *fs,
+                               T *bc, T *zc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= cl.nx || J >= cl.ny) return;
+    T s = 0;
+    for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++) {
+            const int i = 2 * I + di, j = 2 * J + dj, q = j * f.zp + i, c = j * f.cp + i;
+            T diag;
+            const T off = mac_offdiag<T>(z, fw, fs, q, c, f.zp, f.cp, T(f.kx), T(f.ky), diag);
+            s += b[q] - off - diag * z[q];
+        }
+    bc[J * cl.zp + I] = T(0.25) * s;
+    zc[J * cl.zp + I] = 0;
+}
+
+// FEM: residual into r, then full-weighting restriction (the transpose of
+// bilinear interpolation, weights 1, 1/2, 1/4).
+template <class T>
+__global__ void k_fem_residual(LevelArgs a, FemC k, const T *z, const T *b, const T *kc, T *r) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    T diag;
+    const T off = fem_offdiag<T>(z, kc, q, c, a.zp, a.cp, k, diag);
+    r[q] = b[q] - off - diag * z[q];
+}
+template <class T>
+__global__ void k_fem_restrict(LevelArgs f, LevelArgs cl, const T *r, T *bc, T *zc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= cl.nx || J >= cl.ny) return;
+    const int q = 2 * J * f.zp + 2 * I, p = f.zp;
+    // ghost-ring entries of r are zero, so boundary children drop out
+    const T s = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - p] + r[q + p])
+                + T(0.25) * (r[q - p - 1] + r[q - p + 1] + r[q + p - 1] + r[q + p + 1]);
+    bc[J * cl.zp + I] = s;
+    zc[J * cl.zp + I] = 0;
+}
+
+// MAC: bilinear cell-centred prolongation with constant extrapolation at
+// the walls, added to the fine iterate.
+template <class T>
+__global__ void k_mac_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= f.nx || j >= f.ny) return;
+    const int I = i >> 1, J = j >> 1;
+    int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+    if (I2 < 0 || I2 >= cl.nx) I2 = I;
+    if (J2 < 0 || J2 >= cl.ny) J2 = J;
+    const int p = cl.zp;
+    z[j * f.zp + i] += T(0.5625) * e[J * p + I] + T(0.1875) * e[J * p + I2] + T(0.1875) * e[J2 * p + I]
+                       + T(0.0625) * e[J2 * p + I2];
+}
+
+// FEM: bilinear vertex prolongation, added to the fine iterate.
+template <class T>
+__global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= f.nx || j >= f.ny) return;
+    const int I = i >> 1, J = j >> 1, p = cl.zp, q = J * p + I;
+    T v;
+    if (!(i & 1) && !(j & 1)) v = e[q];
+    else if (!(j & 1)) v = T(0.5) * (e[q] + e[q + 1]);
+    else if (!(i & 1)) v = T(0.5) * (e[q] + e[q + p]);
+    else v = T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
+    z[j * f.zp + i] += v;
+}
+
+// Coarsest level: one workgroup, mean-free right-hand side, many colored
+// Gauss-Seidel sweeps, mean-free result (the systems are pure Neumann).
+template <class T>
+__global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const T *fw, const T *fs, const T *kc,
+                               int sweeps) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x, N = a.nx * a.ny;
+    double s = 0;
+    for (int p = tid; p < N; p += blockDim.x) s += b[(p / a.nx) * a.zp + p % a.nx];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    const T mb = T(red[0] / N);
+    __syncthreads();
+    for (int p = tid; p < N; p += blockDim.x) {
+        const int q = (p / a.nx) * a.zp + p % a.nx;
+        b[q] -= mb;
+        z[q] = 0;
+    }
+    __syncthreads();
+    const int ncol = kind == kMAC ? 2 : 4;
+    for (int it = 0; it < sweeps; it++)
+        for (int col = 0; col < ncol; col++) {
+            for (int p = tid; p < N; p += blockDim.x) {
+                const int i = p % a.nx, j = p / a.nx;
+                const bool mine = kind == kMAC ? (((i + j) & 1) == col) : ((i & 1) + 2 * (j & 1) == col);
+                if (!mine) continue;
+                const int q = j * a.zp + i, c = j * a.cp + i;
+                T diag, off;
+                if (kind == kMAC) off = mac_offdiag<T>(z, fw, fs, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
+                else off = fem_offdiag<T>(z, kc, q, c, a.zp, a.cp, k, diag);
+                if (diag != T(0)) z[q] = (b[q] - off) / diag;
+            }
+            __syncthreads();
+        }
+    s = 0;
+    for (int p = tid; p < N; p += blockDim.x) s += z[(p / a.nx) * a.zp + p % a.nx];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    const T mz = T(red[0] / N);
+    __syncthreads();
+    for (int p = tid; p < N; p += blockDim.x) z[(p / a.nx) * a.zp + p % a.nx] -= mz;
+}
+
+// ------------------------------------------------------ coefficient setup
+// MAC fine faces from cell densities: 0.5/rho_c + 0.5/rho_nb
+// (fluid_2d.cc:1140-1160), zeroed on the walls.
+__global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *fw, double *fs) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > a.nx || j > a.ny) return;
+    const int c = j * a.cp + i;
+    if (j < a.ny) {
+        const int ij = g.at(i, j);
+        fw[c] = (i == 0 || i == a.nx) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
+    }
+    if (i < a.nx) {
+        const int ij = g.at(i, j);
+        fs[c] = (j == 0 || j == a.ny) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - g.P];
+    }
+}
+// FEM fine cell weights 1/rho (fluid_2d.cc:1163-1170).
+__global__ void k_fem_coef_fine(Grid g, const double *rho, LevelArgs a, double *kc) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx - 1 || j >= a.ny - 1) return;
+    kc[j * a.cp + i] = 1. / rho[g.at(i, j)];
+}
+__global__ void k_mac_coef_coarsen(LevelArgs f, LevelArgs c, const double *fwf, const double *fsf, double *fwc,
+                                   double *fsc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I > c.nx || J > c.ny) return;
+    if (J < c.ny) fwc[J * c.cp + I] = 0.5 * (fwf[2 * J * f.cp + 2 * I] + fwf[(2 * J + 1) * f.cp + 2 * I]);
+    if (I < c.nx) fsc[J * c.cp + I] = 0.5 * (fsf[2 * J * f.cp + 2 * I] + fsf[2 * J * f.cp + 2 * I + 1]);
+}
+__global__ void k_fem_coef_coarsen(LevelArgs f, LevelArgs c, const double *kf, double *kcc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= c.nx - 1 || J >= c.ny - 1) return;
+    const int q = 2 * J * f.cp + 2 * I;
+    kcc[J * c.cp + I] = 0.25 * (kf[q] + kf[q + 1] + kf[q + f.cp] + kf[q + f.cp + 1]);
+}
+__global__ void k_to_f32(const double *a, float *b, long long n) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = float(a[k]);
+}
+
+__global__ void k_apply(int kind, LevelArgs a, FemC k, const double *x, const double *fw, const double *fs,
+                        const double *kc, double *y) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    double diag, off;
+    if (kind == kMAC) off = mac_offdiag<double>(x, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+    else off = fem_offdiag<double>(x, kc, q, c, a.zp, a.cp, k, diag);
+    y[q] = off + diag * x[q];
+}
+
+// ------------------------------------------------------------- reductions
+// Partial sums per block of (b^2, r^2, b) over level 0 with r = b - A z.
+__global__ void k_norms(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *fw,
+                        const double *fs, const double *kc, double *part) {
+    __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
+    double bb = 0, rr = 0, bs = 0;
+    const int N = a.nx * a.ny;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int i = p % a.nx, j = p / a.nx, q = j * a.zp + i, c = j * a.cp + i;
+        double diag, off;
+        if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+        else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
+        const double bq = b[q], r = bq - off - diag * z[q];
+        bb += bq * bq; rr += r * r; bs += bq;
+    }
+    s0[threadIdx.x] = bb; s1[threadIdx.x] = rr; s2[threadIdx.x] = bs;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+// Fixed-order sum of the partials into out[0..2].
+__global__ void k_sum_partials(const double *part, int nb, double *out) {
+    __shared__ double s[3][kRedThreads];
+    for (int k = 0; k < 3; k++) {
+        double v = 0;
+        for (int p = threadIdx.x; p < nb; p += blockDim.x) v += part[3 * p + k];
+        s[k][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 3; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[k] = s[k][0];
+}
+__global__ void k_sub_mean(LevelArgs a, double *b, const double *sums) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    b[j * a.zp + i] -= sums[2] / (double(a.nx) * a.ny);
+}
+// Mixed precision: r = b - A z in FP64, handed to the FP32 cycle; z += e.
+__global__ void k_residual_f32(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *fw,
+                               const double *fs, const double *kc, float *bf, float *zf) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    double diag, off;
+    if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+    else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
+    bf[q] = float(b[q] - off - diag * z[q]);
+    zf[q] = 0.f;
+}
+__global__ void k_add_f32(LevelArgs a, double *z, const float *e) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i;
+    z[q] += double(e[q]);
+}
+
+FemC fem_consts(double dx, double dy) {
+    // mgs_fem.cc:12-21
+    const double dydx = dy / dx, dxdy = dx / dy;
+    FemC k;
+    k.fm = 1. / 3. * (dxdy + dydx);
+    const double fey = 1. / 3. * (-2 * dxdy + dydx), fex = 1. / 3. * (-2 * dydx + dxdy);
+    k.hey = 0.5 * fey;
+    k.hex = 0.5 * fex;
+    k.fc = -1. / 6. * (dxdy + dydx);
+    return k;
+}
+
+LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky}; }
+
+}  // namespace
+
+#define KCHECK() HIP_CHECK(hipGetLastError())
+
+Multigrid::~Multigrid() {
+    for (auto &e : ev_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (void *p : allocs) (void)hipFree(p);
+    if (h_red) (void)hipHostFree(h_red);
+}
+
+int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s) {
+    kind_ = kind; s_ = s; dx_ = dx; dy_ = dy;
+    int nx = m, ny = n;
+    double kx = 1. / (dx * dx), ky = 1. / (dy * dy);
+    while (true) {
+        MgLevel l{};
+        l.nx = nx; l.ny = ny;
+        l.pitch = round_up(nx + 2, 32);
+        l.cp = l.pitch;
+        l.kx = kx; l.ky = ky;
+        const size_t len = (size_t)(ny + 2) * l.pitch + 64;
+        auto dalloc = [&](double *&p) -> int {
+            void *q;
+            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+            HIP_CHECK(hipMemset(q, 0, len * sizeof(double)));
+            allocs.push_back(q);
+            p = (double *)q + l.pitch + 1;
+            return 0;
+        };
+        auto falloc = [&](float *&p) -> int {
+            void *q;
+            HIP_CHECK(hipMalloc(&q, len * sizeof(float)));
+            HIP_CHECK(hipMemset(q, 0, len * sizeof(float)));
+            allocs.push_back(q);
+            p = (float *)q + l.pitch + 1;
+            return 0;
+        };
+        int rc;
+        if ((rc = dalloc(l.z)) || (rc = dalloc(l.b)) || (rc = dalloc(l.r))) return rc;
+        if ((rc = falloc(l.zf)) || (rc = falloc(l.bf)) || (rc = falloc(l.rf))) return rc;
+        if (kind == kMAC) {
+            if ((rc = dalloc(l.fw)) || (rc = dalloc(l.fs))) return rc;
+            if ((rc = falloc(l.fwf)) || (rc = falloc(l.fsf))) return rc;
+        } else {
+            if ((rc = dalloc(l.kc))) return rc;
+            if ((rc = falloc(l.kcf))) return rc;
+        }
+        L.push_back(l);
+        bool more = kind == kMAC ? (nx % 2 == 0 && ny % 2 == 0 && nx >= 4 && ny >= 4)
+                                 : (nx % 2 == 1 && ny % 2 == 1 && nx >= 5 && ny >= 5);
+        if (!more) break;
+        if (kind == kMAC) { nx /= 2; ny /= 2; kx *= 0.25; ky *= 0.25; }
+        else { nx = (nx + 1) / 2; ny = (ny + 1) / 2; }
+    }
+    void *q;
+    HIP_CHECK(hipMalloc(&q, (3 * kRedBlocks + 8) * sizeof(double)));
+    allocs.push_back(q);
+    d_red = (double *)q;
+    HIP_CHECK(hipHostMalloc((void **)&h_red, 8 * sizeof(double), hipHostMallocDefault));
+    return set_irho_host(nullptr);
+}
+
+int Multigrid::coarsen_coefs() {
+    for (size_t l = 0; l + 1 < L.size(); l++) {
+        const MgLevel &f = L[l];
+        MgLevel &c = L[l + 1];
+        if (kind_ == kMAC) {
+            k_mac_coef_coarsen<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.fw, f.fs,
+                                                                                    c.fw, c.fs);
+        } else {
+            k_fem_coef_coarsen<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc);
+        }
+        KCHECK();
+    }
+    // single-precision copies for the mixed cycle
+    for (auto &l : L) {
+        const long long len = (long long)(l.ny + 2) * l.pitch;
+        const int nb = (int)((len + 255) / 256);
+        if (kind_ == kMAC) {
+            k_to_f32<<<nb, 256, 0, s_>>>(l.fw - l.pitch - 1, l.fwf - l.pitch - 1, len);
+            k_to_f32<<<nb, 256, 0, s_>>>(l.fs - l.pitch - 1, l.fsf - l.pitch - 1, len);
+        } else {
+            k_to_f32<<<nb, 256, 0, s_>>>(l.kc - l.pitch - 1, l.kcf - l.pitch - 1, len);
+        }
+        KCHECK();
+    }
+    return 0;
+}
+
+int Multigrid::set_density(const Grid &g, const double *rho) {
+    MgLevel &f = L[0];
+    if (kind_ == kMAC)
+        k_mac_coef_fine<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.fw, f.fs);
+    else
+        k_fem_coef_fine<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.kc);
+    KCHECK();
+    return coarsen_coefs();
+}
+
+int Multigrid::set_irho_host(const double *irho) {
+    MgLevel &f = L[0];
+    const int nx = f.nx, ny = f.ny;
+    std::vector<double> h((size_t)(ny + 2) * f.pitch, 0.0);
+    auto at = [&](int i, int j) -> double & { return h[(size_t)(j + 1) * f.pitch + i + 1]; };
+    if (kind_ == kMAC) {
+        // fw from the left-edge entries, walls zero (mgs_mac.hh:64-102)
+        for (int j = 0; j < ny; j++)
+            for (int i = 1; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i)] : 1.0;
+        HIP_CHECK(hipMemcpy(f.fw - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::fill(h.begin(), h.end(), 0.0);
+        for (int j = 1; j < ny; j++)
+            for (int i = 0; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i) + 1] : 1.0;
+        HIP_CHECK(hipMemcpy(f.fs - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        // cell (i,j) weight = irho of its upper-right node (mgs_fem.hh:99-106)
+        for (int j = 0; j < ny - 1; j++)
+            for (int i = 0; i < nx - 1; i++) at(i, j) = irho ? irho[(j + 1) * nx + i + 1] : 1.0;
+        HIP_CHECK(hipMemcpy(f.kc - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return coarsen_coefs();
+}
+
+int Multigrid::record_begin() {
+    if (!time_smoother) return 0;
+    if (ev_used_ == ev_.size()) {
+        hipEvent_t a, b;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        ev_.push_back({a, b});
+    }
+    HIP_CHECK(hipEventRecord(ev_[ev_used_].first, s_));
+    return 0;
+}
+int Multigrid::record_end() {
+    if (!time_smoother) return 0;
+    HIP_CHECK(hipEventRecord(ev_[ev_used_].second, s_));
+    ev_used_++;
+    return 0;
+}
+int Multigrid::collect_events() {
+    for (size_t k = 0; k < ev_used_; k++) {
+        float ms = 0;
+        HIP_CHECK(hipEventSynchronize(ev_[k].second));
+        HIP_CHECK(hipEventElapsedTime(&ms, ev_[k].first, ev_[k].second));
+        smoother_ms += ms;
+        smoother_launches++;
+    }
+    ev_used_ = 0;
+    return 0;
+}
+
+int Multigrid::smooth(size_t l, int sweeps, bool reverse) {
+    const MgLevel &v = L[l];
+    const LevelArgs a = args_of(v);
+    const int ncol = kind_ == kMAC ? 2 : 4;
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int cc = 0; cc < ncol; cc++) {
+            const int col = reverse ? ncol - 1 - cc : cc;
+            const bool timed = l == 0;
+            if (timed && record_begin()) return 6;
+            if (kind_ == kMAC)
+                k_mac_smooth<double><<<grid2((v.nx + 1) / 2, v.ny), dim3(BX, BY), 0, s_>>>(a, v.z, v.b, v.fw, v.fs, col);
+            else
+                k_fem_smooth<double><<<grid2((v.nx + 1) / 2, (v.ny + 1) / 2), dim3(BX, BY), 0, s_>>>(
+                    a, fem_consts(dx_, dy_), v.z, v.b, v.kc, col);
+            KCHECK();
+            if (timed && record_end()) return 6;
+        }
+    return 0;
+}
+
+int Multigrid::smooth_f(size_t l, int sweeps, bool reverse) {
+    const MgLevel &v = L[l];
+    const LevelArgs a = args_of(v);
+    const int ncol = kind_ == kMAC ? 2 : 4;
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int cc = 0; cc < ncol; cc++) {
+            const int col = reverse ? ncol - 1 - cc : cc;
+            const bool timed = l == 0;
+            if (timed && record_begin()) return 6;
+            if (kind_ == kMAC)
+                k_mac_smooth<float><<<grid2((v.nx + 1) / 2, v.ny), dim3(BX, BY), 0, s_>>>(a, v.zf, v.bf, v.fwf, v.fsf, col);
+            else
+                k_fem_smooth<float><<<grid2((v.nx + 1) / 2, (v.ny + 1) / 2), dim3(BX, BY), 0, s_>>>(
+                    a, fem_consts(dx_, dy_), v.zf, v.bf, v.kcf, col);
+            KCHECK();
+            if (timed && record_end()) return 6;
+        }
+    return 0;
+}
+
+int Multigrid::apply(const double *x, double *y) {
+    const MgLevel &f = L[0];
+    k_apply<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), fem_consts(dx_, dy_), x, f.fw, f.fs, f.kc, y);
+    KCHECK();
+    return 0;
+}
+
+int Multigrid::time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches) {
+    const bool was = time_smoother;
+    const double ms0 = smoother_ms;
+    const long n0 = smoother_launches;
+    time_smoother = true;
+    int rc = smooth_fine(sweeps, fp32);
+    if (!rc) rc = collect_events();
+    time_smoother = was;
+    if (rc) return rc;
+    const long n = smoother_launches - n0;
+    if (launches) *launches = n;
+    if (ms_per_launch) *ms_per_launch = n ? (smoother_ms - ms0) / n : 0;
+    smoother_ms = ms0;
+    smoother_launches = n0;
+    return 0;
+}
+
+int Multigrid::smooth_fine(int sweeps, bool fp32) { return fp32 ? smooth_f(0, sweeps, false) : smooth(0, sweeps, false); }
+
+int Multigrid::vcycle(size_t l) {
+    const MgLevel &f = L[l];
+    const FemC k = fem_consts(dx_, dy_);
+    if (l + 1 == L.size()) {
+        k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 64);
+        KCHECK();
+        return 0;
+    }
+    int rc;
+    if ((rc = smooth(l, nu1, false))) return rc;
+    const MgLevel &c = L[l + 1];
+    if (kind_ == kMAC) {
+        k_mac_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, f.b, f.fw,
+                                                                            f.fs, c.b, c.z);
+    } else {
+        k_fem_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.z, f.b, f.kc, f.r);
+        KCHECK();
+        k_fem_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b, c.z);
+    }
+    KCHECK();
+    if ((rc = vcycle(l + 1))) return rc;
+    if (kind_ == kMAC) k_mac_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
+    else k_fem_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
+    KCHECK();
+    return smooth(l, nu2, true);
+}
+
+int Multigrid::vcycle_f(size_t l) {
+    const MgLevel &f = L[l];
+    const FemC k = fem_consts(dx_, dy_);
+    if (l + 1 == L.size()) {
+        k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 64);
+        KCHECK();
+        return 0;
+    }
+    int rc;
+    if ((rc = smooth_f(l, nu1, false))) return rc;
+    const MgLevel &c = L[l + 1];
+    if (kind_ == kMAC) {
+        k_mac_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, f.bf, f.fwf,
+                                                                           f.fsf, c.bf, c.zf);
+    } else {
+        k_fem_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.zf, f.bf, f.kcf, f.rf);
+        KCHECK();
+        k_fem_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf, c.zf);
+    }
+    KCHECK();
+    if ((rc = vcycle_f(l + 1))) return rc;
+    if (kind_ == kMAC) k_mac_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
+    else k_fem_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
+    KCHECK();
+    return smooth_f(l, nu2, true);
+}
+
+int Multigrid::reduce_norms(double *out3) {
+    const MgLevel &f = L[0];
+    k_norms<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), fem_consts(dx_, dy_), f.z, f.b, f.fw, f.fs, f.kc,
+                                                d_red + 8);
+    KCHECK();
+    k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_red);
+    KCHECK();
+    HIP_CHECK(hipMemcpyAsync(h_red, d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    HIP_CHECK(hipStreamSynchronize(s_));
+    for (int k = 0; k < 3; k++) out3[k] = h_red[k];
+    return 0;
+}
+
+int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
+    const MgLevel &f = L[0];
+    double nr[3];
+    int rc;
+    // project the right-hand side onto the range (pure Neumann system)
+    if ((rc = reduce_norms(nr))) return rc;
+    k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
+    KCHECK();
+    if ((rc = reduce_norms(nr))) return rc;
+    const double bn = std::sqrt(nr[0]);
+    double rel = bn > 0 ? std::sqrt(nr[1]) / bn : 0;
+    int cycles = 0;
+    while (rel > tol && bn > 0) {
+        if (cycles >= max_cycles) break;
+        if ((rc = vcycle(0))) return rc;
+        cycles++;
+        if ((rc = reduce_norms(nr))) return rc;
+        rel = std::sqrt(nr[1]) / bn;
+    }
+    if (time_smoother && (rc = collect_events())) return rc;
+    if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
+    return rel <= tol || bn == 0 ? 0 : 2;
+}
+
+int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
+    const MgLevel &f = L[0];
+    double nr[3];
+    int rc;
+    if ((rc = reduce_norms(nr))) return rc;
+    k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
+    KCHECK();
+    if ((rc = reduce_norms(nr))) return rc;
+    const double bn = std::sqrt(nr[0]);
+    double rel = bn > 0 ? std::sqrt(nr[1]) / bn : 0;
+    int cycles = 0;
+    const FemC k = fem_consts(dx_, dy_);
+    while (rel > tol && bn > 0) {
+        if (cycles >= max_cycles) break;
+        // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
+        k_residual_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc,
+                                                                    f.bf, f.zf);
+        KCHECK();
+        if ((rc = vcycle_f(0))) return rc;
+        k_add_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, f.zf);
+        KCHECK();
+        cycles++;
+        if ((rc = reduce_norms(nr))) return rc;
+        rel = std::sqrt(nr[1]) / bn;
+    }
+    if (time_smoother && (rc = collect_events())) return rc;
+    if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
+    return rel <= tol || bn == 0 ? 0 : 2;
+}
+
+}  // namespace incrmt
