@@ -83,7 +83,8 @@ double incrmt_fluid_time(const incrmt_fluid *f);
 int incrmt_fluid_timestep_select(const incrmt_fluid *f, double interval, double *adt);
 /* State on the reference's ghosted (m+4)x(n+4) grid, row-major, fbase /
  * sbase indexing (fluid_2d.cc:37, obj_field.cc:18). Fluid names: u v p rho
- * s11 s12 s21 s22; solid names (object index obj): X Y nX nY phi s. */
+ * s11 s12 s21 s22, c0 c1 (u* of the last step), ul vd (its MAC-projected
+ * left/down faces); solid names (object index obj): X Y nX nY phi s. */
 int incrmt_fluid_download(incrmt_fluid *f, const char *name, int obj, double *host);
 int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double *host);
 /* The current simulation time (fluid_2d::time) may be set for restarts. */

@@ -357,6 +357,10 @@ double *field_ptr(FluidEngine &e, const std::string &nm, int k, bool &is_int) {
     if (nm == "s12") return f.s12;
     if (nm == "s21") return f.s21;
     if (nm == "s22") return f.s22;
+    if (nm == "c0") return f.c0;    // u* after the step (fluid_2d.cc:1282-1288)
+    if (nm == "c1") return f.c1;
+    if (nm == "ul") return f.ulc;   // MAC-projected faces (fluid_2d.cc:1193-1235)
+    if (nm == "vd") return f.vdc;
     if (k < 0 || k >= (int)e.obj.size()) return nullptr;
     ObjArrays &a = e.obj[k].a;
     if (nm == "X") return a.X;

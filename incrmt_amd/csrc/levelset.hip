@@ -27,7 +27,6 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 constexpr int kListBlocks = 1024, kListThreads = 256;
-constexpr int kDilate = 10;  // candidate radius (cells) around the interface
 
 inline dim3 grid2(int nx, int ny) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY); }
 
@@ -256,14 +255,15 @@ __global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) {
     }
 }
 
-// Candidate points: non-interface points within kDilate cells (max norm) of
-// an interface point; separable dilation, rows then columns.
-__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w) {
+// Candidate points: non-interface points within R cells (max norm) of an
+// interface point, R covering the band cut-offs; separable dilation, rows
+// then columns.
+__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= g.ml || J >= g.nl) return;
     const int ij = g.ls(I, J);
     bool hit = false;
-    for (int d = -kDilate; d <= kDilate && !hit; d++) {
+    for (int d = -R; d <= R && !hit; d++) {
         const int I2 = I + d;
         if (I2 < 0 || I2 >= g.ml) continue;
         const int s = a.st[ij + d];
@@ -271,13 +271,13 @@ __global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w) {
     }
     w.mark[ij] = hit;
 }
-__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w) {
+__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= g.ml || J >= g.nl) return;
     const int ij = g.ls(I, J), s = a.st[ij];
     if (s == 3 || s == 4) return;
     bool hit = false;
-    for (int d = -kDilate; d <= kDilate && !hit; d++) {
+    for (int d = -R; d <= R && !hit; d++) {
         const int J2 = J + d;
         if (J2 < 0 || J2 >= g.nl) continue;
         hit = w.mark[ij + d * g.P] != 0;
@@ -504,9 +504,13 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     KCHECK();
     k_newton<<<kListBlocks, kListThreads, 0, s>>>(g, geom_of(g), a, w);
     KCHECK();
-    k_dilate_rows<<<gg, bb, 0, s>>>(g, a, w);
+    // the march cannot accept a point further than limit/h + 2 cells (max
+    // norm) from the interface layer: beyond it every path is longer
+    const double reach = fmax(-c.phi_down, c.phi_up) / fmin(g.dx, g.dy) + 3;
+    const int R = reach > g.ml + g.nl ? g.ml + g.nl : (int)reach;
+    k_dilate_rows<<<gg, bb, 0, s>>>(g, a, w, R < g.ml ? R : g.ml);
     KCHECK();
-    k_dilate_cols<<<gg, bb, 0, s>>>(g, a, w);
+    k_dilate_cols<<<gg, bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl);
     KCHECK();
     EikGeom eg;
     eg.init(g.dx, g.dy);

@@ -149,31 +149,51 @@ __global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     z[j * f.zp + i] += v;
 }
 
-// Coarsest level: one workgroup, mean-free right-hand side, many colored
-// Gauss-Seidel sweeps, mean-free result (the systems are pure Neumann).
+// Coarsest level: one workgroup, mean-free right-hand side, colored
+// Gauss-Seidel sweeps until the residual has dropped by `reduce` (checked
+// every 8 sweeps, at most max_sweeps), mean-free result (the systems are
+// pure Neumann).
 template <class T>
-__global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const T *fw, const T *fs, const T *kc,
-                               int sweeps) {
-    __shared__ double red[256];
-    const int tid = threadIdx.x, N = a.nx * a.ny;
-    double s = 0;
-    for (int p = tid; p < N; p += blockDim.x) s += b[(p / a.nx) * a.zp + p % a.nx];
-    red[tid] = s;
+__device__ double block_sum(double v, double *red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) red[tid] += red[tid + w];
         __syncthreads();
     }
-    const T mb = T(red[0] / N);
+    const double s = red[0];
     __syncthreads();
-    for (int p = tid; p < N; p += blockDim.x) {
-        const int q = (p / a.nx) * a.zp + p % a.nx;
-        b[q] -= mb;
-        z[q] = 0;
-    }
+    return s;
+}
+template <class T>
+__global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const T *fw, const T *fs, const T *kc,
+                               double reduce, int max_sweeps) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x, N = a.nx * a.ny;
+    auto Q = [&](int p) { return (p / a.nx) * a.zp + p % a.nx; };
+    double s = 0;
+    for (int p = tid; p < N; p += blockDim.x) s += b[Q(p)];
+    const T mb = T(block_sum<T>(s, red) / N);
+    // z is the correction the restriction zeroed, or, when the finest level
+    // is also the coarsest, the current iterate: it is not reset here
+    for (int p = tid; p < N; p += blockDim.x) b[Q(p)] -= mb;
     __syncthreads();
+    auto resid2 = [&]() -> double {
+        double r2 = 0;
+        for (int p = tid; p < N; p += blockDim.x) {
+            const int q = Q(p), c = (p / a.nx) * a.cp + p % a.nx;
+            T diag, off;
+            if (kind == kMAC) off = mac_offdiag<T>(z, fw, fs, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
+            else off = fem_offdiag<T>(z, kc, q, c, a.zp, a.cp, k, diag);
+            const double r = double(b[q] - off - diag * z[q]);
+            r2 += r * r;
+        }
+        return block_sum<T>(r2, red);
+    };
+    const double r0 = resid2();
     const int ncol = kind == kMAC ? 2 : 4;
-    for (int it = 0; it < sweeps; it++)
+    for (int it = 0; it < max_sweeps && r0 > 0; it++) {
         for (int col = 0; col < ncol; col++) {
             for (int p = tid; p < N; p += blockDim.x) {
                 const int i = p % a.nx, j = p / a.nx;
@@ -187,17 +207,12 @@ __global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const 
             }
             __syncthreads();
         }
-    s = 0;
-    for (int p = tid; p < N; p += blockDim.x) s += z[(p / a.nx) * a.zp + p % a.nx];
-    red[tid] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
+        if ((it & 7) == 7 && resid2() <= reduce * reduce * r0) break;
     }
-    const T mz = T(red[0] / N);
-    __syncthreads();
-    for (int p = tid; p < N; p += blockDim.x) z[(p / a.nx) * a.zp + p % a.nx] -= mz;
+    s = 0;
+    for (int p = tid; p < N; p += blockDim.x) s += z[Q(p)];
+    const T mz = T(block_sum<T>(s, red) / N);
+    for (int p = tid; p < N; p += blockDim.x) z[Q(p)] -= mz;
 }
 
 // ------------------------------------------------------ coefficient setup
@@ -556,7 +571,7 @@ int Multigrid::vcycle(size_t l) {
     const MgLevel &f = L[l];
     const FemC k = fem_consts(dx_, dy_);
     if (l + 1 == L.size()) {
-        k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 64);
+        k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 1e-6, 20000);
         KCHECK();
         return 0;
     }
@@ -583,7 +598,7 @@ int Multigrid::vcycle_f(size_t l) {
     const MgLevel &f = L[l];
     const FemC k = fem_consts(dx_, dy_);
     if (l + 1 == L.size()) {
-        k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 64);
+        k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 1e-5, 20000);
         KCHECK();
         return 0;
     }

@@ -789,6 +789,10 @@ int fluid::step(double dt) {  // fluid_2d.cc:298-369
     return 0;
 }
 
+double mono_diff_probe(double f0, double f1, double f2, double f3, double f4) {
+    return fluid::mono_diff(f0, f1, f2, f3, f4);
+}
+
 fluid *fluid_create(const sim_spec &sp, const std::vector<object_spec> &objs) { return new fluid(sp, objs); }
 void fluid_destroy(fluid *f) { delete f; }
 double fluid_dt_reg(fluid *f) { return f->dt_reg; }
@@ -808,6 +812,10 @@ static bool field_ptr(fluid *f, const std::string &nm, int k, size_t &off, bool 
     else if (nm == "s12") off = offsetof(cellf, s12);
     else if (nm == "s21") off = offsetof(cellf, s21);
     else if (nm == "s22") off = offsetof(cellf, s22);
+    else if (nm == "c0") off = offsetof(cellf, c0);
+    else if (nm == "c1") off = offsetof(cellf, c1);
+    else if (nm == "ul") off = offsetof(cellf, ul);
+    else if (nm == "vd") off = offsetof(cellf, vd);
     else {
         if (k < 0 || k >= (int)f->obj.size()) return false;
         solidf = true;

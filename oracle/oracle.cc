@@ -115,3 +115,13 @@ int orc_fluid_set(void *h, const char *name, int k, const double *in) { return f
 void orc_fluid_stats(void *h, double *out4) { fluid_solver_stats((fluid *)h, out4); }
 
 }
+
+namespace orc {
+double mono_diff_probe(double, double, double, double, double);
+}
+extern "C" {
+// fluid_2d::mono_diff (fluid_2d.cc:604-634), for tests
+double orc_mono_diff(double f0, double f1, double f2, double f3, double f4) {
+    return orc::mono_diff_probe(f0, f1, f2, f3, f4);
+}
+}

@@ -1,0 +1,124 @@
+"""GPU projection solvers (mgs_mac_* / mgs_fem_* replacements).
+
+Operators: equal to the definitions assembled in tests/operators.py to
+1e-13 relative (the same operators the oracle is pinned to). Solves: the
+relative residual reaches the requested tolerance, and the mean-free
+solution agrees with a direct solve / the oracle to 1e-8 relative at
+tolerance 1e-10. The reference's own solver (TGMG) is not in the reference
+tree: its smoother, coarse operators and cycle counts are not reproduced,
+only the solution it is asked to produce.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import incrmt_amd as ia
+from tests import operators as ops
+
+pytestmark = pytest.mark.gpu
+
+
+def _mac(m, n, varying, seed=1):
+    dx, dy = 1.0 / m, 1.3 / n
+    irho = ops.mac_irho_from_rho(ops.blob_density(m, n, seed=seed), m, n) if varying else None
+    s = ia.MgSolver(ia.MG_MAC, m, n, dx, dy)
+    s.set_irho(irho)
+    return s, ops.mac_matrix(m, n, dx, dy, irho)
+
+
+def _fem(m, n, varying, seed=1):
+    dx, dy = 1.0 / (m - 1), 1.3 / (n - 1)
+    irho = ops.fem_irho_from_rho(ops.blob_density(m - 1, n - 1, seed=seed), m - 1, n - 1) if varying else None
+    s = ia.MgSolver(ia.MG_FEM, m, n, dx, dy)
+    s.set_irho(irho)
+    return s, ops.fem_matrix(m, n, dx, dy, irho)
+
+
+@pytest.mark.parametrize("m,n", [(16, 16), (64, 40), (30, 18), (7, 9)])
+@pytest.mark.parametrize("varying", [False, True])
+def test_mac_operator(m, n, varying):
+    s, A = _mac(m, n, varying)
+    x = np.random.default_rng(0).standard_normal(m * n)
+    assert np.abs(s.apply(x) - A @ x).max() <= 1e-13 * np.abs(A @ x).max()
+
+
+@pytest.mark.parametrize("m,n", [(17, 17), (65, 41), (31, 19), (8, 9)])
+@pytest.mark.parametrize("varying", [False, True])
+def test_fem_operator(m, n, varying):
+    s, A = _fem(m, n, varying)
+    x = np.random.default_rng(0).standard_normal(m * n)
+    assert np.abs(s.apply(x) - A @ x).max() <= 1e-13 * np.abs(A @ x).max()
+
+
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+@pytest.mark.parametrize("m,n", [(32, 32), (48, 20), (33, 17)])
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
+def test_solve_matches_direct(kind, m, n, precision):
+    s, A = (_mac if kind == "mac" else _fem)(m, n, True, seed=2)
+    b = np.random.default_rng(3).standard_normal(m * n)
+    z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=400, precision=precision)
+    assert rr <= 1e-10
+    zd = ops.neumann_solve(A, b)
+    assert np.linalg.norm((z - z.mean()) - zd) <= 1e-8 * np.linalg.norm(zd)
+
+
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+def test_warm_start_and_zero_rhs(kind):
+    s, _ = (_mac if kind == "mac" else _fem)(64, 64, True)
+    b = np.random.default_rng(4).standard_normal(64 * 64)
+    z1, c1, _ = s.solve(b, rel_tol=1e-10)
+    z2, c2, r2 = s.solve(b, rel_tol=1e-10)  # warm start from z1
+    assert c1 > 0 and c2 == 0 and r2 <= 1e-10
+    z3, c3, r3 = s.solve(np.zeros(64 * 64), rel_tol=1e-10)
+    assert c3 == 0
+
+
+def test_not_converged_is_reported():
+    s, _ = _mac(64, 64, True)
+    b = np.random.default_rng(5).standard_normal(64 * 64)
+    with pytest.raises(ia.IncrmtError) as e:
+        s.solve(b, rel_tol=1e-15, max_cycles=1)
+    assert e.value.code == ia.ERR_NOT_CONVERGED
+
+
+def fourier_rhs(nodes, seed=20260101, modes=32):
+    """Divergence of a seeded smooth velocity (32 Fourier modes, amplitudes
+    ~ |k|^-2) on the unit square, by the FEM source formula
+    (fluid_2d.cc:1303-1308) with dt = 1; mean projected out."""
+    m = nodes - 1
+    h = 1.0 / (nodes - 1)
+    rng = np.random.default_rng(seed)
+    ks = rng.choice(256, size=modes, replace=False)
+    k1, k2 = ks % 16 + 1, ks // 16 + 1
+    kn = np.sqrt(k1**2 + k2**2)
+    a = rng.standard_normal(modes) / kn**2
+    bq = rng.standard_normal(modes) / kn**2
+    x = (np.arange(m) + 0.5) * h
+    X, Y = np.meshgrid(x, x)
+    u = np.zeros((m, m))
+    v = np.zeros((m, m))
+    for q in range(modes):
+        u += a[q] * np.sin(np.pi * k1[q] * X) * np.sin(np.pi * k2[q] * Y)
+        v += bq[q] * np.sin(np.pi * k1[q] * X) * np.sin(np.pi * k2[q] * Y)
+    c0 = np.zeros((m + 2, m + 2))
+    c1 = np.zeros((m + 2, m + 2))
+    c0[1:-1, 1:-1] = u
+    c1[1:-1, 1:-1] = v
+    hx = hy = 0.5 * h
+    b = (hx * (c0[:-1, :-1] + c0[1:, :-1] - c0[:-1, 1:] - c0[1:, 1:])
+         + hy * (c1[:-1, :-1] - c1[1:, :-1] + c1[:-1, 1:] - c1[1:, 1:]))
+    b = b.ravel()
+    return b - b.mean()
+
+
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
+def test_fem_1024_fourier_projection(precision):
+    # the standalone FEM projection on 1024x1024 cells (1025^2 nodes, the
+    # step's FEM size): converges to a relative residual of 1e-10 from zero
+    n = 1025
+    s = ia.MgSolver(ia.MG_FEM, n, n, 1.0 / (n - 1), 1.0 / (n - 1))
+    b = fourier_rhs(n)
+    z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=100, precision=precision)
+    assert rr <= 1e-10 and cyc < 40
+    assert np.isfinite(z).all()

@@ -1,0 +1,149 @@
+"""GPU fluid_2d::step_forward vs the oracle, per step.
+
+Both sides solve the projections to a relative residual far below the
+stated tolerances (GPU 1e-12, oracle 1e-13). Stated per-step tolerances:
+u, v, p relative L2 <= 1e-6; X, Y max |difference| <= 1e-9; band status
+codes equal on at least 99.5% of the band points and phi within 0.05 dx
+(the GPU band rebuild equals the reference's heap march except on exact
+phi ties, which the mirror-symmetric initial rotor has). After the first
+steps the differences are roundoff plus the solver tolerance (~1e-10).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import incrmt_amd as ia
+from tests import divergence as dv
+from tests import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+
+
+def compare(g, o, m, dx, tag):
+    inner = (slice(2, -2), slice(2, -2))
+    for nm in ("u", "v", "p"):
+        r = rel(g.get(nm)[inner], o.get(nm)[inner])
+        assert r <= 1e-6, f"{tag}: {nm} relative L2 {r:.3e}"
+    for k in range(g.nobj):
+        for nm in ("X", "Y"):
+            d = np.abs(g.get(nm, k) - o.get(nm, k))[inner].max()
+            assert d <= 1e-9, f"{tag}: {nm}[{k}] max diff {d:.3e}"
+        sg, so = g.get("s", k), o.get("s", k)
+        band = (so >= 2) & (so <= 5)
+        assert (sg[band] != so[band]).mean() <= 0.005, f"{tag}: status mismatch"
+        dphi = np.abs(g.get("phi", k) - o.get("phi", k))[band & (sg == so)].max()
+        assert dphi <= 0.05 * dx, f"{tag}: phi diff {dphi:.3e}"
+
+
+def gpu_from_spec(spec, tol=1e-12):
+    f = ia.Fluid2D(spec["m"], spec["n"], visc=spec["visc"], rhof=spec["rhof"], tw=spec["tw"], pinw=spec["pinw"],
+                   ax=spec["ax"], bx=spec["bx"], ay=spec["ay"], by=spec["by"])
+    for kind, p, G, rhos, evm in spec["objects"]:
+        if kind == ol.SPIN:
+            f.add_spin(*p, G=G, rhos=rhos, ev_trans_mult=evm)
+        else:
+            f.add_circle(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
+    f.set_solver(rel_tol=tol, max_cycles=200)
+    f.initialize(spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"])
+    return f
+
+
+@pytest.mark.parametrize("m", [64, 128])
+def test_simple_spin_per_step(m):
+    spec = ol.simple_spin_spec(m)
+    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    assert g.dt_reg == o.dt_reg
+    l, adt = g.timestep_select(4 * math.pi / 240)
+    dx = 2.0 / m
+    compare(g, o, m, dx, "init")
+    for k in range(1, 21):
+        g.step_forward(adt)
+        o.step(adt)
+        compare(g, o, m, dx, f"step {k}")
+    assert g.time == pytest.approx(o.time, rel=1e-15)
+
+
+def circles_spec(m, centres):
+    # sediment-style solids: rho_s/rho_f = 2, G = 1, mu = 0.01, gravity 1
+    return dict(m=m, n=m, ax=-1.0, bx=1.0, ay=-1.0, by=1.0, visc=0.01, rhof=1.0, tw=2.5, pinw=2.5,
+                w_scale=3.0, w_force=20.0, ev_mult=0.8, dt_pad=0.4, dt_ev_pad=0.8,
+                objects=[(ol.CIRCLE, [cx, cy, r, 1.0, 0, 0], 1.0, 2.0, 1.0) for cx, cy, r in centres])
+
+
+@pytest.mark.parametrize("centres", [[(0.13, 0.07, 0.3)], [(-0.31, 0.02, 0.27), (0.29, -0.04, 0.3)]])
+def test_sedimenting_circles_per_step(centres):
+    m = 64
+    spec = circles_spec(m, centres)
+    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    dx = 2.0 / m
+    for k in range(1, 11):
+        g.step_forward(g.dt_reg)
+        o.step(o.dt_reg)
+        compare(g, o, m, dx, f"step {k}")
+
+
+def test_projected_divergence_matches_oracle():
+    m = 64
+    spec = ol.simple_spin_spec(m)
+    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    l, adt = g.timestep_select(4 * math.pi / 240)
+    for _ in range(5):
+        g.step_forward(adt)
+        o.step(adt)
+    h = 2.0 / m
+    dg = dv.mac_div(g.get("ul"), g.get("vd"), m, m, h, h)
+    assert np.abs(dg).max() <= 1e-9 * np.abs(g.get("ul")).max() / h
+    fg = dv.fem_div(g.get("u"), g.get("v"), m, m, h, h)
+    fo = dv.fem_div(o.get("u"), o.get("v"), m, m, h, h)
+    assert rel(fg, fo) <= 1e-6
+
+
+def test_rest_state():
+    g = ia.simple_spin(128, omega=0.0, rel_tol=1e-12)
+    g.steps(g.dt_reg, 4)
+    assert np.abs(g.get("u")).max() < 1e-8 and np.abs(g.get("v")).max() < 1e-8
+
+
+def test_1024_properties():
+    # the bench workload at full size, checked by size-independent properties
+    g = ia.simple_spin(1024, rel_tol=1e-8)
+    l, adt = g.timestep_select(4 * math.pi / 240)
+    g.steps(adt, 3)
+    st = g.stats()
+    assert st["mac_rel_res"] <= 1e-8 and st["fem_rel_res"] <= 1e-8
+    u, v = g.get("u"), g.get("v")
+    assert np.isfinite(u).all() and np.isfinite(v).all()
+    h = 2.0 / 1024
+    dg = dv.mac_div(g.get("ul"), g.get("vd"), 1024, 1024, h, h)
+    assert np.abs(dg).max() <= 1e-6 * np.abs(g.get("ul")).max() / h
+    # interface layers sit on sign changes: every status-3/4 point has a
+    # 4-neighbour of the other sign
+    s, phi = g.get("s"), g.get("phi")
+    iface = (s == 3) | (s == 4)
+    pos = phi > 0
+    other = np.zeros_like(iface)
+    other[1:, :] |= pos[1:, :] != pos[:-1, :]
+    other[:-1, :] |= pos[:-1, :] != pos[1:, :]
+    other[:, 1:] |= pos[:, 1:] != pos[:, :-1]
+    other[:, :-1] |= pos[:, :-1] != pos[:, 1:]
+    assert np.all(other[iface])
+    # the anchor turns the rotor counter-clockwise (theta = pi(1-cos t),
+    # object.cc:175-181): the solid's angular momentum about the pivot is positive
+    x = -1 + h * (np.arange(1028) - 1.5)
+    X, Y = np.meshgrid(x, x)
+    solid = phi < 0
+    assert (X * v - Y * u)[solid].sum() > 0
+
+
+def test_solver_failure_is_reported():
+    g = ia.simple_spin(64)
+    g.set_solver(rel_tol=1e-15, max_cycles=1)
+    with pytest.raises(ia.IncrmtError) as e:
+        g.step_forward(g.dt_reg)
+    assert e.value.code == ia.ERR_NOT_CONVERGED
