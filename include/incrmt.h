@@ -46,7 +46,8 @@ typedef struct incrmt_stats {
     long total_mac_cycles, total_fem_cycles, steps;
     int reinit_passes, extend_passes; /* fixed-point passes of the last step */
     double smoother_ms;               /* summed fine-level smoother time (timing on) */
-    long smoother_launches;
+    long smoother_launches;           /* fine-level smoother kernel launches timed */
+    double smoother_bytes;            /* their algorithmic bytes (z, b, coefficients read; z written) */
 } incrmt_stats;
 
 /* Last error message of the calling thread. */
@@ -77,7 +78,9 @@ int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int
 int incrmt_fluid_step(incrmt_fluid *f, double dt);
 /* k calls of step_forward(dt), the inner loop of fluid_2d::solve (fluid_2d.cc:274). */
 int incrmt_fluid_steps(incrmt_fluid *f, double dt, int k);
-/* fluid_2d::dt_reg, fluid_2d::time, fluid_2d::timestep_select (fluid_2d.hh:173-177). */
+/* fluid_2d::dt_reg, fluid_2d::time, fluid_2d::timestep_select (fluid_2d.hh:173-177);
+ * timestep_select returns the number of steps l (adt = interval/l), or -1
+ * before initialize. */
 double incrmt_fluid_dt_reg(const incrmt_fluid *f);
 double incrmt_fluid_time(const incrmt_fluid *f);
 int incrmt_fluid_timestep_select(const incrmt_fluid *f, double interval, double *adt);

@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("mac_cycles", _I), ("fem_cycles", _I), ("mac_rel_res", _D), ("fem_rel_res", _D),
                 ("total_mac_cycles", ctypes.c_long), ("total_fem_cycles", ctypes.c_long),
                 ("steps", ctypes.c_long), ("reinit_passes", _I), ("extend_passes", _I),
-                ("smoother_ms", _D), ("smoother_launches", ctypes.c_long)]
+                ("smoother_ms", _D), ("smoother_launches", ctypes.c_long), ("smoother_bytes", _D)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

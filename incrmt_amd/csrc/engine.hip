@@ -541,6 +541,7 @@ int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out) {
     *out = f->e.st;
     out->smoother_ms = f->e.mac.smoother_ms + f->e.fem.smoother_ms;
     out->smoother_launches = f->e.mac.smoother_launches + f->e.fem.smoother_launches;
+    out->smoother_bytes = f->e.mac.smoother_bytes + f->e.fem.smoother_bytes;
     return 0;
 }
 
@@ -549,6 +550,7 @@ int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable) {
     f->e.mac.time_smoother = f->e.fem.time_smoother = enable != 0;
     f->e.mac.smoother_ms = f->e.fem.smoother_ms = 0;
     f->e.mac.smoother_launches = f->e.fem.smoother_launches = 0;
+    f->e.mac.smoother_bytes = f->e.fem.smoother_bytes = 0;
     return 0;
 }
 

@@ -8,9 +8,12 @@
 namespace incrmt {
 
 // Counter slots in LsWork::cnt
-enum { kCntIface = 0, kCntCand, kCntPband, kCntErr, kCntIter };
+enum { kCntIface = 0, kCntCand, kCntPband, kCntErr, kCntTileFim, kCntTileExt, kCntIter };
 constexpr int kIterSlots = 64;
 enum { kErrNewton = 1, kErrSingular = 2 };
+
+// The fixed-point passes run on 32x8 tiles of the ghosted grid held in LDS.
+constexpr int kTileX = 32, kTileY = 8;
 
 struct LsWork {
     double *va, *vb;      // fixed-point buffers (ghosted grid)
@@ -18,6 +21,10 @@ struct LsWork {
     double *fx, *fy;      // extension buffers, pong
     int *iface, *cand, *pband;  // point lists
     unsigned char *mark;  // dilation scratch
+    unsigned char *cflag; // 1 where a point takes part in the march
+    int *tflag;           // per-tile flags: bit 1 march, bit 2 extension
+    int *tiles_fim, *tiles_ext;  // active tile lists
+    int ntx, nty;         // tile grid
     int *cnt;             // counters (kCntIter + kIterSlots ints)
     int *hcnt;            // pinned host mirror
 };

@@ -27,6 +27,7 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 constexpr int kListBlocks = 1024, kListThreads = 256;
+constexpr int kTileBlocks = 512;
 
 inline dim3 grid2(int nx, int ny) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY); }
 
@@ -275,16 +276,31 @@ __global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= g.ml || J >= g.nl) return;
     const int ij = g.ls(I, J), s = a.st[ij];
-    if (s == 3 || s == 4) return;
     bool hit = false;
-    for (int d = -R; d <= R && !hit; d++) {
-        const int J2 = J + d;
-        if (J2 < 0 || J2 >= g.nl) continue;
-        hit = w.mark[ij + d * g.P] != 0;
-    }
+    if (s != 3 && s != 4)
+        for (int d = -R; d <= R && !hit; d++) {
+            const int J2 = J + d;
+            if (J2 < 0 || J2 >= g.nl) continue;
+            hit = w.mark[ij + d * g.P] != 0;
+        }
+    w.cflag[ij] = hit;
     if (hit) {
         const int k = atomicAdd(&w.cnt[kCntCand], 1);
         w.cand[k] = ij;
+        atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 1);
+    }
+}
+
+// Collect the tiles whose flag has `bit` set into a list, clearing the bit.
+__global__ void k_compact_tiles(LsWork w, int bit, int *list, int slot) {
+    const int n = w.ntx * w.nty;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int f = w.tflag[t];
+        if (f & bit) {
+            const int k = atomicAdd(&w.cnt[slot], 1);
+            list[k] = t;
+            w.tflag[t] = f & ~bit;
+        }
     }
 }
 
@@ -305,27 +321,80 @@ struct FimAcc {
     __device__ double val(int q) const { return v[q]; }
 };
 
-__global__ void k_fim(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w, const double *src, double *dst,
-                      int slot) {
-    const int n = w.cnt[kCntCand], P = g.P;
-    int changed = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int ij = w.cand[k];
-        int I, J;
-        ls_coords(g, ij, I, J);
-        const bool pos = a.st[ij] == 7;
-        const double limit = pos ? c.phi_up : c.phi_down;
-        FimAcc acc{src, a.st, src[ij], limit, pos};
-        const bool any = (I > 0 && acc.in(ij - 1)) || (I < g.ml - 1 && acc.in(ij + 1)) ||
-                         (J > 0 && acc.in(ij - P)) || (J < g.nl - 1 && acc.in(ij + P));
-        double r;
-        if (!any) r = pos ? INFINITY : -INFINITY;
-        else if (pos) { Eik<true, FimAcc> e{acc, eg, g.ml, g.nl, P}; r = e.calc(I, J, ij); }
-        else { Eik<false, FimAcc> e{acc, eg, g.ml, g.nl, P}; r = e.calc(I, J, ij); }
-        dst[ij] = r;
-        if (__double_as_longlong(r) != __double_as_longlong(src[ij])) changed = 1;
+// One pass of the band march over the active tiles. A workgroup loads its
+// tile and a 2-point halo (the reach of the local update) into LDS and
+// repeats the Jacobi update on the tile until it stops changing, the halo
+// held at the previous pass's values; passes repeat until no tile changes.
+// Any order of such relaxations reaches the same fixed point.
+constexpr int kFH = 2, kFW = kTileX + 2 * kFH, kFHt = kTileY + 2 * kFH, kFN = kFW * kFHt;
+
+struct LdsAcc {
+    const double *v;
+    const int *st;
+    double pval, limit;
+    bool pos;
+    __device__ bool in(int q) const {
+        const int s = st[q];
+        if (s < 0) return false;
+        if (pos) return s <= 4 || (v[q] < pval && v[q] < limit);
+        return s >= 3 || (v[q] > pval && v[q] > limit);
     }
-    if (__any(changed) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
+    __device__ double val(int q) const { return v[q]; }
+};
+
+__global__ __launch_bounds__(256) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
+                                                   const double *src, double *dst, int slot, int max_local) {
+    __shared__ double sv[2][kFN];
+    __shared__ int sst[kFN];
+    const int nt = w.cnt[kCntTileFim];
+    int changed_any = 0;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int tile = w.tiles_fim[t], tx = tile % w.ntx, ty = tile / w.ntx;
+        const int I0 = tx * kTileX - kFH, J0 = ty * kTileY - kFH;
+        for (int e = threadIdx.x; e < kFN; e += blockDim.x) {
+            const int I = I0 + e % kFW, J = J0 + e / kFW;
+            if (I >= 0 && I < g.ml && J >= 0 && J < g.nl) {
+                const int q = g.ls(I, J);
+                sv[0][e] = sv[1][e] = src[q];
+                sst[e] = a.st[q];
+            } else {
+                sv[0][e] = sv[1][e] = 0;
+                sst[e] = -1;
+            }
+        }
+        __syncthreads();
+        const int li = threadIdx.x % kTileX + kFH, lj = threadIdx.x / kTileX + kFH, e = lj * kFW + li;
+        const int I = I0 + li, J = J0 + lj;
+        const bool mine = I < g.ml && J < g.nl && w.cflag[g.ls(I, J)];
+        const bool pos = mine && sst[e] == 7;
+        const double limit = pos ? c.phi_up : c.phi_down;
+        int cur = 0;
+        for (int it = 0; it < max_local; it++) {
+            int changed = 0;
+            if (mine) {
+                LdsAcc acc{sv[cur], sst, sv[cur][e], limit, pos};
+                const bool any = (I > 0 && acc.in(e - 1)) || (I < g.ml - 1 && acc.in(e + 1)) ||
+                                 (J > 0 && acc.in(e - kFW)) || (J < g.nl - 1 && acc.in(e + kFW));
+                double r;
+                if (!any) r = pos ? INFINITY : -INFINITY;
+                else if (pos) { Eik<true, LdsAcc> k{acc, eg, g.ml, g.nl, kFW}; r = k.calc(I, J, e); }
+                else { Eik<false, LdsAcc> k{acc, eg, g.ml, g.nl, kFW}; r = k.calc(I, J, e); }
+                sv[cur ^ 1][e] = r;
+                changed = __double_as_longlong(r) != __double_as_longlong(sv[cur][e]);
+            }
+            const int ch = __syncthreads_or(changed);
+            cur ^= 1;
+            if (!ch) break;
+        }
+        if (mine) {
+            const int q = g.ls(I, J);
+            const double v = sv[cur][e];
+            dst[q] = v;
+            if (__double_as_longlong(v) != __double_as_longlong(src[q])) changed_any = 1;
+        }
+        __syncthreads();
+    }
+    if (__any(changed_any) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
 }
 
 // Final band (levelset.cc:300-335, 408-413): accepted points get their
@@ -365,6 +434,7 @@ __global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const doub
     if (ns == 4 || ns == 5) {
         const int k = atomicAdd(&w.cnt[kCntPband], 1);
         w.pband[k] = ij;
+        atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 2);
     }
 }
 __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w) {
@@ -374,60 +444,105 @@ __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w) {
     a.st[ij] = w.mark[ij];
 }
 
-// One Jacobi pass of the extension (traverse.hh:169-231): outside band
-// points take a weighted least-squares linear fit of (nX,nY) over band
-// neighbours with smaller phi, in a (2r+1)^2 box, r = 3 growing to 7 while
-// the fit is singular. Outside points read the previous pass (sx,sy);
-// inside points read their own predictor values in nX,nY.
-__global__ void k_extend(Grid g, ObjArrays a, LsWork w, const double *sx, const double *sy, double *dx_,
-                         double *dy_, int slot) {
-    const int n = w.cnt[kCntPband], P = g.P, m = g.ml, nn = g.nl;
-    int changed = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int ij = w.pband[k];
-        int i, j;
-        ls_coords(g, ij, i, j);
-        const double pp = a.phi[ij];
-        double S, Sx, Sy, Sxx, Sxy, Syy, det, p0, p1, p2, p3, p4, p5;
-        int r = 3;
-        do {
-            const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
-                      jhi = j + r >= nn ? nn - 1 - j : r;
-            S = Sx = Sy = Sxx = Sxy = Syy = 0;
-            p0 = p1 = p2 = p3 = p4 = p5 = 0;
-            for (int dj = jlo; dj <= jhi; dj++)
-                for (int di = ilo; di <= ihi; di++) {
-                    const int q = ij + dj * P + di, s = a.st[q];
-                    if (s < 2 || s > 5) continue;
-                    double wgt = a.phi[q];
-                    if (!(wgt < pp)) continue;
-                    wgt = pp - wgt;
-                    S += wgt; Sx += di * wgt; Sy += dj * wgt;
-                    Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
-                    const double fa = s >= 4 ? sx[q] : a.nX[q], fb = s >= 4 ? sy[q] : a.nY[q];
-                    const double xx = di, yy = dj;
-                    p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
-                    p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
-                }
-            det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
-            r++;
-        } while (fabs(det) < 1e-10 && r <= 7);
-        if (fabs(det) < 1e-10) {
-            atomicOr(&w.cnt[kCntErr], kErrSingular);
-            continue;
+// One pass of the extension (traverse.hh:169-231) over the active tiles:
+// outside band points take a weighted least-squares linear fit of (nX,nY)
+// over band neighbours with smaller phi, in a (2r+1)^2 box, r = 3 growing to
+// 7 while the fit is singular. A workgroup holds its tile and a 7-point halo
+// in LDS and repeats the fit until the tile stops changing; outside points
+// read the previous values (sx,sy), inside points their predictor values.
+constexpr int kEH = 7, kEW = kTileX + 2 * kEH, kEHt = kTileY + 2 * kEH, kEN = kEW * kEHt;
+
+__global__ __launch_bounds__(256) void k_extend_tiles(Grid g, ObjArrays a, LsWork w, const double *srcx,
+                                                      const double *srcy, double *dstx, double *dsty, int slot,
+                                                      int max_local) {
+    __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
+    __shared__ int sst[kEN];
+    const int nt = w.cnt[kCntTileExt], m = g.ml, nn = g.nl;
+    int changed_any = 0;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int tile = w.tiles_ext[t], tx = tile % w.ntx, ty = tile / w.ntx;
+        const int I0 = tx * kTileX - kEH, J0 = ty * kTileY - kEH;
+        for (int e = threadIdx.x; e < kEN; e += blockDim.x) {
+            const int I = I0 + e % kEW, J = J0 + e / kEW;
+            if (I >= 0 && I < m && J >= 0 && J < nn) {
+                const int q = g.ls(I, J), st = a.st[q];
+                sst[e] = st;
+                sphi[e] = a.phi[q];
+                const double vx = st >= 4 ? srcx[q] : a.nX[q], vy = st >= 4 ? srcy[q] : a.nY[q];
+                sx[0][e] = sx[1][e] = vx;
+                sy[0][e] = sy[1][e] = vy;
+            } else {
+                sst[e] = -1;
+                sphi[e] = 0;
+                sx[0][e] = sx[1][e] = sy[0][e] = sy[1][e] = 0;
+            }
         }
-        det = 1 / det;
-        const double al = (Sxy * Sxy - Sxx * Syy) * det, be = (Sx * Syy - Sxy * Sy) * det,
-                     ga = (Sxx * Sy - Sxy * Sx) * det;
-        const double vx = p0 * al + p1 * be + p2 * ga, vy = p3 * al + p4 * be + p5 * ga;
-        dx_[ij] = vx;
-        dy_[ij] = vy;
-        if (__double_as_longlong(vx) != __double_as_longlong(sx[ij]) ||
-            __double_as_longlong(vy) != __double_as_longlong(sy[ij]))
-            changed = 1;
+        __syncthreads();
+        const int li = threadIdx.x % kTileX + kEH, lj = threadIdx.x / kTileX + kEH, e = lj * kEW + li;
+        const int i = I0 + li, j = J0 + lj;
+        const bool mine = i < m && j < nn && (sst[e] == 4 || sst[e] == 5);
+        const double pp = sphi[e];
+        int cur = 0;
+        for (int it = 0; it < max_local; it++) {
+            int changed = 0;
+            if (mine) {
+                const double *cx = sx[cur], *cy = sy[cur];
+                double S, Sx, Sy, Sxx, Sxy, Syy, det, p0, p1, p2, p3, p4, p5;
+                int r = 3;
+                do {
+                    const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
+                              jhi = j + r >= nn ? nn - 1 - j : r;
+                    S = Sx = Sy = Sxx = Sxy = Syy = 0;
+                    p0 = p1 = p2 = p3 = p4 = p5 = 0;
+                    for (int dj = jlo; dj <= jhi; dj++)
+                        for (int di = ilo; di <= ihi; di++) {
+                            const int q = e + dj * kEW + di, s = sst[q];
+                            if (s < 2 || s > 5) continue;
+                            double wgt = sphi[q];
+                            if (!(wgt < pp)) continue;
+                            wgt = pp - wgt;
+                            S += wgt; Sx += di * wgt; Sy += dj * wgt;
+                            Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
+                            const double fa = cx[q], fb = cy[q], xx = di, yy = dj;
+                            p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                            p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                        }
+                    det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
+                    r++;
+                } while (fabs(det) < 1e-10 && r <= 7);
+                if (fabs(det) < 1e-10) {
+                    atomicOr(&w.cnt[kCntErr], kErrSingular);
+                    sx[cur ^ 1][e] = cx[e];
+                    sy[cur ^ 1][e] = cy[e];
+                } else {
+                    det = 1 / det;
+                    const double al = (Sxy * Sxy - Sxx * Syy) * det, be = (Sx * Syy - Sxy * Sy) * det,
+                                 ga = (Sxx * Sy - Sxy * Sx) * det;
+                    const double vx = p0 * al + p1 * be + p2 * ga, vy = p3 * al + p4 * be + p5 * ga;
+                    sx[cur ^ 1][e] = vx;
+                    sy[cur ^ 1][e] = vy;
+                    changed = __double_as_longlong(vx) != __double_as_longlong(cx[e]) ||
+                              __double_as_longlong(vy) != __double_as_longlong(cy[e]);
+                }
+            }
+            const int ch = __syncthreads_or(changed);
+            cur ^= 1;
+            if (!ch) break;
+        }
+        if (mine) {
+            const int q = g.ls(i, j);
+            const double vx = sx[cur][e], vy = sy[cur][e];
+            dstx[q] = vx;
+            dsty[q] = vy;
+            if (__double_as_longlong(vx) != __double_as_longlong(srcx[q]) ||
+                __double_as_longlong(vy) != __double_as_longlong(srcy[q]))
+                changed_any = 1;
+        }
+        __syncthreads();
     }
-    if (__any(changed) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
+    if (__any(changed_any) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
 }
+
 // Seed both extension buffers with the current nX,nY of the outside band,
 // or copy the converged buffer back into nX,nY.
 __global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double *fy, int to_obj) {
@@ -482,13 +597,23 @@ int ls_alloc(const Grid &g, LsWork &w) {
     HIP_CHECK(hipMalloc((void **)&w.pband, L * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.mark, N));
     HIP_CHECK(hipMemset(w.mark, 0, N));
+    HIP_CHECK(hipMalloc((void **)&w.cflag, N));
+    HIP_CHECK(hipMemset(w.cflag, 0, N));
+    w.ntx = (g.ml + kTileX - 1) / kTileX;
+    w.nty = (g.nl + kTileY - 1) / kTileY;
+    const size_t T = (size_t)w.ntx * w.nty + 64;
+    HIP_CHECK(hipMalloc((void **)&w.tflag, T * sizeof(int)));
+    HIP_CHECK(hipMemset(w.tflag, 0, T * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&w.tiles_fim, T * sizeof(int)));
+    HIP_CHECK(hipMalloc((void **)&w.tiles_ext, T * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.cnt, (kCntIter + kIterSlots) * sizeof(int)));
     HIP_CHECK(hipHostMalloc((void **)&w.hcnt, (kCntIter + kIterSlots) * sizeof(int), hipHostMallocDefault));
     return 0;
 }
 
 void ls_free(LsWork &w) {
-    void *p[] = {w.va, w.vb, w.ex, w.ey, w.fx, w.fy, w.iface, w.cand, w.pband, w.mark, w.cnt};
+    void *p[] = {w.va, w.vb, w.ex, w.ey, w.fx, w.fy, w.iface, w.cand, w.pband, w.mark, w.cflag, w.tflag,
+                 w.tiles_fim, w.tiles_ext, w.cnt};
     for (auto q : p)
         if (q) (void)hipFree(q);
     if (w.hcnt) (void)hipHostFree(w.hcnt);
@@ -512,12 +637,14 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     KCHECK();
     k_dilate_cols<<<gg, bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl);
     KCHECK();
+    k_compact_tiles<<<64, 256, 0, s>>>(w, 1, w.tiles_fim, kCntTileFim);
+    KCHECK();
     EikGeom eg;
     eg.init(g.dx, g.dy);
     double *bufs[2] = {w.va, w.vb};
     int passes = 0;
-    int rc = iterate(s, w, 8, 4096, [&](int p, int slot) -> int {
-        k_fim<<<kListBlocks, kListThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot);
+    int rc = iterate(s, w, 2, 4096, [&](int p, int slot) -> int {
+        k_fim_tiles<<<kTileBlocks, 256, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
         KCHECK();
         return 0;
     }, &passes);
@@ -527,6 +654,8 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
     KCHECK();
     k_finalize<<<gg, bb, 0, s>>>(g, c, a, w, bufs[passes & 1]);
+    KCHECK();
+    k_compact_tiles<<<64, 256, 0, s>>>(w, 2, w.tiles_ext, kCntTileExt);
     KCHECK();
     k_commit_status<<<gg, bb, 0, s>>>(g, a, w);
     KCHECK();
@@ -549,9 +678,9 @@ int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *
     KCHECK();
     double *bx[2] = {w.ex, w.fx}, *by[2] = {w.ey, w.fy};
     int passes = 0;
-    int rc = iterate(s, w, 8, 4096, [&](int p, int slot) -> int {
-        k_extend<<<kListBlocks, kListThreads, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1],
-                                                      by[(p + 1) & 1], slot);
+    int rc = iterate(s, w, 2, 4096, [&](int p, int slot) -> int {
+        k_extend_tiles<<<kTileBlocks, 256, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1], by[(p + 1) & 1],
+                                                   slot, 128);
         KCHECK();
         return 0;
     }, &passes);

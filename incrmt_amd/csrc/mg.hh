@@ -66,10 +66,12 @@ class Multigrid {
     const std::vector<MgLevel> &levels() const { return L; }
     MgKind kind() const { return kind_; }
     int nu1 = 2, nu2 = 2;
+    size_t tail_level() const { return tail_; }
     // timing of fine-level smoothing launches (HIP events on the stream)
     bool time_smoother = false;
     double smoother_ms = 0;
     long smoother_launches = 0;
+    double smoother_bytes = 0;  // algorithmic bytes of the timed sweeps
     double *d_red = nullptr;   // reduction partials
     double *h_red = nullptr;   // pinned host mirror
   private:
@@ -81,6 +83,7 @@ class Multigrid {
     int smooth_f(size_t l, int sweeps, bool reverse);
     MgKind kind_ = kMAC;
     std::vector<MgLevel> L;
+    size_t tail_ = 0;  // first level handled by the single-workgroup tail cycle
     std::vector<void *> allocs;
     hipStream_t s_ = nullptr;
     double dx_ = 1, dy_ = 1;

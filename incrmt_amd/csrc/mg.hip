@@ -78,6 +78,56 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
     if (diag != T(0)) z[q] = (b[q] - off) / diag;
 }
 
+// Fused colored Gauss-Seidel: a whole multi-colour sweep (MAC: two sweeps)
+// in one launch. A 64x16 tile plus a halo of one point per colour phase is
+// loaded into LDS with coalesced row reads; phase p updates its colour on
+// the region shrunk by p points, so after the last phase the tile interior
+// holds exactly what the sequence of single-colour launches would give.
+constexpr int kSX = 64, kSY = 16;
+
+template <class T, int KIND, int H>
+__global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k, T *z, const T *b, const T *c1,
+                                                            const T *c2, int phases, int first_color, int dir) {
+    constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
+    __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
+    const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
+    const int tid = threadIdx.y * kSX + threadIdx.x, nt = kSX * kSY;
+    // region points (li,lj) in [0,RW]x[0,RH]: the extra row/column holds the
+    // faces to the right/top of the region
+    for (int e = tid; e < PN; e += nt) {
+        const int li = e % PW, lj = e / PW, i = i0 + li, j = j0 + lj;
+        const bool inpad = i >= -1 && i <= a.nx && j >= -1 && j <= a.ny;
+        const int q = j * a.zp + i, c = j * a.cp + i;
+        sz[e] = inpad ? z[q] : T(0);
+        sb[e] = inpad ? b[q] : T(0);
+        s1[e] = inpad ? c1[c] : T(0);
+        if (KIND == kMAC) s2[e] = inpad ? c2[c] : T(0);
+    }
+    __syncthreads();
+    const int ncol = KIND == kMAC ? 2 : 4;
+    for (int p = 1; p <= phases; p++) {
+        // colour of phase p: forward or backward through the colours
+        const int step = p - 1;
+        const int col = dir > 0 ? (first_color + step) % ncol : ((first_color - step) % ncol + ncol) % ncol;
+        for (int e = tid; e < RW * RH; e += nt) {
+            const int li = e % RW, lj = e / RW;
+            if (li < p || li >= RW - p || lj < p || lj >= RH - p) continue;
+            const int i = i0 + li, j = j0 + lj;
+            if (i < 0 || i >= a.nx || j < 0 || j >= a.ny) continue;
+            const bool mine = KIND == kMAC ? (((i + j) & 1) == col) : ((i & 1) + 2 * (j & 1) == col);
+            if (!mine) continue;
+            const int q = lj * PW + li;
+            T diag, off;
+            if (KIND == kMAC) off = mac_offdiag<T>(sz, s1, s2, q, q, PW, PW, T(a.kx), T(a.ky), diag);
+            else off = fem_offdiag<T>(sz, s1, q, q, PW, PW, k, diag);
+            if (diag != T(0)) sz[q] = (sb[q] - off) / diag;
+        }
+        __syncthreads();
+    }
+    const int li = threadIdx.x + H, lj = threadIdx.y + H, i = i0 + li, j = j0 + lj;
+    if (i < a.nx && j < a.ny) z[j * a.zp + i] = sz[lj * PW + li];
+}
+
 // ------------------------------------------------- residual and transfers
 // MAC: fused residual of the four children and averaging restriction.
 template <class T>
@@ -213,6 +263,203 @@ __global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const 
     for (int p = tid; p < N; p += blockDim.x) s += z[Q(p)];
     const T mz = T(block_sum<T>(s, red) / N);
     for (int p = tid; p < N; p += blockDim.x) z[Q(p)] -= mz;
+}
+
+// ------------------------------------------------------------ tail cycle
+// The coarse end of a V-cycle (every level from `top` down, when they fit in
+// LDS together) runs in ONE workgroup: the levels' iterates, right-hand
+// sides, residuals and coefficients live in LDS and each smoothing colour,
+// transfer and coarse sweep is a barrier, not a kernel launch.
+constexpr int kTailMaxLev = 12;
+constexpr int kTailThreads = 1024;
+constexpr int kTailArrays = 4;  // z, b, coef1, and coef2 (MAC) or r (FEM)
+
+struct TailLev {
+    int nx, ny, zp, off;   // LDS pitch (nx+2) and offset of the padded array
+    int gcp;               // global pitch
+    const void *c1, *c2;   // global coefficient arrays at (0,0): fw,fs (MAC) or kc (FEM)
+    double kx, ky;
+};
+struct TailArgs {
+    int kind, nlev, nu1, nu2, total;
+    FemC k;
+    TailLev lv[kTailMaxLev];
+    void *z_top;
+    const void *b_top;
+    int top_pitch;
+    double reduce;
+    int max_sweeps;
+};
+
+template <class T>
+__device__ inline double tail_wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ double tail_block_sum(double v, double *red) {
+    v = tail_wave_sum<T>(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int k = 0; k < nw; k++) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+template <class T>
+__global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *pool = (T *)smem;
+    __shared__ double red[16];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto arr = [&](int a, int l) -> T * { return pool + a * A.total + A.lv[l].off + A.lv[l].zp + 1; };
+    for (int p = tid; p < kTailArrays * A.total; p += nt) pool[p] = 0;
+    __syncthreads();
+    // coefficients with their zero ring, and the top level's z and b
+    for (int l = 0; l < A.nlev; l++) {
+        const TailLev &v = A.lv[l];
+        const int W = v.nx + 2, H = v.ny + 2;
+        const T *g1 = (const T *)v.c1, *g2 = (const T *)v.c2;
+        for (int p = tid; p < W * H; p += nt) {
+            const int i = p % W - 1, j = p / W - 1;
+            arr(2, l)[j * v.zp + i] = g1[j * v.gcp + i];
+            if (A.kind == kMAC) arr(3, l)[j * v.zp + i] = g2[j * v.gcp + i];
+        }
+    }
+    {
+        const TailLev &v = A.lv[0];
+        const T *gb = (const T *)A.b_top, *gz = (const T *)A.z_top;
+        for (int p = tid; p < v.nx * v.ny; p += nt) {
+            const int i = p % v.nx, j = p / v.nx;
+            arr(1, 0)[j * v.zp + i] = gb[j * A.top_pitch + i];
+            arr(0, 0)[j * v.zp + i] = gz[j * A.top_pitch + i];
+        }
+    }
+    __syncthreads();
+    auto offd = [&](int l, int i, int j, T &diag) -> T {
+        const TailLev &v = A.lv[l];
+        const int q = j * v.zp + i;
+        if (A.kind == kMAC) return mac_offdiag<T>(arr(0, l), arr(2, l), arr(3, l), q, q, v.zp, v.zp, T(v.kx), T(v.ky), diag);
+        return fem_offdiag<T>(arr(0, l), arr(2, l), q, q, v.zp, v.zp, A.k, diag);
+    };
+    auto smooth = [&](int l, int sweeps, bool reverse) {
+        const TailLev &v = A.lv[l];
+        const int ncol = A.kind == kMAC ? 2 : 4, N = v.nx * v.ny;
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int cc = 0; cc < ncol; cc++) {
+                const int col = reverse ? ncol - 1 - cc : cc;
+                for (int p = tid; p < N; p += nt) {
+                    const int i = p % v.nx, j = p / v.nx;
+                    const bool mine = A.kind == kMAC ? (((i + j) & 1) == col) : ((i & 1) + 2 * (j & 1) == col);
+                    if (!mine) continue;
+                    T diag;
+                    const T off = offd(l, i, j, diag);
+                    if (diag != T(0)) arr(0, l)[j * v.zp + i] = (arr(1, l)[j * v.zp + i] - off) / diag;
+                }
+                __syncthreads();
+            }
+    };
+    auto residual = [&](int l, int i, int j) -> T {
+        const TailLev &v = A.lv[l];
+        T diag;
+        const T off = offd(l, i, j, diag);
+        return arr(1, l)[j * v.zp + i] - off - diag * arr(0, l)[j * v.zp + i];
+    };
+    // descend
+    for (int l = 0; l + 1 < A.nlev; l++) {
+        smooth(l, A.nu1, false);
+        const TailLev &f = A.lv[l], &c = A.lv[l + 1];
+        if (A.kind == kMAC) {
+            for (int p = tid; p < c.nx * c.ny; p += nt) {
+                const int I = p % c.nx, J = p / c.nx;
+                T s = 0;
+                for (int dj = 0; dj < 2; dj++)
+                    for (int di = 0; di < 2; di++) s += residual(l, 2 * I + di, 2 * J + dj);
+                arr(1, l + 1)[J * c.zp + I] = T(0.25) * s;
+                arr(0, l + 1)[J * c.zp + I] = 0;
+            }
+        } else {
+            for (int p = tid; p < f.nx * f.ny; p += nt) {
+                const int i = p % f.nx, j = p / f.nx;
+                arr(3, l)[j * f.zp + i] = residual(l, i, j);
+            }
+            __syncthreads();
+            const T *r = arr(3, l);
+            const int pz = f.zp;
+            for (int p = tid; p < c.nx * c.ny; p += nt) {
+                const int I = p % c.nx, J = p / c.nx, q = 2 * J * pz + 2 * I;
+                arr(1, l + 1)[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
+                                              + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
+                arr(0, l + 1)[J * c.zp + I] = 0;
+            }
+        }
+        __syncthreads();
+    }
+    // coarsest: mean-free right-hand side, sweeps to a residual reduction
+    {
+        const int l = A.nlev - 1;
+        const TailLev &v = A.lv[l];
+        const int N = v.nx * v.ny;
+        double sb = 0;
+        for (int p = tid; p < N; p += nt) sb += double(arr(1, l)[(p / v.nx) * v.zp + p % v.nx]);
+        const T mb = T(tail_block_sum<T>(sb, red) / N);
+        for (int p = tid; p < N; p += nt) arr(1, l)[(p / v.nx) * v.zp + p % v.nx] -= mb;
+        __syncthreads();
+        auto r2 = [&]() -> double {
+            double a = 0;
+            for (int p = tid; p < N; p += nt) {
+                const double r = double(residual(l, p % v.nx, p / v.nx));
+                a += r * r;
+            }
+            return tail_block_sum<T>(a, red);
+        };
+        const double r0 = r2();
+        for (int it = 0; it < A.max_sweeps && r0 > 0; it++) {
+            smooth(l, 1, false);
+            if ((it & 7) == 7 && r2() <= A.reduce * A.reduce * r0) break;
+        }
+        double sz = 0;
+        for (int p = tid; p < N; p += nt) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
+        const T mz = T(tail_block_sum<T>(sz, red) / N);
+        for (int p = tid; p < N; p += nt) arr(0, l)[(p / v.nx) * v.zp + p % v.nx] -= mz;
+        __syncthreads();
+    }
+    // ascend
+    for (int l = A.nlev - 2; l >= 0; l--) {
+        const TailLev &f = A.lv[l], &c = A.lv[l + 1];
+        const T *e = arr(0, l + 1);
+        const int p2 = c.zp;
+        for (int p = tid; p < f.nx * f.ny; p += nt) {
+            const int i = p % f.nx, j = p / f.nx, I = i >> 1, J = j >> 1;
+            T add;
+            if (A.kind == kMAC) {
+                int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+                if (I2 < 0 || I2 >= c.nx) I2 = I;
+                if (J2 < 0 || J2 >= c.ny) J2 = J;
+                add = T(0.5625) * e[J * p2 + I] + T(0.1875) * e[J * p2 + I2] + T(0.1875) * e[J2 * p2 + I]
+                      + T(0.0625) * e[J2 * p2 + I2];
+            } else {
+                const int q = J * p2 + I;
+                if (!(i & 1) && !(j & 1)) add = e[q];
+                else if (!(j & 1)) add = T(0.5) * (e[q] + e[q + 1]);
+                else if (!(i & 1)) add = T(0.5) * (e[q] + e[q + p2]);
+                else add = T(0.25) * (e[q] + e[q + 1] + e[q + p2] + e[q + p2 + 1]);
+            }
+            arr(0, l)[j * f.zp + i] += add;
+        }
+        __syncthreads();
+        smooth(l, A.nu2, true);
+    }
+    {
+        const TailLev &v = A.lv[0];
+        T *gz = (T *)A.z_top;
+        for (int p = tid; p < v.nx * v.ny; p += nt) {
+            const int i = p % v.nx, j = p / v.nx;
+            gz[j * A.top_pitch + i] = arr(0, 0)[j * v.zp + i];
+        }
+    }
 }
 
 // ------------------------------------------------------ coefficient setup
@@ -354,6 +601,45 @@ LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp
 
 #define KCHECK() HIP_CHECK(hipGetLastError())
 
+// The first level from which the remaining levels fit in LDS together.
+static size_t tail_start(const std::vector<MgLevel> &L) {
+    size_t t = L.size();
+    long long need = 0;
+    const long long budget = 60 * 1024 / (kTailArrays * sizeof(double));
+    while (t > 0) {
+        const long long sz = (long long)(L[t - 1].nx + 2) * (L[t - 1].ny + 2);
+        if (need + sz > budget || L.size() - (t - 1) > (size_t)kTailMaxLev) break;
+        need += sz;
+        t--;
+    }
+    return t;
+}
+
+template <class T>
+static TailArgs tail_args(const std::vector<MgLevel> &L, size_t t, MgKind kind, FemC k, int nu1, int nu2, bool f32,
+                          size_t *shmem) {
+    TailArgs A{};
+    A.kind = kind; A.nlev = (int)(L.size() - t); A.nu1 = nu1; A.nu2 = nu2; A.k = k;
+    int off = 0;
+    for (int q = 0; q < A.nlev; q++) {
+        const MgLevel &v = L[t + q];
+        TailLev &d = A.lv[q];
+        d.nx = v.nx; d.ny = v.ny; d.zp = v.nx + 2; d.off = off; d.gcp = v.cp; d.kx = v.kx; d.ky = v.ky;
+        if (kind == kMAC) { d.c1 = f32 ? (const void *)v.fwf : v.fw; d.c2 = f32 ? (const void *)v.fsf : v.fs; }
+        else { d.c1 = f32 ? (const void *)v.kcf : v.kc; d.c2 = nullptr; }
+        off += (v.nx + 2) * (v.ny + 2);
+    }
+    A.total = off;
+    A.z_top = f32 ? (void *)L[t].zf : (void *)L[t].z;
+    A.b_top = f32 ? (const void *)L[t].bf : (const void *)L[t].b;
+    A.top_pitch = L[t].pitch;
+    A.reduce = f32 ? 1e-5 : 1e-6;
+    A.max_sweeps = 20000;
+    *shmem = (size_t)kTailArrays * off * sizeof(T);
+    return A;
+}
+
+
 Multigrid::~Multigrid() {
     for (auto &e : ev_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (void *p : allocs) (void)hipFree(p);
@@ -409,6 +695,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
     allocs.push_back(q);
     d_red = (double *)q;
     HIP_CHECK(hipHostMalloc((void **)&h_red, 8 * sizeof(double), hipHostMallocDefault));
+    tail_ = tail_start(L);
     return set_irho_host(nullptr);
 }
 
@@ -495,49 +782,63 @@ int Multigrid::collect_events() {
         HIP_CHECK(hipEventSynchronize(ev_[k].second));
         HIP_CHECK(hipEventElapsedTime(&ms, ev_[k].first, ev_[k].second));
         smoother_ms += ms;
-        smoother_launches++;
     }
     ev_used_ = 0;
     return 0;
 }
 
+template <class T>
+static int launch_smooth(MgKind kind, const MgLevel &v, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1,
+                         const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch) {
+    *nlaunch = 0;
+    const dim3 grid((v.nx + kSX - 1) / kSX, (v.ny + kSY - 1) / kSY), block(kSX, kSY);
+    if (kind == kMAC) {
+        // red-black: up to two sweeps (four phases) per launch
+        for (int sw = 0; sw < sweeps; sw += 2) {
+            const int ph = sweeps - sw >= 2 ? 4 : 2;
+            k_smooth_fused<T, kMAC, 4><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, ph, reverse ? 1 : 0, reverse ? -1 : 1);
+            HIP_CHECK(hipGetLastError());
+            ++*nlaunch;
+        }
+    } else {
+        for (int sw = 0; sw < sweeps; sw++) {
+            k_smooth_fused<T, kFEM, 4><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, 4, reverse ? 3 : 0, reverse ? -1 : 1);
+            HIP_CHECK(hipGetLastError());
+            ++*nlaunch;
+        }
+    }
+    return 0;
+}
+
 int Multigrid::smooth(size_t l, int sweeps, bool reverse) {
     const MgLevel &v = L[l];
-    const LevelArgs a = args_of(v);
-    const int ncol = kind_ == kMAC ? 2 : 4;
-    for (int sw = 0; sw < sweeps; sw++)
-        for (int cc = 0; cc < ncol; cc++) {
-            const int col = reverse ? ncol - 1 - cc : cc;
-            const bool timed = l == 0;
-            if (timed && record_begin()) return 6;
-            if (kind_ == kMAC)
-                k_mac_smooth<double><<<grid2((v.nx + 1) / 2, v.ny), dim3(BX, BY), 0, s_>>>(a, v.z, v.b, v.fw, v.fs, col);
-            else
-                k_fem_smooth<double><<<grid2((v.nx + 1) / 2, (v.ny + 1) / 2), dim3(BX, BY), 0, s_>>>(
-                    a, fem_consts(dx_, dy_), v.z, v.b, v.kc, col);
-            KCHECK();
-            if (timed && record_end()) return 6;
-        }
+    const bool timed = l == 0;
+    if (timed && record_begin()) return 6;
+    int nl = 0;
+    int rc = launch_smooth<double>(kind_, v, args_of(v), fem_consts(dx_, dy_), v.z, v.b, kind_ == kMAC ? v.fw : v.kc,
+                                   v.fs, sweeps, reverse, s_, &nl);
+    if (rc) return rc;
+    if (timed && time_smoother) {
+        if (record_end()) return 6;
+        smoother_bytes += (kind_ == kMAC ? 40.0 : 32.0) * v.nx * v.ny * sweeps;
+        smoother_launches += nl;
+    }
     return 0;
 }
 
 int Multigrid::smooth_f(size_t l, int sweeps, bool reverse) {
     const MgLevel &v = L[l];
-    const LevelArgs a = args_of(v);
-    const int ncol = kind_ == kMAC ? 2 : 4;
-    for (int sw = 0; sw < sweeps; sw++)
-        for (int cc = 0; cc < ncol; cc++) {
-            const int col = reverse ? ncol - 1 - cc : cc;
-            const bool timed = l == 0;
-            if (timed && record_begin()) return 6;
-            if (kind_ == kMAC)
-                k_mac_smooth<float><<<grid2((v.nx + 1) / 2, v.ny), dim3(BX, BY), 0, s_>>>(a, v.zf, v.bf, v.fwf, v.fsf, col);
-            else
-                k_fem_smooth<float><<<grid2((v.nx + 1) / 2, (v.ny + 1) / 2), dim3(BX, BY), 0, s_>>>(
-                    a, fem_consts(dx_, dy_), v.zf, v.bf, v.kcf, col);
-            KCHECK();
-            if (timed && record_end()) return 6;
-        }
+    const bool timed = l == 0;
+    if (timed && record_begin()) return 6;
+    int nl = 0;
+    int rc = launch_smooth<float>(kind_, v, args_of(v), fem_consts(dx_, dy_), v.zf, v.bf, kind_ == kMAC ? v.fwf : v.kcf,
+                                  v.fsf, sweeps, reverse, s_, &nl);
+    if (rc) return rc;
+    if (timed && time_smoother) {
+        if (record_end()) return 6;
+        smoother_bytes += (kind_ == kMAC ? 20.0 : 16.0) * v.nx * v.ny * sweeps;
+        smoother_launches += nl;
+    }
     return 0;
 }
 
@@ -550,7 +851,7 @@ int Multigrid::apply(const double *x, double *y) {
 
 int Multigrid::time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches) {
     const bool was = time_smoother;
-    const double ms0 = smoother_ms;
+    const double ms0 = smoother_ms, b0 = smoother_bytes;
     const long n0 = smoother_launches;
     time_smoother = true;
     int rc = smooth_fine(sweeps, fp32);
@@ -562,6 +863,7 @@ int Multigrid::time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, 
     if (ms_per_launch) *ms_per_launch = n ? (smoother_ms - ms0) / n : 0;
     smoother_ms = ms0;
     smoother_launches = n0;
+    smoother_bytes = b0;
     return 0;
 }
 
@@ -570,6 +872,13 @@ int Multigrid::smooth_fine(int sweeps, bool fp32) { return fp32 ? smooth_f(0, sw
 int Multigrid::vcycle(size_t l) {
     const MgLevel &f = L[l];
     const FemC k = fem_consts(dx_, dy_);
+    if (l == tail_) {
+        size_t sh;
+        const TailArgs A = tail_args<double>(L, l, kind_, k, nu1, nu2, false, &sh);
+        k_tail<double><<<1, kTailThreads, sh, s_>>>(A);
+        KCHECK();
+        return 0;
+    }
     if (l + 1 == L.size()) {
         k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 1e-6, 20000);
         KCHECK();
@@ -597,6 +906,13 @@ int Multigrid::vcycle(size_t l) {
 int Multigrid::vcycle_f(size_t l) {
     const MgLevel &f = L[l];
     const FemC k = fem_consts(dx_, dy_);
+    if (l == tail_) {
+        size_t sh;
+        const TailArgs A = tail_args<float>(L, l, kind_, k, nu1, nu2, true, &sh);
+        k_tail<float><<<1, kTailThreads, sh, s_>>>(A);
+        KCHECK();
+        return 0;
+    }
     if (l + 1 == L.size()) {
         k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 1e-5, 20000);
         KCHECK();

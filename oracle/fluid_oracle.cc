@@ -12,7 +12,9 @@
 //     overwritten by compute_half_time_edge_reference_map before any read;
 //   - the r0..r3 tangential terms of compute_tangential_stability_derivatives:
 //     computed but never read anywhere in the reference.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -24,6 +26,132 @@ namespace orc {
 namespace {
 
 const double small_number = 100. * std::numeric_limits<double>::epsilon();
+const double rand_max = 2147483647.;  // RAND_MAX of glibc
+
+// glibc's rand(), i.e. random(3) with its default TYPE_3 state: an additive
+// lagged-Fibonacci generator r[k] = r[k-31] + r[k-3] (mod 2^32), output
+// r[k] >> 1, whose 31 words are seeded by the Park-Miller sequence
+// 16807^k * seed mod (2^31 - 1) and then stepped 310 times. The reference
+// never calls srand(), which is seed 1 (common.hh:28-30 draws from it).
+struct glibc_rand {
+    int32_t s[31];
+    int f = 3, r = 0;
+    explicit glibc_rand(uint32_t seed = 1) {
+        int32_t w = seed ? (int32_t)seed : 1;
+        s[0] = w;
+        for (int i = 1; i < 31; i++) {  // Schrage's form of 16807 w mod (2^31-1)
+            const int32_t hi = w / 127773, lo = w % 127773;
+            w = 16807 * lo - 2836 * hi;
+            if (w < 0) w += 2147483647;
+            s[i] = w;
+        }
+        for (int i = 0; i < 310; i++) next();
+    }
+    int next() {
+        const uint32_t v = (uint32_t)s[f] + (uint32_t)s[r];
+        s[f] = (int32_t)v;
+        f = f == 30 ? 0 : f + 1;
+        r = r == 30 ? 0 : r + 1;
+        return (int)(v >> 1);
+    }
+    double rnd(double a, double b) { return a + ((b - a) / rand_max) * static_cast<double>(next()); }  // common.hh:28-30
+};
+
+// Two-component value of bi_interp.cc's vec arithmetic, evaluated component
+// by component in the reference's operator order.
+struct v2 {
+    double x, y;
+};
+inline v2 operator+(v2 a, v2 b) { return {a.x + b.x, a.y + b.y}; }
+inline v2 operator-(v2 a, v2 b) { return {a.x - b.x, a.y - b.y}; }
+inline v2 operator*(double e, v2 a) { return {e * a.x, e * a.y}; }
+
+// bicubic_interp (bi_interp.cc:4-194) over an m x n point grid spanning
+// [ax,bx] x [ay,by]: cubic in the bulk, quadratic in the first and last
+// intervals of each direction. val(ij) gives the two components at point ij.
+template <class F>
+struct bicubic2 {
+    int m, n;
+    double ax, ay, xsp, ysp;
+    F val;
+    v2 a[16];
+    bicubic2(int m_, int n_, double ax_, double bx_, double ay_, double by_, F v)
+        : m(m_), n(n_), ax(ax_), ay(ay_), xsp((m - 1) / (bx_ - ax)), ysp((n - 1) / (by_ - ay)), val(v) {}
+    void column(int i, int ij, v2 &c0, v2 &c1, v2 &c2, v2 &c3) {  // bi_interp.cc:105-171
+        c0 = val(ij);
+        if (i == 0) {
+            c1 = (-1.5 * val(ij) + 2.0 * val(ij + 1)) - 0.5 * val(ij + 2);
+            c2 = (0.5 * val(ij) - val(ij + 1)) + 0.5 * val(ij + 2);
+            c3 = {0, 0};
+        } else if (i == m - 2) {
+            c1 = -0.5 * val(ij - 1) + 0.5 * val(ij + 1);
+            c2 = (0.5 * val(ij - 1) - val(ij)) + 0.5 * val(ij + 1);
+            c3 = {0, 0};
+        } else {
+            c1 = -0.5 * val(ij - 1) + 0.5 * val(ij + 1);
+            c2 = ((val(ij - 1) - 2.5 * val(ij)) + 2.0 * val(ij + 1)) - 0.5 * val(ij + 2);
+            c3 = ((-0.5 * val(ij - 1) + 1.5 * val(ij)) - 1.5 * val(ij + 1)) + 0.5 * val(ij + 2);
+        }
+    }
+    static void fill_ad(v2 *p, v2 c1, v2 c2, v2 c3) {  // bi_interp.cc:153-158
+        p[0] = c1;
+        p[1] = (-1.5 * c1 + 2.0 * c2) - 0.5 * c3;
+        p[2] = (0.5 * c1 - c2) + 0.5 * c3;
+        p[3] = {0, 0};
+    }
+    static void fill_au(v2 *p, v2 c0, v2 c1, v2 c2) {  // bi_interp.cc:165-170
+        p[0] = c1;
+        p[1] = -0.5 * c0 + 0.5 * c2;
+        p[2] = (0.5 * c0 - c1) + 0.5 * c2;
+        p[3] = {0, 0};
+    }
+    static void fill_a(v2 *p, v2 c0, v2 c1, v2 c2, v2 c3) {  // bi_interp.cc:178-183
+        p[0] = c1;
+        p[1] = -0.5 * c0 + 0.5 * c2;
+        p[2] = ((c0 - 2.5 * c1) + 2.0 * c2) - 0.5 * c3;
+        p[3] = ((-0.5 * c0 + 1.5 * c1) - 1.5 * c2) + 0.5 * c3;
+    }
+    void index(double &x, double &y) {  // bi_interp.cc:30-44
+        x = (x - ax) * xsp;
+        y = (y - ay) * ysp;
+        int i = static_cast<int>(x), j = static_cast<int>(y);
+        if (i < 0) i = 0; else if (i > m - 2) i = m - 2;
+        if (j < 0) j = 0; else if (j > n - 2) j = n - 2;
+        const int ij = i + m * j;
+        v2 c[4][4];  // c[q][row]: the q-th polynomial coefficient of row j-1..j+2
+        column(i, ij, c[0][1], c[1][1], c[2][1], c[3][1]);
+        column(i, ij + m, c[0][2], c[1][2], c[2][2], c[3][2]);
+        if (j == 0) {
+            column(i, ij + 2 * m, c[0][3], c[1][3], c[2][3], c[3][3]);
+            for (int q = 0; q < 4; q++) fill_ad(a + 4 * q, c[q][1], c[q][2], c[q][3]);
+        } else if (j == n - 2) {
+            column(i, ij - m, c[0][0], c[1][0], c[2][0], c[3][0]);
+            for (int q = 0; q < 4; q++) fill_au(a + 4 * q, c[q][0], c[q][1], c[q][2]);
+        } else {
+            column(i, ij - m, c[0][0], c[1][0], c[2][0], c[3][0]);
+            column(i, ij + 2 * m, c[0][3], c[1][3], c[2][3], c[3][3]);
+            for (int q = 0; q < 4; q++) fill_a(a + 4 * q, c[q][0], c[q][1], c[q][2], c[q][3]);
+        }
+        x -= i;
+        y -= j;
+    }
+    static v2 yl(const v2 *p, double y) { return p[0] + y * (p[1] + y * (p[2] + y * p[3])); }
+    static v2 dyl(const v2 *p, double y) { return p[1] + y * (2.0 * p[2] + (3.0 * y) * p[3]); }
+    v2 f(double x, double y) {  // bi_interp.cc:137-140
+        index(x, y);
+        return yl(a, y) + x * (yl(a + 4, y) + x * (yl(a + 8, y) + x * yl(a + 12, y)));
+    }
+    v2 f_grad_f(double x, double y, v2 &fx, v2 &fy) {  // bi_interp.cc:146-151
+        index(x, y);
+        fx = xsp * (yl(a + 4, y) + x * (2.0 * yl(a + 8, y) + (3 * x) * yl(a + 12, y)));
+        fy = ysp * (dyl(a, y) + x * (dyl(a + 4, y) + x * (dyl(a + 8, y) + x * dyl(a + 12, y))));
+        return yl(a, y) + x * (yl(a + 4, y) + x * (yl(a + 8, y) + x * yl(a + 12, y)));
+    }
+};
+template <class F>
+bicubic2<F> make_bicubic(int m, int n, double ax, double bx, double ay, double by, F v) {
+    return bicubic2<F>(m, n, ax, bx, ay, by, v);
+}
 
 struct cellf {  // fields.hh:7-49 (the members that the step reads)
     double u, v, p, rho, ux, uy, vx, vy;
@@ -47,6 +175,12 @@ struct solid {
     std::vector<int> st;      // band status
     std::vector<int> band;    // band traversal order
     ls_grid g;
+    // obj_track(1,10,...) of obj_simple_spin (object.hh:314-318, 456-461):
+    // position, improved-Euler stage velocity, anchored reference map
+    bool has_track = false, t_correction = true;
+    double tk[6] = {0, 0, 0, 0, 0, 0};
+    int count = 1, every = 10, prev_q = 0, wind = 0;
+    std::vector<double> rec;  // track.dat records: time, x, y, unwound angle
     solid(const ls_grid &g_) : g(g_) {}
 
     double phi_of(double X, double Y) const {
@@ -103,6 +237,9 @@ class fluid {
     std::vector<double> src, zmac, zfem, irho_mac, irho_fem;
     int mac_cycles = 0, fem_cycles = 0;
     double mac_res = 0, fem_res = 0;
+    int ntrace = 0;
+    std::vector<double> tm;  // fluid tracers: x, y, stage velocity (fluid_2d.cc:1376-1386)
+    glibc_rand rng;
 
     int at(int i, int j) const { return (j + 2) * ml + (i + 2); }
     cellf &F(int i, int j) { return fb[at(i, j)]; }
@@ -134,6 +271,17 @@ class fluid {
     void copy_pressure();
     void update_velocity(double dt);
     int step(double dt);
+    // tracers (fluid_2d.cc:1376-1484, object.cc:313-417)
+    void init_tracers(int nt);
+    bool inside_object(double tx, double ty);
+    v2 vel(double x, double y);
+    void tracer_update_1(double *tp);
+    void tracer_update_2(double *tp, double dt);
+    void tracer_remap(double &x, double &y);
+    void update_tracers(int stage, double dt);
+    bool correct_tracer(solid &o, double xref, double yref, double &xx, double &yy);
+    void track_output(solid &o);
+    void frame_output();
 };
 
 fluid::fluid(const sim_spec &s, const std::vector<object_spec> &os)
@@ -157,6 +305,9 @@ fluid::fluid(const sim_spec &s, const std::vector<object_spec> &os)
             so.c0 = scf * (1 - a2); so.c1 = scf * a1; so.c2 = 2 * scf * a2;
             so.piv_r = o.p[3]; so.th_max = o.p[4]; so.omega = o.p[5];
             so.cth = 1; so.sth = 0;
+            so.has_track = true;  // tracer at (0.9 r_scale, 0), X = x, Y = y (object.hh:456-461)
+            so.tk[0] = 0.9 * rs; so.tk[1] = 0;
+            so.tk[4] = so.tk[0]; so.tk[5] = so.tk[1];
         } else {
             so.cx = o.p[0]; so.cy = o.p[1]; so.cr = o.p[2]; so.grav = o.p[3];
         }
@@ -762,9 +913,119 @@ void fluid::update_velocity(double dt) {  // fluid_2d.cc:1354-1374
             }
 }
 
+void fluid::init_tracers(int nt) {  // fluid_2d.cc:1376-1386
+    ntrace = nt;
+    tm.assign(4 * (size_t)nt, 0.0);
+    for (size_t q = 0; q < tm.size();) {
+        double tx = rng.rnd(ax, bx);
+        double ty = rng.rnd(ay, by);
+        if (!inside_object(tx, ty)) { tm[q] = tx; tm[q + 1] = ty; q += 4; }
+    }
+}
+
+bool fluid::inside_object(double tx, double ty) {  // fluid_2d.cc:1393-1430
+    double x = (tx - ax) * xsp - 0.5, y = (ty - ay) * ysp - 0.5;
+    int i = static_cast<int>(x), j = static_cast<int>(y);
+    if (i < -1) i = -1; else if (i > m - 1) i = m - 1;
+    if (j < -1) j = -1; else if (j > n - 1) j = n - 1;
+    x -= i; y -= j;
+    const double fx = 1 - x, fy = 1 - y;
+    for (auto &o : obj) {
+        const double *p = o.phi.data() + at(i, j);
+        if (fy * (*p * fx + p[1] * x) + y * (p[ml] * fx + p[ml + 1] * x) < o.eps) return true;
+    }
+    return false;
+}
+
+v2 fluid::vel(double x, double y) {  // fluid_2d.cc:43, 1435-1438
+    const cellf *f = fb.data();
+    auto b = make_bicubic(ml, nl, ax - 1.5 * dx, bx + 1.5 * dx, ay - 1.5 * dy, by + 1.5 * dy,
+                          [f](int ij) { return v2{f[ij].u, f[ij].v}; });
+    return b.f(x, y);
+}
+
+void fluid::tracer_update_1(double *tp) {  // fluid_2d.cc:1445-1447
+    v2 v = vel(tp[0], tp[1]);
+    tp[2] = v.x; tp[3] = v.y;
+}
+
+void fluid::tracer_update_2(double *tp, double dt) {  // fluid_2d.cc:1454-1459
+    double tx = tp[0] + dt * tp[2], ty = tp[1] + dt * tp[3];
+    v2 v = vel(tx, ty);
+    tp[0] += 0.5 * dt * (tp[2] + v.x);
+    tp[1] += 0.5 * dt * (tp[3] + v.y);
+}
+
+void fluid::tracer_remap(double &x, double &y) {  // sim_type.cc:12-17
+    if (x < ax) x += (bx - ax), y = rng.rnd(ay, by);
+    if (x > bx) x -= (bx - ax), y = rng.rnd(ay, by);
+    if (y < ay) y += (by - ay), x = rng.rnd(ax, bx);
+    if (y > by) y -= (by - ay), x = rng.rnd(ax, bx);
+}
+
+bool fluid::correct_tracer(solid &o, double xref, double yref, double &xx, double &yy) {  // object.cc:383-417
+    const double tolnewt = small_number * small_number * 100, bigstep = 1;
+    const scell *sb = o.sb.data();
+    auto b = make_bicubic(ml, nl, ax - 1.5 * dx, bx + 1.5 * dx, ay - 1.5 * dy, by + 1.5 * dy,
+                          [sb](int ij) { return v2{sb[ij].X, sb[ij].Y}; });
+    int count = 0, iter = 0;
+    while (count < 4) {
+        if (iter > 100) return false;
+        v2 fx, fy, f = b.f_grad_f(xx, yy, fx, fy);
+        f = f - v2{xref, yref};
+        if (f.x * f.x + f.y * f.y < tolnewt) count++;
+        const double det = fx.x * fy.y - fy.x * fx.y;
+        if (det < small_number && det > -small_number) return false;
+        const double delx = (fy.y * f.x - fy.x * f.y) / det, dely = (-fx.y * f.x + fx.x * f.y) / det;
+        if (delx * delx + dely * dely > bigstep) return false;
+        xx -= delx; yy -= dely; iter++;
+    }
+    return true;
+}
+
+void fluid::track_output(solid &o) {  // obj_simple_spin::file_output (object.cc:202-219)
+    const double ang = std::atan2(o.tk[1], o.tk[0]);
+    if (o.t_correction && !correct_tracer(o, o.tk[4], o.tk[5], o.tk[0], o.tk[1]))  // object.cc:367-377
+        o.t_correction = false;
+    const int q = std::fabs(ang) > 0.5 * M_PI ? (ang > 0 ? 1 : -1) : 0;
+    if (q == 1 && o.prev_q == -1) o.wind--;
+    else if (q == -1 && o.prev_q == 1) o.wind++;
+    o.prev_q = q;
+    o.rec.push_back(time);
+    o.rec.push_back(o.tk[0]);
+    o.rec.push_back(o.tk[1]);
+    o.rec.push_back(ang + 2 * M_PI * o.wind);
+    o.count = 1;
+}
+
+void fluid::frame_output() {  // the object part of write_files (fluid_2d_io.cc:101), obj_track::write
+    for (auto &o : obj)
+        if (o.has_track) track_output(o);
+}
+
+void fluid::update_tracers(int stage, double dt) {  // fluid_2d.cc:1464-1484, object.cc:351-362
+    if (stage == 1) {
+        for (int q = 0; q < ntrace; q++) tracer_update_1(&tm[4 * q]);
+        for (auto &o : obj)
+            if (o.has_track) tracer_update_1(o.tk);
+    } else {
+        for (int q = 0; q < ntrace; q++) {
+            tracer_update_2(&tm[4 * q], dt);
+            tracer_remap(tm[4 * q], tm[4 * q + 1]);
+        }
+        for (auto &o : obj)
+            if (o.has_track) {
+                tracer_update_2(o.tk, dt);
+                if (o.count == o.every) track_output(o);
+                else o.count++;
+            }
+    }
+}
+
 int fluid::step(double dt) {  // fluid_2d.cc:298-369
     double ntime = time + dt;
     int rc;
+    update_tracers(1, dt);
     normal_derivatives(dt);
     godunov(true);
     edge_bc();
@@ -786,7 +1047,49 @@ int fluid::step(double dt) {  // fluid_2d.cc:298-369
     if ((rc = pin())) return rc;
     set_boundaries();
     compute_stress();
+    update_tracers(2, dt);
     return 0;
+}
+
+void fluid_init_tracers(fluid *f, int ntrace) { f->init_tracers(ntrace); }
+void fluid_frame_output(fluid *f) { f->frame_output(); }
+int fluid_ntrace(fluid *f) { return f->ntrace; }
+void fluid_get_tracers(fluid *f, double *out) { std::memcpy(out, f->tm.data(), sizeof(double) * f->tm.size()); }
+void fluid_set_tracers(fluid *f, const double *in) { std::memcpy(f->tm.data(), in, sizeof(double) * f->tm.size()); }
+// track state of object k: x, y, stage u, stage v, X, Y, count, prev_q, wind, t_correction
+int fluid_get_track(fluid *f, int k, double *st10) {
+    if (k < 0 || k >= (int)f->obj.size() || !f->obj[k].has_track) return 1;
+    const solid &o = f->obj[k];
+    for (int q = 0; q < 6; q++) st10[q] = o.tk[q];
+    st10[6] = o.count; st10[7] = o.prev_q; st10[8] = o.wind; st10[9] = o.t_correction;
+    return 0;
+}
+int fluid_set_track(fluid *f, int k, const double *st10) {
+    if (k < 0 || k >= (int)f->obj.size() || !f->obj[k].has_track) return 1;
+    solid &o = f->obj[k];
+    for (int q = 0; q < 6; q++) o.tk[q] = st10[q];
+    o.count = (int)st10[6]; o.prev_q = (int)st10[7]; o.wind = (int)st10[8]; o.t_correction = st10[9] != 0;
+    return 0;
+}
+int fluid_track_records(fluid *f, int k, double *out, int cap) {
+    if (k < 0 || k >= (int)f->obj.size()) return -1;
+    const std::vector<double> &r = f->obj[k].rec;
+    const int nrec = (int)(r.size() / 4);
+    for (int q = 0; q < 4 * std::min(nrec, cap); q++) out[q] = r[q];
+    return nrec;
+}
+void interp2(int m, int n, double ax, double bx, double ay, double by, const double *fx, const double *fy, double x,
+             double y, double *out6) {
+    auto b = make_bicubic(m, n, ax, bx, ay, by, [fx, fy](int ij) { return v2{fx[ij], fy[ij]}; });
+    v2 gx, gy, v = b.f_grad_f(x, y, gx, gy);
+    v2 w = b.f(x, y);
+    out6[0] = w.x; out6[1] = w.y; out6[2] = gx.x; out6[3] = gx.y; out6[4] = gy.x; out6[5] = gy.y;
+    (void)v;
+}
+int glibc_rand_probe(int n, int *out) {
+    glibc_rand r;
+    for (int q = 0; q < n; q++) out[q] = r.next();
+    return n;
 }
 
 double mono_diff_probe(double f0, double f1, double f2, double f3, double f4) {

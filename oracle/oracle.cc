@@ -113,6 +113,19 @@ int orc_fluid_step(void *h, double dt) { return fluid_step((fluid *)h, dt); }
 int orc_fluid_get(void *h, const char *name, int k, double *out) { return fluid_get((fluid *)h, name, k, out); }
 int orc_fluid_set(void *h, const char *name, int k, const double *in) { return fluid_set((fluid *)h, name, k, in); }
 void orc_fluid_stats(void *h, double *out4) { fluid_solver_stats((fluid *)h, out4); }
+void orc_fluid_init_tracers(void *h, int ntrace) { fluid_init_tracers((fluid *)h, ntrace); }
+void orc_fluid_frame_output(void *h) { fluid_frame_output((fluid *)h); }
+int orc_fluid_ntrace(void *h) { return fluid_ntrace((fluid *)h); }
+void orc_fluid_get_tracers(void *h, double *out) { fluid_get_tracers((fluid *)h, out); }
+void orc_fluid_set_tracers(void *h, const double *in) { fluid_set_tracers((fluid *)h, in); }
+int orc_fluid_get_track(void *h, int k, double *st10) { return fluid_get_track((fluid *)h, k, st10); }
+int orc_fluid_set_track(void *h, int k, const double *st10) { return fluid_set_track((fluid *)h, k, st10); }
+int orc_fluid_track_records(void *h, int k, double *out, int cap) { return fluid_track_records((fluid *)h, k, out, cap); }
+void orc_interp2(int m, int n, double ax, double bx, double ay, double by, const double *fx, const double *fy, double x,
+                 double y, double *out6) {
+    interp2(m, n, ax, bx, ay, by, fx, fy, x, y, out6);
+}
+int orc_glibc_rand(int n, int *out) { return glibc_rand_probe(n, out); }
 
 }
 

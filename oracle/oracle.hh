@@ -109,6 +109,21 @@ int fluid_step(fluid *f, double dt);
 int fluid_get(fluid *f, const char *name, int k, double *out);
 int fluid_set(fluid *f, const char *name, int k, const double *in);
 void fluid_solver_stats(fluid *f, double *out4);
+// Tracers: fluid_2d::init_tracers with glibc's unseeded rand() sequence
+// (fluid_2d.cc:1376-1386); the per-step improved-Euler update is part of
+// fluid_step. frame_output is the object part of write_files (track.dat).
+void fluid_init_tracers(fluid *f, int ntrace);
+void fluid_frame_output(fluid *f);
+int fluid_ntrace(fluid *f);
+void fluid_get_tracers(fluid *f, double *out);
+void fluid_set_tracers(fluid *f, const double *in);
+int fluid_get_track(fluid *f, int k, double *st10);
+int fluid_set_track(fluid *f, int k, const double *st10);
+int fluid_track_records(fluid *f, int k, double *out, int cap);
+// bicubic_interp::f and f_grad_f (bi_interp.cc) of two fields on an m x n grid
+void interp2(int m, int n, double ax, double bx, double ay, double by, const double *fx, const double *fy, double x,
+             double y, double *out6);
+int glibc_rand_probe(int n, int *out);
 
 }  // namespace orc
 

@@ -12,6 +12,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libreflevel.so")
+REF_INTERP_SO = os.path.join(ROOT, "oracle", "_ref", "librefinterp.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -157,8 +158,81 @@ class OracleFluid:
         self.lib.orc_fluid_stats(self.h, dptr(out))
         return dict(mac_cycles=int(out[0]), mac_res=out[1], fem_cycles=int(out[2]), fem_res=out[3])
 
+    # tracers (fluid_2d.cc:1376-1484, object.cc:313-417)
+    def init_tracers(self, ntrace):
+        self.lib.orc_fluid_init_tracers.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self.lib.orc_fluid_init_tracers(self.h, ntrace)
+
+    def frame_output(self):
+        self.lib.orc_fluid_frame_output.argtypes = [ctypes.c_void_p]
+        self.lib.orc_fluid_frame_output(self.h)
+
+    def tracers(self):
+        self.lib.orc_fluid_ntrace.argtypes = [ctypes.c_void_p]
+        nt = self.lib.orc_fluid_ntrace(self.h)
+        out = np.zeros(4 * nt)
+        self.lib.orc_fluid_get_tracers.argtypes = [ctypes.c_void_p, _dp]
+        self.lib.orc_fluid_get_tracers(self.h, dptr(out))
+        return out.reshape(nt, 4)
+
+    def set_tracers(self, tr):
+        a = np.ascontiguousarray(tr, dtype=np.float64).ravel()
+        self.lib.orc_fluid_set_tracers.argtypes = [ctypes.c_void_p, _dp]
+        self.lib.orc_fluid_set_tracers(self.h, dptr(a))
+
+    def track_state(self, k=0):
+        out = np.zeros(10)
+        self.lib.orc_fluid_get_track.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        if self.lib.orc_fluid_get_track(self.h, k, dptr(out)):
+            raise KeyError(k)
+        return out
+
+    def set_track_state(self, st, k=0):
+        a = np.ascontiguousarray(st, dtype=np.float64)
+        self.lib.orc_fluid_set_track.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        if self.lib.orc_fluid_set_track(self.h, k, dptr(a)):
+            raise KeyError(k)
+
+    def track(self, k=0):
+        """track.dat records of object k: (time, x, y, unwound angle)."""
+        f = self.lib.orc_fluid_track_records
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int]
+        n = f(self.h, k, dptr(np.zeros(4)), 0)
+        out = np.zeros(4 * max(n, 1))
+        f(self.h, k, dptr(out), n)
+        return out[: 4 * n].reshape(n, 4)
+
 
 def timestep_select(interval, dt_reg):
     """fluid_2d::timestep_select (fluid_2d.hh:173-177)."""
     l = int(interval / dt_reg) + 1
     return l, interval / l
+
+
+def interp2(which, m, n, ax, bx, ay, by, fx, fy, x, y, solid=True):
+    """bicubic_interp (bi_interp.cc) of (fx, fy) at (x, y): value, d/dx, d/dy.
+    which: "oracle" or "reference" (the reference's bi_interp.cc compiled
+    from its sources)."""
+    fx = np.ascontiguousarray(fx, dtype=np.float64).ravel()
+    fy = np.ascontiguousarray(fy, dtype=np.float64).ravel()
+    out = np.zeros(6)
+    D = ctypes.c_double
+    if which == "oracle":
+        f = oracle().orc_interp2
+        f.argtypes = [ctypes.c_int] * 2 + [D] * 4 + [_dp, _dp, D, D, _dp]
+        f(m, n, ax, bx, ay, by, dptr(fx), dptr(fy), x, y, dptr(out))
+        if not solid:
+            out[2:] = 0
+    else:
+        f = load(REF_INTERP_SO).ref_interp2
+        f.argtypes = [ctypes.c_int] * 2 + [D] * 4 + [_dp, _dp, D, D, ctypes.c_int, _dp]
+        f(m, n, ax, bx, ay, by, dptr(fx), dptr(fy), x, y, int(solid), dptr(out))
+    return out
+
+
+def glibc_rand_sequence(n):
+    out = np.zeros(n, dtype=np.int32)
+    f = oracle().orc_glibc_rand
+    f.argtypes = [ctypes.c_int, _ip]
+    f(n, iptr(out))
+    return out

@@ -66,3 +66,12 @@ def test_status_codes_match_header():
     assert int(codes["INCRMT_ERR_NEWTON"]) == ia.ERR_NEWTON
     assert int(codes["INCRMT_ERR_SINGULAR_FIT"]) == ia.ERR_SINGULAR_FIT
     assert int(codes["INCRMT_ERR_HIP"]) == ia.ERR_HIP
+
+
+def test_library_resolves_its_own_symbols():
+    # every C++ symbol of the engine is defined inside the library (a missing
+    # definition would only surface when the loader binds it)
+    import incrmt_amd as ia
+    r = subprocess.run(["nm", "-D", "--undefined-only", "-C", ia.LIB_PATH], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "incrmt::" not in r.stdout
