@@ -98,6 +98,29 @@ int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
 void incrmt_fluid_destroy(incrmt_fluid *f);
 
+/* ---- tracers -------------------------------------------------------------
+ * fluid_2d's ntrace (fluid_2d.cc:26-43): passive tracers placed by
+ * initialize at random positions outside the solids, drawn from the
+ * reference's unseeded glibc rand() sequence (fluid_2d.cc:1376-1386), and
+ * moved by the improved Euler method on the bicubic velocity every step
+ * (fluid_2d.cc:1464-1484). Call before initialize. */
+int incrmt_fluid_set_ntrace(incrmt_fluid *f, int ntrace);
+/* fluid_2d::tm: per tracer x, y and the stage velocity (4 doubles each). */
+int incrmt_fluid_get_tracers(incrmt_fluid *f, double *xy4);
+int incrmt_fluid_set_tracers(incrmt_fluid *f, const double *xy4);
+/* The track tracer of an obj_simple_spin solid (object.hh:456-461,
+ * object.cc:351-417): state = x, y, stage u, stage v, anchored X, Y, step
+ * count, previous quadrant, winding number, correction still on. */
+int incrmt_fluid_get_track_state(incrmt_fluid *f, int obj, double *st10);
+int incrmt_fluid_set_track_state(incrmt_fluid *f, int obj, const double *st10);
+/* The object part of fluid_2d::write_files (fluid_2d_io.cc:101,
+ * obj_track::write, object.cc:330-336): one track.dat record per track. */
+int incrmt_fluid_frame_output(incrmt_fluid *f);
+/* track.dat of solid obj: up to cap records (time, x, y, unwound angle) into
+ * rec (4 doubles each); *nrec = records written so far (the device keeps
+ * the first 16384). */
+int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec);
+
 /* ---- mgs_mac_* / mgs_fem_* ------------------------------------------------
  * Standalone solver, the constructor mgs_*_const_rho(m,n,x_prd,y_prd,dx,dy,src)
  * (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM). */

@@ -70,6 +70,13 @@ _PROTOS = [
     ("incrmt_fluid_get_stats", _I, [_V, ctypes.POINTER(Stats)]),
     ("incrmt_fluid_time_smoothers", _I, [_V, _I]),
     ("incrmt_fluid_destroy", None, [_V]),
+    ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
+    ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
+    ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
+    ("incrmt_fluid_get_track_state", _I, [_V, _I, _dp]),
+    ("incrmt_fluid_set_track_state", _I, [_V, _I, _dp]),
+    ("incrmt_fluid_frame_output", _I, [_V]),
+    ("incrmt_fluid_track", _I, [_V, _I, _dp, _I, _ip]),
     ("incrmt_mg_create", _I, [_I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_V)]),
     ("incrmt_mg_set_irho", _I, [_V, _dp]),
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
@@ -135,6 +142,7 @@ class Fluid2D:
         self.m, self.n = m, n
         self.ml, self.nl = m + 4, n + 4
         self.nobj = 0
+        self.ntrace = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -213,6 +221,45 @@ class Fluid2D:
 
     def time_smoothers(self, enable=True):
         _check(lib().incrmt_fluid_time_smoothers(self._h, int(enable)))
+
+    # tracers (fluid_2d.cc:1376-1484, object.cc:202-219, 313-417)
+    def set_ntrace(self, ntrace):
+        """fluid_2d's ntrace: passive tracers placed by initialize."""
+        _check(lib().incrmt_fluid_set_ntrace(self._h, ntrace))
+        self.ntrace = ntrace
+
+    def tracers(self):
+        out = np.zeros(4 * self.ntrace)
+        _check(lib().incrmt_fluid_get_tracers(self._h, _d(out)))
+        return out.reshape(self.ntrace, 4)
+
+    def set_tracers(self, tr):
+        a = np.ascontiguousarray(tr, dtype=np.float64).ravel()
+        if a.size != 4 * self.ntrace:
+            raise IncrmtError(ERR_ARG, "tracer array has the wrong size")
+        _check(lib().incrmt_fluid_set_tracers(self._h, _d(a)))
+
+    def track_state(self, k=0):
+        out = np.zeros(10)
+        _check(lib().incrmt_fluid_get_track_state(self._h, k, _d(out)))
+        return out
+
+    def set_track_state(self, st, k=0):
+        a = np.ascontiguousarray(st, dtype=np.float64)
+        _check(lib().incrmt_fluid_set_track_state(self._h, k, _d(a)))
+
+    def frame_output(self):
+        """The object part of fluid_2d::write_files: one track.dat record."""
+        _check(lib().incrmt_fluid_frame_output(self._h))
+
+    def track(self, k=0):
+        """track.dat records of solid k: (time, x, y, unwound angle)."""
+        n = ctypes.c_int(0)
+        _check(lib().incrmt_fluid_track(self._h, k, None, 0, ctypes.byref(n)))
+        cap = min(n.value, 16384)
+        out = np.zeros(4 * max(cap, 1))
+        _check(lib().incrmt_fluid_track(self._h, k, _d(out), cap, ctypes.byref(n)))
+        return out[: 4 * cap].reshape(cap, 4)
 
 
 def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64):

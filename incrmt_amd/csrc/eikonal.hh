@@ -216,6 +216,151 @@ struct Eik {
         if (hl) return L(va) ? ct(va, g.dx) * ti : ct(va, g.dx);
         return 0;
     }
+
+    // ---- the same update with the memory reads and the square root taken
+    // out of the decision tree (for SIMD execution): every branch of calc
+    // ends in corner(h,v,x,y) or in a pair of one-sided values (corner_bail
+    // with per-side spacings), so the tree only picks those arguments; the
+    // one arithmetic tail below then evaluates them with selects. Each
+    // expression keeps the operation order of corner() and corner_bail(),
+    // multiplying by 1 where a case has no factor (exact in IEEE
+    // arithmetic), so the result is bit-identical to calc().
+    struct Pick {
+        bool corner;       // corner(h,v,x,y), else the two one-sided values
+        bool h, v;         // second-order flags
+        double x, y;       // arguments
+        double sx, sy;     // spacings of the one-sided values
+    };
+    __host__ __device__ static double far() { return POS ? INFINITY : -INFINITY; }
+    __host__ __device__ void pick_corner(Pick &p, bool h, bool v, double x, double y) const {
+        p.corner = true; p.h = h; p.v = v; p.x = x; p.y = y; p.sx = g.dx; p.sy = g.dy;
+    }
+    __host__ __device__ void pick_sides(Pick &p, bool h, double x, double sx, bool v, double y, double sy) const {
+        p.corner = false; p.h = h; p.v = v; p.x = x; p.y = y; p.sx = sx; p.sy = sy;
+    }
+    __host__ __device__ double eval(const Pick &p) const {
+        const double ti = 2.0 / 3.0;
+        // one-sided values (corner_bail, heap.cc:371-388)
+        const double hb = ct(p.x, p.sx), vb = ct(p.y, p.sy);
+        const double ht = p.h ? hb * ti : hb, vt = p.v ? vb * ti : vb;
+        const double bail = cm(ht, vt) ? ht : vt;
+        if (!p.corner) return bail;
+        // corner (heap.cc:349-370): constants of the four cases
+        const bool h = p.h, v = p.v;
+        const double fs = h && v ? 1.5 : 1.0, fx = !h && v ? 1.5 : 1.0, fy = h && !v ? 1.5 : 1.0;
+        const double nx = h ? 1.5 : 1.0, ny = v ? 1.5 : 1.0;
+        const double B = h ? (v ? g.b2 : g.b3) : (v ? g.b1 : g.b2);
+        const double F = h ? (v ? g.f2a : g.f2b) : (v ? g.f2c : g.f2d);
+        const double I = h ? (v ? g.i2a : g.i2b) : (v ? g.i2c : g.i2d);
+        double f = fs * (p.x * fx - p.y * fy);
+        f *= f;
+        const bool ok = f * B < F;
+        const double r = ct(p.x * nx + (p.y * ny) * g.dx2bydy2, g.dxbydy * sqrt(ok ? F - f : 0.0)) * I;
+        return ok ? r : bail;
+    }
+    __host__ __device__ void fourway_p(Pick &p, double x1, double y1, double x2, double y2) const {
+        double e = 2 * (x2 - x1), f = 2 * (y2 - y1), gg, hh;
+        if (cm(e, f)) {
+            gg = e - x1; hh = e - y1; clamp0(gg); clamp0(hh);
+            if (small(gg, hh)) {
+                gg = 1.5 * f - x2; hh = f - y1; clamp0(gg); clamp0(hh);
+                if (small(gg, hh)) pick_corner(p, true, true, x2, y2);
+                else pick_corner(p, true, false, x2, y1);
+                return;
+            }
+            pick_corner(p, false, false, x1, y1);
+            return;
+        }
+        gg = f - x1; hh = f - y1; clamp0(gg); clamp0(hh);
+        if (small(gg, hh)) {
+            gg = e - x1; hh = 1.5 * e - y2; clamp0(gg); clamp0(hh);
+            if (small(gg, hh)) pick_corner(p, true, true, x2, y2);
+            else pick_corner(p, false, true, x1, y2);
+            return;
+        }
+        pick_corner(p, false, false, x1, y1);
+    }
+    __host__ __device__ void threeway1_p(Pick &p, bool h1, bool h2, bool v, double l, double r, double b) const {
+        if (h1 && h2) { pick_corner(p, true, v, cm(l, r) ? l : r, b); return; }
+        if (!h1 && !h2) { pick_corner(p, false, v, cm(l, r) ? l : r, b); return; }
+        double d, e, f;
+        if (h1) {
+            d = 2 * (l - r); e = d - r; f = (v ? 1.5 * d : d) - b;
+            clamp0(e); clamp0(f);
+            if (e * e + f * f * g.dx2bydy2 < g.dx2) pick_corner(p, true, v, l, b);
+            else pick_corner(p, false, v, r, b);
+            return;
+        }
+        d = 2 * (r - l); e = d - l; f = (v ? 1.5 * d : d) - b;
+        clamp0(e); clamp0(f);
+        if (e * e + f * f * g.dx2bydy2 < g.dx2) pick_corner(p, true, v, r, b);
+        else pick_corner(p, false, v, l, b);
+    }
+    __host__ __device__ void threeway2_p(Pick &p, bool h, bool v1, bool v2, double b, double x, double y) const {
+        if (v1 && v2) { pick_corner(p, h, true, b, cm(x, y) ? x : y); return; }
+        if (!v1 && !v2) { pick_corner(p, h, false, b, cm(x, y) ? x : y); return; }
+        double d, e, f;
+        if (v1) {
+            d = 2 * (x - y); e = d - y; f = (h ? 1.5 * d : d) - b;
+            clamp0(e); clamp0(f);
+            if (f * f + e * e * g.dx2bydy2 < g.dx2) pick_corner(p, h, true, b, x);
+            else pick_corner(p, h, false, b, y);
+            return;
+        }
+        d = 2 * (y - x); e = d - x; f = (h ? 1.5 * d : d) - b;
+        clamp0(e); clamp0(f);
+        if (f * f + e * e * g.dx2bydy2 < g.dx2) pick_corner(p, h, true, b, y);
+        else pick_corner(p, h, false, b, x);
+    }
+    // calc() with the neighbour reads done first: kX known, sX second order,
+    // eX the value look() gives (heap.cc:177-262)
+    __host__ __device__ double calc_fast(int I, int J, int ij) const {
+        const bool hd = J > 0 && a.in(ij - P), hu = J < n - 1 && a.in(ij + P);
+        const bool hl = I > 0 && a.in(ij - 1), hr = I < m - 1 && a.in(ij + 1);
+        double eL = 0, eR = 0, eD = 0, eU = 0;
+        const bool sL = hl && look(ij, -1, I > 1, eL), sR = hr && look(ij, 1, I < m - 2, eR);
+        const bool sD = hd && look(ij, -P, J > 1, eD), sU = hu && look(ij, P, J < n - 2, eU);
+        Pick p;
+        pick_sides(p, false, far(), g.dx, false, far(), g.dy);
+        if (hd) {
+            if (hu) {
+                if (hr) {
+                    if (hl) {
+                        if (sD && sU) threeway1_p(p, sL, sR, true, eL, eR, cm(eD, eU) ? eD : eU);
+                        else if (sD) {
+                            const double mlr = cm(eL, eR) ? eL : eR;
+                            if (sL && sR) threeway2_p(p, true, true, false, mlr, eD, eU);
+                            else if (sL) fourway_p(p, eR, eU, eL, eD);
+                            else if (sR) fourway_p(p, eL, eU, eR, eD);
+                            else threeway2_p(p, false, true, false, mlr, eD, eU);
+                        } else if (sU) {
+                            const double mlr = cm(eL, eR) ? eL : eR;
+                            if (sL && sR) threeway2_p(p, true, true, false, mlr, eU, eD);
+                            else if (sL) fourway_p(p, eR, eD, eL, eU);
+                            else if (sR) fourway_p(p, eL, eD, eR, eU);
+                            else threeway2_p(p, false, true, false, mlr, eU, eD);
+                        } else threeway1_p(p, sL, sR, false, eL, eR, cm(eD, eU) ? eD : eU);
+                    } else threeway2_p(p, sR, sU, sD, eR, eU, eD);
+                } else if (hl) threeway2_p(p, sL, sU, sD, eL, eU, eD);
+                else pick_sides(p, sD, eD, g.dy, sU, eU, g.dy);
+            } else if (hr) {
+                if (hl) threeway1_p(p, sL, sR, sD, eL, eR, eD);
+                else pick_corner(p, sR, sD, eR, eD);
+            } else if (hl) pick_corner(p, sL, sD, eL, eD);
+            else pick_sides(p, sD, eD, g.dy, false, far(), g.dy);
+        } else if (hu) {
+            if (hr) {
+                if (hl) threeway1_p(p, sL, sR, sU, eL, eR, eU);
+                else pick_corner(p, sR, sU, eR, eU);
+            } else if (hl) pick_corner(p, sL, sU, eL, eU);
+            else pick_sides(p, sU, eU, g.dy, false, far(), g.dy);
+        } else if (hr) {
+            if (hl) pick_sides(p, sL, eL, g.dx, sR, eR, g.dx);
+            else pick_sides(p, sR, eR, g.dx, false, far(), g.dx);
+        } else if (hl) pick_sides(p, sL, eL, g.dx, false, far(), g.dx);
+        else return 0;
+        return eval(p);
+    }
 };
 
 }  // namespace incrmt

@@ -16,6 +16,7 @@
 #include "step.hh"
 #include "mg.hh"
 #include "levelset.hh"
+#include "tracers.hh"
 #include "../../include/incrmt.h"
 
 namespace incrmt {
@@ -78,6 +79,7 @@ struct Solid {
     ObjConst c;
     ObjArrays a;
     LsWork w;
+    double r_scale = 0;  // obj_simple_spin's r_scale (its tracer starts at 0.9 r_scale)
 };
 
 struct FluidEngine {
@@ -97,6 +99,8 @@ struct FluidEngine {
     double mg_tol = 1e-8;
     int max_cycles = 100, precision = INCRMT_PREC_FP64;
     incrmt_stats st{};
+    int ntrace_req = 0;   // fluid tracers asked for before initialize
+    TracerSet tr;
 
     ~FluidEngine() {
         for (auto &o : obj) ls_free(o.w);
@@ -148,9 +152,84 @@ struct FluidEngine {
     int set_rho() { return launch_set_rho(s, g, f, solids(), rhof); }
 
     int initialize(double ev_mult, double dt_pad, double dt_ev_pad);
+    int init_tracers();
     int step(double dt);
     int pin_all(int *passes);
 };
+
+// fluid_2d::init_tracers (fluid_2d.cc:1376-1408) with the reference's
+// unseeded rand() sequence, on the host as the reference does it, against
+// the level sets just built; and the obj_track of every obj_simple_spin
+// (object.hh:456-461: one tracer at (0.9 r_scale, 0), X = x, Y = y,
+// recorded every 10 steps).
+int FluidEngine::init_tracers() {
+    tr.ntrace = ntrace_req;
+    GlibcRand rng;
+    rng.seed(1);
+    const int nt = ntrace_req;
+    std::vector<double> tm(4 * (size_t)nt, 0.0);
+    if (nt > 0) {
+        std::vector<std::vector<double>> phi(obj.size(), std::vector<double>(g.N));
+        for (size_t k = 0; k < obj.size(); k++)
+            HIP_CHECK(hipMemcpy(phi[k].data(), obj[k].a.phi, g.N * sizeof(double), hipMemcpyDeviceToHost));
+        auto inside = [&](double tx, double ty) {
+            double x = (tx - g.ax) * g.xsp - 0.5, y = (ty - g.ay) * g.ysp - 0.5;
+            int i = static_cast<int>(x), j = static_cast<int>(y);
+            if (i < -1) i = -1; else if (i > g.m - 1) i = g.m - 1;
+            if (j < -1) j = -1; else if (j > g.n - 1) j = g.n - 1;
+            x -= i; y -= j;
+            const double fx = 1 - x, fy = 1 - y;
+            for (size_t k = 0; k < obj.size(); k++) {
+                const double *p = phi[k].data();
+                const int a = g.at(i, j), b = g.at(i + 1, j), c = g.at(i, j + 1), d = g.at(i + 1, j + 1);
+                if (fy * (p[a] * fx + p[b] * x) + y * (p[c] * fx + p[d] * x) < obj[k].c.eps) return true;
+            }
+            return false;
+        };
+        for (size_t q = 0; q < tm.size();) {
+            double tx = rng.rnd(g.ax, g.bx);
+            double ty = rng.rnd(g.ay, g.by);
+            if (!inside(tx, ty)) { tm[q] = tx; tm[q + 1] = ty; q += 4; }
+        }
+    }
+    std::vector<TrackDev> tk;
+    for (size_t k = 0; k < obj.size(); k++) {
+        if (obj[k].c.kind != kSpin) continue;
+        TrackDev t{};
+        t.tk[0] = 0.9 * obj[k].r_scale;
+        t.tk[1] = 0;
+        t.tk[4] = t.tk[0];
+        t.tk[5] = t.tk[1];
+        t.obj = (int)k;
+        t.count = 1; t.every = 10; t.prev_q = 0; t.wind = 0; t.tcorr = 1; t.nrec = 0;
+        tk.push_back(t);
+    }
+    tr.ntrack = (int)tk.size();
+    tr.rec_cap = 16384;
+    void *q;
+    HIP_CHECK(hipMalloc(&q, sizeof(GlibcRand) + 64));
+    allocs.push_back(q);
+    tr.rng = (GlibcRand *)q;
+    tr.nout = (int *)((char *)q + ((sizeof(GlibcRand) + 15) / 16) * 16);
+    HIP_CHECK(hipMemcpy(tr.rng, &rng, sizeof(GlibcRand), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(tr.nout, 0, sizeof(int)));
+    if (nt > 0) {
+        HIP_CHECK(hipMalloc(&q, tm.size() * sizeof(double)));
+        allocs.push_back(q);
+        tr.tm = (double *)q;
+        HIP_CHECK(hipMemcpy(tr.tm, tm.data(), tm.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (tr.ntrack > 0) {
+        HIP_CHECK(hipMalloc(&q, tk.size() * sizeof(TrackDev)));
+        allocs.push_back(q);
+        tr.trk = (TrackDev *)q;
+        HIP_CHECK(hipMemcpy(tr.trk, tk.data(), tk.size() * sizeof(TrackDev), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMalloc(&q, (size_t)tr.ntrack * tr.rec_cap * 4 * sizeof(double)));
+        allocs.push_back(q);
+        tr.rec = (double *)q;
+    }
+    return 0;
+}
 
 int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (obj.empty()) return fail(INCRMT_ERR_ARG, "no solid added");
@@ -242,6 +321,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (int rc = compute_stress()) return rc;
     if (int rc = set_rho()) return rc;
     HIP_CHECK(hipStreamSynchronize(s));
+    if (int rc = init_tracers()) return rc;
     initialized = true;
     return 0;
 }
@@ -261,6 +341,7 @@ int FluidEngine::step(double dt) {
     const double ntime = time + dt;
     int rc;
     const Solids o = solids();
+    if ((rc = tracers_stage1(s, g, f.u, f.v, tr))) return rc;
     if ((rc = launch_normal_derivs(s, g, f, o, dt))) return rc;
     if ((rc = launch_godunov(s, g, f, o, true))) return rc;
     if ((rc = launch_tang_half(s, g, f, o, ph, dt))) return rc;
@@ -308,6 +389,7 @@ int FluidEngine::step(double dt) {
     if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
     if ((rc = launch_set_boundaries(s, g, f))) return rc;
     if ((rc = compute_stress())) return rc;
+    if ((rc = tracers_stage2(s, g, f.u, f.v, d_oa, tr, dt, time))) return rc;
     st.steps++;
     return 0;
 }
@@ -390,6 +472,12 @@ int add_object(incrmt_fluid *f, const ObjConst &c0) {
     return 0;
 }
 
+int check_ready(incrmt_fluid *f) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (!f->e.initialized) return fail(INCRMT_ERR_ARG, "call before initialize");
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -444,7 +532,9 @@ int incrmt_fluid_add_spin(incrmt_fluid *f, double r_scale, double a1, double a2,
     c.c0 = scf * (1 - a2); c.c1 = scf * a1; c.c2 = 2 * scf * a2;
     c.piv_r = piv_r; c.th_max = th_max; c.omega = omega;
     c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
-    return add_object(f, c);
+    if (int rc = add_object(f, c)) return rc;
+    f->e.obj.back().r_scale = r_scale;
+    return 0;
 }
 
 int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
@@ -551,6 +641,94 @@ int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable) {
     f->e.mac.smoother_ms = f->e.fem.smoother_ms = 0;
     f->e.mac.smoother_launches = f->e.fem.smoother_launches = 0;
     f->e.mac.smoother_bytes = f->e.fem.smoother_bytes = 0;
+    return 0;
+}
+
+int incrmt_fluid_set_ntrace(incrmt_fluid *f, int ntrace) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "tracers are placed by initialize: set them before");
+    if (ntrace < 0 || ntrace > (1 << 24)) return fail(INCRMT_ERR_ARG, "bad tracer count");
+    f->e.ntrace_req = ntrace;
+    return 0;
+}
+
+int incrmt_fluid_get_tracers(incrmt_fluid *f, double *xy4) {
+    if (int rc = check_ready(f)) return rc;
+    if (!xy4) return fail(INCRMT_ERR_ARG, "null argument");
+    FluidEngine &e = f->e;
+    if (e.tr.ntrace == 0) return 0;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    HIP_CHECK(hipMemcpy(xy4, e.tr.tm, 4 * (size_t)e.tr.ntrace * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int incrmt_fluid_set_tracers(incrmt_fluid *f, const double *xy4) {
+    if (int rc = check_ready(f)) return rc;
+    if (!xy4) return fail(INCRMT_ERR_ARG, "null argument");
+    FluidEngine &e = f->e;
+    if (e.tr.ntrace == 0) return 0;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    HIP_CHECK(hipMemcpy(e.tr.tm, xy4, 4 * (size_t)e.tr.ntrace * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// track state: x, y, stage u, stage v, X, Y, count, prev_q, wind, correction on
+static int find_track(incrmt_fluid *f, int obj, int *k) {
+    FluidEngine &e = f->e;
+    if (obj < 0 || obj >= (int)e.obj.size() || e.obj[obj].c.kind != kSpin) return fail(INCRMT_ERR_ARG, "object has no track");
+    int q = 0;
+    for (int i = 0; i < obj; i++) q += e.obj[i].c.kind == kSpin;
+    *k = q;
+    return 0;
+}
+
+int incrmt_fluid_get_track_state(incrmt_fluid *f, int obj, double *st10) {
+    if (int rc = check_ready(f)) return rc;
+    int k;
+    if (int rc = find_track(f, obj, &k)) return rc;
+    FluidEngine &e = f->e;
+    TrackDev t;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    HIP_CHECK(hipMemcpy(&t, e.tr.trk + k, sizeof t, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 6; q++) st10[q] = t.tk[q];
+    st10[6] = t.count; st10[7] = t.prev_q; st10[8] = t.wind; st10[9] = t.tcorr;
+    return 0;
+}
+
+int incrmt_fluid_set_track_state(incrmt_fluid *f, int obj, const double *st10) {
+    if (int rc = check_ready(f)) return rc;
+    int k;
+    if (int rc = find_track(f, obj, &k)) return rc;
+    FluidEngine &e = f->e;
+    TrackDev t;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    HIP_CHECK(hipMemcpy(&t, e.tr.trk + k, sizeof t, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 6; q++) t.tk[q] = st10[q];
+    t.count = (int)st10[6]; t.prev_q = (int)st10[7]; t.wind = (int)st10[8]; t.tcorr = st10[9] != 0;
+    HIP_CHECK(hipMemcpy(e.tr.trk + k, &t, sizeof t, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int incrmt_fluid_frame_output(incrmt_fluid *f) {
+    if (int rc = check_ready(f)) return rc;
+    FluidEngine &e = f->e;
+    return tracers_frame_output(e.s, e.g, e.d_oa, e.tr, e.time);
+}
+
+int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec) {
+    if (int rc = check_ready(f)) return rc;
+    int k;
+    if (int rc = find_track(f, obj, &k)) return rc;
+    FluidEngine &e = f->e;
+    TrackDev t;
+    HIP_CHECK(hipStreamSynchronize(e.s));
+    HIP_CHECK(hipMemcpy(&t, e.tr.trk + k, sizeof t, hipMemcpyDeviceToHost));
+    const int have = t.nrec < e.tr.rec_cap ? t.nrec : e.tr.rec_cap;
+    if (nrec) *nrec = t.nrec;
+    const int n = have < cap ? have : cap;
+    if (n > 0 && rec)
+        HIP_CHECK(hipMemcpy(rec, e.tr.rec + (size_t)k * e.tr.rec_cap * 4, 4 * (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -27,6 +27,7 @@ struct LsWork {
     int ntx, nty;         // tile grid
     int *cnt;             // counters (kCntIter + kIterSlots ints)
     int *hcnt;            // pinned host mirror
+    int pred_fim = 2, pred_ext = 2;  // passes the last march / extension needed
 };
 
 int ls_alloc(const Grid &g, LsWork &w);

@@ -328,10 +328,14 @@ struct FimAcc {
 // Any order of such relaxations reaches the same fixed point.
 constexpr int kFH = 2, kFW = kTileX + 2 * kFH, kFHt = kTileY + 2 * kFH, kFN = kFW * kFHt;
 
+// Known-set test on the LDS tile. val() hands the update values with the
+// inward march's sign flipped, so both marches run the outward update:
+// negating every input of the inward update negates its result exactly
+// (IEEE rounding is symmetric), which keeps one code path per wavefront.
 struct LdsAcc {
     const double *v;
     const int *st;
-    double pval, limit;
+    double pval, limit, sg;
     bool pos;
     __device__ bool in(int q) const {
         const int s = st[q];
@@ -339,13 +343,14 @@ struct LdsAcc {
         if (pos) return s <= 4 || (v[q] < pval && v[q] < limit);
         return s >= 3 || (v[q] > pval && v[q] > limit);
     }
-    __device__ double val(int q) const { return v[q]; }
+    __device__ double val(int q) const { return sg * v[q]; }
 };
 
 __global__ __launch_bounds__(256) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
                                                    const double *src, double *dst, int slot, int max_local) {
     __shared__ double sv[2][kFN];
     __shared__ int sst[kFN];
+    if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
     const int nt = w.cnt[kCntTileFim];
     int changed_any = 0;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
@@ -372,13 +377,16 @@ __global__ __launch_bounds__(256) void k_fim_tiles(Grid g, EikGeom eg, ObjConst 
         for (int it = 0; it < max_local; it++) {
             int changed = 0;
             if (mine) {
-                LdsAcc acc{sv[cur], sst, sv[cur][e], limit, pos};
+                LdsAcc acc{sv[cur], sst, sv[cur][e], limit, pos ? 1.0 : -1.0, pos};
                 const bool any = (I > 0 && acc.in(e - 1)) || (I < g.ml - 1 && acc.in(e + 1)) ||
                                  (J > 0 && acc.in(e - kFW)) || (J < g.nl - 1 && acc.in(e + kFW));
                 double r;
                 if (!any) r = pos ? INFINITY : -INFINITY;
-                else if (pos) { Eik<true, LdsAcc> k{acc, eg, g.ml, g.nl, kFW}; r = k.calc(I, J, e); }
-                else { Eik<false, LdsAcc> k{acc, eg, g.ml, g.nl, kFW}; r = k.calc(I, J, e); }
+                else {
+                    Eik<true, LdsAcc> k{acc, eg, g.ml, g.nl, kFW};
+                    r = k.calc_fast(I, J, e);
+                    if (!pos) r = -r;
+                }
                 sv[cur ^ 1][e] = r;
                 changed = __double_as_longlong(r) != __double_as_longlong(sv[cur][e]);
             }
@@ -457,6 +465,7 @@ __global__ __launch_bounds__(256) void k_extend_tiles(Grid g, ObjArrays a, LsWor
                                                       int max_local) {
     __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
     __shared__ int sst[kEN];
+    if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
     const int nt = w.cnt[kCntTileExt], m = g.ml, nn = g.nl;
     int changed_any = 0;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
@@ -482,43 +491,91 @@ __global__ __launch_bounds__(256) void k_extend_tiles(Grid g, ObjArrays a, LsWor
         const int i = I0 + li, j = J0 + lj;
         const bool mine = i < m && j < nn && (sst[e] == 4 || sst[e] == 5);
         const double pp = sphi[e];
+        // The fit's weights and moments depend on phi and the status only,
+        // which the extension does not change: for the usual radius 3 they
+        // are formed once, the contributing neighbours kept as a 7x7 bit
+        // mask, and every iteration only re-accumulates the value sums in
+        // the same neighbour order (bit-identical to forming everything).
+        unsigned long long mask = 0;
+        double al = 0, be = 0, ga = 0;
+        bool wide = false, singular = false;
+        if (mine) {
+            const int r = 3, ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
+                      jhi = j + r >= nn ? nn - 1 - j : r;
+            double S = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0, Syy = 0;
+            for (int dj = jlo; dj <= jhi; dj++)
+                for (int di = ilo; di <= ihi; di++) {
+                    const int q = e + dj * kEW + di, s = sst[q];
+                    if (s < 2 || s > 5) continue;
+                    double wgt = sphi[q];
+                    if (!(wgt < pp)) continue;
+                    wgt = pp - wgt;
+                    S += wgt; Sx += di * wgt; Sy += dj * wgt;
+                    Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
+                    mask |= 1ull << ((dj + 3) * 7 + di + 3);
+                }
+            double det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
+            if (fabs(det) < 1e-10) wide = true;  // the radius grows (traverse.hh:211-222)
+            else {
+                det = 1 / det;
+                al = (Sxy * Sxy - Sxx * Syy) * det;
+                be = (Sx * Syy - Sxy * Sy) * det;
+                ga = (Sxx * Sy - Sxy * Sx) * det;
+            }
+        }
         int cur = 0;
         for (int it = 0; it < max_local; it++) {
             int changed = 0;
             if (mine) {
                 const double *cx = sx[cur], *cy = sy[cur];
-                double S, Sx, Sy, Sxx, Sxy, Syy, det, p0, p1, p2, p3, p4, p5;
-                int r = 3;
-                do {
-                    const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
-                              jhi = j + r >= nn ? nn - 1 - j : r;
-                    S = Sx = Sy = Sxx = Sxy = Syy = 0;
-                    p0 = p1 = p2 = p3 = p4 = p5 = 0;
-                    for (int dj = jlo; dj <= jhi; dj++)
-                        for (int di = ilo; di <= ihi; di++) {
-                            const int q = e + dj * kEW + di, s = sst[q];
-                            if (s < 2 || s > 5) continue;
-                            double wgt = sphi[q];
-                            if (!(wgt < pp)) continue;
-                            wgt = pp - wgt;
-                            S += wgt; Sx += di * wgt; Sy += dj * wgt;
-                            Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
-                            const double fa = cx[q], fb = cy[q], xx = di, yy = dj;
-                            p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
-                            p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
-                        }
-                    det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
-                    r++;
-                } while (fabs(det) < 1e-10 && r <= 7);
-                if (fabs(det) < 1e-10) {
+                double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+                double a0 = al, b0 = be, g0 = ga;
+                if (!wide) {
+                    for (unsigned long long mm = mask; mm; mm &= mm - 1) {
+                        const int bit = __builtin_ctzll(mm), dj = bit / 7 - 3, di = bit % 7 - 3;
+                        const int q = e + dj * kEW + di;
+                        const double wgt = pp - sphi[q], fa = cx[q], fb = cy[q], xx = di, yy = dj;
+                        p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                        p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                    }
+                } else {
+                    double S, Sx, Sy, Sxx, Sxy, Syy, det;
+                    int r = 4;
+                    do {
+                        const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
+                                  jhi = j + r >= nn ? nn - 1 - j : r;
+                        S = Sx = Sy = Sxx = Sxy = Syy = 0;
+                        p0 = p1 = p2 = p3 = p4 = p5 = 0;
+                        for (int dj = jlo; dj <= jhi; dj++)
+                            for (int di = ilo; di <= ihi; di++) {
+                                const int q = e + dj * kEW + di, s = sst[q];
+                                if (s < 2 || s > 5) continue;
+                                double wgt = sphi[q];
+                                if (!(wgt < pp)) continue;
+                                wgt = pp - wgt;
+                                S += wgt; Sx += di * wgt; Sy += dj * wgt;
+                                Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
+                                const double fa = cx[q], fb = cy[q], xx = di, yy = dj;
+                                p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                                p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                            }
+                        det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
+                        r++;
+                    } while (fabs(det) < 1e-10 && r <= 7);
+                    if (fabs(det) < 1e-10) singular = true;
+                    else {
+                        det = 1 / det;
+                        a0 = (Sxy * Sxy - Sxx * Syy) * det;
+                        b0 = (Sx * Syy - Sxy * Sy) * det;
+                        g0 = (Sxx * Sy - Sxy * Sx) * det;
+                    }
+                }
+                if (singular) {
                     atomicOr(&w.cnt[kCntErr], kErrSingular);
                     sx[cur ^ 1][e] = cx[e];
                     sy[cur ^ 1][e] = cy[e];
                 } else {
-                    det = 1 / det;
-                    const double al = (Sxy * Sxy - Sxx * Syy) * det, be = (Sx * Syy - Sxy * Sy) * det,
-                                 ga = (Sxx * Sy - Sxy * Sx) * det;
-                    const double vx = p0 * al + p1 * be + p2 * ga, vy = p3 * al + p4 * be + p5 * ga;
+                    const double vx = p0 * a0 + p1 * b0 + p2 * g0, vy = p3 * a0 + p4 * b0 + p5 * g0;
                     sx[cur ^ 1][e] = vx;
                     sy[cur ^ 1][e] = vy;
                     changed = __double_as_longlong(vx) != __double_as_longlong(cx[e]) ||
@@ -556,30 +613,43 @@ __global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double
 
 #define KCHECK() HIP_CHECK(hipGetLastError())
 
-// Run a fixed-point iteration in batches of `batch` passes; after each batch
-// read the change count of its last pass. Returns passes, or -1 on no
-// convergence within max_passes, or an error (>0 codes negated below -1).
+// Run a fixed-point iteration in batches of passes, reading the change
+// counts back once per batch. A pass whose predecessor in the batch changed
+// nothing returns at once (the fixed point is reached and both ping-pong
+// buffers hold it), so a batch may safely run past convergence; its length
+// is the number of passes the previous iteration needed, which usually
+// makes one host round trip per iteration. *passes_out: passes up to and
+// including the first one that changed nothing. Returns 0, 7 when
+// max_passes pass without convergence, or a HIP error.
 template <class F>
-int iterate(hipStream_t s, LsWork &w, int batch, int max_passes, F pass, int *passes_out) {
+int iterate(hipStream_t s, LsWork &w, int &predict, int max_passes, F pass, int *passes_out) {
     int passes = 0;
     while (passes < max_passes) {
+        int batch = predict < 2 ? 2 : predict;
+        if (batch > kIterSlots) batch = kIterSlots;
         k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntIter, kIterSlots);
         KCHECK();
         for (int b = 0; b < batch; b++) {
-            int rc = pass(passes, b);
+            int rc = pass(passes + b, b);
             if (rc) return rc;
-            passes++;
         }
         HIP_CHECK(hipMemcpyAsync(w.hcnt, w.cnt, (kCntIter + kIterSlots) * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        if (w.hcnt[kCntErr]) break;
-        if (w.hcnt[kCntIter + batch - 1] == 0) {
-            *passes_out = passes;
+        if (w.hcnt[kCntErr]) {
+            *passes_out = passes + batch;
             return 0;
         }
+        for (int b = 0; b < batch; b++)
+            if (w.hcnt[kCntIter + b] == 0) {
+                *passes_out = passes + b + 1;
+                predict = passes + b + 1;
+                return 0;
+            }
+        passes += batch;
+        predict = passes + 2;
     }
     *passes_out = passes;
-    return w.hcnt[kCntErr] ? 0 : 7;
+    return 7;
 }
 
 }  // namespace
@@ -643,7 +713,7 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     eg.init(g.dx, g.dy);
     double *bufs[2] = {w.va, w.vb};
     int passes = 0;
-    int rc = iterate(s, w, 2, 4096, [&](int p, int slot) -> int {
+    int rc = iterate(s, w, w.pred_fim, 4096, [&](int p, int slot) -> int {
         k_fim_tiles<<<kTileBlocks, 256, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
         KCHECK();
         return 0;
@@ -678,7 +748,7 @@ int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *
     KCHECK();
     double *bx[2] = {w.ex, w.fx}, *by[2] = {w.ey, w.fy};
     int passes = 0;
-    int rc = iterate(s, w, 2, 4096, [&](int p, int slot) -> int {
+    int rc = iterate(s, w, w.pred_ext, 4096, [&](int p, int slot) -> int {
         k_extend_tiles<<<kTileBlocks, 256, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1], by[(p + 1) & 1],
                                                    slot, 128);
         KCHECK();

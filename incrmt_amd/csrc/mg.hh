@@ -79,6 +79,10 @@ class Multigrid {
     int vcycle_f(size_t l);
     int coarsen_coefs();
     int reduce_norms(double *out3);  // ||b||^2, ||r||^2, sum b of level 0
+    int reduce_dev();                // the same into d_red, no host copy
+    template <class Cycle>
+    int solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle);
+    int predict_ = 0;                // V-cycles run before the first residual check
     int smooth(size_t l, int sweeps, bool reverse);
     int smooth_f(size_t l, int sweeps, bool reverse);
     MgKind kind_ = kMAC;

@@ -937,71 +937,80 @@ int Multigrid::vcycle_f(size_t l) {
     return smooth_f(l, nu2, true);
 }
 
-int Multigrid::reduce_norms(double *out3) {
+int Multigrid::reduce_dev() {
     const MgLevel &f = L[0];
     k_norms<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), fem_consts(dx_, dy_), f.z, f.b, f.fw, f.fs, f.kc,
                                                 d_red + 8);
     KCHECK();
     k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_red);
     KCHECK();
+    return 0;
+}
+
+int Multigrid::reduce_norms(double *out3) {
+    if (int rc = reduce_dev()) return rc;
     HIP_CHECK(hipMemcpyAsync(h_red, d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIP_CHECK(hipStreamSynchronize(s_));
     for (int k = 0; k < 3; k++) out3[k] = h_red[k];
     return 0;
 }
 
-int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
+// The solve loop shared by both arithmetics. The right-hand side's mean is
+// projected out on the device (pure Neumann system) and the starting
+// residual is checked. Then, as TGMG's tgmg_predict does for the reference
+// (mgs_common.hh:37-77), the number of V-cycles the last solve needed, less
+// one, is run before the next residual check, so a solve costs two or three
+// host round trips instead of one per cycle; after that the residual is
+// checked after every cycle.
+template <class Cycle>
+int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) {
     const MgLevel &f = L[0];
-    double nr[3];
     int rc;
-    // project the right-hand side onto the range (pure Neumann system)
-    if ((rc = reduce_norms(nr))) return rc;
+    if ((rc = reduce_dev())) return rc;
     k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
     KCHECK();
+    double nr[3];
     if ((rc = reduce_norms(nr))) return rc;
     const double bn = std::sqrt(nr[0]);
     double rel = bn > 0 ? std::sqrt(nr[1]) / bn : 0;
     int cycles = 0;
+    if (rel > tol && bn > 0 && predict_ > 0) {
+        const int ahead = predict_ < max_cycles ? predict_ : max_cycles;
+        for (; cycles < ahead; cycles++)
+            if ((rc = cycle())) return rc;
+        if ((rc = reduce_norms(nr))) return rc;
+        rel = std::sqrt(nr[1]) / bn;
+    }
     while (rel > tol && bn > 0) {
         if (cycles >= max_cycles) break;
-        if ((rc = vcycle(0))) return rc;
+        if ((rc = cycle())) return rc;
         cycles++;
         if ((rc = reduce_norms(nr))) return rc;
         rel = std::sqrt(nr[1]) / bn;
     }
+    if (cycles > 0) predict_ = cycles - 1;
     if (time_smoother && (rc = collect_events())) return rc;
     if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
     return rel <= tol || bn == 0 ? 0 : 2;
 }
 
+int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
+    return solve_loop(tol, max_cycles, st, [&]() { return vcycle(0); });
+}
+
 int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
     const MgLevel &f = L[0];
-    double nr[3];
-    int rc;
-    if ((rc = reduce_norms(nr))) return rc;
-    k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
-    KCHECK();
-    if ((rc = reduce_norms(nr))) return rc;
-    const double bn = std::sqrt(nr[0]);
-    double rel = bn > 0 ? std::sqrt(nr[1]) / bn : 0;
-    int cycles = 0;
     const FemC k = fem_consts(dx_, dy_);
-    while (rel > tol && bn > 0) {
-        if (cycles >= max_cycles) break;
-        // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
+    // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
+    return solve_loop(tol, max_cycles, st, [&]() -> int {
         k_residual_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc,
                                                                     f.bf, f.zf);
         KCHECK();
-        if ((rc = vcycle_f(0))) return rc;
+        if (int rc = vcycle_f(0)) return rc;
         k_add_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, f.zf);
         KCHECK();
-        cycles++;
-        if ((rc = reduce_norms(nr))) return rc;
-        rel = std::sqrt(nr[1]) / bn;
-    }
-    if (time_smoother && (rc = collect_events())) return rc;
-    if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
-    return rel <= tol || bn == 0 ? 0 : 2;
+        return 0;
+    });
 }
 
 }  // namespace incrmt
