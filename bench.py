@@ -38,11 +38,13 @@ def gpu_run(m, steps, warmup, precision, timing=False):
     f.sync()
     dt = time.perf_counter() - t0
     s1 = f.stats()
+    shapes = f.describe()
     out = dict(seconds=dt, ms_per_step=dt / steps * 1e3, cell_steps_per_s=m * m * steps / dt,
                mac_cycles_per_step=(s1["total_mac_cycles"] - s0["total_mac_cycles"]) / steps,
                fem_cycles_per_step=(s1["total_fem_cycles"] - s0["total_fem_cycles"]) / steps,
                mac_rel_res=s1["mac_rel_res"], fem_rel_res=s1["fem_rel_res"],
-               reinit_passes=s1["reinit_passes"], extend_passes=s1["extend_passes"])
+               reinit_passes=s1["reinit_passes"], extend_passes=s1["extend_passes"],
+               smoother_shapes=shapes)
     if timing:
         # second pass with HIP events around every fine-level smoother launch
         f.time_smoothers(True)
@@ -74,6 +76,11 @@ def cpu_baseline(m, steps):
     return dict(value=m * m * steps / dt, unit="cell-steps/s", cores=threads, kind="port",
                 sample=f"{steps} oracle steps at {m}x{m} after 1 warm-up step, OpenMP {threads} threads, "
                        f"projections to relative residual 1e-8")
+
+
+def sig(v):
+    """Floats to 5 significant digits (residuals of 1e-9 stay visible)."""
+    return float(f"{v:.5g}") if isinstance(v, float) else v
 
 
 def main():
@@ -120,8 +127,8 @@ def main():
         "data": "synthetic (simple-spin rotor initial state)",
         "config": {"workload": f"simple-spin rotor, full step_forward, {m}x{m} cells, projections to rel. residual 1e-8",
                    "solver": best_name, "grid": [m, m]},
-        "variants": {"fp64": {k: round(v, 4) if isinstance(v, float) else v for k, v in fp64.items()},
-                     "mixed": {k: round(v, 4) if isinstance(v, float) else v for k, v in mixed.items()}},
+        "variants": {"fp64": {k: sig(v) for k, v in fp64.items()},
+                     "mixed": {k: sig(v) for k, v in mixed.items()}},
         "roofline": roofline,
         "device": name,
     }

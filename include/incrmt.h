@@ -94,6 +94,10 @@ int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double
 int incrmt_fluid_set_time(incrmt_fluid *f, double t);
 int incrmt_fluid_sync(incrmt_fluid *f);
 int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
+/* No reference counterpart: the fused-smoother launch shape the autotuner
+ * picked for each multigrid level ("nx x ny: SXxSY hH/SXxSY hH", FP64/FP32
+ * sweeps), for reports. */
+int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len);
 /* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
 void incrmt_fluid_destroy(incrmt_fluid *f);

@@ -71,6 +71,7 @@ _PROTOS = [
     ("incrmt_fluid_time_smoothers", _I, [_V, _I]),
     ("incrmt_fluid_destroy", None, [_V]),
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
+    ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_get_track_state", _I, [_V, _I, _dp]),
@@ -222,6 +223,12 @@ class Fluid2D:
     def time_smoothers(self, enable=True):
         _check(lib().incrmt_fluid_time_smoothers(self._h, int(enable)))
 
+    def describe(self):
+        """The autotuned fused-smoother shapes per multigrid level."""
+        buf = ctypes.create_string_buffer(2048)
+        _check(lib().incrmt_fluid_describe(self._h, buf, 2048))
+        return buf.value.decode()
+
     # tracers (fluid_2d.cc:1376-1484, object.cc:202-219, 313-417)
     def set_ntrace(self, ntrace):
         """fluid_2d's ntrace: passive tracers placed by initialize."""
@@ -345,3 +352,19 @@ def extrapolate(m, n, ax, ay, dx, dy, phi_down, phi_up, phi, fx, fy):
     _check(lib().incrmt_levelset_extrapolate(m, n, ax, ay, dx, dy, phi_down, phi_up, _d(p), _i(s), _d(a), _d(b),
                                              ctypes.byref(passes)))
     return p, s, a, b, passes.value
+
+
+def multibody(m=2048, count=64, seed=20260102, rel_tol=1e-8, precision=PREC_MIXED, max_cycles=100):
+    """The sediment-style multibody configuration: `count` obj_circle solids
+    (object.hh:128-138) placed by workloads.sediment_circles in a no-slip
+    [-1,1]^2 box at m x m cells; solid/fluid density ratio 2, shear modulus
+    1, viscosity 0.01, gravity 1 (obj_circle's accel, object.cc:31-34);
+    transition and pinning widths 2.5 cells, ev_mult 0.8, dt_pad 0.4,
+    dt_ev_pad 0.8 (the sediment.cc settings)."""
+    from .workloads import sediment_circles
+    f = Fluid2D(m, m, visc=0.01, rhof=1.0, tw=2.5, pinw=2.5)
+    for cx, cy, r in sediment_circles(count, seed, m):
+        f.add_circle(cx, cy, r, 1.0, G=1.0, rhos=2.0, ev_trans_mult=1.0)
+    f.set_solver(rel_tol=rel_tol, max_cycles=max_cycles, precision=precision)
+    f.initialize(0.8, 0.4, 0.8)
+    return f

@@ -732,6 +732,14 @@ int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec
     return 0;
 }
 
+int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len) {
+    if (int rc = check_ready(f)) return rc;
+    if (!buf || len <= 0) return fail(INCRMT_ERR_ARG, "null buffer");
+    const std::string d = "MAC " + f->e.mac.shapes() + "; FEM " + f->e.fem.shapes();
+    snprintf(buf, len, "%s", d.c_str());
+    return 0;
+}
+
 void incrmt_fluid_destroy(incrmt_fluid *f) { delete f; }
 
 // ------------------------------------------------------------ standalone MG

@@ -10,6 +10,7 @@
 #ifndef INCRMT_MG_HH
 #define INCRMT_MG_HH
 
+#include <string>
 #include <vector>
 #include "common.hh"
 
@@ -31,6 +32,20 @@ struct MgLevel {
     float *fwf, *fsf, *kcf;
     int cp;       // coefficient pitch
     double kx, ky;  // MAC 1/dx^2, 1/dy^2 of this level
+    double hx, hy;  // FEM node spacing of this level
+    int shape64 = 0, shape32 = 0;  // fused-smoother launch shape (FP64, FP32 sweeps)
+};
+
+// Transfer tables between a FEM level and a uniform coarse level whose
+// nodes do not coincide with every other fine node (node counts that are
+// not 2^k+1). Per direction: fine node i lies in coarse interval i0[i] at
+// fraction t[i]; coarse node I gathers fine nodes lo[I]..hi[I]; coarse cell
+// I overlaps fine cells clo[I]..chi[I].
+struct Resample {
+    int *i0, *lo, *hi, *clo, *chi;
+    double *t;
+    int nf, nc;
+    double hf, hc;
 };
 
 struct MgStats {
@@ -83,6 +98,17 @@ class Multigrid {
     template <class Cycle>
     int solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle);
     int predict_ = 0;                // V-cycles run before the first residual check
+    bool resampled_ = false;         // level 1 is a uniform resampling of level 0
+    Resample rx_{}, ry_{};
+    int make_resample(int nf, double hf, int nc, Resample &r);
+  public:
+    // Time every fused-smoother launch shape on each level large enough to
+    // matter and keep the fastest (results do not depend on the shape).
+    int autotune();
+    bool tuned = false;
+    // "nx x ny: SXxSY hH / SXxSY hH" per fused level (FP64 / FP32 shapes)
+    std::string shapes() const;
+  private:
     int smooth(size_t l, int sweeps, bool reverse);
     int smooth_f(size_t l, int sweeps, bool reverse);
     MgKind kind_ = kMAC;

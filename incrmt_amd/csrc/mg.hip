@@ -7,8 +7,10 @@
 // level's coefficients and writes z once. Reductions (norms, means) are
 // deterministic two-pass block reductions, so a solve is reproducible
 // launch to launch.
+#include <algorithm>
 #include <cstring>
 #include <cstdio>
+#include <type_traits>
 #include "mg.hh"
 
 namespace incrmt {
@@ -16,6 +18,7 @@ namespace incrmt {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+constexpr int INCRMT_ERR_ARG_SHAPE = 1;  // unknown smoother shape
 constexpr int kRedBlocks = 512;   // partial sums per reduction
 constexpr int kRedThreads = 256;
 
@@ -83,9 +86,7 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
 // loaded into LDS with coalesced row reads; phase p updates its colour on
 // the region shrunk by p points, so after the last phase the tile interior
 // holds exactly what the sequence of single-colour launches would give.
-constexpr int kSX = 64, kSY = 16;
-
-template <class T, int KIND, int H>
+template <class T, int KIND, int H, int kSX, int kSY>
 __global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k, T *z, const T *b, const T *c1,
                                                             const T *c2, int phases, int first_color, int dir) {
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
@@ -197,6 +198,57 @@ __global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     else if (!(i & 1)) v = T(0.5) * (e[q] + e[q + p]);
     else v = T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
     z[j * f.zp + i] += v;
+}
+
+// FEM transfers to and from a uniform coarse level that is not nested in
+// the fine one (fine node counts other than 2^k+1): tensor-product linear
+// interpolation by node position, and its transpose as the restriction.
+struct RsArgs {
+    const int *i0, *lo, *hi;
+    const double *t;
+};
+template <class T>
+__global__ void k_fem_restrict_rs(LevelArgs f, LevelArgs cl, const T *r, T *bc, T *zc, RsArgs x, RsArgs y) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= cl.nx || J >= cl.ny) return;
+    T s = 0;
+    for (int j = y.lo[J]; j <= y.hi[J]; j++) {
+        const T wy = y.i0[j] == J ? T(1 - y.t[j]) : (y.i0[j] == J - 1 ? T(y.t[j]) : T(0));
+        T sr = 0;
+        for (int i = x.lo[I]; i <= x.hi[I]; i++) {
+            const T wx = x.i0[i] == I ? T(1 - x.t[i]) : (x.i0[i] == I - 1 ? T(x.t[i]) : T(0));
+            sr += wx * r[j * f.zp + i];
+        }
+        s += wy * sr;
+    }
+    bc[J * cl.zp + I] = s;
+    zc[J * cl.zp + I] = 0;
+}
+template <class T>
+__global__ void k_fem_prolong_rs(LevelArgs f, LevelArgs cl, T *z, const T *e, RsArgs x, RsArgs y) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= f.nx || j >= f.ny) return;
+    const int I = x.i0[i], J = y.i0[j], p = cl.zp;
+    const T tx = T(x.t[i]), ty = T(y.t[j]);
+    const T lo = (T(1) - tx) * e[J * p + I] + tx * e[J * p + I + 1];
+    const T up = (T(1) - tx) * e[(J + 1) * p + I] + tx * e[(J + 1) * p + I + 1];
+    z[j * f.zp + i] += (T(1) - ty) * lo + ty * up;
+}
+// coarse cell coefficient: fine cell values averaged with their overlap
+__global__ void k_fem_coef_rs(LevelArgs f, LevelArgs c, const double *kf, double *kcc, int *cxlo, int *cxhi,
+                              int *cylo, int *cyhi, double hfx, double hcx, double hfy, double hcy) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= c.nx - 1 || J >= c.ny - 1) return;
+    double s = 0;
+    for (int j = cylo[J]; j <= cyhi[J]; j++) {
+        const double oy = fmin((j + 1) * hfy, (J + 1) * hcy) - fmax(j * hfy, J * hcy);
+        if (!(oy > 0)) continue;
+        for (int i = cxlo[I]; i <= cxhi[I]; i++) {
+            const double ox = fmin((i + 1) * hfx, (I + 1) * hcx) - fmax(i * hfx, I * hcx);
+            if (ox > 0) s += ox * oy * kf[j * f.cp + i];
+        }
+    }
+    kcc[J * c.cp + I] = s / (hcx * hcy);
 }
 
 // Coarsest level: one workgroup, mean-free right-hand side, colored
@@ -596,6 +648,7 @@ FemC fem_consts(double dx, double dy) {
 }
 
 LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky}; }
+RsArgs rs_args(const Resample &r) { return RsArgs{r.i0, r.lo, r.hi, r.t}; }
 
 }  // namespace
 
@@ -646,10 +699,52 @@ Multigrid::~Multigrid() {
     if (h_red) (void)hipHostFree(h_red);
 }
 
+int Multigrid::make_resample(int nf, double hf, int nc, Resample &r) {
+    // the nc nodes span the nf fine nodes' extent; when nf = 2nc-1 the levels
+    // are nested and these tables reduce to the usual 1/2 weights
+    const double L0 = (nf - 1) * hf, hc = L0 / (nc - 1);
+    std::vector<int> i0(nf), lo(nc, nf), hi(nc, -1), clo(nc, nf), chi(nc, -1);
+    std::vector<double> t(nf);
+    for (int i = 0; i < nf; i++) {
+        const double x = i * hf / hc;
+        int I = (int)x;
+        if (I > nc - 2) I = nc - 2;
+        i0[i] = I;
+        t[i] = x - I;
+        if (i == nf - 1) { i0[i] = nc - 2; t[i] = 1.0; }
+        for (int J = i0[i]; J <= i0[i] + 1; J++) {
+            lo[J] = std::min(lo[J], i);
+            hi[J] = std::max(hi[J], i);
+        }
+    }
+    for (int i = 0; i + 1 < nf; i++) {  // fine cell i = [i hf, (i+1) hf]
+        const int a = (int)(i * hf / hc), b = std::min((int)((i + 1) * hf / hc), nc - 2);
+        for (int J = std::max(a - 1, 0); J <= std::min(b + 1, nc - 2); J++) {
+            clo[J] = std::min(clo[J], i);
+            chi[J] = std::max(chi[J], i);
+        }
+    }
+    auto up = [&](auto &v, auto *&d) -> int {
+        using E = typename std::remove_reference<decltype(v[0])>::type;
+        void *q;
+        HIP_CHECK(hipMalloc(&q, v.size() * sizeof(E)));
+        allocs.push_back(q);
+        HIP_CHECK(hipMemcpy(q, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+        d = (decltype(d))q;
+        return 0;
+    };
+    int rc;
+    if ((rc = up(i0, r.i0)) || (rc = up(lo, r.lo)) || (rc = up(hi, r.hi)) || (rc = up(clo, r.clo)) ||
+        (rc = up(chi, r.chi)) || (rc = up(t, r.t)))
+        return rc;
+    r.nf = nf; r.nc = nc; r.hf = hf; r.hc = hc;
+    return 0;
+}
+
 int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s) {
     kind_ = kind; s_ = s; dx_ = dx; dy_ = dy;
     int nx = m, ny = n;
-    double kx = 1. / (dx * dx), ky = 1. / (dy * dy);
+    double kx = 1. / (dx * dx), ky = 1. / (dy * dy), hx = dx, hy = dy;
     while (true) {
         MgLevel l{};
         l.nx = nx; l.ny = ny;
@@ -683,12 +778,24 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
             if ((rc = dalloc(l.kc))) return rc;
             if ((rc = falloc(l.kcf))) return rc;
         }
+        l.hx = hx; l.hy = hy;
         L.push_back(l);
+        if (kind == kFEM && L.size() == 1 && nx >= 5 && ny >= 5 && (nx % 2 == 0 || ny % 2 == 0)) {
+            // node counts that vertex coarsening cannot halve: the next level
+            // is the largest uniform 2^k+1 grid on the same box
+            auto pow2p1 = [](int n) { int c = 2; while (2 * (c - 1) + 1 <= n - 1) c = 2 * (c - 1) + 1; return c; };
+            const int cx = nx % 2 ? (nx + 1) / 2 : pow2p1(nx), cy = ny % 2 ? (ny + 1) / 2 : pow2p1(ny);
+            if ((rc = make_resample(nx, hx, cx, rx_)) || (rc = make_resample(ny, hy, cy, ry_))) return rc;
+            resampled_ = true;
+            hx = rx_.hc; hy = ry_.hc;
+            nx = cx; ny = cy;
+            continue;
+        }
         bool more = kind == kMAC ? (nx % 2 == 0 && ny % 2 == 0 && nx >= 4 && ny >= 4)
                                  : (nx % 2 == 1 && ny % 2 == 1 && nx >= 5 && ny >= 5);
         if (!more) break;
         if (kind == kMAC) { nx /= 2; ny /= 2; kx *= 0.25; ky *= 0.25; }
-        else { nx = (nx + 1) / 2; ny = (ny + 1) / 2; }
+        else { nx = (nx + 1) / 2; ny = (ny + 1) / 2; hx *= 2; hy *= 2; }
     }
     void *q;
     HIP_CHECK(hipMalloc(&q, (3 * kRedBlocks + 8) * sizeof(double)));
@@ -696,8 +803,11 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
     d_red = (double *)q;
     HIP_CHECK(hipHostMalloc((void **)&h_red, 8 * sizeof(double), hipHostMallocDefault));
     tail_ = tail_start(L);
-    return set_irho_host(nullptr);
+    if (resampled_ && tail_ == 0) tail_ = 1;  // the tail's transfers assume nested levels
+    if (int rc = set_irho_host(nullptr)) return rc;
+    return autotune();
 }
+
 
 int Multigrid::coarsen_coefs() {
     for (size_t l = 0; l + 1 < L.size(); l++) {
@@ -706,6 +816,10 @@ int Multigrid::coarsen_coefs() {
         if (kind_ == kMAC) {
             k_mac_coef_coarsen<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.fw, f.fs,
                                                                                     c.fw, c.fs);
+        } else if (l == 0 && resampled_) {
+            k_fem_coef_rs<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc, rx_.clo,
+                                                                      rx_.chi, ry_.clo, ry_.chi, rx_.hf, rx_.hc,
+                                                                      ry_.hf, ry_.hc);
         } else {
             k_fem_coef_coarsen<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc);
         }
@@ -787,27 +901,112 @@ int Multigrid::collect_events() {
     return 0;
 }
 
-template <class T>
-static int launch_smooth(MgKind kind, const MgLevel &v, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1,
-                         const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch) {
-    *nlaunch = 0;
-    const dim3 grid((v.nx + kSX - 1) / kSX, (v.ny + kSY - 1) / kSY), block(kSX, kSY);
-    if (kind == kMAC) {
-        // red-black: up to two sweeps (four phases) per launch
-        for (int sw = 0; sw < sweeps; sw += 2) {
-            const int ph = sweeps - sw >= 2 ? 4 : 2;
-            k_smooth_fused<T, kMAC, 4><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, ph, reverse ? 1 : 0, reverse ? -1 : 1);
-            HIP_CHECK(hipGetLastError());
-            ++*nlaunch;
-        }
-    } else {
-        for (int sw = 0; sw < sweeps; sw++) {
-            k_smooth_fused<T, kFEM, 4><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, 4, reverse ? 3 : 0, reverse ? -1 : 1);
-            HIP_CHECK(hipGetLastError());
-            ++*nlaunch;
-        }
+// The fused smoother's launch shapes: a tile of SX x SY points per
+// workgroup and a halo of H points, i.e. H colour phases per launch (MAC:
+// H/2 red-black sweeps, FEM: H/4 four-colour sweeps). Every shape gives
+// bit-identical results; Multigrid::autotune picks one per level and
+// arithmetic by timing them on the level itself.
+struct SmoothShape {
+    int sx, sy, h;
+};
+constexpr SmoothShape kShapes[] = {{64, 16, 4}, {64, 8, 4}, {64, 4, 4}, {64, 16, 2}, {64, 8, 2},
+                                   {64, 4, 2},  {64, 8, 8},  {64, 4, 8}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+inline bool shape_valid(MgKind kind, const SmoothShape &v) { return kind == kMAC ? v.h <= 4 : v.h >= 4; }
+
+template <class T, int KIND, int H, int SX, int SY>
+static int launch_shape(const LevelArgs &a, FemC k, T *z, const T *b, const T *c1, const T *c2, int sweeps,
+                        bool reverse, hipStream_t s, int *nlaunch) {
+    const dim3 grid((a.nx + SX - 1) / SX, (a.ny + SY - 1) / SY), block(SX, SY);
+    const int ncol = KIND == kMAC ? 2 : 4, per = H / ncol;  // sweeps per launch
+    for (int sw = 0; sw < sweeps; sw += per) {
+        const int nsw = sweeps - sw < per ? sweeps - sw : per;
+        k_smooth_fused<T, KIND, H, SX, SY><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, nsw * ncol,
+                                                                  reverse ? ncol - 1 : 0, reverse ? -1 : 1);
+        HIP_CHECK(hipGetLastError());
+        ++*nlaunch;
     }
     return 0;
+}
+
+template <class T, int KIND>
+static int launch_kind(int shape, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1, const T *c2, int sweeps,
+                       bool reverse, hipStream_t s, int *nl) {
+    switch (shape) {
+    case 0: return launch_shape<T, KIND, 4, 64, 16>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
+    case 1: return launch_shape<T, KIND, 4, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
+    case 2: return launch_shape<T, KIND, 4, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
+    case 3: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 16>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    case 4: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    case 5: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    case 6: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    case 7: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    }
+    return INCRMT_ERR_ARG_SHAPE;
+}
+
+template <class T>
+static int launch_smooth(MgKind kind, int shape, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1,
+                         const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch) {
+    *nlaunch = 0;
+    if (kind == kMAC) return launch_kind<T, kMAC>(shape, a, k, z, b, c1, c2, sweeps, reverse, s, nlaunch);
+    return launch_kind<T, kFEM>(shape, a, k, z, b, c1, c2, sweeps, reverse, s, nlaunch);
+}
+
+int Multigrid::autotune() {
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int rc = 0;
+    for (size_t l = 0; l < L.size() && l < tail_ && !rc; l++) {
+        MgLevel &v = L[l];
+        if ((long long)v.nx * v.ny < 64 * 64) continue;
+        const LevelArgs a = args_of(v);
+        const FemC k = fem_consts(v.hx, v.hy);
+        for (int prec = 0; prec < 2 && !rc; prec++) {
+            float best = 1e30f;
+            int pick = 0;
+            for (int sh = 0; sh < kNumShapes && !rc; sh++) {
+                if (!shape_valid(kind_, kShapes[sh])) continue;
+                // scratch iterate: the level's residual array (always written
+                // before it is read); b and the coefficients are the level's
+                auto run = [&]() -> int {
+                    int nl;
+                    if (prec == 0)
+                        return launch_smooth<double>(kind_, sh, a, k, v.r, v.b, kind_ == kMAC ? v.fw : v.kc, v.fs, nu1,
+                                                     false, s_, &nl);
+                    return launch_smooth<float>(kind_, sh, a, k, v.rf, v.bf, kind_ == kMAC ? v.fwf : v.kcf, v.fsf, nu1,
+                                                false, s_, &nl);
+                };
+                if ((rc = run())) break;
+                HIP_CHECK(hipEventRecord(e0, s_));
+                for (int r = 0; r < 4 && !rc; r++) rc = run();
+                HIP_CHECK(hipEventRecord(e1, s_));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; pick = sh; }
+            }
+            (prec == 0 ? v.shape64 : v.shape32) = pick;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tuned = true;
+    return rc;
+}
+
+std::string Multigrid::shapes() const {
+    std::string s;
+    char buf[96];
+    for (size_t l = 0; l < L.size() && l < tail_; l++) {
+        const SmoothShape &a = kShapes[L[l].shape64], &b = kShapes[L[l].shape32];
+        snprintf(buf, sizeof buf, "%s%dx%d:%dx%dh%d/%dx%dh%d", s.empty() ? "" : " ", L[l].nx, L[l].ny, a.sx, a.sy, a.h,
+                 b.sx, b.sy, b.h);
+        s += buf;
+    }
+    return s;
 }
 
 int Multigrid::smooth(size_t l, int sweeps, bool reverse) {
@@ -815,8 +1014,8 @@ int Multigrid::smooth(size_t l, int sweeps, bool reverse) {
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
-    int rc = launch_smooth<double>(kind_, v, args_of(v), fem_consts(dx_, dy_), v.z, v.b, kind_ == kMAC ? v.fw : v.kc,
-                                   v.fs, sweeps, reverse, s_, &nl);
+    int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), v.z, v.b,
+                                   kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl);
     if (rc) return rc;
     if (timed && time_smoother) {
         if (record_end()) return 6;
@@ -831,8 +1030,8 @@ int Multigrid::smooth_f(size_t l, int sweeps, bool reverse) {
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
-    int rc = launch_smooth<float>(kind_, v, args_of(v), fem_consts(dx_, dy_), v.zf, v.bf, kind_ == kMAC ? v.fwf : v.kcf,
-                                  v.fsf, sweeps, reverse, s_, &nl);
+    int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), v.zf, v.bf,
+                                  kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl);
     if (rc) return rc;
     if (timed && time_smoother) {
         if (record_end()) return 6;
@@ -871,7 +1070,7 @@ int Multigrid::smooth_fine(int sweeps, bool fp32) { return fp32 ? smooth_f(0, sw
 
 int Multigrid::vcycle(size_t l) {
     const MgLevel &f = L[l];
-    const FemC k = fem_consts(dx_, dy_);
+    const FemC k = fem_consts(f.hx, f.hy);
     if (l == tail_) {
         size_t sh;
         const TailArgs A = tail_args<double>(L, l, kind_, k, nu1, nu2, false, &sh);
@@ -893,11 +1092,18 @@ int Multigrid::vcycle(size_t l) {
     } else {
         k_fem_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.z, f.b, f.kc, f.r);
         KCHECK();
-        k_fem_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b, c.z);
+        if (l == 0 && resampled_)
+            k_fem_restrict_rs<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b,
+                                                                               c.z, rs_args(rx_), rs_args(ry_));
+        else
+            k_fem_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b, c.z);
     }
     KCHECK();
     if ((rc = vcycle(l + 1))) return rc;
     if (kind_ == kMAC) k_mac_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
+    else if (l == 0 && resampled_)
+        k_fem_prolong_rs<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z,
+                                                                          rs_args(rx_), rs_args(ry_));
     else k_fem_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
     KCHECK();
     return smooth(l, nu2, true);
@@ -905,7 +1111,7 @@ int Multigrid::vcycle(size_t l) {
 
 int Multigrid::vcycle_f(size_t l) {
     const MgLevel &f = L[l];
-    const FemC k = fem_consts(dx_, dy_);
+    const FemC k = fem_consts(f.hx, f.hy);
     if (l == tail_) {
         size_t sh;
         const TailArgs A = tail_args<float>(L, l, kind_, k, nu1, nu2, true, &sh);
@@ -927,11 +1133,18 @@ int Multigrid::vcycle_f(size_t l) {
     } else {
         k_fem_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.zf, f.bf, f.kcf, f.rf);
         KCHECK();
-        k_fem_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf, c.zf);
+        if (l == 0 && resampled_)
+            k_fem_restrict_rs<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf,
+                                                                               c.zf, rs_args(rx_), rs_args(ry_));
+        else
+            k_fem_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf, c.zf);
     }
     KCHECK();
     if ((rc = vcycle_f(l + 1))) return rc;
     if (kind_ == kMAC) k_mac_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
+    else if (l == 0 && resampled_)
+        k_fem_prolong_rs<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf,
+                                                                          rs_args(rx_), rs_args(ry_));
     else k_fem_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
     KCHECK();
     return smooth_f(l, nu2, true);

@@ -73,6 +73,19 @@ def fem_matrix(m, n, dx, dy, irho=None):
     return sp.csr_matrix((vals, (rows, cols)), shape=(m * n, m * n))
 
 
+def fem_matrix_sparse(m, n, dx, dy):
+    """fem_matrix for constant density, assembled with array operations
+    (for grids of a million nodes)."""
+    Ke = bilinear_element(dx, dy)
+    ci, cj = np.meshgrid(np.arange(m - 1), np.arange(n - 1))
+    ci, cj = ci.ravel(), cj.ravel()
+    nodes = [cj * m + ci, cj * m + ci + 1, (cj + 1) * m + ci, (cj + 1) * m + ci + 1]
+    rows = np.concatenate([nodes[p] for p in range(4) for q in range(4)])
+    cols = np.concatenate([nodes[q] for p in range(4) for q in range(4)])
+    vals = np.concatenate([np.full(ci.size, Ke[p, q]) for p in range(4) for q in range(4)])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(m * n, m * n))
+
+
 def neumann_solve(A, b):
     """Mean-free solution of a singular pure-Neumann system (one node pinned,
     then the mean removed)."""

@@ -82,34 +82,21 @@ def test_not_converged_is_reported():
     assert e.value.code == ia.ERR_NOT_CONVERGED
 
 
-def fourier_rhs(nodes, seed=20260101, modes=32):
-    """Divergence of a seeded smooth velocity (32 Fourier modes, amplitudes
-    ~ |k|^-2) on the unit square, by the FEM source formula
-    (fluid_2d.cc:1303-1308) with dt = 1; mean projected out."""
-    m = nodes - 1
-    h = 1.0 / (nodes - 1)
-    rng = np.random.default_rng(seed)
-    ks = rng.choice(256, size=modes, replace=False)
-    k1, k2 = ks % 16 + 1, ks // 16 + 1
-    kn = np.sqrt(k1**2 + k2**2)
-    a = rng.standard_normal(modes) / kn**2
-    bq = rng.standard_normal(modes) / kn**2
-    x = (np.arange(m) + 0.5) * h
-    X, Y = np.meshgrid(x, x)
-    u = np.zeros((m, m))
-    v = np.zeros((m, m))
-    for q in range(modes):
-        u += a[q] * np.sin(np.pi * k1[q] * X) * np.sin(np.pi * k2[q] * Y)
-        v += bq[q] * np.sin(np.pi * k1[q] * X) * np.sin(np.pi * k2[q] * Y)
-    c0 = np.zeros((m + 2, m + 2))
-    c1 = np.zeros((m + 2, m + 2))
-    c0[1:-1, 1:-1] = u
-    c1[1:-1, 1:-1] = v
-    hx = hy = 0.5 * h
-    b = (hx * (c0[:-1, :-1] + c0[1:, :-1] - c0[:-1, 1:] - c0[1:, 1:])
-         + hy * (c1[:-1, :-1] - c1[1:, :-1] + c1[:-1, 1:] - c1[1:, 1:]))
+def fourier_rhs(nodes, seed=20260101):
+    """The standalone FEM configuration's right-hand side (incrmt_amd.workloads):
+    the FEM source (fluid_2d.cc:1303-1308, dt = 1) of a seeded smooth velocity
+    of 32 Fourier modes with amplitudes ~ |k|^-2; mean projected out."""
+    from incrmt_amd.workloads import fourier_fem_rhs
+    b, h = fourier_fem_rhs(nodes, seed)
     b = b.ravel()
     return b - b.mean()
+
+
+def _fem_residual(z, b, n, h):
+    # independent check of ||b - A z|| / ||b|| with a sparse assembly of the
+    # bilinear-FEM operator (the definition tests/operators.py assembles)
+    A = ops.fem_matrix_sparse(n, n, h, h)
+    return np.linalg.norm(b - A @ z) / np.linalg.norm(b)
 
 
 @pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
@@ -122,3 +109,35 @@ def test_fem_1024_fourier_projection(precision):
     z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=100, precision=precision)
     assert rr <= 1e-10 and cyc < 40
     assert np.isfinite(z).all()
+
+
+@pytest.mark.parametrize("m,n", [(34, 34), (40, 28), (64, 33), (20, 64)])
+@pytest.mark.parametrize("varying", [False, True])
+def test_fem_even_nodes_matches_direct(m, n, varying):
+    # node counts vertex coarsening cannot halve: the first coarse level is a
+    # uniform resampling of the box (non-nested transfers); the solution is
+    # still the fine system's to the requested tolerance
+    s, A = _fem(m, n, varying, seed=3)
+    b = np.random.default_rng(9).standard_normal(m * n)
+    b -= b.mean()
+    for precision in (ia.PREC_FP64, ia.PREC_MIXED):
+        s.z[:] = 0
+        z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=200, precision=precision)
+        assert rr <= 1e-10 and cyc < 60, (precision, cyc, rr)
+        zd = ops.neumann_solve(A, b)
+        assert np.linalg.norm((z - z.mean()) - zd) <= 1e-8 * np.linalg.norm(zd)
+
+
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
+def test_fem_1024_nodes_config(precision):
+    # the standalone FEM configuration: 1024 x 1024 nodes on the unit square
+    # (h = 1/1023), seeded Fourier right-hand side, zero start, relative
+    # residual 1e-10, FP64 and FP32-smoothing refinement
+    n = 1024
+    h = 1.0 / (n - 1)
+    s = ia.MgSolver(ia.MG_FEM, n, n, h, h)
+    b = fourier_rhs(n)
+    z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=100, precision=precision)
+    print(f"1024^2 nodes precision={precision}: {cyc} V-cycles, rel. residual {rr:.3e}", flush=True)
+    assert rr <= 1e-10 and cyc < 40
+    assert _fem_residual(z, b, n, h) <= 2e-10
