@@ -78,6 +78,39 @@ def cpu_baseline(m, steps):
                        f"projections to relative residual 1e-8")
 
 
+def smoother_probe(m, sweeps):
+    """Fine-level smoother sweeps of the two projection systems at the bench
+    size (MAC: m x m cells, FEM: (m+1)^2 nodes, variable density of the
+    rotor's 3:1 ratio as a smooth blob), timed with HIP events on the
+    library stream, plus one device copy of known size (512 MiB read and
+    written) for calibrating memory counters. For hardware counter runs."""
+    import numpy as np
+    import incrmt_amd as ia
+    out = {}
+    h = 2.0 / m
+    x = (np.arange(m) + 0.5) * h - 1
+    rho = 1 + 2 * np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / 0.2)
+    irho_mac = np.zeros(2 * m * m + 2 * m)
+    irho_mac[0:2 * m * m:2] = 1 / rho.ravel()
+    irho_mac[1:2 * m * m:2] = 1 / rho.ravel()
+    for name, kind, n, irho in (("mac", ia.MG_MAC, m, irho_mac),
+                                ("fem", ia.MG_FEM, m + 1, np.ones((m + 1) * (m + 1) + m + 1))):
+        s = ia.MgSolver(kind, n, n, h, h)
+        s.set_irho(irho)
+        for prec, tag in ((ia.PREC_FP64, "fp64"), (ia.PREC_MIXED, "fp32")):
+            ms, nl = s.time_smoother(sweeps, prec)
+            per_point = (40.0 if kind == ia.MG_MAC else 32.0) * (0.5 if tag == "fp32" else 1.0)
+            sweeps_per_launch = sweeps / nl
+            gbs = per_point * n * n * sweeps_per_launch / (ms * 1e-3) / 1e9
+            out[f"{name}_{tag}"] = dict(us_per_launch=round(ms * 1e3, 2), launches=nl,
+                                       sweeps_per_launch=sweeps_per_launch, algorithmic_GBs=round(gbs, 1))
+        s.close()
+    nbytes = 512 * 1024 * 1024
+    ms = ia.copy_probe(nbytes, 4)
+    out["copy"] = dict(bytes_each_way=nbytes, ms=round(ms, 4), GBs_read_plus_write=round(2 * nbytes / (ms * 1e-3) / 1e9, 1))
+    return out
+
+
 def sig(v):
     """Floats to 5 significant digits (residuals of 1e-9 stay visible)."""
     return float(f"{v:.5g}") if isinstance(v, float) else v
@@ -91,6 +124,8 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
+                    help="only run SWEEPS fine-level smoother sweeps of each system (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU sharding is a design "
@@ -98,6 +133,9 @@ def main():
         sys.exit(2)
     import incrmt_amd as ia
     m = a.size
+    if a.smoother_probe:
+        print(json.dumps(smoother_probe(m, a.smoother_probe)))
+        return
     fp64 = gpu_run(m, a.steps, a.warmup, ia.PREC_FP64, timing=True)
     mixed = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED)
     best = fp64 if fp64["cell_steps_per_s"] >= mixed["cell_steps_per_s"] else mixed

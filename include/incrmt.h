@@ -54,6 +54,10 @@ typedef struct incrmt_stats {
 const char *incrmt_last_error(void);
 /* Device name and HBM size of device 0, for reports. */
 int incrmt_device_info(char *name, int len, double *hbm_gib);
+/* No reference counterpart: `reps` device-to-device copies of `bytes` bytes
+ * timed with HIP events (a copy of known size, for calibrating memory
+ * counters and for the achievable-bandwidth figure). */
+int incrmt_copy_probe(long long bytes, int reps, double *ms_per_copy);
 
 /* ---- fluid_2d ------------------------------------------------------------
  * fluid_2d::fluid_2d(m,n,sim_type&,visc,rhof,tw,pinw,ntrace,fflags,dir)
@@ -94,6 +98,13 @@ int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double
 int incrmt_fluid_set_time(incrmt_fluid *f, double t);
 int incrmt_fluid_sync(incrmt_fluid *f);
 int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
+/* fluid_2d::write_files(sn) (fluid_2d_io.cc:60-102): the fields selected by
+ * the reference's fflags bits (1 u, 2 v, 4 spd, 8 p, 16 w, 32 rho, 64 X,
+ * 128 Y, 256 phi, 512 solid fields on the solids' bounding box, 1024 whole
+ * grid, 2048 ds/dp, 4096 div, 8192 J) as "<dir>/<name>.<sn>" in the
+ * reference's float32 Gnuplot matrix format, the tracers as "trace.<sn>",
+ * and the objects' output (a track.dat record; the file is rewritten whole). */
+int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned int fflags);
 /* No reference counterpart: the fused-smoother launch shape the autotuner
  * picked for each multigrid level ("nx x ny: SXxSY hH/SXxSY hH", FP64/FP32
  * sweeps), for reports. */

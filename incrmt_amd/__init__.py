@@ -53,6 +53,7 @@ class Stats(ctypes.Structure):
 _PROTOS = [
     ("incrmt_last_error", ctypes.c_char_p, []),
     ("incrmt_device_info", _I, [ctypes.c_char_p, _I, _dp]),
+    ("incrmt_copy_probe", _I, [ctypes.c_longlong, _I, _dp]),
     ("incrmt_fluid_create", _I, [_I, _I, _D, _D, _D, _D, _I, _I, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_V)]),
     ("incrmt_fluid_add_spin", _I, [_V, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("incrmt_fluid_add_circle", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
@@ -72,6 +73,7 @@ _PROTOS = [
     ("incrmt_fluid_destroy", None, [_V]),
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
+    ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_get_track_state", _I, [_V, _I, _dp]),
@@ -121,6 +123,13 @@ def _d(a):
 def _i(a):
     assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_ip)
+
+
+def copy_probe(nbytes, reps=10):
+    """Milliseconds per device-to-device copy of nbytes (HIP events)."""
+    ms = _D(0)
+    _check(lib().incrmt_copy_probe(nbytes, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 def device_info():
@@ -222,6 +231,11 @@ class Fluid2D:
 
     def time_smoothers(self, enable=True):
         _check(lib().incrmt_fluid_time_smoothers(self._h, int(enable)))
+
+    def write_frame(self, directory, sn, fflags=8 | 16 | 64 | 128 | 256 | 512):
+        """fluid_2d::write_files(sn): the fflags-selected fields as float32
+        Gnuplot matrices in `directory` (ftest's default flags)."""
+        _check(lib().incrmt_fluid_write_frame(self._h, str(directory).encode(), sn, fflags))
 
     def describe(self):
         """The autotuned fused-smoother shapes per multigrid level."""

@@ -1,0 +1,107 @@
+"""Runs of the benchmark configurations other than the headline step
+(bench.py measures that one), printing one JSON line each.
+
+    python -m incrmt_amd.configs fem        standalone FEM projection, 1024^2 nodes
+    python -m incrmt_amd.configs multibody  2048^2 cells, 64 circles, mixed vs FP64
+    python -m incrmt_amd.configs ensemble   256 rotor instances of uneven size
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+import incrmt_amd as ia
+from incrmt_amd import workloads as wl
+
+
+def run_fem(nodes=1024, tol=1e-10, repeats=3):
+    """Seeded Fourier right-hand side (workloads.fourier_fem_rhs), zero start,
+    relative residual tol; FP64 V-cycles vs FP32 V-cycles in FP64 iterative
+    refinement. Times the whole solve (host copies excluded)."""
+    b, h = wl.fourier_fem_rhs(nodes)
+    b = b.ravel() - b.mean()
+    out = dict(config="standalone FEM projection", nodes=[nodes, nodes], rel_tol=tol)
+    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
+        s = ia.MgSolver(ia.MG_FEM, nodes, nodes, h, h)
+        best = None
+        for _ in range(repeats):
+            s.z[:] = 0
+            t0 = time.perf_counter()
+            z, cyc, rr = s.solve(b, rel_tol=tol, max_cycles=100, precision=prec)
+            dt = time.perf_counter() - t0
+            best = dt if best is None or dt < best else best
+        out[name] = dict(v_cycles=cyc, rel_residual=rr, seconds=best, ms_per_v_cycle=1e3 * best / cyc)
+        s.close()
+    return out
+
+
+def run_multibody(m=2048, count=64, steps=200, every=1):
+    """The mixed-precision multibody run against an FP64 run of the same case,
+    every `every` steps: relative divergence of both projections and the
+    relative L2 distance of the mixed run's velocity from the FP64 run's."""
+    a = ia.multibody(m, count, precision=ia.PREC_MIXED)
+    b = ia.multibody(m, count, precision=ia.PREC_FP64)
+    inner = (slice(2, -2), slice(2, -2))
+    rows = []
+    ta = tb = 0.0
+    for k in range(1, steps + 1):
+        t0 = time.perf_counter()
+        a.step_forward(a.dt_reg)
+        a.sync()
+        t1 = time.perf_counter()
+        b.step_forward(b.dt_reg)
+        b.sync()
+        t2 = time.perf_counter()
+        ta += t1 - t0
+        tb += t2 - t1
+        if k % every == 0 or k == steps:
+            sa, sb = a.stats(), b.stats()
+            ua, ub, va, vb = a.get("u")[inner], b.get("u")[inner], a.get("v")[inner], b.get("v")[inner]
+            err = math.sqrt((np.sum((ua - ub) ** 2) + np.sum((va - vb) ** 2)) / (np.sum(ub ** 2) + np.sum(vb ** 2)))
+            rows.append(dict(step=k, mac_rel_div=sa["mac_rel_res"], fem_rel_div=sa["fem_rel_res"],
+                             fp64_mac_rel_div=sb["mac_rel_res"], fp64_fem_rel_div=sb["fem_rel_res"],
+                             rel_l2_vs_fp64=err, mac_cycles=sa["mac_cycles"], fem_cycles=sa["fem_cycles"]))
+            print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    out = dict(config="multibody", cells=[m, m], solids=count, steps=steps,
+               mixed_ms_per_step=1e3 * ta / steps, fp64_ms_per_step=1e3 * tb / steps,
+               mixed_cell_steps_per_s=m * m * steps / ta, fp64_cell_steps_per_s=m * m * steps / tb,
+               max_rel_l2_vs_fp64=max(r["rel_l2_vs_fp64"] for r in rows),
+               max_rel_div=max(max(r["mac_rel_div"], r["fem_rel_div"]) for r in rows), per_step=rows)
+    a.close()
+    b.close()
+    return out
+
+
+def run_ensemble(count=256, steps=10, threads=16, precision=ia.PREC_FP64):
+    from incrmt_amd.ensemble import Ensemble
+    specs = wl.rotor_ensemble(count)
+    e = Ensemble(specs, rel_tol=1e-8, precision=precision, threads=threads)
+    e.steps(1)
+    r = e.throughput(steps)
+    r.update(config="batched ensemble", threads=threads, precision="fp64" if precision == ia.PREC_FP64 else "mixed")
+    e.close()
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["fem", "multibody", "ensemble"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--every", type=int, default=1)
+    ap.add_argument("--threads", type=int, default=16)
+    a = ap.parse_args()
+    if a.which == "fem":
+        r = run_fem()
+    elif a.which == "multibody":
+        r = run_multibody(steps=a.steps or 200, every=a.every)
+    else:
+        r = run_ensemble(steps=a.steps or 10, threads=a.threads)
+    r["device"] = ia.device_info()[0]
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

@@ -488,11 +488,43 @@ int incrmt_device_info(char *name, int len, double *hbm_gib) {
     hipDeviceProp_t p;
     HIP_CHECK(hipGetDeviceProperties(&p, 0));
     if (name && len > 0) {
-        strncpy(name, p.name, len - 1);
-        name[len - 1] = 0;
+        // the marketing name, with the ISA appended (some drivers leave the name empty)
+        snprintf(name, len, "%s%s%s (%d CUs)", p.name, p.name[0] ? " " : "", p.gcnArchName, p.multiProcessorCount);
     }
     if (hbm_gib) *hbm_gib = p.totalGlobalMem / 1073741824.0;
     return 0;
+}
+
+int incrmt_copy_probe(long long bytes, int reps, double *ms_per_copy) {
+    if (bytes <= 0 || reps <= 0 || !ms_per_copy) return fail(INCRMT_ERR_ARG, "bad copy probe");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto run = [&]() -> int {
+        HIP_CHECK(hipMalloc(&a, bytes));
+        HIP_CHECK(hipMalloc(&b, bytes));
+        HIP_CHECK(hipStreamCreate(&s));
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipMemsetAsync(a, 0, bytes, s));
+        HIP_CHECK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) HIP_CHECK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / reps;
+        return 0;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
@@ -641,6 +673,201 @@ int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable) {
     f->e.mac.smoother_ms = f->e.fem.smoother_ms = 0;
     f->e.mac.smoother_launches = f->e.fem.smoother_launches = 0;
     f->e.mac.smoother_bytes = f->e.fem.smoother_bytes = 0;
+    return 0;
+}
+
+// ---- frame output: fluid_2d::write_files (fluid_2d_io.cc:60-179) ----------
+// The fields are copied to the host once per frame and written in the
+// reference's Gnuplot matrix-binary format (float32; first row: column count
+// then the x coordinates; every further row: y then the values).
+namespace {
+
+struct HostFrame {
+    int m, n, ml, nl;
+    double ax, ay, dx, dy, xsp, ysp;
+    std::vector<double> u, v, p, rho, c0, c1;
+    std::vector<std::vector<double>> phi, X, Y;
+    std::vector<std::vector<int>> st;
+    double eps;
+};
+
+int write_matrix(const std::string &path, const HostFrame &h, int mode, int li, int ui, int lj, int uj) {
+    const int si = ui - li, sj = uj - lj;
+    if (si <= 0 || sj <= 0) return 0;
+    FILE *fp = fopen(path.c_str(), "wb");
+    if (!fp) return fail(INCRMT_ERR_ARG, ("cannot open " + path).c_str());
+    const bool cen = !((1 << mode) & (8 | 16 | 512 | 2048));  // fluid_2d_io.cc:130-133
+    const double lx = h.ax + h.dx * (li - (cen ? 1.5 : 2.)), ly = h.ay + h.dy * (lj - (cen ? 1.5 : 2.));
+    std::vector<float> buf(si + 1);
+    buf[0] = (float)si;
+    for (int i = 0; i < si; i++) buf[i + 1] = (float)(lx + i * h.dx);
+    fwrite(buf.data(), sizeof(float), si + 1, fp);
+    const int ml = h.ml;
+    auto minobj = [&](int ij) {  // min_phi_obj (fluid_2d.cc:1502-1508)
+        size_t k = 0;
+        double mp = h.phi[0][ij];
+        for (size_t q = 1; q < h.phi.size(); q++)
+            if (h.phi[q][ij] < mp) { mp = h.phi[q][ij]; k = q; }
+        return k;
+    };
+    auto corner_phi = [&](size_t k, int ij) {  // fluid_2d.hh:266-268
+        const double *p = h.phi[k].data() + ij;
+        return p[-ml - 1] + p[-ml] + p[-1] + *p;
+    };
+    for (int j = 0; j < sj; j++) {
+        buf[0] = (float)(ly + j * h.dy);
+        const int ijr = li + ml * (lj + j);
+        for (int i = 0; i < si; i++) {
+            const int ij = ijr + i;
+            double val = 0;
+            switch (mode) {
+            case 0: val = h.u[ij]; break;
+            case 1: val = h.v[ij]; break;
+            case 2: val = sqrt(h.u[ij] * h.u[ij] + h.v[ij] * h.v[ij]); break;
+            case 3: val = h.p[ij]; break;
+            case 4:  // vorticity at the lower-left corner (fluid_2d.hh:252-255)
+                val = 0.5 * (h.xsp * (h.v[ij] - h.v[ij - 1] + h.v[ij - ml] - h.v[ij - ml - 1]) -
+                             h.ysp * (h.u[ij] + h.u[ij - 1] - h.u[ij - ml] - h.u[ij - ml - 1]));
+                break;
+            case 5: val = h.rho[ij]; break;
+            case 6: val = h.X[minobj(ij)][ij]; break;
+            case 7: val = h.Y[minobj(ij)][ij]; break;
+            case 8: val = h.phi[minobj(ij)][ij]; break;
+            case 9: {  // vol_change (fluid_2d_io.cc:303-316)
+                size_t k = 0;
+                double mp = corner_phi(0, ij);
+                for (size_t q = 1; q < h.phi.size(); q++)
+                    if (corner_phi(q, ij) < mp) { mp = corner_phi(q, ij); k = q; }
+                if (corner_phi(k, ij) > 0) { val = 0; break; }
+                const double *X = h.X[k].data() + ij, *Y = h.Y[k].data() + ij;
+                const double Xx = 0.5 * h.xsp * (X[0] + X[-ml] - X[-1] - X[-ml - 1]),
+                             Xy = 0.5 * h.ysp * (X[0] + X[-1] - X[-ml] - X[-ml - 1]),
+                             Yx = 0.5 * h.xsp * (Y[0] + Y[-ml] - Y[-1] - Y[-ml - 1]),
+                             Yy = 0.5 * h.ysp * (Y[0] + Y[-1] - Y[-ml] - Y[-ml - 1]);
+                val = 1. / (Xx * Yy - Xy * Yx) - 1;
+                break;
+            }
+            case 10: val = h.c0[ij]; break;
+            case 11: val = h.c1[ij]; break;
+            case 12:  // divergence at the lower-left corner (fluid_2d.hh:259-262)
+                val = 0.5 * (h.xsp * (h.u[ij] - h.u[ij - 1] + h.u[ij - ml] - h.u[ij - ml - 1]) +
+                             h.ysp * (h.v[ij] + h.v[ij - 1] - h.v[ij - ml] - h.v[ij - ml - 1]));
+                break;
+            }
+            buf[i + 1] = (float)val;
+        }
+        fwrite(buf.data(), sizeof(float), si + 1, fp);
+    }
+    fclose(fp);
+    return 0;
+}
+
+}  // namespace
+
+int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned int fflags) {
+    if (int rc = check_ready(f)) return rc;
+    if (!dir) return fail(INCRMT_ERR_ARG, "null directory");
+    FluidEngine &e = f->e;
+    const Grid &g = e.g;
+    HostFrame h;
+    h.m = g.m; h.n = g.n; h.ml = g.ml; h.nl = g.nl;
+    h.ax = g.ax; h.ay = g.ay; h.dx = g.dx; h.dy = g.dy; h.xsp = g.xsp; h.ysp = g.ysp;
+    const size_t N = (size_t)g.ml * g.nl;
+    auto get = [&](const char *nm, int k, std::vector<double> &out) -> int {
+        out.assign(N, 0.0);
+        return incrmt_fluid_download(f, nm, k, out.data());
+    };
+    int rc;
+    if ((rc = get("u", 0, h.u)) || (rc = get("v", 0, h.v)) || (rc = get("p", 0, h.p)) || (rc = get("rho", 0, h.rho)) ||
+        (rc = get("c0", 0, h.c0)) || (rc = get("c1", 0, h.c1)))
+        return rc;
+    const int nobj = (int)e.obj.size();
+    h.phi.resize(nobj); h.X.resize(nobj); h.Y.resize(nobj); h.st.resize(nobj);
+    for (int k = 0; k < nobj; k++) {
+        std::vector<double> s;
+        if ((rc = get("phi", k, h.phi[k])) || (rc = get("X", k, h.X[k])) || (rc = get("Y", k, h.Y[k])) ||
+            (rc = get("s", k, s)))
+            return rc;
+        h.st[k].resize(N);
+        for (size_t q = 0; q < N; q++) h.st[k][q] = (int)s[q];
+    }
+    const std::string base = std::string(dir) + "/";
+    auto out = [&](const char *prefix, int mode) -> int {
+        const std::string path = base + prefix + "." + std::to_string(sn);
+        if (fflags & 1024) {  // whole grid (fluid_2d_io.cc:109-111)
+            const int lo = mode == 4 ? 1 : 0;
+            return write_matrix(path, h, mode, lo, g.ml, lo, g.nl);
+        }
+        const int dis = (1 << mode) & (8 | 16 | 512 | 2048) ? 1 : 2;
+        return write_matrix(path, h, mode, 2, g.ml - dis, 2, g.nl - dis);
+    };
+    if ((fflags & 1) && (rc = out("u", 0))) return rc;
+    if ((fflags & 2) && (rc = out("v", 1))) return rc;
+    if ((fflags & 4) && (rc = out("spd", 2))) return rc;
+    if ((fflags & 8) && (rc = out("p", 3))) return rc;
+    if ((fflags & 16) && (rc = out("w", 4))) return rc;
+    if (!e.const_rho && (fflags & 32) && (rc = out("rho", 5))) return rc;
+    if ((fflags & 4096) && (rc = out("div", 12))) return rc;
+    if (nobj > 0 && (fflags & 8640)) {
+        if (fflags & 512) {  // bounding box of the solids' inner band, padded by 2 (fluid_2d_io.cc:75-83)
+            int li = g.ml, ui = 0, lj = g.nl, uj = 0;
+            for (int k = 0; k < nobj; k++)
+                for (int J = 0; J < g.nl; J++)
+                    for (int I = 0; I < g.ml; I++)
+                        if (h.st[k][J * g.ml + I] < 4) {
+                            if (I < li) li = I;
+                            if (I >= ui) ui = I + 1;
+                            if (J < lj) lj = J;
+                            if (J >= uj) uj = J + 1;
+                        }
+            li -= 2; ui += 2; lj -= 2; uj += 2;
+            if (li < 0) li = 0;
+            if (ui > g.ml) ui = g.ml;
+            if (lj < 0) lj = 0;
+            if (uj > g.nl) uj = g.nl;
+            const std::string sfx = "." + std::to_string(sn);
+            if ((fflags & 64) && (rc = write_matrix(base + "X" + sfx, h, 6, li, ui, lj, uj))) return rc;
+            if ((fflags & 128) && (rc = write_matrix(base + "Y" + sfx, h, 7, li, ui, lj, uj))) return rc;
+            if ((fflags & 256) && (rc = write_matrix(base + "phi" + sfx, h, 8, li, ui, lj, uj))) return rc;
+        } else {
+            if ((fflags & 64) && (rc = out("X", 6))) return rc;
+            if ((fflags & 128) && (rc = out("Y", 7))) return rc;
+            if ((fflags & 256) && (rc = out("phi", 8))) return rc;
+        }
+        if ((fflags & 8192) && (rc = out("J", 9))) return rc;
+    }
+    if (fflags & 2048) {
+        if ((rc = out("ds", 10)) || (rc = out("dp", 11))) return rc;
+    }
+    // tracer positions (fluid_2d_io.cc:24-55): x, y as float32 pairs
+    if (e.tr.ntrace > 0) {
+        std::vector<double> tm(4 * (size_t)e.tr.ntrace);
+        if ((rc = incrmt_fluid_get_tracers(f, tm.data()))) return rc;
+        std::vector<float> b(2 * (size_t)e.tr.ntrace);
+        for (int q = 0; q < e.tr.ntrace; q++) { b[2 * q] = (float)tm[4 * q]; b[2 * q + 1] = (float)tm[4 * q + 1]; }
+        FILE *fp = fopen((base + "trace." + std::to_string(sn)).c_str(), "wb");
+        if (!fp) return fail(INCRMT_ERR_ARG, "cannot open the tracer file");
+        fwrite(b.data(), sizeof(float), b.size(), fp);
+        fclose(fp);
+    }
+    // the objects' own output: a track.dat record, the file rewritten whole
+    if (e.tr.ntrack > 0) {
+        if ((rc = incrmt_fluid_frame_output(f))) return rc;
+        for (size_t k = 0, t = 0; k < e.obj.size(); k++) {
+            if (e.obj[k].c.kind != kSpin) continue;
+            int nrec = 0;
+            if ((rc = incrmt_fluid_track(f, (int)k, nullptr, 0, &nrec))) return rc;
+            const int have = nrec < e.tr.rec_cap ? nrec : e.tr.rec_cap;
+            std::vector<double> rec(4 * (size_t)have);
+            if ((rc = incrmt_fluid_track(f, (int)k, rec.data(), have, &nrec))) return rc;
+            const std::string nm = t == 0 ? "track.dat" : "track" + std::to_string(t) + ".dat";
+            FILE *fp = fopen((base + nm).c_str(), "wb");
+            if (!fp) return fail(INCRMT_ERR_ARG, "cannot open track.dat");
+            fwrite(rec.data(), sizeof(double), rec.size(), fp);
+            fclose(fp);
+            t++;
+        }
+    }
     return 0;
 }
 
