@@ -78,6 +78,14 @@ int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, doub
 /* Projection stopping rule: relative residual ||b-Az||/||b|| (the
  * reference's TGMG rule is not available; see mgs_common.hh:37). */
 int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int precision);
+/* Starting iterate of the two projections: the previous step's solution
+ * (INCRMT_GUESS_PREVIOUS, the reference's warm start, mgs_common.hh:48-50;
+ * the default) or the linear extrapolation in time of the last two
+ * (INCRMT_GUESS_EXTRAPOLATE: 2 z_n - z_(n-1); no reference counterpart).
+ * Either way each solve stops at the same relative residual. */
+#define INCRMT_GUESS_PREVIOUS 0
+#define INCRMT_GUESS_EXTRAPOLATE 1
+int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
 /* fluid_2d::step_forward(dt) (fluid_2d.cc:298-369). */
 int incrmt_fluid_step(incrmt_fluid *f, double dt);
 /* k calls of step_forward(dt), the inner loop of fluid_2d::solve (fluid_2d.cc:274). */
@@ -148,6 +156,11 @@ int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
  * its input, to ||b-Az|| <= rel_tol ||b|| (b's mean is projected out first). */
 int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
                     int *cycles, double *rel_res);
+/* No reference counterpart (for checking the smoother): `sweeps` forward
+ * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused
+ * launch shape `shape` (0..7; see incrmt_fluid_describe) or one launch per
+ * colour when shape < 0. */
+int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape);
 /* y = A x with the solver's operator (mgs_*::mul_a + a_cc, full row). */
 int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y);
 /* Time `sweeps` fine-level smoother sweeps on the resident problem with HIP

@@ -27,6 +27,7 @@ OK = 0
 ERR_ARG, ERR_NOT_CONVERGED, ERR_NEWTON, ERR_BAND, ERR_SINGULAR_FIT, ERR_HIP, ERR_UNSUPPORTED = range(1, 8)
 MG_MAC, MG_FEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+GUESS_PREVIOUS, GUESS_EXTRAPOLATE = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -73,6 +74,7 @@ _PROTOS = [
     ("incrmt_fluid_destroy", None, [_V]),
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
+    ("incrmt_fluid_set_guess", _I, [_V, _I]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
@@ -84,6 +86,7 @@ _PROTOS = [
     ("incrmt_mg_set_irho", _I, [_V, _dp]),
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
     ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
+    ("incrmt_mg_smooth", _I, [_V, _dp, _dp, _I, _I]),
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
@@ -176,6 +179,11 @@ class Fluid2D:
 
     def set_solver(self, rel_tol=1e-8, max_cycles=100, precision=PREC_FP64):
         _check(lib().incrmt_fluid_set_solver(self._h, rel_tol, max_cycles, precision))
+
+    def set_guess(self, mode):
+        """GUESS_PREVIOUS (the reference's warm start) or GUESS_EXTRAPOLATE
+        (2 z_n - z_(n-1)) as the projections' starting iterate."""
+        _check(lib().incrmt_fluid_set_guess(self._h, mode))
 
     def step_forward(self, dt):
         _check(lib().incrmt_fluid_step(self._h, dt))
@@ -283,7 +291,7 @@ class Fluid2D:
         return out[: 4 * cap].reshape(cap, 4)
 
 
-def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64):
+def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64, guess=GUESS_PREVIOUS):
     """`ftest simple-spin m` (ftest.cc:116-143): G=48, visc=1e-2, rhos=3, the
     three-pronged rotor obj_simple_spin(0.8,0.5,0.125,0.2,pi,omega) in a
     no-slip [-1,1]^2 box, initialized with ev_mult 0.8, dt_pad 0.4,
@@ -291,6 +299,7 @@ def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64):
     f = Fluid2D(m, m, visc=1e-2, rhof=1.0, tw=2.5, pinw=2.5)
     f.add_spin(0.8, 0.5, 0.125, 0.2, math.pi, omega, G=48.0, rhos=3.0, ev_trans_mult=1.0)
     f.set_solver(rel_tol=rel_tol, precision=precision)
+    f.set_guess(guess)
     f.initialize(0.8, 0.4, 0.8)
     return f
 
@@ -338,6 +347,14 @@ class MgSolver:
         y = np.zeros_like(xx)
         _check(lib().incrmt_mg_apply(self._h, _d(xx), _d(y)))
         return y
+
+    def smooth(self, b, z, sweeps, shape):
+        """`sweeps` Gauss-Seidel sweeps from z with fused launch shape
+        `shape`, or one launch per colour (shape -1). Returns the new z."""
+        bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        zz = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
+        _check(lib().incrmt_mg_smooth(self._h, _d(bb), _d(zz), sweeps, shape))
+        return zz
 
     def time_smoother(self, sweeps, precision=PREC_FP64):
         ms, n = _D(0), _I(0)

@@ -489,7 +489,8 @@ int incrmt_device_info(char *name, int len, double *hbm_gib) {
     HIP_CHECK(hipGetDeviceProperties(&p, 0));
     if (name && len > 0) {
         // the marketing name, with the ISA appended (some drivers leave the name empty)
-        snprintf(name, len, "%s%s%s (%d CUs)", p.name, p.name[0] ? " " : "", p.gcnArchName, p.multiProcessorCount);
+        snprintf(name, len, "%s%s%s (%d CUs, LDS %zu KiB per workgroup)", p.name, p.name[0] ? " " : "", p.gcnArchName,
+                 p.multiProcessorCount, (size_t)p.sharedMemPerBlock / 1024);
     }
     if (hbm_gib) *hbm_gib = p.totalGlobalMem / 1073741824.0;
     return 0;
@@ -871,6 +872,13 @@ int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned 
     return 0;
 }
 
+int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE) return fail(INCRMT_ERR_ARG, "bad guess mode");
+    f->e.mac.extrapolate = f->e.fem.extrapolate = mode == INCRMT_GUESS_EXTRAPOLATE;
+    return 0;
+}
+
 int incrmt_fluid_set_ntrace(incrmt_fluid *f, int ntrace) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (f->e.initialized) return fail(INCRMT_ERR_ARG, "tracers are placed by initialize: set them before");
@@ -1034,6 +1042,15 @@ int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, in
     if (cycles) *cycles = st.cycles;
     if (rel_res) *rel_res = st.rel_res;
     return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;
+}
+
+int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape) {
+    if (!h || !b || !z || sweeps < 1) return fail(INCRMT_ERR_ARG, "bad smooth arguments");
+    MgLevel &l = h->mg.fine();
+    if (int rc = mg_copy(h, l.b, b, nullptr)) return rc;
+    if (int rc = mg_copy(h, l.z, z, nullptr)) return rc;
+    if (int rc = h->mg.smooth_check(sweeps, shape)) return rc == 1 ? fail(INCRMT_ERR_ARG, "unknown shape") : rc;
+    return mg_copy(h, l.z, nullptr, z);
 }
 
 int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y) {

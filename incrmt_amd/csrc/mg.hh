@@ -25,6 +25,8 @@ struct MgLevel {
     int nx, ny, pitch;
     double *z, *b, *r;
     float *zf, *bf, *rf;  // single-precision copies for the mixed cycle
+    double *z2;           // second iterate buffer: the fused smoother reads one and writes the other
+    float *zf2;
     // MAC: face inverse densities, zero on walls: fw(i,j) face between cells
     // (i-1,j),(i,j), i in [0,nx]; fs(i,j) face between (i,j-1),(i,j).
     // FEM: kc(i,j) = 1/rho of cell (i,j), i in [0,nx-2], zero ring around.
@@ -73,6 +75,10 @@ class Multigrid {
     int solve_mixed(double tol, int max_cycles, MgStats *st);
     // Launch `sweeps` fine-level smoothing sweeps (for timing the smoother).
     int smooth_fine(int sweeps, bool fp32);
+    // `sweeps` forward FP64 sweeps on level 0 with fused launch shape
+    // `shape`, or with one launch per colour when shape < 0 (for checking
+    // that every form gives the same sweep)
+    int smooth_check(int sweeps, int shape);
     // y = A x on level 0; x and y are padded level arrays (pointers at (0,0)).
     int apply(const double *x, double *y);
     // Time `sweeps` fine-level sweeps with HIP events: mean ms per launch.
@@ -84,6 +90,8 @@ class Multigrid {
     size_t tail_level() const { return tail_; }
     // timing of fine-level smoothing launches (HIP events on the stream)
     bool time_smoother = false;
+    // start each solve from 2 z_n - z_(n-1) (set for the step's projections)
+    bool extrapolate = false;
     double smoother_ms = 0;
     long smoother_launches = 0;
     double smoother_bytes = 0;  // algorithmic bytes of the timed sweeps
@@ -98,6 +106,10 @@ class Multigrid {
     template <class Cycle>
     int solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle);
     int predict_ = 0;                // V-cycles run before the first residual check
+    double *zprev_ = nullptr;        // the solution before the last one (extrapolation)
+    bool f32_stale_ = true;          // coefficient copies for the FP32 cycle out of date
+    int make_f32_coefs();
+    long nsolved_ = 0;
     bool resampled_ = false;         // level 1 is a uniform resampling of level 0
     Resample rx_{}, ry_{};
     int make_resample(int nf, double hf, int nc, Resample &r);
@@ -109,8 +121,9 @@ class Multigrid {
     // "nx x ny: SXxSY hH / SXxSY hH" per fused level (FP64 / FP32 shapes)
     std::string shapes() const;
   private:
-    int smooth(size_t l, int sweeps, bool reverse);
-    int smooth_f(size_t l, int sweeps, bool reverse);
+    // correct: add level l+1's correction (prolonged) as the sweep loads z
+    int smooth(size_t l, int sweeps, bool reverse, bool correct = false);
+    int smooth_f(size_t l, int sweeps, bool reverse, bool correct = false);
     MgKind kind_ = kMAC;
     std::vector<MgLevel> L;
     size_t tail_ = 0;  // first level handled by the single-workgroup tail cycle

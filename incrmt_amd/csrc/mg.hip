@@ -82,13 +82,41 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
 }
 
 // Fused colored Gauss-Seidel: a whole multi-colour sweep (MAC: two sweeps)
-// in one launch. A 64x16 tile plus a halo of one point per colour phase is
-// loaded into LDS with coalesced row reads; phase p updates its colour on
-// the region shrunk by p points, so after the last phase the tile interior
-// holds exactly what the sequence of single-colour launches would give.
+// in one launch. A tile plus a halo of one point per colour phase is loaded
+// into LDS with coalesced row reads; phase p updates its colour on the
+// region shrunk by p points, so after the last phase the tile interior
+// holds exactly what the sequence of single-colour launches would give. The
+// launch reads z and writes zout (a second buffer): no workgroup can see a
+// neighbour's result while loading its halo.
+// Coarse-grid correction of a nested level, as the prolongation kernels
+// below add it (same expressions, so fusing it into a smoother's load is
+// bit-identical to a separate prolongation launch).
+struct ProArgs {
+    const void *e;   // coarse correction at (0,0), or null for none
+    int ezp, cnx, cny;
+};
+template <class T, int KIND>
+__device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
+    const T *e = (const T *)pa.e;
+    const int I = i >> 1, J = j >> 1, p = pa.ezp;
+    if (KIND == kMAC) {
+        int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+        if (I2 < 0 || I2 >= pa.cnx) I2 = I;
+        if (J2 < 0 || J2 >= pa.cny) J2 = J;
+        return T(0.5625) * e[J * p + I] + T(0.1875) * e[J * p + I2] + T(0.1875) * e[J2 * p + I]
+               + T(0.0625) * e[J2 * p + I2];
+    }
+    const int q = J * p + I;
+    if (!(i & 1) && !(j & 1)) return e[q];
+    if (!(j & 1)) return T(0.5) * (e[q] + e[q + 1]);
+    if (!(i & 1)) return T(0.5) * (e[q] + e[q + p]);
+    return T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
+}
+
 template <class T, int KIND, int H, int kSX, int kSY>
-__global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k, T *z, const T *b, const T *c1,
-                                                            const T *c2, int phases, int first_color, int dir) {
+__global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k, const T *z, T *zout, const T *b,
+                                                            const T *c1, const T *c2, int phases, int first_color,
+                                                            int dir, ProArgs pa) {
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
     __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
     const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
@@ -99,24 +127,34 @@ __global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k,
         const int li = e % PW, lj = e / PW, i = i0 + li, j = j0 + lj;
         const bool inpad = i >= -1 && i <= a.nx && j >= -1 && j <= a.ny;
         const int q = j * a.zp + i, c = j * a.cp + i;
-        sz[e] = inpad ? z[q] : T(0);
+        T zv = inpad ? z[q] : T(0);
+        if (pa.e && i >= 0 && i < a.nx && j >= 0 && j < a.ny) zv += prolong_at<T, KIND>(pa, i, j);
+        sz[e] = zv;
         sb[e] = inpad ? b[q] : T(0);
         s1[e] = inpad ? c1[c] : T(0);
         if (KIND == kMAC) s2[e] = inpad ? c2[c] : T(0);
     }
     __syncthreads();
     const int ncol = KIND == kMAC ? 2 : 4;
+    // i0, j0, RW and RH are even, so a point's colour follows from its
+    // region coordinates; each phase enumerates only its own colour
+    constexpr int NPH = KIND == kMAC ? (RW / 2) * RH : (RW / 2) * (RH / 2);
     for (int p = 1; p <= phases; p++) {
         // colour of phase p: forward or backward through the colours
         const int step = p - 1;
         const int col = dir > 0 ? (first_color + step) % ncol : ((first_color - step) % ncol + ncol) % ncol;
-        for (int e = tid; e < RW * RH; e += nt) {
-            const int li = e % RW, lj = e / RW;
+        for (int e = tid; e < NPH; e += nt) {
+            int li, lj;
+            if (KIND == kMAC) {
+                lj = e / (RW / 2);
+                li = 2 * (e % (RW / 2)) + ((col + lj) & 1);  // (i + j) & 1 == col
+            } else {
+                lj = 2 * (e / (RW / 2)) + (col >> 1);         // (i & 1) + 2 (j & 1) == col
+                li = 2 * (e % (RW / 2)) + (col & 1);
+            }
             if (li < p || li >= RW - p || lj < p || lj >= RH - p) continue;
             const int i = i0 + li, j = j0 + lj;
             if (i < 0 || i >= a.nx || j < 0 || j >= a.ny) continue;
-            const bool mine = KIND == kMAC ? (((i + j) & 1) == col) : ((i & 1) + 2 * (j & 1) == col);
-            if (!mine) continue;
             const int q = lj * PW + li;
             T diag, off;
             if (KIND == kMAC) off = mac_offdiag<T>(sz, s1, s2, q, q, PW, PW, T(a.kx), T(a.ky), diag);
@@ -126,7 +164,7 @@ __global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k,
         __syncthreads();
     }
     const int li = threadIdx.x + H, lj = threadIdx.y + H, i = i0 + li, j = j0 + lj;
-    if (i < a.nx && j < a.ny) z[j * a.zp + i] = sz[lj * PW + li];
+    if (i < a.nx && j < a.ny) zout[j * a.zp + i] = sz[lj * PW + li];
 }
 
 // ------------------------------------------------- residual and transfers
@@ -198,6 +236,29 @@ __global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     else if (!(i & 1)) v = T(0.5) * (e[q] + e[q + p]);
     else v = T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
     z[j * f.zp + i] += v;
+}
+
+// FEM residual and full-weighting restriction in one pass: each coarse
+// node forms the nine fine residuals it gathers (the same expressions as
+// k_fem_residual, summed as k_fem_restrict sums them). Residuals outside
+// the level are the zero ghost ring's.
+template <class T>
+__global__ void k_fem_restrict_fused(LevelArgs f, LevelArgs cl, FemC k, const T *z, const T *b, const T *kc, T *bc,
+                                     T *zc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= cl.nx || J >= cl.ny) return;
+    auto res = [&](int i, int j) -> T {
+        if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return T(0);
+        const int q = j * f.zp + i, c = j * f.cp + i;
+        T diag;
+        const T off = fem_offdiag<T>(z, kc, q, c, f.zp, f.cp, k, diag);
+        return b[q] - off - diag * z[q];
+    };
+    const int i = 2 * I, j = 2 * J;
+    const T s = res(i, j) + T(0.5) * (res(i - 1, j) + res(i + 1, j) + res(i, j - 1) + res(i, j + 1))
+                + T(0.25) * (res(i - 1, j - 1) + res(i + 1, j - 1) + res(i - 1, j + 1) + res(i + 1, j + 1));
+    bc[J * cl.zp + I] = s;
+    zc[J * cl.zp + I] = 0;
 }
 
 // FEM transfers to and from a uniform coarse level that is not nested in
@@ -628,6 +689,20 @@ __global__ void k_residual_f32(int kind, LevelArgs a, FemC k, const double *z, c
     bf[q] = float(b[q] - off - diag * z[q]);
     zf[q] = 0.f;
 }
+// Starting iterate from the two previous solutions: z <- 2 z - zp, zp <- z.
+__global__ void k_extrapolate(LevelArgs a, double *z, double *zp) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i;
+    const double t = z[q];
+    z[q] = 2 * t - zp[q];
+    zp[q] = t;
+}
+__global__ void k_copy_level(LevelArgs a, const double *z, double *zp) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    zp[j * a.zp + i] = z[j * a.zp + i];
+}
 __global__ void k_add_f32(LevelArgs a, double *z, const float *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
@@ -769,8 +844,8 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
             return 0;
         };
         int rc;
-        if ((rc = dalloc(l.z)) || (rc = dalloc(l.b)) || (rc = dalloc(l.r))) return rc;
-        if ((rc = falloc(l.zf)) || (rc = falloc(l.bf)) || (rc = falloc(l.rf))) return rc;
+        if ((rc = dalloc(l.z)) || (rc = dalloc(l.z2)) || (rc = dalloc(l.b)) || (rc = dalloc(l.r))) return rc;
+        if ((rc = falloc(l.zf)) || (rc = falloc(l.zf2)) || (rc = falloc(l.bf)) || (rc = falloc(l.rf))) return rc;
         if (kind == kMAC) {
             if ((rc = dalloc(l.fw)) || (rc = dalloc(l.fs))) return rc;
             if ((rc = falloc(l.fwf)) || (rc = falloc(l.fsf))) return rc;
@@ -825,7 +900,12 @@ int Multigrid::coarsen_coefs() {
         }
         KCHECK();
     }
-    // single-precision copies for the mixed cycle
+    f32_stale_ = true;  // the single-precision copies are made when a mixed solve needs them
+    return 0;
+}
+
+int Multigrid::make_f32_coefs() {
+    f32_stale_ = false;
     for (auto &l : L) {
         const long long len = (long long)(l.ny + 2) * l.pitch;
         const int nb = (int)((len + 255) / 256);
@@ -916,14 +996,18 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 inline bool shape_valid(MgKind kind, const SmoothShape &v) { return kind == kMAC ? v.h <= 4 : v.h >= 4; }
 
 template <class T, int KIND, int H, int SX, int SY>
-static int launch_shape(const LevelArgs &a, FemC k, T *z, const T *b, const T *c1, const T *c2, int sweeps,
-                        bool reverse, hipStream_t s, int *nlaunch) {
+static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2, int sweeps,
+                        bool reverse, hipStream_t s, int *nlaunch, ProArgs pa) {
     const dim3 grid((a.nx + SX - 1) / SX, (a.ny + SY - 1) / SY), block(SX, SY);
     const int ncol = KIND == kMAC ? 2 : 4, per = H / ncol;  // sweeps per launch
     for (int sw = 0; sw < sweeps; sw += per) {
         const int nsw = sweeps - sw < per ? sweeps - sw : per;
-        k_smooth_fused<T, KIND, H, SX, SY><<<grid, block, 0, s>>>(a, k, z, b, c1, c2, nsw * ncol,
-                                                                  reverse ? ncol - 1 : 0, reverse ? -1 : 1);
+        const ProArgs p = sw == 0 ? pa : ProArgs{nullptr, 0, 0, 0};  // the correction enters once
+        k_smooth_fused<T, KIND, H, SX, SY><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
+                                                                  reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
+        T *t = *z;  // the result is the level's iterate from now on
+        *z = *z2;
+        *z2 = t;
         HIP_CHECK(hipGetLastError());
         ++*nlaunch;
     }
@@ -931,30 +1015,34 @@ static int launch_shape(const LevelArgs &a, FemC k, T *z, const T *b, const T *c
 }
 
 template <class T, int KIND>
-static int launch_kind(int shape, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1, const T *c2, int sweeps,
-                       bool reverse, hipStream_t s, int *nl) {
+static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2,
+                       int sweeps, bool reverse, hipStream_t s, int *nl, ProArgs pa) {
     switch (shape) {
-    case 0: return launch_shape<T, KIND, 4, 64, 16>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
-    case 1: return launch_shape<T, KIND, 4, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
-    case 2: return launch_shape<T, KIND, 4, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl);
-    case 3: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 16>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
-    case 4: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
-    case 5: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
-    case 6: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 8>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
-    case 7: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 4>(a, k, z, b, c1, c2, sweeps, reverse, s, nl); break;
+    case 0: return launch_shape<T, KIND, 4, 64, 16>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
+    case 1: return launch_shape<T, KIND, 4, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
+    case 2: return launch_shape<T, KIND, 4, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
+    case 3: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 16>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
+    case 4: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
+    case 5: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
+    case 6: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
+    case 7: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
     }
     return INCRMT_ERR_ARG_SHAPE;
 }
 
+// z, z2: the level's iterate and its second buffer, swapped after every launch
 template <class T>
-static int launch_smooth(MgKind kind, int shape, const LevelArgs &a, FemC k, T *z, const T *b, const T *c1,
-                         const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch) {
+static int launch_smooth(MgKind kind, int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1,
+                         const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch,
+                         ProArgs pa = ProArgs{nullptr, 0, 0, 0}) {
     *nlaunch = 0;
-    if (kind == kMAC) return launch_kind<T, kMAC>(shape, a, k, z, b, c1, c2, sweeps, reverse, s, nlaunch);
-    return launch_kind<T, kFEM>(shape, a, k, z, b, c1, c2, sweeps, reverse, s, nlaunch);
+    if (kind == kMAC) return launch_kind<T, kMAC>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa);
+    return launch_kind<T, kFEM>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa);
 }
 
 int Multigrid::autotune() {
+    if (f32_stale_)
+        if (int rc = make_f32_coefs()) return rc;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -973,11 +1061,13 @@ int Multigrid::autotune() {
                 // before it is read); b and the coefficients are the level's
                 auto run = [&]() -> int {
                     int nl;
+                    double *d0 = v.r, *d1 = v.z2;
+                    float *f0 = v.rf, *f1 = v.zf2;
                     if (prec == 0)
-                        return launch_smooth<double>(kind_, sh, a, k, v.r, v.b, kind_ == kMAC ? v.fw : v.kc, v.fs, nu1,
-                                                     false, s_, &nl);
-                    return launch_smooth<float>(kind_, sh, a, k, v.rf, v.bf, kind_ == kMAC ? v.fwf : v.kcf, v.fsf, nu1,
-                                                false, s_, &nl);
+                        return launch_smooth<double>(kind_, sh, a, k, &d0, &d1, v.b, kind_ == kMAC ? v.fw : v.kc, v.fs,
+                                                     nu1, false, s_, &nl);
+                    return launch_smooth<float>(kind_, sh, a, k, &f0, &f1, v.bf, kind_ == kMAC ? v.fwf : v.kcf, v.fsf,
+                                                nu1, false, s_, &nl);
                 };
                 if ((rc = run())) break;
                 HIP_CHECK(hipEventRecord(e0, s_));
@@ -1009,33 +1099,66 @@ std::string Multigrid::shapes() const {
     return s;
 }
 
-int Multigrid::smooth(size_t l, int sweeps, bool reverse) {
+int Multigrid::smooth_check(int sweeps, int shape) {
+    MgLevel &v = L[0];
+    const LevelArgs a = args_of(v);
+    const FemC k = fem_consts(v.hx, v.hy);
+    if (shape < 0) {  // one launch per colour: the reference form of the sweep
+        const int ncol = kind_ == kMAC ? 2 : 4;
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int c = 0; c < ncol; c++) {
+                if (kind_ == kMAC) {
+                    const dim3 grid((v.nx / 2 + 1 + BX - 1) / BX, (v.ny + BY - 1) / BY);
+                    k_mac_smooth<double><<<grid, dim3(BX, BY), 0, s_>>>(a, v.z, v.b, v.fw, v.fs, c);
+                } else {
+                    const dim3 grid(((v.nx + 1) / 2 + BX - 1) / BX, ((v.ny + 1) / 2 + BY - 1) / BY);
+                    k_fem_smooth<double><<<grid, dim3(BX, BY), 0, s_>>>(a, k, v.z, v.b, v.kc, c);
+                }
+                KCHECK();
+            }
+        return 0;
+    }
+    if (shape >= kNumShapes || !shape_valid(kind_, kShapes[shape])) return INCRMT_ERR_ARG_SHAPE;
+    int nl;
+    return launch_smooth<double>(kind_, shape, a, k, &v.z, &v.z2, v.b, kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps,
+                                 false, s_, &nl);
+}
+
+int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
     const MgLevel &v = L[l];
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
-    int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), v.z, v.b,
-                                   kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl);
+    ProArgs pa{nullptr, 0, 0, 0};
+    if (correct) pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny};
+    MgLevel &w = L[l];
+    int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), &w.z, &w.z2, v.b,
+                                   kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl, pa);
     if (rc) return rc;
     if (timed && time_smoother) {
         if (record_end()) return 6;
-        smoother_bytes += (kind_ == kMAC ? 40.0 : 32.0) * v.nx * v.ny * sweeps;
+        // z, b, coefficients read and z written per sweep; a fused coarse
+        // correction adds the coarse level's read (a quarter point per point)
+        smoother_bytes += (kind_ == kMAC ? 40.0 : 32.0) * v.nx * v.ny * sweeps + (correct ? 2.0 * v.nx * v.ny : 0.0);
         smoother_launches += nl;
     }
     return 0;
 }
 
-int Multigrid::smooth_f(size_t l, int sweeps, bool reverse) {
+int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
     const MgLevel &v = L[l];
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
-    int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), v.zf, v.bf,
-                                  kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl);
+    ProArgs pa{nullptr, 0, 0, 0};
+    if (correct) pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny};
+    MgLevel &w = L[l];
+    int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), &w.zf, &w.zf2, v.bf,
+                                  kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl, pa);
     if (rc) return rc;
     if (timed && time_smoother) {
         if (record_end()) return 6;
-        smoother_bytes += (kind_ == kMAC ? 20.0 : 16.0) * v.nx * v.ny * sweeps;
+        smoother_bytes += (kind_ == kMAC ? 20.0 : 16.0) * v.nx * v.ny * sweeps + (correct ? 1.0 * v.nx * v.ny : 0.0);
         smoother_launches += nl;
     }
     return 0;
@@ -1049,6 +1172,8 @@ int Multigrid::apply(const double *x, double *y) {
 }
 
 int Multigrid::time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches) {
+    if (fp32 && f32_stale_)
+        if (int rc = make_f32_coefs()) return rc;
     const bool was = time_smoother;
     const double ms0 = smoother_ms, b0 = smoother_bytes;
     const long n0 = smoother_launches;
@@ -1090,23 +1215,33 @@ int Multigrid::vcycle(size_t l) {
         k_mac_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, f.b, f.fw,
                                                                             f.fs, c.b, c.z);
     } else {
-        k_fem_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.z, f.b, f.kc, f.r);
-        KCHECK();
-        if (l == 0 && resampled_)
+        if (l == 0 && resampled_) {
+            k_fem_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.z, f.b, f.kc, f.r);
+            KCHECK();
             k_fem_restrict_rs<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b,
                                                                                c.z, rs_args(rx_), rs_args(ry_));
-        else
-            k_fem_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b, c.z);
+        } else {
+            k_fem_restrict_fused<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), k, f.z, f.b,
+                                                                                  f.kc, c.b, c.z);
+        }
     }
     KCHECK();
     if ((rc = vcycle(l + 1))) return rc;
-    if (kind_ == kMAC) k_mac_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
-    else if (l == 0 && resampled_)
+    // the coarse-grid correction enters in the post-smoother's load, except
+    // across a resampled (non-nested) level pair
+    if (l == 0 && resampled_) {
         k_fem_prolong_rs<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z,
                                                                           rs_args(rx_), rs_args(ry_));
-    else k_fem_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
-    KCHECK();
-    return smooth(l, nu2, true);
+        KCHECK();
+        return smooth(l, nu2, true, false);
+    }
+    if (nu2 == 0) {
+        if (kind_ == kMAC) k_mac_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
+        else k_fem_prolong<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z);
+        KCHECK();
+        return 0;
+    }
+    return smooth(l, nu2, true, true);
 }
 
 int Multigrid::vcycle_f(size_t l) {
@@ -1131,23 +1266,33 @@ int Multigrid::vcycle_f(size_t l) {
         k_mac_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, f.bf, f.fwf,
                                                                            f.fsf, c.bf, c.zf);
     } else {
-        k_fem_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.zf, f.bf, f.kcf, f.rf);
-        KCHECK();
-        if (l == 0 && resampled_)
+        if (l == 0 && resampled_) {
+            k_fem_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.zf, f.bf, f.kcf, f.rf);
+            KCHECK();
             k_fem_restrict_rs<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf,
                                                                                c.zf, rs_args(rx_), rs_args(ry_));
-        else
-            k_fem_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.rf, c.bf, c.zf);
+        } else {
+            k_fem_restrict_fused<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), k, f.zf, f.bf,
+                                                                                  f.kcf, c.bf, c.zf);
+        }
     }
     KCHECK();
     if ((rc = vcycle_f(l + 1))) return rc;
-    if (kind_ == kMAC) k_mac_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
-    else if (l == 0 && resampled_)
+    // the coarse-grid correction enters in the post-smoother's load, except
+    // across a resampled (non-nested) level pair
+    if (l == 0 && resampled_) {
         k_fem_prolong_rs<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf,
                                                                           rs_args(rx_), rs_args(ry_));
-    else k_fem_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
-    KCHECK();
-    return smooth_f(l, nu2, true);
+        KCHECK();
+        return smooth_f(l, nu2, true, false);
+    }
+    if (nu2 == 0) {
+        if (kind_ == kMAC) k_mac_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
+        else k_fem_prolong<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf);
+        KCHECK();
+        return 0;
+    }
+    return smooth_f(l, nu2, true, true);
 }
 
 int Multigrid::reduce_dev() {
@@ -1179,6 +1324,25 @@ template <class Cycle>
 int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) {
     const MgLevel &f = L[0];
     int rc;
+    if (extrapolate) {
+        // start from the linear extrapolation in time of the last two
+        // solutions (the system is solved once per step): the start is
+        // O(dt^2) from the new solution instead of O(dt)
+        if (!zprev_) {
+            void *q;
+            const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+            HIP_CHECK(hipMemset(q, 0, len * sizeof(double)));
+            allocs.push_back(q);
+            zprev_ = (double *)q + f.pitch + 1;
+        }
+        if (nsolved_ >= 2)
+            k_extrapolate<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_);
+        else
+            k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_);
+        KCHECK();
+    }
+    nsolved_++;
     if ((rc = reduce_dev())) return rc;
     k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
     KCHECK();
@@ -1213,6 +1377,8 @@ int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
 
 int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
     const MgLevel &f = L[0];
+    if (f32_stale_)
+        if (int rc = make_f32_coefs()) return rc;
     const FemC k = fem_consts(dx_, dy_);
     // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
     return solve_loop(tol, max_cycles, st, [&]() -> int {

@@ -52,8 +52,13 @@ def test_frames_match_oracle_fields(tmp_path):
     assert np.abs(PHI - o.get("phi")[lj:uj, li:ui]).max() <= 0.05 * h
     _, _, X = fr.read_matrix(os.path.join(tmp_path, "X.3"))
     assert np.abs(X - o.get("X")[lj:uj, li:ui]).max() <= 1e-6
-    _, _, D = fr.read_matrix(os.path.join(tmp_path, "div.3"))
-    assert D.shape == (m + 1, m + 1)
+    # div is evaluated at lower-left corners but written over the m x m
+    # interior with cell-centre coordinates: the reference's output() keys
+    # its region and coordinates on the mode bits 3, 4, 9, 11 only
+    # (fluid_2d_io.cc:113, 130)
+    x, _, D = fr.read_matrix(os.path.join(tmp_path, "div.3"))
+    assert D.shape == (m, m)
+    assert np.array_equal(x, np.float32(-1 + h * (np.arange(m) + 0.5)))
 
 
 def test_rigid_rotation_vorticity(tmp_path):
@@ -69,4 +74,4 @@ def test_rigid_rotation_vorticity(tmp_path):
     _, _, W = fr.read_matrix(os.path.join(tmp_path, "w.0"))
     assert np.allclose(W, 2 * om, rtol=1e-6)
     _, _, D = fr.read_matrix(os.path.join(tmp_path, "div.0"))
-    assert np.abs(D).max() <= 1e-6
+    assert D.shape == (m, m) and np.abs(D).max() <= 1e-6

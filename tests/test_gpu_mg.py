@@ -141,3 +141,21 @@ def test_fem_1024_nodes_config(precision):
     print(f"1024^2 nodes precision={precision}: {cyc} V-cycles, rel. residual {rr:.3e}", flush=True)
     assert rr <= 1e-10 and cyc < 40
     assert _fem_residual(z, b, n, h) <= 2e-10
+
+
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_fused_smoother_shapes_bit_identical(kind, sweeps):
+    # every fused launch shape the autotuner may pick gives exactly the
+    # sweep of one launch per colour (the reference form of the sweep)
+    m, n = (200, 136) if kind == "mac" else (201, 137)
+    s, _ = (_mac if kind == "mac" else _fem)(m, n, True, seed=5)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal(m * n)
+    z = rng.standard_normal(m * n)
+    ref = s.smooth(b, z, sweeps, -1)
+    shapes = range(6) if kind == "mac" else (0, 1, 2, 6, 7)
+    for sh in shapes:
+        assert np.array_equal(s.smooth(b, z, sweeps, sh), ref), sh
+    with pytest.raises(ia.IncrmtError):
+        s.smooth(b, z, sweeps, 6 if kind == "mac" else 3)

@@ -147,3 +147,22 @@ def test_solver_failure_is_reported():
     with pytest.raises(ia.IncrmtError) as e:
         g.step_forward(g.dt_reg)
     assert e.value.code == ia.ERR_NOT_CONVERGED
+
+
+def test_extrapolated_guess_same_solution():
+    # starting the projections from 2 z_n - z_(n-1) changes the iterates,
+    # not the answer: per-step parity with the oracle as above, fewer cycles
+    m = 64
+    spec = ol.simple_spin_spec(m)
+    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    g.set_guess(ia.GUESS_EXTRAPOLATE)
+    base = gpu_from_spec(spec)
+    l, adt = g.timestep_select(4 * math.pi / 240)
+    dx = 2.0 / m
+    for k in range(1, 16):
+        g.step_forward(adt)
+        o.step(adt)
+        base.step_forward(adt)
+        compare(g, o, m, dx, f"step {k}")
+    sg, sb = g.stats(), base.stats()
+    assert sg["total_mac_cycles"] + sg["total_fem_cycles"] < sb["total_mac_cycles"] + sb["total_fem_cycles"]
