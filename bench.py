@@ -57,6 +57,11 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0):
         out["mac_cycles_timed"] = st["total_mac_cycles"] - s1["total_mac_cycles"]
         out["fem_cycles_timed"] = st["total_fem_cycles"] - s1["total_fem_cycles"]
         f.time_smoothers(False)
+        # the same kernels back to back (no event between launches), for
+        # comparison with a profiler's per-dispatch durations
+        mac_ms, mac_n = f.time_fine_smoother(ia.MG_MAC, 40)
+        fem_ms, fem_n = f.time_fine_smoother(ia.MG_FEM, 40)
+        out["back_to_back_us_per_launch"] = {"mac": mac_ms * 1e3, "fem": fem_ms * 1e3}
     f.close()
     return out
 
@@ -111,6 +116,32 @@ def smoother_probe(m, sweeps):
     return out
 
 
+def measured_traffic(shapes, mac_cycles, fem_cycles, nu=2):
+    """HBM bytes per fine-level smoother launch in the timed pass, from the
+    calibrated FETCH_SIZE/WRITE_SIZE counts per launch shape measured on the
+    same fine levels (profiles/smoother_fine_1024_traffic.json), weighted by
+    the launches each system made (pre- and post-smoothing per V-cycle).
+    None when a shape in use was not measured."""
+    import re
+    path = os.path.join(ROOT, "profiles", "smoother_fine_1024_traffic.json")
+    if not os.path.exists(path):
+        return None
+    tab = json.load(open(path))["per_dispatch"]
+    total = 0.0
+    count = 0
+    for kind, part, cycles in ((0, shapes.split(";")[0], mac_cycles), (1, shapes.split(";")[1], fem_cycles)):
+        m = re.search(r"(\d+)x(\d+):(\d+)x(\d+)h(\d+)/", part)
+        sx, sy, h = int(m.group(3)), int(m.group(4)), int(m.group(5))
+        name = f"k_smooth_fused<double, {kind}, {h}, {sx}, {sy}>"
+        if name not in tab:
+            return None
+        per_launch = h // (2 if kind == 0 else 4)
+        n = cycles * 2 * ((nu + per_launch - 1) // per_launch)
+        total += n * tab[name]
+        count += n
+    return round(total / count) if count else None
+
+
 def sig(v):
     """Floats to 5 significant digits (residuals of 1e-9 stay visible)."""
     return float(f"{v:.5g}") if isinstance(v, float) else v
@@ -153,10 +184,12 @@ def main():
     launches = fp64["smoother_launches"]
     avg_ms = fp64["smoother_ms"] / launches
     achieved = fp64["smoother_bytes"] / (fp64["smoother_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(fp64["smoother_shapes"], fp64["mac_cycles_timed"], fp64["fem_cycles_timed"])
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="fine-level projection smoother k_smooth_fused<double> (MAC and FEM)",
                     avg_launch_us=round(avg_ms * 1e3, 2),
+                    back_to_back_us_per_launch={k: round(v, 2) for k, v in fp64["back_to_back_us_per_launch"].items()},
                     algorithmic_bytes_per_launch=round(fp64["smoother_bytes"] / launches))
     name, hbm = ia.device_info()
     line = {
@@ -171,7 +204,7 @@ def main():
         "config": {"workload": f"simple-spin rotor, full step_forward, {m}x{m} cells, projections to rel. residual 1e-8",
                    "solver": best_name, "baseline_solver": "fp64 multigrid, previous-solution warm start",
                    "grid": [m, m]},
-        "variants": {"fp64": {k: sig(v) for k, v in fp64.items()},
+        "variants": {"fp64": {k: sig(v) for k, v in fp64.items() if k != "back_to_back_us_per_launch"},
                      "mixed": {k: sig(v) for k, v in mixed.items()},
                      "mixed+predicted_guess": {k: sig(v) for k, v in pred.items()}},
         "speedup_vs_plain_fp64": round(pred["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),

@@ -117,6 +117,12 @@ int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned 
  * picked for each multigrid level ("nx x ny: SXxSY hH/SXxSY hH", FP64/FP32
  * sweeps), for reports. */
 int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len);
+/* No reference counterpart: time `sweeps` fine-level smoother sweeps of one
+ * projection system (INCRMT_MG_MAC / INCRMT_MG_FEM) back to back between two
+ * HIP events on the handle's stream (the sweeps act on that system's current
+ * iterate); returns the mean milliseconds per launch. */
+int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int precision, double *ms_per_launch,
+                                    int *launches);
 /* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
 void incrmt_fluid_destroy(incrmt_fluid *f);

@@ -75,6 +75,7 @@ _PROTOS = [
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
     ("incrmt_fluid_set_guess", _I, [_V, _I]),
+    ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
@@ -244,6 +245,14 @@ class Fluid2D:
         """fluid_2d::write_files(sn): the fflags-selected fields as float32
         Gnuplot matrices in `directory` (ftest's default flags)."""
         _check(lib().incrmt_fluid_write_frame(self._h, str(directory).encode(), sn, fflags))
+
+    def time_fine_smoother(self, kind, sweeps, precision=PREC_FP64):
+        """Back-to-back fine-level sweeps of one projection system between two
+        HIP events: (ms per launch, launches)."""
+        ms, n = _D(0), _I(0)
+        _check(lib().incrmt_fluid_time_fine_smoother(self._h, kind, sweeps, precision, ctypes.byref(ms),
+                                                     ctypes.byref(n)))
+        return ms.value, n.value
 
     def describe(self):
         """The autotuned fused-smoother shapes per multigrid level."""

@@ -872,6 +872,17 @@ int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned 
     return 0;
 }
 
+int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int precision, double *ms_per_launch,
+                                    int *launches) {
+    if (int rc = check_ready(f)) return rc;
+    if ((kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) || sweeps < 1) return fail(INCRMT_ERR_ARG, "bad arguments");
+    Multigrid &mg = kind == INCRMT_MG_MAC ? f->e.mac : f->e.fem;
+    long n = 0;
+    int rc = mg.time_fine_smoother(sweeps, precision == INCRMT_PREC_MIXED, ms_per_launch, &n);
+    if (launches) *launches = (int)n;
+    return rc;
+}
+
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE) return fail(INCRMT_ERR_ARG, "bad guess mode");
