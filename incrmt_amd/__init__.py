@@ -156,6 +156,8 @@ class Fluid2D:
         self.m, self.n = m, n
         self.ml, self.nl = m + 4, n + 4
         self.nobj = 0
+        dx, dy = (bx - ax) / m, (by - ay) / n
+        self.eps = tw * (dx if dx < dy else dy)  # the transition width (fluid_2d.cc:35)
         self.ntrace = 0
 
     def close(self):

@@ -4,6 +4,7 @@
     python -m incrmt_amd.configs fem        standalone FEM projection, 1024^2 nodes
     python -m incrmt_amd.configs multibody  2048^2 cells, 64 circles, mixed vs FP64
     python -m incrmt_amd.configs ensemble   256 rotor instances of uneven size
+    python -m incrmt_amd.configs conv       resolution ladder of the rotor (conv_test)
 """
 import argparse
 import json
@@ -86,9 +87,24 @@ def run_ensemble(count=256, steps=10, threads=16, precision=ia.PREC_FP64):
     return r
 
 
+def run_conv(sizes=(128, 256, 512, 1024), duration=0.2):
+    """conv_test's procedure (conv_test.cc:100-194) on the simple-spin rotor
+    in its no-slip box: each size run to `duration` and compared with the
+    finest by l_comparison; observed orders of the total velocity L2 and
+    pressure L2 errors between successive sizes."""
+    from incrmt_amd import conv
+    t0 = time.perf_counter()
+    errs, steps = conv.ladder(lambda m: ia.simple_spin(m, rel_tol=1e-10), sizes, duration)
+    out = dict(config="conv ladder, simple-spin rotor", sizes=list(sizes), duration=duration, steps=steps,
+               errors={m: [float(x) for x in e] for m, e in errs.items()},
+               velocity_l2_orders=conv.orders(errs, 11), pressure_l2_orders=conv.orders(errs, 15),
+               seconds=time.perf_counter() - t0)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["fem", "multibody", "ensemble"])
+    ap.add_argument("which", choices=["fem", "multibody", "ensemble", "conv"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--threads", type=int, default=16)
@@ -97,6 +113,8 @@ def main():
         r = run_fem()
     elif a.which == "multibody":
         r = run_multibody(steps=a.steps or 200, every=a.every)
+    elif a.which == "conv":
+        r = run_conv()
     else:
         r = run_ensemble(steps=a.steps or 10, threads=a.threads)
     r["device"] = ia.device_info()[0]

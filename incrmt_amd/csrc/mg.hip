@@ -113,14 +113,19 @@ __device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
     return T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
 }
 
+// Threads per workgroup of a fused launch: one per tile point up to 1024.
+template <int SX, int SY>
+constexpr int smooth_threads() { return SX * SY < 1024 ? SX * SY : 1024; }
+
 template <class T, int KIND, int H, int kSX, int kSY>
-__global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k, const T *z, T *zout, const T *b,
-                                                            const T *c1, const T *c2, int phases, int first_color,
-                                                            int dir, ProArgs pa) {
+__global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(LevelArgs a, FemC k, const T *z, T *zout,
+                                                                            const T *b, const T *c1, const T *c2,
+                                                                            int phases, int first_color, int dir,
+                                                                            ProArgs pa) {
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
     __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
     const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
-    const int tid = threadIdx.y * kSX + threadIdx.x, nt = kSX * kSY;
+    const int tid = threadIdx.x, nt = smooth_threads<kSX, kSY>();
     // region points (li,lj) in [0,RW]x[0,RH]: the extra row/column holds the
     // faces to the right/top of the region
     for (int e = tid; e < PN; e += nt) {
@@ -163,8 +168,10 @@ __global__ __launch_bounds__(kSX * kSY) void k_smooth_fused(LevelArgs a, FemC k,
         }
         __syncthreads();
     }
-    const int li = threadIdx.x + H, lj = threadIdx.y + H, i = i0 + li, j = j0 + lj;
-    if (i < a.nx && j < a.ny) zout[j * a.zp + i] = sz[lj * PW + li];
+    for (int e = tid; e < kSX * kSY; e += nt) {
+        const int li = e % kSX + H, lj = e / kSX + H, i = i0 + li, j = j0 + lj;
+        if (i < a.nx && j < a.ny) zout[j * a.zp + i] = sz[lj * PW + li];
+    }
 }
 
 // ------------------------------------------------- residual and transfers
@@ -989,8 +996,11 @@ int Multigrid::collect_events() {
 struct SmoothShape {
     int sx, sy, h;
 };
-constexpr SmoothShape kShapes[] = {{64, 16, 4}, {64, 8, 4}, {64, 4, 4}, {64, 16, 2}, {64, 8, 2},
-                                   {64, 4, 2},  {64, 8, 8},  {64, 4, 8}};
+// gfx950 gives a workgroup up to 160 KiB of LDS; the larger tiles (up to
+// 110 KiB of FP64 tile data) halve the halo re-reads of the small ones.
+constexpr SmoothShape kShapes[] = {{64, 16, 4}, {64, 8, 4},  {64, 4, 4},  {64, 16, 2}, {64, 8, 2},
+                                   {64, 4, 2},  {64, 8, 8},  {64, 4, 8},  {64, 32, 4}, {128, 16, 4},
+                                   {64, 32, 8}, {128, 16, 8}, {128, 8, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 inline bool shape_valid(MgKind kind, const SmoothShape &v) { return kind == kMAC ? v.h <= 4 : v.h >= 4; }
@@ -998,7 +1008,7 @@ inline bool shape_valid(MgKind kind, const SmoothShape &v) { return kind == kMAC
 template <class T, int KIND, int H, int SX, int SY>
 static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2, int sweeps,
                         bool reverse, hipStream_t s, int *nlaunch, ProArgs pa) {
-    const dim3 grid((a.nx + SX - 1) / SX, (a.ny + SY - 1) / SY), block(SX, SY);
+    const dim3 grid((a.nx + SX - 1) / SX, (a.ny + SY - 1) / SY), block(smooth_threads<SX, SY>());
     const int ncol = KIND == kMAC ? 2 : 4, per = H / ncol;  // sweeps per launch
     for (int sw = 0; sw < sweeps; sw += per) {
         const int nsw = sweeps - sw < per ? sweeps - sw : per;
@@ -1017,16 +1027,31 @@ static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, c
 template <class T, int KIND>
 static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2,
                        int sweeps, bool reverse, hipStream_t s, int *nl, ProArgs pa) {
+#define SHAPE(id, H, SX, SY) \
+    case id: return launch_shape<T, KIND, H, SX, SY>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
     switch (shape) {
-    case 0: return launch_shape<T, KIND, 4, 64, 16>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
-    case 1: return launch_shape<T, KIND, 4, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
-    case 2: return launch_shape<T, KIND, 4, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
-    case 3: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 16>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
-    case 4: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
-    case 5: if (KIND == kMAC) return launch_shape<T, KIND, 2, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
-    case 6: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 8>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
-    case 7: if (KIND == kFEM) return launch_shape<T, KIND, 8, 64, 4>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); break;
+        SHAPE(0, 4, 64, 16)
+        SHAPE(1, 4, 64, 8)
+        SHAPE(2, 4, 64, 4)
+        SHAPE(8, 4, 64, 32)
+        SHAPE(9, 4, 128, 16)
     }
+    if constexpr (KIND == kMAC) {
+        switch (shape) {
+            SHAPE(3, 2, 64, 16)
+            SHAPE(4, 2, 64, 8)
+            SHAPE(5, 2, 64, 4)
+        }
+    } else {
+        switch (shape) {
+            SHAPE(6, 8, 64, 8)
+            SHAPE(7, 8, 64, 4)
+            SHAPE(10, 8, 64, 32)
+            SHAPE(11, 8, 128, 16)
+            SHAPE(12, 8, 128, 8)
+        }
+    }
+#undef SHAPE
     return INCRMT_ERR_ARG_SHAPE;
 }
 

@@ -123,6 +123,8 @@ class OracleFluid:
             spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"], mg_tol, len(objs),
             iptr(kinds), dptr(params))
         self.nobj = len(objs)
+        dx, dy = (spec["bx"] - spec["ax"]) / self.m, (spec["by"] - spec["ay"]) / self.n
+        self.eps = spec["tw"] * (dx if dx < dy else dy)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -201,6 +203,17 @@ class OracleFluid:
         out = np.zeros(4 * max(n, 1))
         f(self.h, k, dptr(out), n)
         return out[: 4 * n].reshape(n, 4)
+
+
+    def timestep_select(self, interval):
+        return timestep_select(interval, self.dt_reg)
+
+    def steps(self, dt, k):
+        for _ in range(k):
+            self.step(dt)
+
+    def close(self):
+        pass
 
 
 def timestep_select(interval, dt_reg):
