@@ -638,6 +638,9 @@ int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double
     if (!p) return fail(INCRMT_ERR_ARG, "unknown field");
     FluidEngine &e = f->e;
     HIP_CHECK(hipStreamSynchronize(e.s));
+    const std::string nm = name;
+    if (nm == "X" || nm == "Y" || nm == "nX" || nm == "nY" || nm == "phi" || nm == "s")
+        ls_reset_window(e.obj[obj].w);  // the next band rebuild of this solid covers the whole grid
     if (!is_int) return copy_field(e, p, const_cast<double *>(host), false);
     std::vector<int> tmp(e.g.N, 0);
     HIP_CHECK(hipMemcpy(tmp.data(), p, e.g.N * sizeof(int), hipMemcpyDeviceToHost));

@@ -28,6 +28,13 @@ struct LsWork {
     int *cnt;             // counters (kCntIter + kIterSlots ints)
     int *hcnt;            // pinned host mirror
     int pred_fim = 2, pred_ext = 2;  // passes the last march / extension needed
+    // window of the next rebuild (see rebuild_common): level-set points
+    // [win0,win1) x [win2,win3); bbox of the last band on the device, its
+    // host copy and the event that marks the copy done
+    bool have_win = false, bbox_pending = false, pin_first = false;
+    int win[4] = {0, 0, 0, 0};
+    int *bbox = nullptr, *hbbox = nullptr;
+    hipEvent_t ev_bbox = nullptr;
 };
 
 int ls_alloc(const Grid &g, LsWork &w);
@@ -37,6 +44,9 @@ void ls_free(LsWork &w);
 // levelset.cc:199-335). Returns 0, or an INCRMT_ERR_* code.
 int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
                    int *fim_iters);
+// Make the next rebuild cover the whole grid (after phi or the status
+// codes were replaced from outside).
+void ls_reset_window(LsWork &w);
 // levelset::build_band alone on the current phi (initialisation).
 int ls_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, int *fim_iters);
 // obj_field::extrapolate (traverse.hh:169-260) over the band of the last

@@ -26,6 +26,12 @@ namespace incrmt {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+
+// Rectangle of level-set points [I0,I1) x [J0,J1) a rebuild processes.
+struct Win {
+    int I0, I1, J0, J1;
+};
+inline dim3 grid_win(const Win &v) { return dim3((v.I1 - v.I0 + BX - 1) / BX, (v.J1 - v.J0 + BY - 1) / BY); }
 constexpr int kListBlocks = 1024, kListThreads = 256;
 constexpr int kTileBlocks = 512;
 
@@ -199,9 +205,9 @@ __global__ void k_zero_counts(int *cnt, int n) {
 }
 
 // obj_field::pin (obj_field.cc:92-97) over the ghosted grid
-__global__ void k_pin(Grid g, ObjConst c, ObjArrays a) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn) {
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
     const double p = a.phi[ij];
     if (p < c.pinw && p > -c.pinw) a.phi[ij] = obj_phi(c, a.X[ij], a.Y[ij]);
@@ -210,9 +216,9 @@ __global__ void k_pin(Grid g, ObjConst c, ObjArrays a) {
 // Sign-change scan (levelset.cc:211-263): a point is on an interface layer
 // when a 4-neighbour has the other sign. Interface points are listed for
 // the Newton solves; the fixed-point buffers start at +-inf elsewhere.
-__global__ void k_scan(Grid g, ObjArrays a, LsWork w) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__global__ void k_scan(Grid g, ObjArrays a, LsWork w, Win wn) {
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), P = g.P;
     const double *phi = a.phi;
     const bool pos = phi[ij] > 0;
@@ -259,9 +265,9 @@ __global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) {
 // Candidate points: non-interface points within R cells (max norm) of an
 // interface point, R covering the band cut-offs; separable dilation, rows
 // then columns.
-__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
     bool hit = false;
     for (int d = -R; d <= R && !hit; d++) {
@@ -272,9 +278,9 @@ __global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R) {
     }
     w.mark[ij] = hit;
 }
-__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), s = a.st[ij];
     bool hit = false;
     if (s != 3 && s != 4)
@@ -288,6 +294,13 @@ __global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R) {
         const int k = atomicAdd(&w.cnt[kCntCand], 1);
         w.cand[k] = ij;
         atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 1);
+    }
+}
+
+__global__ void k_bbox_init(LsWork w) {
+    if (threadIdx.x == 0) {
+        w.bbox[0] = w.bbox[2] = 1 << 30;
+        w.bbox[1] = w.bbox[3] = -1;
     }
 }
 
@@ -409,9 +422,35 @@ __global__ __launch_bounds__(256) void k_fim_tiles(Grid g, EikGeom eg, ObjConst 
 // fixed-point values and status 5/2, points the heap would have left in its
 // queue status 6/1, everything outside the band the cut-off value. Outside
 // band points are listed for the extension.
-__global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, int I, int J,
+                               int (*bb)[BX * BY], int lane);
+__global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, Win wn) {
+    __shared__ int bb[4][BX * BY];
+    const int t = threadIdx.y * BX + threadIdx.x;
+    bb[0][t] = bb[2][t] = 1 << 30;
+    bb[1][t] = bb[3][t] = -1;
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I < wn.I1 && J < wn.J1) finalize_point(g, c, a, w, v, I, J, bb, t);
+    // bounding box of the band and its trial points, for the next window
+    __syncthreads();
+    for (int h = BX * BY / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            bb[0][t] = min(bb[0][t], bb[0][t + h]);
+            bb[1][t] = max(bb[1][t], bb[1][t + h]);
+            bb[2][t] = min(bb[2][t], bb[2][t + h]);
+            bb[3][t] = max(bb[3][t], bb[3][t + h]);
+        }
+        __syncthreads();
+    }
+    if (t == 0 && bb[1][0] >= 0) {
+        atomicMin(&w.bbox[0], bb[0][0]);
+        atomicMax(&w.bbox[1], bb[1][0]);
+        atomicMin(&w.bbox[2], bb[2][0]);
+        atomicMax(&w.bbox[3], bb[3][0]);
+    }
+}
+__device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, int I, int J,
+                               int (*bb)[BX * BY], int lane) {
     const int ij = g.ls(I, J), P = g.P, s = a.st[ij];
     const double lim = c.phi_up, dlim = c.phi_down;
     auto acc_pos = [&](int q) { const int t = a.st[q]; return t == 4 || (t == 7 && v[q] < lim); };
@@ -438,6 +477,9 @@ __global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const doub
         }
     }
     w.mark[ij] = (unsigned char)ns;  // statuses go to st after every thread read the old ones
+    if (ns >= 1 && ns <= 6) {
+        bb[0][lane] = I; bb[1][lane] = I; bb[2][lane] = J; bb[3][lane] = J;
+    }
     a.phi[ij] = phi;
     if (ns == 4 || ns == 5) {
         const int k = atomicAdd(&w.cnt[kCntPband], 1);
@@ -445,9 +487,9 @@ __global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const doub
         atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 2);
     }
 }
-__global__ void k_commit_status(Grid g, ObjArrays a, LsWork w) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= g.ml || J >= g.nl) return;
+__global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
+    if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
     a.st[ij] = w.mark[ij];
 }
@@ -678,34 +720,77 @@ int ls_alloc(const Grid &g, LsWork &w) {
     HIP_CHECK(hipMalloc((void **)&w.tiles_ext, T * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.cnt, (kCntIter + kIterSlots) * sizeof(int)));
     HIP_CHECK(hipHostMalloc((void **)&w.hcnt, (kCntIter + kIterSlots) * sizeof(int), hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void **)&w.bbox, 4 * sizeof(int)));
+    HIP_CHECK(hipHostMalloc((void **)&w.hbbox, 4 * sizeof(int), hipHostMallocDefault));
+    HIP_CHECK(hipEventCreateWithFlags(&w.ev_bbox, hipEventDisableTiming));
     return 0;
 }
 
 void ls_free(LsWork &w) {
     void *p[] = {w.va, w.vb, w.ex, w.ey, w.fx, w.fy, w.iface, w.cand, w.pband, w.mark, w.cflag, w.tflag,
-                 w.tiles_fim, w.tiles_ext, w.cnt};
+                 w.tiles_fim, w.tiles_ext, w.cnt, w.bbox};
     for (auto q : p)
         if (q) (void)hipFree(q);
     if (w.hcnt) (void)hipHostFree(w.hcnt);
+    if (w.hbbox) (void)hipHostFree(w.hbbox);
+    if (w.ev_bbox) (void)hipEventDestroy(w.ev_bbox);
     w = LsWork{};
+}
+
+// The points a rebuild may change lie within the reach of the march from
+// the previous band. The window is the union of every earlier window and
+// the previous band's bounding box (statuses 1..6) widened by that reach
+// plus two, aligned outward to whole tiles; it never shrinks, so every
+// point outside it still holds what the first, whole-grid rebuild left
+// there (status 0 or 7, phi at the cut-off, +-inf in the march buffers).
+static Win next_window(const Grid &g, LsWork &w, int R) {
+    const Win full{0, g.ml, 0, g.nl};
+    if (!w.have_win) return full;
+    Win v{w.win[0], w.win[1], w.win[2], w.win[3]};
+    if (w.bbox_pending) {
+        (void)hipEventSynchronize(w.ev_bbox);
+        w.bbox_pending = false;
+        const int *b = w.hbbox;
+        if (b[1] >= 0) {
+            const int m = R + 2;
+            int I0 = (b[0] - m) / kTileX * kTileX, I1 = ((b[1] + 1 + m + kTileX - 1) / kTileX) * kTileX;
+            int J0 = (b[2] - m) / kTileY * kTileY, J1 = ((b[3] + 1 + m + kTileY - 1) / kTileY) * kTileY;
+            if (b[0] - m < 0) I0 = 0;
+            if (b[2] - m < 0) J0 = 0;
+            if (I1 > g.ml) I1 = g.ml;
+            if (J1 > g.nl) J1 = g.nl;
+            if (v.I1 <= v.I0) v = Win{I0, I1, J0, J1};
+            else v = Win{I0 < v.I0 ? I0 : v.I0, I1 > v.I1 ? I1 : v.I1, J0 < v.J0 ? J0 : v.J0, J1 > v.J1 ? J1 : v.J1};
+        }
+    }
+    return v;
 }
 
 static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
                           int *fim_iters) {
-    const dim3 gg = grid2(g.ml, g.nl), bb(BX, BY);
-    k_zero_counts<<<1, 64, 0, s>>>(w.cnt, kCntIter + kIterSlots);
-    KCHECK();
-    k_scan<<<gg, bb, 0, s>>>(g, a, w);
-    KCHECK();
-    k_newton<<<kListBlocks, kListThreads, 0, s>>>(g, geom_of(g), a, w);
-    KCHECK();
+    const dim3 bb(BX, BY);
     // the march cannot accept a point further than limit/h + 2 cells (max
     // norm) from the interface layer: beyond it every path is longer
     const double reach = fmax(-c.phi_down, c.phi_up) / fmin(g.dx, g.dy) + 3;
     const int R = reach > g.ml + g.nl ? g.ml + g.nl : (int)reach;
-    k_dilate_rows<<<gg, bb, 0, s>>>(g, a, w, R < g.ml ? R : g.ml);
+    const bool whole = !w.have_win;
+    const Win wn = next_window(g, w, R);
+    if (wn.I1 <= wn.I0 || wn.J1 <= wn.J0) return 0;  // nothing of this solid left on the grid
+    // the row dilation also covers R rows either side, which the column pass reads
+    const Win wr{wn.I0, wn.I1, wn.J0 - R < 0 ? 0 : wn.J0 - R, wn.J1 + R > g.nl ? g.nl : wn.J1 + R};
+    if (w.pin_first) {
+        k_pin<<<grid_win(wn), bb, 0, s>>>(g, c, a, wn);
+        KCHECK();
+    }
+    k_zero_counts<<<1, 64, 0, s>>>(w.cnt, kCntIter + kIterSlots);
     KCHECK();
-    k_dilate_cols<<<gg, bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl);
+    k_scan<<<grid_win(wn), bb, 0, s>>>(g, a, w, wn);
+    KCHECK();
+    k_newton<<<kListBlocks, kListThreads, 0, s>>>(g, geom_of(g), a, w);
+    KCHECK();
+    k_dilate_rows<<<grid_win(wr), bb, 0, s>>>(g, a, w, R < g.ml ? R : g.ml, wr);
+    KCHECK();
+    k_dilate_cols<<<grid_win(wn), bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl, wn);
     KCHECK();
     k_compact_tiles<<<64, 256, 0, s>>>(w, 1, w.tiles_fim, kCntTileFim);
     KCHECK();
@@ -723,24 +808,42 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     if (fim_iters) *fim_iters = passes;
     k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
     KCHECK();
-    k_finalize<<<gg, bb, 0, s>>>(g, c, a, w, bufs[passes & 1]);
+    k_bbox_init<<<1, 64, 0, s>>>(w);
     KCHECK();
+    k_finalize<<<grid_win(wn), bb, 0, s>>>(g, c, a, w, bufs[passes & 1], wn);
+    KCHECK();
+    HIP_CHECK(hipMemcpyAsync(w.hbbox, w.bbox, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipEventRecord(w.ev_bbox, s));
+    w.bbox_pending = true;
+    w.have_win = true;
+    if (whole) {  // windows grow from the band's box, not from the whole grid
+        w.win[0] = w.win[1] = w.win[2] = w.win[3] = 0;
+    } else {
+        w.win[0] = wn.I0; w.win[1] = wn.I1; w.win[2] = wn.J0; w.win[3] = wn.J1;
+    }
     k_compact_tiles<<<64, 256, 0, s>>>(w, 2, w.tiles_ext, kCntTileExt);
     KCHECK();
-    k_commit_status<<<gg, bb, 0, s>>>(g, a, w);
+    k_commit_status<<<grid_win(wn), bb, 0, s>>>(g, a, w, wn);
     KCHECK();
     return 0;
 }
 
 int ls_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, int *fim_iters) {
+    ls_reset_window(w);  // a new field: the whole grid
+    w.pin_first = false;
     return rebuild_common(s, g, c, a, w, fim_iters);
 }
 
 int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
                    int *fim_iters) {
-    k_pin<<<grid2(g.ml, g.nl), dim3(BX, BY), 0, s>>>(g, c, a);
-    KCHECK();
+    w.pin_first = true;
     return rebuild_common(s, g, c, a, w, fim_iters);
+}
+
+void ls_reset_window(LsWork &w) {
+    if (w.bbox_pending) (void)hipEventSynchronize(w.ev_bbox);
+    w.bbox_pending = false;
+    w.have_win = false;
 }
 
 int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *ext_iters) {
