@@ -26,9 +26,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def gpu_run(m, steps, warmup, precision, timing=False, guess=0):
+def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=1):
     import incrmt_amd as ia
     f = ia.simple_spin(m, rel_tol=1e-8, precision=precision, guess=guess)
+    f.set_launch(launch)
     l, adt = f.timestep_select(4 * math.pi / 240)
     f.steps(adt, warmup)
     f.sync()
@@ -174,6 +175,9 @@ def main():
     fp64 = gpu_run(m, a.steps, a.warmup, ia.PREC_FP64, timing=True, guess=ia.GUESS_PREVIOUS)
     mixed = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_PREVIOUS)
     pred = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_EXTRAPOLATE)
+    # the same, V-cycles launched kernel by kernel instead of replayed from
+    # their captured HIP graphs
+    direct = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_EXTRAPOLATE, launch=ia.LAUNCH_DIRECT)
     best = pred
     best_name = "mixed+predicted_guess"
     # roofline of the fine-level smoother (FP64 pass): algorithmic bytes per
@@ -206,7 +210,9 @@ def main():
                    "grid": [m, m]},
         "variants": {"fp64": {k: sig(v) for k, v in fp64.items() if k != "back_to_back_us_per_launch"},
                      "mixed": {k: sig(v) for k, v in mixed.items()},
-                     "mixed+predicted_guess": {k: sig(v) for k, v in pred.items()}},
+                     "mixed+predicted_guess": {k: sig(v) for k, v in pred.items()},
+                     "mixed+predicted_guess, direct launches": {k: sig(v) for k, v in direct.items()
+                                                                if k != "smoother_shapes"}},
         "speedup_vs_plain_fp64": round(pred["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "roofline": roofline,
         "device": name,

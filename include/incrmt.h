@@ -86,6 +86,13 @@ int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int
 #define INCRMT_GUESS_PREVIOUS 0
 #define INCRMT_GUESS_EXTRAPOLATE 1
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
+/* How the projections' V-cycles are issued: captured once into a HIP graph
+ * and replayed (INCRMT_LAUNCH_GRAPH, the default) or launched kernel by
+ * kernel (INCRMT_LAUNCH_DIRECT). Results are bit-identical. No reference
+ * counterpart (the reference's V-cycles are host loops in TGMG). */
+#define INCRMT_LAUNCH_DIRECT 0
+#define INCRMT_LAUNCH_GRAPH 1
+int incrmt_fluid_set_launch(incrmt_fluid *f, int mode);
 /* fluid_2d::step_forward(dt) (fluid_2d.cc:298-369). */
 int incrmt_fluid_step(incrmt_fluid *f, double dt);
 /* k calls of step_forward(dt), the inner loop of fluid_2d::solve (fluid_2d.cc:274). */
@@ -162,9 +169,13 @@ int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
  * its input, to ||b-Az|| <= rel_tol ||b|| (b's mean is projected out first). */
 int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
                     int *cycles, double *rel_res);
+/* V-cycle issue mode of incrmt_mg_solve (INCRMT_LAUNCH_GRAPH, the default,
+ * or INCRMT_LAUNCH_DIRECT; see incrmt_fluid_set_launch). No reference
+ * counterpart. */
+int incrmt_mg_set_launch(incrmt_mg *h, int mode);
 /* No reference counterpart (for checking the smoother): `sweeps` forward
  * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused
- * launch shape `shape` (0..7; see incrmt_fluid_describe) or one launch per
+ * launch shape `shape` (0..12; see incrmt_fluid_describe) or one launch per
  * colour when shape < 0. */
 int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape);
 /* y = A x with the solver's operator (mgs_*::mul_a + a_cc, full row). */

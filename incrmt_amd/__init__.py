@@ -28,6 +28,7 @@ ERR_ARG, ERR_NOT_CONVERGED, ERR_NEWTON, ERR_BAND, ERR_SINGULAR_FIT, ERR_HIP, ERR
 MG_MAC, MG_FEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_PREVIOUS, GUESS_EXTRAPOLATE = 0, 1
+LAUNCH_DIRECT, LAUNCH_GRAPH = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -75,6 +76,7 @@ _PROTOS = [
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
     ("incrmt_fluid_set_guess", _I, [_V, _I]),
+    ("incrmt_fluid_set_launch", _I, [_V, _I]),
     ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
@@ -88,6 +90,7 @@ _PROTOS = [
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
     ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
     ("incrmt_mg_smooth", _I, [_V, _dp, _dp, _I, _I]),
+    ("incrmt_mg_set_launch", _I, [_V, _I]),
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
@@ -187,6 +190,11 @@ class Fluid2D:
         """GUESS_PREVIOUS (the reference's warm start) or GUESS_EXTRAPOLATE
         (2 z_n - z_(n-1)) as the projections' starting iterate."""
         _check(lib().incrmt_fluid_set_guess(self._h, mode))
+
+    def set_launch(self, mode):
+        """LAUNCH_GRAPH (V-cycles captured once as HIP graphs and replayed;
+        the default) or LAUNCH_DIRECT (kernel by kernel); bit-identical."""
+        _check(lib().incrmt_fluid_set_launch(self._h, mode))
 
     def step_forward(self, dt):
         _check(lib().incrmt_fluid_step(self._h, dt))
@@ -352,6 +360,10 @@ class MgSolver:
         _check(lib().incrmt_mg_solve(self._h, _d(bb), _d(self.z), rel_tol, max_cycles, precision,
                                      ctypes.byref(c), ctypes.byref(r)))
         return self.z.copy(), c.value, r.value
+
+    def set_launch(self, mode):
+        """LAUNCH_GRAPH (default) or LAUNCH_DIRECT V-cycles; bit-identical."""
+        _check(lib().incrmt_mg_set_launch(self._h, mode))
 
     def apply(self, x):
         xx = np.ascontiguousarray(x, dtype=np.float64).ravel()

@@ -893,6 +893,13 @@ int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     return 0;
 }
 
+int incrmt_fluid_set_launch(incrmt_fluid *f, int mode) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (mode != INCRMT_LAUNCH_DIRECT && mode != INCRMT_LAUNCH_GRAPH) return fail(INCRMT_ERR_ARG, "bad launch mode");
+    f->e.mac.use_graphs = f->e.fem.use_graphs = mode == INCRMT_LAUNCH_GRAPH;
+    return 0;
+}
+
 int incrmt_fluid_set_ntrace(incrmt_fluid *f, int ntrace) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (f->e.initialized) return fail(INCRMT_ERR_ARG, "tracers are placed by initialize: set them before");
@@ -1056,6 +1063,13 @@ int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, in
     if (cycles) *cycles = st.cycles;
     if (rel_res) *rel_res = st.rel_res;
     return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;
+}
+
+int incrmt_mg_set_launch(incrmt_mg *h, int mode) {
+    if (!h) return fail(INCRMT_ERR_ARG, "null handle");
+    if (mode != INCRMT_LAUNCH_DIRECT && mode != INCRMT_LAUNCH_GRAPH) return fail(INCRMT_ERR_ARG, "bad launch mode");
+    h->mg.use_graphs = mode == INCRMT_LAUNCH_GRAPH;
+    return 0;
 }
 
 int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape) {

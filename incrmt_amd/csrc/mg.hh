@@ -10,6 +10,8 @@
 #ifndef INCRMT_MG_HH
 #define INCRMT_MG_HH
 
+#include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 #include "common.hh"
@@ -92,6 +94,8 @@ class Multigrid {
     bool time_smoother = false;
     // start each solve from 2 z_n - z_(n-1) (set for the step's projections)
     bool extrapolate = false;
+    // replay captured V-cycles (HIP graphs) instead of launching them
+    bool use_graphs = true;
     double smoother_ms = 0;
     long smoother_launches = 0;
     double smoother_bytes = 0;  // algorithmic bytes of the timed sweeps
@@ -109,6 +113,11 @@ class Multigrid {
     double *zprev_ = nullptr;        // the solution before the last one (extrapolation)
     bool f32_stale_ = true;          // coefficient copies for the FP32 cycle out of date
     int make_f32_coefs();
+    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // FP64 cycle, mixed cycle
+    std::vector<uintptr_t> graph_key_[2];                 // launch state each graph was captured in
+    std::vector<uintptr_t> launch_state() const;
+    void restore_buffers(const std::vector<uintptr_t> &key);
+    int run_cycle(int which, const std::function<int()> &cycle);
     long nsolved_ = 0;
     bool resampled_ = false;         // level 1 is a uniform resampling of level 0
     Resample rx_{}, ry_{};

@@ -8,6 +8,7 @@
 // deterministic two-pass block reductions, so a solve is reproducible
 // launch to launch.
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <cstdio>
 #include <type_traits>
@@ -776,6 +777,8 @@ static TailArgs tail_args(const std::vector<MgLevel> &L, size_t t, MgKind kind, 
 
 
 Multigrid::~Multigrid() {
+    for (auto &g : graph_exec_)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto &e : ev_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (void *p : allocs) (void)hipFree(p);
     if (h_red) (void)hipHostFree(h_red);
@@ -1396,8 +1399,74 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     return rel <= tol || bn == 0 ? 0 : 2;
 }
 
+// One V-cycle as a HIP graph. A cycle's launch sequence is fixed by the
+// hierarchy (kernels, launch shapes, sweep counts) and by which of each
+// level's two iterate buffers is current, so it is captured once and
+// replayed while those stay the same; a smoothing or timing call between
+// solves that leaves the other buffer current makes the next solve capture
+// again. Each level's fused smoother swaps its buffers an even number of
+// times per cycle (as many pre- as post-smoothing launches), so a cycle ends
+// in the buffer state it began in; a hierarchy where that fails launches its
+// cycles directly. Nothing in a cycle sets memory with a memset node (every
+// array a cycle clears is cleared by a kernel), and timing runs (HIP events
+// around smoother launches) always launch directly.
+std::vector<uintptr_t> Multigrid::launch_state() const {
+    std::vector<uintptr_t> k{(uintptr_t)nu1, (uintptr_t)nu2, (uintptr_t)tail_};
+    for (const MgLevel &v : L) {
+        k.insert(k.end(), {(uintptr_t)v.z, (uintptr_t)v.z2, (uintptr_t)v.zf, (uintptr_t)v.zf2});
+        k.insert(k.end(), {(uintptr_t)v.shape64, (uintptr_t)v.shape32});
+    }
+    return k;
+}
+
+void Multigrid::restore_buffers(const std::vector<uintptr_t> &key) {
+    size_t i = 3;
+    for (MgLevel &v : L) {
+        v.z = (double *)key[i]; v.z2 = (double *)key[i + 1];
+        v.zf = (float *)key[i + 2]; v.zf2 = (float *)key[i + 3];
+        i += 6;
+    }
+}
+
+int Multigrid::run_cycle(int which, const std::function<int()> &cycle) {
+    if (!use_graphs || time_smoother) return cycle();
+    const std::vector<uintptr_t> key = launch_state();
+    if (graph_exec_[which] && graph_key_[which] != key) {
+        (void)hipGraphExecDestroy(graph_exec_[which]);
+        graph_exec_[which] = nullptr;
+    }
+    if (!graph_exec_[which]) {
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        const int rc = cycle();
+        const hipError_t e = hipStreamEndCapture(s_, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            restore_buffers(key);  // nothing ran
+            return rc ? rc : hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        }
+        if (launch_state() != key) {
+            // the cycle leaves other buffers current: a replay would start
+            // from the wrong ones, so this hierarchy launches directly
+            (void)hipGraphDestroy(g);
+            restore_buffers(key);
+            use_graphs = false;
+            return cycle();
+        }
+        const hipError_t e2 = hipGraphInstantiate(&graph_exec_[which], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            graph_exec_[which] = nullptr;
+            return hip_fail(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        graph_key_[which] = key;
+    }
+    HIP_CHECK(hipGraphLaunch(graph_exec_[which], s_));
+    return 0;
+}
+
 int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
-    return solve_loop(tol, max_cycles, st, [&]() { return vcycle(0); });
+    return solve_loop(tol, max_cycles, st, [&]() { return run_cycle(0, [&]() { return vcycle(0); }); });
 }
 
 int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
@@ -1407,13 +1476,15 @@ int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
     const FemC k = fem_consts(dx_, dy_);
     // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
     return solve_loop(tol, max_cycles, st, [&]() -> int {
-        k_residual_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc,
-                                                                    f.bf, f.zf);
-        KCHECK();
-        if (int rc = vcycle_f(0)) return rc;
-        k_add_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, f.zf);
-        KCHECK();
-        return 0;
+        return run_cycle(1, [&]() -> int {
+            k_residual_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs,
+                                                                        f.kc, f.bf, f.zf);
+            KCHECK();
+            if (int rc = vcycle_f(0)) return rc;
+            k_add_f32<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, f.zf);
+            KCHECK();
+            return 0;
+        });
     });
 }
 
