@@ -20,6 +20,7 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 constexpr int INCRMT_ERR_ARG_SHAPE = 1;  // unknown smoother shape
+constexpr int INCRMT_ERR_GRID = 7;       // grid outside what the kernels index exactly
 constexpr int kRedBlocks = 512;   // partial sums per reduction
 constexpr int kRedThreads = 256;
 
@@ -390,19 +391,27 @@ __global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const 
 // The coarse end of a V-cycle (every level from `top` down, when they fit in
 // LDS together) runs in ONE workgroup: the levels' iterates, right-hand
 // sides, residuals and coefficients live in LDS and each smoothing colour,
-// transfer and coarse sweep is a barrier, not a kernel launch.
+// transfer and coarse sweep is a barrier, not a kernel launch. The levels of
+// at most kTailWavePts points (the bottom of the cycle) run in the first
+// wavefront alone, whose lanes synchronise without a workgroup barrier, and
+// a coarsest level of at most kTailDirect points is solved exactly by dense
+// elimination instead of by sweeps.
 constexpr int kTailMaxLev = 12;
 constexpr int kTailThreads = 1024;
-constexpr int kTailArrays = 4;  // z, b, coef1, and coef2 (MAC) or r (FEM)
+constexpr int kTailArrays = 4;    // z, b, coef1, and coef2 (MAC) or r (FEM)
+constexpr int kTailWavePts = 128;
+constexpr int kTailDirect = 16;
 
 struct TailLev {
     int nx, ny, zp, off;   // LDS pitch (nx+2) and offset of the padded array
     int gcp;               // global pitch
+    unsigned mag;          // ceil(2^20 / ((nx+1)/2)): k -> k / ((nx+1)/2) as (k * mag) >> 20
     const void *c1, *c2;   // global coefficient arrays at (0,0): fw,fs (MAC) or kc (FEM)
     double kx, ky;
 };
 struct TailArgs {
     int kind, nlev, nu1, nu2, total;
+    int wave_from;         // first level run by the first wavefront alone (nlev: none)
     FemC k;
     TailLev lv[kTailMaxLev];
     void *z_top;
@@ -429,57 +438,94 @@ __device__ double tail_block_sum(double v, double *red) {
     return s;
 }
 
+// LDS writes of a wavefront's lanes visible to its other lanes
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
 __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     T *pool = (T *)smem;
     __shared__ double red[16];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ T dm[kTailDirect * kTailDirect];  // reduced system of the direct coarsest solve
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     auto arr = [&](int a, int l) -> T * { return pool + a * A.total + A.lv[l].off + A.lv[l].zp + 1; };
     for (int p = tid; p < kTailArrays * A.total; p += nt) pool[p] = 0;
     __syncthreads();
-    // coefficients with their zero ring, and the top level's z and b
-    for (int l = 0; l < A.nlev; l++) {
-        const TailLev &v = A.lv[l];
-        const int W = v.nx + 2, H = v.ny + 2;
-        const T *g1 = (const T *)v.c1, *g2 = (const T *)v.c2;
-        for (int p = tid; p < W * H; p += nt) {
-            const int i = p % W - 1, j = p / W - 1;
-            arr(2, l)[j * v.zp + i] = g1[j * v.gcp + i];
-            if (A.kind == kMAC) arr(3, l)[j * v.zp + i] = g2[j * v.gcp + i];
+    // coefficients with their zero ring (all levels as one flat range, two
+    // points per thread in flight), and the top level's z and b
+    for (int base = 0; base < A.total; base += 2 * nt) {
+        T v1[2], v2[2];
+        int dst[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int p = base + u * nt + tid;
+            dst[u] = -1;
+            v1[u] = v2[u] = 0;
+            if (p < A.total) {
+                int l = 0;
+                while (l + 1 < A.nlev && p >= A.lv[l + 1].off) l++;
+                const TailLev &v = A.lv[l];
+                const int q = p - v.off, W = v.nx + 2, jj = q / W, ii = q - jj * W;
+                const int g = (jj - 1) * v.gcp + (ii - 1);
+                v1[u] = ((const T *)v.c1)[g];
+                if (A.kind == kMAC) v2[u] = ((const T *)v.c2)[g];
+                dst[u] = p;
+            }
         }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (dst[u] >= 0) {
+                pool[2 * A.total + dst[u]] = v1[u];
+                if (A.kind == kMAC) pool[3 * A.total + dst[u]] = v2[u];
+            }
     }
     {
         const TailLev &v = A.lv[0];
         const T *gb = (const T *)A.b_top, *gz = (const T *)A.z_top;
         for (int p = tid; p < v.nx * v.ny; p += nt) {
             const int i = p % v.nx, j = p / v.nx;
-            arr(1, 0)[j * v.zp + i] = gb[j * A.top_pitch + i];
-            arr(0, 0)[j * v.zp + i] = gz[j * A.top_pitch + i];
+            const T bb = gb[j * A.top_pitch + i], zz = gz[j * A.top_pitch + i];
+            arr(1, 0)[j * v.zp + i] = bb;
+            arr(0, 0)[j * v.zp + i] = zz;
         }
     }
     __syncthreads();
+    // w: the first wavefront alone (lanes 0..63, wave_sync) instead of the
+    // whole workgroup (__syncthreads)
+    auto sync = [&](bool w) {
+        if (w) wave_sync();
+        else __syncthreads();
+    };
     auto offd = [&](int l, int i, int j, T &diag) -> T {
         const TailLev &v = A.lv[l];
         const int q = j * v.zp + i;
         if (A.kind == kMAC) return mac_offdiag<T>(arr(0, l), arr(2, l), arr(3, l), q, q, v.zp, v.zp, T(v.kx), T(v.ky), diag);
         return fem_offdiag<T>(arr(0, l), arr(2, l), q, q, v.zp, v.zp, A.k, diag);
     };
-    auto smooth = [&](int l, int sweeps, bool reverse) {
+    // colour phases enumerate only their own points: the k-th point of a
+    // colour is in colour row r = k / ha, column a = k - r ha
+    auto smooth = [&](int l, int sweeps, bool reverse, bool w) {
         const TailLev &v = A.lv[l];
-        const int ncol = A.kind == kMAC ? 2 : 4, N = v.nx * v.ny;
+        const int ncol = A.kind == kMAC ? 2 : 4, ha = (v.nx + 1) >> 1;
+        const int N = ha * (A.kind == kMAC ? v.ny : (v.ny + 1) >> 1);
+        const int t0 = w ? lane : tid, st = w ? 64 : nt;
         for (int sw = 0; sw < sweeps; sw++)
             for (int cc = 0; cc < ncol; cc++) {
                 const int col = reverse ? ncol - 1 - cc : cc;
-                for (int p = tid; p < N; p += nt) {
-                    const int i = p % v.nx, j = p / v.nx;
-                    const bool mine = A.kind == kMAC ? (((i + j) & 1) == col) : ((i & 1) + 2 * (j & 1) == col);
-                    if (!mine) continue;
+                for (int k = t0; k < N; k += st) {
+                    const int r = (int)(((unsigned)k * v.mag) >> 20), a = k - r * ha;
+                    const int j = A.kind == kMAC ? r : 2 * r + (col >> 1);
+                    const int i = A.kind == kMAC ? 2 * a + ((col + j) & 1) : 2 * a + (col & 1);
+                    if (i >= v.nx || j >= v.ny) continue;
                     T diag;
                     const T off = offd(l, i, j, diag);
                     if (diag != T(0)) arr(0, l)[j * v.zp + i] = (arr(1, l)[j * v.zp + i] - off) / diag;
                 }
-                __syncthreads();
+                sync(w);
             }
     };
     auto residual = [&](int l, int i, int j) -> T {
@@ -488,12 +534,14 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         const T off = offd(l, i, j, diag);
         return arr(1, l)[j * v.zp + i] - off - diag * arr(0, l)[j * v.zp + i];
     };
-    // descend
-    for (int l = 0; l + 1 < A.nlev; l++) {
-        smooth(l, A.nu1, false);
+    // pre-smooth level l, restrict its residual to l+1's right-hand side and
+    // clear l+1's iterate
+    auto down = [&](int l, bool w) {
+        smooth(l, A.nu1, false, w);
         const TailLev &f = A.lv[l], &c = A.lv[l + 1];
+        const int t0 = w ? lane : tid, st = w ? 64 : nt;
         if (A.kind == kMAC) {
-            for (int p = tid; p < c.nx * c.ny; p += nt) {
+            for (int p = t0; p < c.nx * c.ny; p += st) {
                 const int I = p % c.nx, J = p / c.nx;
                 T s = 0;
                 for (int dj = 0; dj < 2; dj++)
@@ -502,57 +550,28 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 arr(0, l + 1)[J * c.zp + I] = 0;
             }
         } else {
-            for (int p = tid; p < f.nx * f.ny; p += nt) {
+            for (int p = t0; p < f.nx * f.ny; p += st) {
                 const int i = p % f.nx, j = p / f.nx;
                 arr(3, l)[j * f.zp + i] = residual(l, i, j);
             }
-            __syncthreads();
+            sync(w);
             const T *r = arr(3, l);
             const int pz = f.zp;
-            for (int p = tid; p < c.nx * c.ny; p += nt) {
+            for (int p = t0; p < c.nx * c.ny; p += st) {
                 const int I = p % c.nx, J = p / c.nx, q = 2 * J * pz + 2 * I;
                 arr(1, l + 1)[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
                                               + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
                 arr(0, l + 1)[J * c.zp + I] = 0;
             }
         }
-        __syncthreads();
-    }
-    // coarsest: mean-free right-hand side, sweeps to a residual reduction
-    {
-        const int l = A.nlev - 1;
-        const TailLev &v = A.lv[l];
-        const int N = v.nx * v.ny;
-        double sb = 0;
-        for (int p = tid; p < N; p += nt) sb += double(arr(1, l)[(p / v.nx) * v.zp + p % v.nx]);
-        const T mb = T(tail_block_sum<T>(sb, red) / N);
-        for (int p = tid; p < N; p += nt) arr(1, l)[(p / v.nx) * v.zp + p % v.nx] -= mb;
-        __syncthreads();
-        auto r2 = [&]() -> double {
-            double a = 0;
-            for (int p = tid; p < N; p += nt) {
-                const double r = double(residual(l, p % v.nx, p / v.nx));
-                a += r * r;
-            }
-            return tail_block_sum<T>(a, red);
-        };
-        const double r0 = r2();
-        for (int it = 0; it < A.max_sweeps && r0 > 0; it++) {
-            smooth(l, 1, false);
-            if ((it & 7) == 7 && r2() <= A.reduce * A.reduce * r0) break;
-        }
-        double sz = 0;
-        for (int p = tid; p < N; p += nt) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
-        const T mz = T(tail_block_sum<T>(sz, red) / N);
-        for (int p = tid; p < N; p += nt) arr(0, l)[(p / v.nx) * v.zp + p % v.nx] -= mz;
-        __syncthreads();
-    }
-    // ascend
-    for (int l = A.nlev - 2; l >= 0; l--) {
+        sync(w);
+    };
+    // add level l+1's correction to level l and post-smooth level l
+    auto up = [&](int l, bool w) {
         const TailLev &f = A.lv[l], &c = A.lv[l + 1];
         const T *e = arr(0, l + 1);
-        const int p2 = c.zp;
-        for (int p = tid; p < f.nx * f.ny; p += nt) {
+        const int p2 = c.zp, t0 = w ? lane : tid, st = w ? 64 : nt;
+        for (int p = t0; p < f.nx * f.ny; p += st) {
             const int i = p % f.nx, j = p / f.nx, I = i >> 1, J = j >> 1;
             T add;
             if (A.kind == kMAC) {
@@ -570,9 +589,139 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             }
             arr(0, l)[j * f.zp + i] += add;
         }
+        sync(w);
+        smooth(l, A.nu2, true, w);
+    };
+    // Exact solve of a coarsest level of N <= kTailDirect points by the first
+    // wavefront: the right-hand side is mean-free, hence compatible, and the
+    // operator is symmetric with the constants as its null space, so fixing
+    // point 0 at zero leaves a definite system in points 1..N-1, eliminated
+    // without pivoting (row r of dm is point r+1). False (nothing written)
+    // when a pivot is zero, not finite or of the other sign.
+    auto direct = [&](int l) -> bool {
+        const TailLev &v = A.lv[l];
+        const int N = v.nx * v.ny, n = N - 1;
+        if (lane < n) {
+            const int p = lane + 1, i = p % v.nx, j = p / v.nx, q = j * v.zp + i;
+            T *row = dm + lane * kTailDirect;
+            for (int c = 0; c < kTailDirect; c++) row[c] = 0;
+            auto put = [&](int di, int dj, T a) {
+                const int ii = i + di, jj = j + dj;
+                if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.ny) return;
+                const int pn = jj * v.nx + ii;
+                if (pn > 0) row[pn - 1] += a;
+            };
+            if (A.kind == kMAC) {
+                const T *fw = arr(2, l), *fs = arr(3, l), kx = T(v.kx), ky = T(v.ky);
+                const T wl = fw[q], wr = fw[q + 1], sd = fs[q], su = fs[q + v.zp];
+                put(0, 0, -(kx * (wl + wr) + ky * (sd + su)));
+                put(-1, 0, kx * wl);
+                put(1, 0, kx * wr);
+                put(0, -1, ky * sd);
+                put(0, 1, ky * su);
+            } else {
+                const T *kc = arr(2, l);
+                const T kdl = kc[q - v.zp - 1], kdr = kc[q - v.zp], kul = kc[q - 1], kur = kc[q];
+                put(0, 0, T(A.k.fm) * (kdl + kdr + kul + kur));
+                put(-1, -1, T(A.k.fc) * kdl);
+                put(1, -1, T(A.k.fc) * kdr);
+                put(-1, 1, T(A.k.fc) * kul);
+                put(1, 1, T(A.k.fc) * kur);
+                put(0, -1, T(A.k.hey) * (kdl + kdr));
+                put(0, 1, T(A.k.hey) * (kul + kur));
+                put(-1, 0, T(A.k.hex) * (kdl + kul));
+                put(1, 0, T(A.k.hex) * (kdr + kur));
+            }
+        }
+        // lane r keeps row r's right-hand side in a register
+        T rhs = 0;
+        if (lane < n) {
+            const int p = lane + 1;
+            rhs = arr(1, l)[(p / v.nx) * v.zp + p % v.nx];
+        }
+        wave_sync();
+        bool ok = true;
+        T s0 = 0;
+        for (int k = 0; k < n; k++) {
+            const T piv = dm[k * kTailDirect + k];
+            if (k == 0) s0 = piv;
+            if (!(piv == piv) || piv == T(0) || (piv > T(0)) != (s0 > T(0)) || !(piv * T(0) == T(0))) {
+                ok = false;
+                break;
+            }
+            const T rk = __shfl(rhs, k, 64);
+            if (lane > k && lane < n) {
+                const T f = dm[lane * kTailDirect + k] / piv;
+                for (int c = k + 1; c < n; c++) dm[lane * kTailDirect + c] -= f * dm[k * kTailDirect + c];
+                rhs -= f * rk;
+            }
+            wave_sync();
+        }
+        if (!ok) return false;
+        // back substitution, one unknown at a time from the last
+        T x = 0;
+        for (int k = n - 1; k >= 0; k--) {
+            // lane k owns row k; the unknowns after it are already in x of their lanes
+            T acc = 0;
+            for (int c = k + 1; c < n; c++) acc += dm[k * kTailDirect + c] * __shfl(x, c, 64);
+            const T xk = (__shfl(rhs, k, 64) - acc) / dm[k * kTailDirect + k];
+            if (lane == k) x = xk;
+        }
+        if (lane < n) {
+            const int p = lane + 1;
+            arr(0, l)[(p / v.nx) * v.zp + p % v.nx] = x;
+        }
+        if (lane == 0) arr(0, l)[0] = 0;  // point 0 (a warm-started single-level tail holds an iterate there)
+        wave_sync();
+        return true;
+    };
+    // coarsest: mean-free right-hand side, then the direct solve or sweeps to
+    // a residual reduction, then a mean-free iterate
+    auto coarsest = [&](bool w) {
+        const int l = A.nlev - 1;
+        const TailLev &v = A.lv[l];
+        const int N = v.nx * v.ny, t0 = w ? lane : tid, st = w ? 64 : nt;
+        auto total = [&](double a) -> double { return w ? tail_wave_sum<T>(a) : tail_block_sum<T>(a, red); };
+        double sb = 0;
+        for (int p = t0; p < N; p += st) sb += double(arr(1, l)[(p / v.nx) * v.zp + p % v.nx]);
+        const T mb = T(total(sb) / N);
+        for (int p = t0; p < N; p += st) arr(1, l)[(p / v.nx) * v.zp + p % v.nx] -= mb;
+        sync(w);
+        const bool solved = w && N <= kTailDirect && N > 1 && direct(l);
+        if (!solved) {
+            auto r2 = [&]() -> double {
+                double a = 0;
+                for (int p = t0; p < N; p += st) {
+                    const double r = double(residual(l, p % v.nx, p / v.nx));
+                    a += r * r;
+                }
+                return total(a);
+            };
+            const double r0 = r2();
+            for (int it = 0; it < A.max_sweeps && r0 > 0; it++) {
+                smooth(l, 1, false, w);
+                if ((it & 7) == 7 && r2() <= A.reduce * A.reduce * r0) break;
+            }
+        }
+        double sz = 0;
+        for (int p = t0; p < N; p += st) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
+        const T mz = T(total(sz) / N);
+        for (int p = t0; p < N; p += st) arr(0, l)[(p / v.nx) * v.zp + p % v.nx] -= mz;
+        sync(w);
+    };
+    const int last = A.nlev - 1, ws = A.wave_from <= last ? A.wave_from : last + 1;
+    for (int l = 0; l < ws && l < last; l++) down(l, false);
+    if (ws <= last) {
+        if (tid < 64) {
+            for (int l = ws; l < last; l++) down(l, true);
+            coarsest(true);
+            for (int l = last - 1; l >= ws; l--) up(l, true);
+        }
         __syncthreads();
-        smooth(l, A.nu2, true);
+    } else {
+        coarsest(false);
     }
+    for (int l = (ws < last ? ws : last) - 1; l >= 0; l--) up(l, false);
     {
         const TailLev &v = A.lv[0];
         T *gz = (T *)A.z_top;
@@ -757,10 +906,14 @@ static TailArgs tail_args(const std::vector<MgLevel> &L, size_t t, MgKind kind, 
     TailArgs A{};
     A.kind = kind; A.nlev = (int)(L.size() - t); A.nu1 = nu1; A.nu2 = nu2; A.k = k;
     int off = 0;
+    A.wave_from = A.nlev;
     for (int q = 0; q < A.nlev; q++) {
         const MgLevel &v = L[t + q];
         TailLev &d = A.lv[q];
         d.nx = v.nx; d.ny = v.ny; d.zp = v.nx + 2; d.off = off; d.gcp = v.cp; d.kx = v.kx; d.ky = v.ky;
+        const unsigned ha = (unsigned)(v.nx + 1) / 2;
+        d.mag = ((1u << 20) + ha - 1) / ha;
+        if (A.wave_from == A.nlev && (long long)v.nx * v.ny <= kTailWavePts) A.wave_from = q;
         if (kind == kMAC) { d.c1 = f32 ? (const void *)v.fwf : v.fw; d.c2 = f32 ? (const void *)v.fsf : v.fs; }
         else { d.c1 = f32 ? (const void *)v.kcf : v.kc; d.c2 = nullptr; }
         off += (v.nx + 2) * (v.ny + 2);
@@ -888,6 +1041,14 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
     d_red = (double *)q;
     HIP_CHECK(hipHostMalloc((void **)&h_red, 8 * sizeof(double), hipHostMallocDefault));
     tail_ = tail_start(L);
+    // the tail kernel divides colour indices k < ha * rows by ha = (nx+1)/2
+    // as (k * ceil(2^20/ha)) >> 20: check it is exact on every tail level
+    for (size_t l = tail_; l < L.size(); l++) {
+        const unsigned ha = (unsigned)(L[l].nx + 1) / 2, mag = ((1u << 20) + ha - 1) / ha;
+        for (unsigned k = 0; k < ha * (unsigned)L[l].ny; k++)
+            if (((unsigned long long)k * mag >> 20) != k / ha || (unsigned long long)k * mag >= (1ull << 32))
+                return INCRMT_ERR_GRID;
+    }
     if (resampled_ && tail_ == 0) tail_ = 1;  // the tail's transfers assume nested levels
     if (int rc = set_irho_host(nullptr)) return rc;
     return autotune();
