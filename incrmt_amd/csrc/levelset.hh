@@ -12,8 +12,16 @@ enum { kCntIface = 0, kCntCand, kCntPband, kCntErr, kCntTileFim, kCntTileExt, kC
 constexpr int kIterSlots = 64;
 enum { kErrNewton = 1, kErrSingular = 2 };
 
-// The fixed-point passes run on 32x8 tiles of the ghosted grid held in LDS.
-constexpr int kTileX = 32, kTileY = 8;
+// The fixed-point passes run on kTileX x kTileY tiles of the ghosted grid
+// held in LDS, one workgroup of kTileX * kTileY threads per tile.
+#ifndef INCRMT_LS_TILE_X
+#define INCRMT_LS_TILE_X 32
+#endif
+#ifndef INCRMT_LS_TILE_Y
+#define INCRMT_LS_TILE_Y 8
+#endif
+constexpr int kTileX = INCRMT_LS_TILE_X, kTileY = INCRMT_LS_TILE_Y, kTileThreads = kTileX * kTileY;
+static_assert(kTileThreads % 64 == 0 && kTileThreads <= 1024, "a tile is whole wavefronts of one workgroup");
 
 struct LsWork {
     double *va, *vb;      // fixed-point buffers (ghosted grid)

@@ -359,7 +359,7 @@ struct LdsAcc {
     __device__ double val(int q) const { return sg * v[q]; }
 };
 
-__global__ __launch_bounds__(256) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
+__global__ __launch_bounds__(kTileThreads) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
                                                    const double *src, double *dst, int slot, int max_local) {
     __shared__ double sv[2][kFN];
     __shared__ int sst[kFN];
@@ -502,7 +502,7 @@ __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
 // read the previous values (sx,sy), inside points their predictor values.
 constexpr int kEH = 7, kEW = kTileX + 2 * kEH, kEHt = kTileY + 2 * kEH, kEN = kEW * kEHt;
 
-__global__ __launch_bounds__(256) void k_extend_tiles(Grid g, ObjArrays a, LsWork w, const double *srcx,
+__global__ __launch_bounds__(kTileThreads) void k_extend_tiles(Grid g, ObjArrays a, LsWork w, const double *srcx,
                                                       const double *srcy, double *dstx, double *dsty, int slot,
                                                       int max_local) {
     __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
@@ -799,7 +799,7 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     double *bufs[2] = {w.va, w.vb};
     int passes = 0;
     int rc = iterate(s, w, w.pred_fim, 4096, [&](int p, int slot) -> int {
-        k_fim_tiles<<<kTileBlocks, 256, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
+        k_fim_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
         KCHECK();
         return 0;
     }, &passes);
@@ -852,7 +852,7 @@ int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *
     double *bx[2] = {w.ex, w.fx}, *by[2] = {w.ey, w.fy};
     int passes = 0;
     int rc = iterate(s, w, w.pred_ext, 4096, [&](int p, int slot) -> int {
-        k_extend_tiles<<<kTileBlocks, 256, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1], by[(p + 1) & 1],
+        k_extend_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, a, w, bx[p & 1], by[p & 1], bx[(p + 1) & 1], by[(p + 1) & 1],
                                                    slot, 128);
         KCHECK();
         return 0;
