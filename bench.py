@@ -158,6 +158,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS fine-level smoother sweeps of each system (for counter runs)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline variant, --warmup then --steps steps (may be 0), no timing "
+                         "passes: two profiler runs of it differ by exactly the steps")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU sharding is a design "
@@ -167,6 +170,19 @@ def main():
     m = a.size
     if a.smoother_probe:
         print(json.dumps(smoother_probe(m, a.smoother_probe)))
+        return
+    if a.headline_only:
+        f = ia.simple_spin(m, rel_tol=1e-8, precision=ia.PREC_MIXED, guess=ia.GUESS_EXTRAPOLATE)
+        l, adt = f.timestep_select(4 * math.pi / 240)
+        f.steps(adt, a.warmup)
+        f.sync()
+        t0 = time.perf_counter()
+        if a.steps:
+            f.steps(adt, a.steps)
+        f.sync()
+        dt = time.perf_counter() - t0
+        print(json.dumps(dict(steps=a.steps, warmup=a.warmup, seconds=round(dt, 6), stats=f.stats())))
+        f.close()
         return
     # plain FP64 multigrid warm-started from the previous step's solution as
     # the reference does (mgs_common.hh:48-50), against FP32 V-cycles in FP64

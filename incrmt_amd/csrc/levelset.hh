@@ -15,10 +15,10 @@ enum { kErrNewton = 1, kErrSingular = 2 };
 // The fixed-point passes run on kTileX x kTileY tiles of the ghosted grid
 // held in LDS, one workgroup of kTileX * kTileY threads per tile.
 #ifndef INCRMT_LS_TILE_X
-#define INCRMT_LS_TILE_X 32
+#define INCRMT_LS_TILE_X 16
 #endif
 #ifndef INCRMT_LS_TILE_Y
-#define INCRMT_LS_TILE_Y 8
+#define INCRMT_LS_TILE_Y 16
 #endif
 constexpr int kTileX = INCRMT_LS_TILE_X, kTileY = INCRMT_LS_TILE_Y, kTileThreads = kTileX * kTileY;
 static_assert(kTileThreads % 64 == 0 && kTileThreads <= 1024, "a tile is whole wavefronts of one workgroup");
