@@ -159,7 +159,11 @@ int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec
 
 /* ---- mgs_mac_* / mgs_fem_* ------------------------------------------------
  * Standalone solver, the constructor mgs_*_const_rho(m,n,x_prd,y_prd,dx,dy,src)
- * (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM). */
+ * (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM).
+ * A periodic direction (mgs_mac.hh:45-56, mgs_fem.hh:47-52; a periodic FEM
+ * direction has m nodes, node m being node 0) needs an even size whose
+ * repeated halving ends on a coarsest level of at most 16 points (e.g.
+ * 2^k or 3 * 2^k); others return INCRMT_ERR_UNSUPPORTED. */
 int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out);
 /* Varying density: the reference's irho layouts (mgs_mac_varying_rho::irho,
  * 2mn+n+m edge values, fluid_2d.cc:1140-1160; mgs_fem_varying_rho::irho, m*n

@@ -1003,7 +1003,6 @@ int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, do
     if (!out) return fail(INCRMT_ERR_ARG, "null output handle");
     *out = nullptr;
     if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
-    if (x_prd || y_prd) return fail(INCRMT_ERR_UNSUPPORTED, "periodic boundaries are not implemented");
     if (m < 2 || n < 2 || !(dx > 0) || !(dy > 0)) return fail(INCRMT_ERR_ARG, "bad grid");
     incrmt_mg *h = new incrmt_mg;
     hipError_t er = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
@@ -1011,7 +1010,9 @@ int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, do
         delete h;
         return hip_fail(er, "hipStreamCreate", __FILE__, __LINE__);
     }
-    int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, m, n, dx, dy, h->s);
+    int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, m, n, dx, dy, h->s, x_prd != 0, y_prd != 0);
+    if (rc == INCRMT_ERR_UNSUPPORTED)
+        rc = fail(rc, "a periodic direction needs an even size whose halvings end at most 16 points");
     if (!rc) {
         const MgLevel &l = h->mg.fine();
         const size_t len = (size_t)(l.ny + 2) * l.pitch + 64;

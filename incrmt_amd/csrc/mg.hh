@@ -2,7 +2,7 @@
 //   MAC: cell-centred 5-point  div((1/rho) grad q) (mgs_mac.hh:14-102)
 //   FEM: vertex-centred 9-point bilinear-FEM stiffness with cell 1/rho
 //        (mgs_fem.hh:15-136)
-// on non-periodic, pure-Neumann boxes. The reference's engine (TGMG) is an
+// on pure-Neumann boxes, periodic in x and/or y or not. The reference's engine (TGMG) is an
 // external library absent from the reference tree; this solver honours its
 // contract (solve A z = b in place, z warm-started from the previous solve)
 // with its own algorithm: colored Gauss-Seidel V-cycles over rediscretized
@@ -38,6 +38,11 @@ struct MgLevel {
     double kx, ky;  // MAC 1/dx^2, 1/dy^2 of this level
     double hx, hy;  // FEM node spacing of this level
     int shape64 = 0, shape32 = 0;  // fused-smoother launch shape (FP64, FP32 sweeps)
+    // periodic directions: the ghost ring of every iterate holds the
+    // opposite edge (each write of an edge point also writes its mirrors),
+    // MAC faces 0 and nx are the two sides of the seam face, FEM has nx
+    // cells with the ring cell mirroring cell nx-1
+    int px = 0, py = 0;
 };
 
 // Transfer tables between a FEM level and a uniform coarse level whose
@@ -61,8 +66,12 @@ class Multigrid {
   public:
     Multigrid() {}
     ~Multigrid();
-    // Allocate a hierarchy for an m x n system (cells for MAC, nodes for FEM).
-    int init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s);
+    // Allocate a hierarchy for an m x n system (cells for MAC, nodes for FEM),
+    // periodic in x and/or y (mgs_mac.hh:45-56, mgs_fem.hh:47-52: a periodic
+    // FEM direction has m nodes, node m being node 0).
+    int init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px = false, bool py = false);
+    // Refresh the periodic ghost ring of a level-0 array (after a host upload).
+    int wrap_fine(double *z);
     // Fine-level coefficients from the ghosted cell densities (fluid_2d.cc:
     // 1140-1170 build the same values), then coarsen.
     int set_density(const Grid &g, const double *rho);
@@ -113,6 +122,7 @@ class Multigrid {
     double *zprev_ = nullptr;        // the solution before the last one (extrapolation)
     bool f32_stale_ = true;          // coefficient copies for the FP32 cycle out of date
     int make_f32_coefs();
+    int wrap_coefs(size_t l);
     hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // FP64 cycle, mixed cycle
     std::vector<uintptr_t> graph_key_[2];                 // launch state each graph was captured in
     std::vector<uintptr_t> launch_state() const;

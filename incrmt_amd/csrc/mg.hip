@@ -54,7 +54,29 @@ __device__ inline T fem_offdiag(const T *z, const T *kc, int q, int c, int zp, i
 struct LevelArgs {
     int nx, ny, zp, cp;
     double kx, ky;
+    int px, py;  // periodic directions
 };
+
+// Periodic directions keep the ghost ring of an iterate equal to the
+// opposite edge: writing edge point (i,j) also writes the ring points that
+// mirror it (column -1 for i = nx-1, column nx for i = 0, rows likewise,
+// and the corner).
+template <class T>
+__device__ inline void put_mirrors(const LevelArgs &a, T *z, int i, int j, T v) {
+    const bool xe = a.px && (i == 0 || i == a.nx - 1), ye = a.py && (j == 0 || j == a.ny - 1);
+    if (!xe && !ye) return;
+    const int mi = i == 0 ? a.nx : -1, mj = j == 0 ? a.ny : -1;
+    if (xe) z[j * a.zp + mi] = v;
+    if (ye) z[mj * a.zp + i] = v;
+    if (xe && ye) z[mj * a.zp + mi] = v;
+}
+template <class T>
+__device__ inline void put_z(const LevelArgs &a, T *z, int i, int j, T v) {
+    z[j * a.zp + i] = v;
+    put_mirrors(a, z, i, j, v);
+}
+// periodic index of a point outside [0, n)
+__device__ inline int wrap_idx(int i, int n) { return ((i % n) + n) % n; }
 
 // --------------------------------------------------------------- smoothers
 // Red-black Gauss-Seidel for the 5-point MAC operator: thread t of row j
@@ -67,7 +89,7 @@ __global__ void k_mac_smooth(LevelArgs a, T *z, const T *b, const T *fw, const T
     const int q = j * a.zp + i, c = j * a.cp + i;
     T diag;
     const T off = mac_offdiag<T>(z, fw, fs, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
-    if (diag != T(0)) z[q] = (b[q] - off) / diag;
+    if (diag != T(0)) put_z(a, z, i, j, (b[q] - off) / diag);
 }
 
 // Four-colour Gauss-Seidel for the 9-point FEM operator (colours (i%2,j%2)
@@ -80,7 +102,7 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
     const int q = j * a.zp + i, c = j * a.cp + i;
     T diag;
     const T off = fem_offdiag<T>(z, kc, q, c, a.zp, a.cp, k, diag);
-    if (diag != T(0)) z[q] = (b[q] - off) / diag;
+    if (diag != T(0)) put_z(a, z, i, j, (b[q] - off) / diag);
 }
 
 // Fused colored Gauss-Seidel: a whole multi-colour sweep (MAC: two sweeps)
@@ -96,15 +118,20 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
 struct ProArgs {
     const void *e;   // coarse correction at (0,0), or null for none
     int ezp, cnx, cny;
+    int px, py;      // periodic directions
 };
+// MAC: bilinear cell-centred interpolation, the neighbour clamped at walls
+// (constant extrapolation) and wrapped across periodic seams. FEM: bilinear
+// vertex interpolation (the next coarse node of a periodic direction's last
+// interval is node 0, which the coarse ring mirrors).
 template <class T, int KIND>
 __device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
     const T *e = (const T *)pa.e;
     const int I = i >> 1, J = j >> 1, p = pa.ezp;
     if (KIND == kMAC) {
         int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-        if (I2 < 0 || I2 >= pa.cnx) I2 = I;
-        if (J2 < 0 || J2 >= pa.cny) J2 = J;
+        if (I2 < 0 || I2 >= pa.cnx) I2 = pa.px ? wrap_idx(I2, pa.cnx) : I;
+        if (J2 < 0 || J2 >= pa.cny) J2 = pa.py ? wrap_idx(J2, pa.cny) : J;
         return T(0.5625) * e[J * p + I] + T(0.1875) * e[J * p + I2] + T(0.1875) * e[J2 * p + I]
                + T(0.0625) * e[J2 * p + I2];
     }
@@ -131,7 +158,16 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
     // region points (li,lj) in [0,RW]x[0,RH]: the extra row/column holds the
     // faces to the right/top of the region
     for (int e = tid; e < PN; e += nt) {
-        const int li = e % PW, lj = e / PW, i = i0 + li, j = j0 + lj;
+        const int li = e % PW, lj = e / PW;
+        int i = i0 + li, j = j0 + lj;
+        // periodic directions read the region's outside points from the
+        // opposite edge: they are real points, updated like the inside ones.
+        // A MAC seam face is then face 0 on both of its sides (the operator's
+        // face nx, used by the residual, is the same value whenever the
+        // density is periodic across the seam; where it is not, the sweep
+        // near the seam smooths a slightly different operator).
+        if (a.px && (i < 0 || i >= a.nx)) i = wrap_idx(i, a.nx);
+        if (a.py && (j < 0 || j >= a.ny)) j = wrap_idx(j, a.ny);
         const bool inpad = i >= -1 && i <= a.nx && j >= -1 && j <= a.ny;
         const int q = j * a.zp + i, c = j * a.cp + i;
         T zv = inpad ? z[q] : T(0);
@@ -161,7 +197,7 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
             }
             if (li < p || li >= RW - p || lj < p || lj >= RH - p) continue;
             const int i = i0 + li, j = j0 + lj;
-            if (i < 0 || i >= a.nx || j < 0 || j >= a.ny) continue;
+            if ((!a.px && (i < 0 || i >= a.nx)) || (!a.py && (j < 0 || j >= a.ny))) continue;
             const int q = lj * PW + li;
             T diag, off;
             if (KIND == kMAC) off = mac_offdiag<T>(sz, s1, s2, q, q, PW, PW, T(a.kx), T(a.ky), diag);
@@ -172,7 +208,7 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
     }
     for (int e = tid; e < kSX * kSY; e += nt) {
         const int li = e % kSX + H, lj = e / kSX + H, i = i0 + li, j = j0 + lj;
-        if (i < a.nx && j < a.ny) zout[j * a.zp + i] = sz[lj * PW + li];
+        if (i < a.nx && j < a.ny) put_z(a, zout, i, j, sz[lj * PW + li]);
     }
 }
 
@@ -192,7 +228,7 @@ __global__ void k_mac_restrict(LevelArgs f, LevelArgs cl, const T *z, const T *b
             s += b[q] - off - diag * z[q];
         }
     bc[J * cl.zp + I] = T(0.25) * s;
-    zc[J * cl.zp + I] = 0;
+    put_z(cl, zc, I, J, T(0));
 }
 
 // FEM: residual into r, then full-weighting restriction (the transpose of
@@ -215,7 +251,7 @@ __global__ void k_fem_restrict(LevelArgs f, LevelArgs cl, const T *r, T *bc, T *
     const T s = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - p] + r[q + p])
                 + T(0.25) * (r[q - p - 1] + r[q - p + 1] + r[q + p - 1] + r[q + p + 1]);
     bc[J * cl.zp + I] = s;
-    zc[J * cl.zp + I] = 0;
+    put_z(cl, zc, I, J, T(0));
 }
 
 // MAC: bilinear cell-centred prolongation with constant extrapolation at
@@ -224,13 +260,8 @@ template <class T>
 __global__ void k_mac_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= f.nx || j >= f.ny) return;
-    const int I = i >> 1, J = j >> 1;
-    int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-    if (I2 < 0 || I2 >= cl.nx) I2 = I;
-    if (J2 < 0 || J2 >= cl.ny) J2 = J;
-    const int p = cl.zp;
-    z[j * f.zp + i] += T(0.5625) * e[J * p + I] + T(0.1875) * e[J * p + I2] + T(0.1875) * e[J2 * p + I]
-                       + T(0.0625) * e[J2 * p + I2];
+    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py};
+    put_z(f, z, i, j, z[j * f.zp + i] + prolong_at<T, kMAC>(pa, i, j));
 }
 
 // FEM: bilinear vertex prolongation, added to the fine iterate.
@@ -238,13 +269,8 @@ template <class T>
 __global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= f.nx || j >= f.ny) return;
-    const int I = i >> 1, J = j >> 1, p = cl.zp, q = J * p + I;
-    T v;
-    if (!(i & 1) && !(j & 1)) v = e[q];
-    else if (!(j & 1)) v = T(0.5) * (e[q] + e[q + 1]);
-    else if (!(i & 1)) v = T(0.5) * (e[q] + e[q + p]);
-    else v = T(0.25) * (e[q] + e[q + 1] + e[q + p] + e[q + p + 1]);
-    z[j * f.zp + i] += v;
+    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py};
+    put_z(f, z, i, j, z[j * f.zp + i] + prolong_at<T, kFEM>(pa, i, j));
 }
 
 // FEM residual and full-weighting restriction in one pass: each coarse
@@ -257,6 +283,9 @@ __global__ void k_fem_restrict_fused(LevelArgs f, LevelArgs cl, FemC k, const T 
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= cl.nx || J >= cl.ny) return;
     auto res = [&](int i, int j) -> T {
+        // children across a periodic seam are the opposite edge's nodes
+        if (f.px) i = wrap_idx(i, f.nx);
+        if (f.py) j = wrap_idx(j, f.ny);
         if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return T(0);
         const int q = j * f.zp + i, c = j * f.cp + i;
         T diag;
@@ -267,7 +296,7 @@ __global__ void k_fem_restrict_fused(LevelArgs f, LevelArgs cl, FemC k, const T 
     const T s = res(i, j) + T(0.5) * (res(i - 1, j) + res(i + 1, j) + res(i, j - 1) + res(i, j + 1))
                 + T(0.25) * (res(i - 1, j - 1) + res(i + 1, j - 1) + res(i - 1, j + 1) + res(i + 1, j + 1));
     bc[J * cl.zp + I] = s;
-    zc[J * cl.zp + I] = 0;
+    put_z(cl, zc, I, J, T(0));
 }
 
 // FEM transfers to and from a uniform coarse level that is not nested in
@@ -412,6 +441,7 @@ struct TailLev {
 struct TailArgs {
     int kind, nlev, nu1, nu2, total;
     int wave_from;         // first level run by the first wavefront alone (nlev: none)
+    int px, py;            // periodic directions (the LDS iterates keep mirrored rings)
     FemC k;
     TailLev lv[kTailMaxLev];
     void *z_top;
@@ -453,6 +483,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     __shared__ T dm[kTailDirect * kTailDirect];  // reduced system of the direct coarsest solve
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     auto arr = [&](int a, int l) -> T * { return pool + a * A.total + A.lv[l].off + A.lv[l].zp + 1; };
+    // a level's LDS arrays as a LevelArgs, for put_z (mirrored periodic rings)
+    auto lds = [&](int l) -> LevelArgs {
+        const TailLev &v = A.lv[l];
+        return LevelArgs{v.nx, v.ny, v.zp, v.zp, v.kx, v.ky, A.px, A.py};
+    };
     for (int p = tid; p < kTailArrays * A.total; p += nt) pool[p] = 0;
     __syncthreads();
     // coefficients with their zero ring (all levels as one flat range, two
@@ -490,7 +525,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             const int i = p % v.nx, j = p / v.nx;
             const T bb = gb[j * A.top_pitch + i], zz = gz[j * A.top_pitch + i];
             arr(1, 0)[j * v.zp + i] = bb;
-            arr(0, 0)[j * v.zp + i] = zz;
+            put_z(lds(0), arr(0, 0), i, j, zz);
         }
     }
     __syncthreads();
@@ -523,7 +558,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                     if (i >= v.nx || j >= v.ny) continue;
                     T diag;
                     const T off = offd(l, i, j, diag);
-                    if (diag != T(0)) arr(0, l)[j * v.zp + i] = (arr(1, l)[j * v.zp + i] - off) / diag;
+                    if (diag != T(0)) put_z(lds(l), arr(0, l), i, j, (arr(1, l)[j * v.zp + i] - off) / diag);
                 }
                 sync(w);
             }
@@ -547,12 +582,12 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 for (int dj = 0; dj < 2; dj++)
                     for (int di = 0; di < 2; di++) s += residual(l, 2 * I + di, 2 * J + dj);
                 arr(1, l + 1)[J * c.zp + I] = T(0.25) * s;
-                arr(0, l + 1)[J * c.zp + I] = 0;
+                put_z(lds(l + 1), arr(0, l + 1), I, J, T(0));
             }
         } else {
             for (int p = t0; p < f.nx * f.ny; p += st) {
                 const int i = p % f.nx, j = p / f.nx;
-                arr(3, l)[j * f.zp + i] = residual(l, i, j);
+                put_z(lds(l), arr(3, l), i, j, residual(l, i, j));
             }
             sync(w);
             const T *r = arr(3, l);
@@ -561,7 +596,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 const int I = p % c.nx, J = p / c.nx, q = 2 * J * pz + 2 * I;
                 arr(1, l + 1)[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
                                               + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
-                arr(0, l + 1)[J * c.zp + I] = 0;
+                put_z(lds(l + 1), arr(0, l + 1), I, J, T(0));
             }
         }
         sync(w);
@@ -576,8 +611,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             T add;
             if (A.kind == kMAC) {
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-                if (I2 < 0 || I2 >= c.nx) I2 = I;
-                if (J2 < 0 || J2 >= c.ny) J2 = J;
+                if (I2 < 0 || I2 >= c.nx) I2 = A.px ? wrap_idx(I2, c.nx) : I;
+                if (J2 < 0 || J2 >= c.ny) J2 = A.py ? wrap_idx(J2, c.ny) : J;
                 add = T(0.5625) * e[J * p2 + I] + T(0.1875) * e[J * p2 + I2] + T(0.1875) * e[J2 * p2 + I]
                       + T(0.0625) * e[J2 * p2 + I2];
             } else {
@@ -587,7 +622,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 else if (!(i & 1)) add = T(0.5) * (e[q] + e[q + p2]);
                 else add = T(0.25) * (e[q] + e[q + 1] + e[q + p2] + e[q + p2 + 1]);
             }
-            arr(0, l)[j * f.zp + i] += add;
+            put_z(lds(l), arr(0, l), i, j, arr(0, l)[j * f.zp + i] + add);
         }
         sync(w);
         smooth(l, A.nu2, true, w);
@@ -606,7 +641,9 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             T *row = dm + lane * kTailDirect;
             for (int c = 0; c < kTailDirect; c++) row[c] = 0;
             auto put = [&](int di, int dj, T a) {
-                const int ii = i + di, jj = j + dj;
+                int ii = i + di, jj = j + dj;
+                if (A.px) ii = wrap_idx(ii, v.nx);
+                if (A.py) jj = wrap_idx(jj, v.ny);
                 if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.ny) return;
                 const int pn = jj * v.nx + ii;
                 if (pn > 0) row[pn - 1] += a;
@@ -669,9 +706,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         }
         if (lane < n) {
             const int p = lane + 1;
-            arr(0, l)[(p / v.nx) * v.zp + p % v.nx] = x;
+            put_z(lds(l), arr(0, l), p % v.nx, p / v.nx, x);
         }
-        if (lane == 0) arr(0, l)[0] = 0;  // point 0 (a warm-started single-level tail holds an iterate there)
+        // point 0 (a warm-started single-level tail holds an iterate there)
+        if (lane == 0) put_z(lds(l), arr(0, l), 0, 0, T(0));
         wave_sync();
         return true;
     };
@@ -706,7 +744,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         double sz = 0;
         for (int p = t0; p < N; p += st) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
         const T mz = T(total(sz) / N);
-        for (int p = t0; p < N; p += st) arr(0, l)[(p / v.nx) * v.zp + p % v.nx] -= mz;
+        // (total() synchronises: every lane has read z before it changes)
+        for (int p = t0; p < N; p += st) {
+            const int i = p % v.nx, j = p / v.nx;
+            put_z(lds(l), arr(0, l), i, j, arr(0, l)[j * v.zp + i] - mz);
+        }
         sync(w);
     };
     const int last = A.nlev - 1, ws = A.wave_from <= last ? A.wave_from : last + 1;
@@ -723,36 +765,43 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     }
     for (int l = (ws < last ? ws : last) - 1; l >= 0; l--) up(l, false);
     {
+        // the top level's iterate back, with its periodic ring mirrors
         const TailLev &v = A.lv[0];
         T *gz = (T *)A.z_top;
+        const LevelArgs ga{v.nx, v.ny, A.top_pitch, A.top_pitch, v.kx, v.ky, A.px, A.py};
         for (int p = tid; p < v.nx * v.ny; p += nt) {
             const int i = p % v.nx, j = p / v.nx;
-            gz[j * A.top_pitch + i] = arr(0, 0)[j * v.zp + i];
+            put_z(ga, gz, i, j, arr(0, 0)[j * v.zp + i]);
         }
     }
 }
 
 // ------------------------------------------------------ coefficient setup
 // MAC fine faces from cell densities: 0.5/rho_c + 0.5/rho_nb
-// (fluid_2d.cc:1140-1160), zeroed on the walls.
+// (fluid_2d.cc:1140-1160), zeroed on the walls. Across a periodic seam the
+// two sides are the reference's left-edge and right-edge values: face 0
+// from cells 0 and -1, face nx from cells nx-1 and nx (ghost densities).
 __global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *fw, double *fs) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i > a.nx || j > a.ny) return;
     const int c = j * a.cp + i;
     if (j < a.ny) {
         const int ij = g.at(i, j);
-        fw[c] = (i == 0 || i == a.nx) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
+        fw[c] = (!a.px && (i == 0 || i == a.nx)) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
     }
     if (i < a.nx) {
         const int ij = g.at(i, j);
-        fs[c] = (j == 0 || j == a.ny) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - g.P];
+        fs[c] = (!a.py && (j == 0 || j == a.ny)) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - g.P];
     }
 }
-// FEM fine cell weights 1/rho (fluid_2d.cc:1163-1170).
+// FEM fine cell weights 1/rho (fluid_2d.cc:1163-1170). A periodic
+// direction has nx cells; the seam cell's weight is the ghost cell -1's,
+// as mgs_fem_varying_rho wraps its corner lookup (mgs_fem.hh:99-106).
 __global__ void k_fem_coef_fine(Grid g, const double *rho, LevelArgs a, double *kc) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.nx - 1 || j >= a.ny - 1) return;
-    kc[j * a.cp + i] = 1. / rho[g.at(i, j)];
+    if (i >= a.nx - (a.px ? 0 : 1) || j >= a.ny - (a.py ? 0 : 1)) return;
+    const int ci = a.px && i == a.nx - 1 ? -1 : i, cj = a.py && j == a.ny - 1 ? -1 : j;
+    kc[j * a.cp + i] = 1. / rho[g.at(ci, cj)];
 }
 __global__ void k_mac_coef_coarsen(LevelArgs f, LevelArgs c, const double *fwf, const double *fsf, double *fwc,
                                    double *fsc) {
@@ -763,7 +812,7 @@ __global__ void k_mac_coef_coarsen(LevelArgs f, LevelArgs c, const double *fwf, 
 }
 __global__ void k_fem_coef_coarsen(LevelArgs f, LevelArgs c, const double *kf, double *kcc) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= c.nx - 1 || J >= c.ny - 1) return;
+    if (I >= c.nx - (c.px ? 0 : 1) || J >= c.ny - (c.py ? 0 : 1)) return;
     const int q = 2 * J * f.cp + 2 * I;
     kcc[J * c.cp + I] = 0.25 * (kf[q] + kf[q + 1] + kf[q + f.cp] + kf[q + f.cp + 1]);
 }
@@ -844,7 +893,7 @@ __global__ void k_residual_f32(int kind, LevelArgs a, FemC k, const double *z, c
     if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
     else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
     bf[q] = float(b[q] - off - diag * z[q]);
-    zf[q] = 0.f;
+    put_z(a, zf, i, j, 0.0f);
 }
 // Starting iterate from the two previous solutions: z <- 2 z - zp, zp <- z.
 __global__ void k_extrapolate(LevelArgs a, double *z, double *zp) {
@@ -852,7 +901,7 @@ __global__ void k_extrapolate(LevelArgs a, double *z, double *zp) {
     if (i >= a.nx || j >= a.ny) return;
     const int q = j * a.zp + i;
     const double t = z[q];
-    z[q] = 2 * t - zp[q];
+    put_z(a, z, i, j, 2 * t - zp[q]);
     zp[q] = t;
 }
 __global__ void k_copy_level(LevelArgs a, const double *z, double *zp) {
@@ -864,7 +913,26 @@ __global__ void k_add_f32(LevelArgs a, double *z, const float *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
     const int q = j * a.zp + i;
-    z[q] += double(e[q]);
+    put_z(a, z, i, j, z[q] + double(e[q]));
+}
+
+// Periodic ghost ring of a level array from its edges (one thread per ring
+// point; corners from the opposite corner).
+template <class T>
+__global__ void k_wrap_ring(LevelArgs a, T *z) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = a.nx + 2, H = a.ny + 2, n = 2 * W + 2 * H;
+    if (t >= n) return;
+    int i, j;
+    if (t < W) { i = t - 1; j = -1; }
+    else if (t < 2 * W) { i = t - W - 1; j = a.ny; }
+    else if (t < 2 * W + H) { i = -1; j = t - 2 * W - 1; }
+    else { i = a.nx; j = t - 2 * W - H - 1; }
+    int si = i, sj = j;
+    if (a.px) si = wrap_idx(i, a.nx);
+    if (a.py) sj = wrap_idx(j, a.ny);
+    if (si < 0 || si >= a.nx || sj < 0 || sj >= a.ny) return;  // a wall side keeps its ring
+    z[j * a.zp + i] = z[sj * a.zp + si];
 }
 
 FemC fem_consts(double dx, double dy) {
@@ -879,7 +947,7 @@ FemC fem_consts(double dx, double dy) {
     return k;
 }
 
-LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky}; }
+LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky, l.px, l.py}; }
 RsArgs rs_args(const Resample &r) { return RsArgs{r.i0, r.lo, r.hi, r.t}; }
 
 }  // namespace
@@ -905,6 +973,7 @@ static TailArgs tail_args(const std::vector<MgLevel> &L, size_t t, MgKind kind, 
                           size_t *shmem) {
     TailArgs A{};
     A.kind = kind; A.nlev = (int)(L.size() - t); A.nu1 = nu1; A.nu2 = nu2; A.k = k;
+    A.px = L[t].px; A.py = L[t].py;
     int off = 0;
     A.wave_from = A.nlev;
     for (int q = 0; q < A.nlev; q++) {
@@ -979,13 +1048,17 @@ int Multigrid::make_resample(int nf, double hf, int nc, Resample &r) {
     return 0;
 }
 
-int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s) {
+int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px, bool py) {
     kind_ = kind; s_ = s; dx_ = dx; dy_ = dy;
     int nx = m, ny = n;
     double kx = 1. / (dx * dx), ky = 1. / (dy * dy), hx = dx, hy = dy;
+    // a periodic direction is coloured across its seam: it needs an even
+    // size on every level that is smoothed
+    if ((px && nx % 2) || (py && ny % 2)) return INCRMT_ERR_GRID;
     while (true) {
         MgLevel l{};
         l.nx = nx; l.ny = ny;
+        l.px = px; l.py = py;
         l.pitch = round_up(nx + 2, 32);
         l.cp = l.pitch;
         l.kx = kx; l.ky = ky;
@@ -1018,7 +1091,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         }
         l.hx = hx; l.hy = hy;
         L.push_back(l);
-        if (kind == kFEM && L.size() == 1 && nx >= 5 && ny >= 5 && (nx % 2 == 0 || ny % 2 == 0)) {
+        if (kind == kFEM && !px && !py && L.size() == 1 && nx >= 5 && ny >= 5 && (nx % 2 == 0 || ny % 2 == 0)) {
             // node counts that vertex coarsening cannot halve: the next level
             // is the largest uniform 2^k+1 grid on the same box
             auto pow2p1 = [](int n) { int c = 2; while (2 * (c - 1) + 1 <= n - 1) c = 2 * (c - 1) + 1; return c; };
@@ -1029,12 +1102,18 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
             nx = cx; ny = cy;
             continue;
         }
+        // MAC halves even cell counts; FEM halves 2^k+1 node counts, or even
+        // node counts along a periodic direction (nodes 2I are kept)
+        auto fem_more = [](int k, bool p) { return p ? k % 2 == 0 && k >= 4 : k % 2 == 1 && k >= 5; };
         bool more = kind == kMAC ? (nx % 2 == 0 && ny % 2 == 0 && nx >= 4 && ny >= 4)
-                                 : (nx % 2 == 1 && ny % 2 == 1 && nx >= 5 && ny >= 5);
+                                 : (fem_more(nx, px) && fem_more(ny, py));
         if (!more) break;
         if (kind == kMAC) { nx /= 2; ny /= 2; kx *= 0.25; ky *= 0.25; }
-        else { nx = (nx + 1) / 2; ny = (ny + 1) / 2; hx *= 2; hy *= 2; }
+        else { nx = px ? nx / 2 : (nx + 1) / 2; ny = py ? ny / 2 : (ny + 1) / 2; hx *= 2; hy *= 2; }
     }
+    // a periodic coarsest level is solved exactly (dense elimination in the
+    // tail cycle), so it must be tiny
+    if ((px || py) && (long long)L.back().nx * L.back().ny > 16) return INCRMT_ERR_GRID;
     void *q;
     HIP_CHECK(hipMalloc(&q, (3 * kRedBlocks + 8) * sizeof(double)));
     allocs.push_back(q);
@@ -1070,6 +1149,7 @@ int Multigrid::coarsen_coefs() {
             k_fem_coef_coarsen<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc);
         }
         KCHECK();
+        if (int rc = wrap_coefs(l + 1)) return rc;
     }
     f32_stale_ = true;  // the single-precision copies are made when a mixed solve needs them
     return 0;
@@ -1098,7 +1178,27 @@ int Multigrid::set_density(const Grid &g, const double *rho) {
     else
         k_fem_coef_fine<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.kc);
     KCHECK();
+    if (int rc = wrap_coefs(0)) return rc;
     return coarsen_coefs();
+}
+
+// FEM cell weights of a periodic level: ring cells mirror the last cells
+int Multigrid::wrap_coefs(size_t l) {
+    const MgLevel &v = L[l];
+    if (kind_ != kFEM || !(v.px || v.py)) return 0;
+    const int n = 2 * (v.nx + 2) + 2 * (v.ny + 2);
+    k_wrap_ring<double><<<(n + 255) / 256, 256, 0, s_>>>(args_of(v), v.kc);
+    KCHECK();
+    return 0;
+}
+
+int Multigrid::wrap_fine(double *z) {
+    const MgLevel &v = L[0];
+    if (!(v.px || v.py)) return 0;
+    const int n = 2 * (v.nx + 2) + 2 * (v.ny + 2);
+    k_wrap_ring<double><<<(n + 255) / 256, 256, 0, s_>>>(args_of(v), z);
+    KCHECK();
+    return 0;
 }
 
 int Multigrid::set_irho_host(const double *irho) {
@@ -1106,20 +1206,30 @@ int Multigrid::set_irho_host(const double *irho) {
     const int nx = f.nx, ny = f.ny;
     std::vector<double> h((size_t)(ny + 2) * f.pitch, 0.0);
     auto at = [&](int i, int j) -> double & { return h[(size_t)(j + 1) * f.pitch + i + 1]; };
+    const bool px = f.px, py = f.py;
     if (kind_ == kMAC) {
-        // fw from the left-edge entries, walls zero (mgs_mac.hh:64-102)
-        for (int j = 0; j < ny; j++)
-            for (int i = 1; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i)] : 1.0;
+        // fw from the left-edge entries, walls zero (mgs_mac.hh:64-102); a
+        // periodic seam's right side is the right-edge entry irhor
+        const long long mn = (long long)nx * ny;
+        for (int j = 0; j < ny; j++) {
+            for (int i = px ? 0 : 1; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i)] : 1.0;
+            if (px) at(nx, j) = irho ? irho[2 * mn + j] : 1.0;
+        }
         HIP_CHECK(hipMemcpy(f.fw - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         std::fill(h.begin(), h.end(), 0.0);
-        for (int j = 1; j < ny; j++)
+        for (int j = py ? 0 : 1; j < ny; j++)
             for (int i = 0; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i) + 1] : 1.0;
+        if (py)
+            for (int i = 0; i < nx; i++) at(i, ny) = irho ? irho[2 * mn + ny + i] : 1.0;
         HIP_CHECK(hipMemcpy(f.fs - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
-        // cell (i,j) weight = irho of its upper-right node (mgs_fem.hh:99-106)
-        for (int j = 0; j < ny - 1; j++)
-            for (int i = 0; i < nx - 1; i++) at(i, j) = irho ? irho[(j + 1) * nx + i + 1] : 1.0;
+        // cell (i,j) weight = irho of its upper-right node (mgs_fem.hh:99-106),
+        // node nx being node 0 along a periodic direction
+        for (int j = 0; j < ny - (py ? 0 : 1); j++)
+            for (int i = 0; i < nx - (px ? 0 : 1); i++)
+                at(i, j) = irho ? irho[(size_t)((j + 1) % ny) * nx + (i + 1) % nx] : 1.0;
         HIP_CHECK(hipMemcpy(f.kc - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (int rc = wrap_coefs(0)) return rc;
     }
     return coarsen_coefs();
 }
@@ -1292,6 +1402,7 @@ int Multigrid::smooth_check(int sweeps, int shape) {
     MgLevel &v = L[0];
     const LevelArgs a = args_of(v);
     const FemC k = fem_consts(v.hx, v.hy);
+    if (int rc = wrap_fine(v.z)) return rc;
     if (shape < 0) {  // one launch per colour: the reference form of the sweep
         const int ncol = kind_ == kMAC ? 2 : 4;
         for (int sw = 0; sw < sweeps; sw++)
@@ -1319,7 +1430,7 @@ int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
     if (timed && record_begin()) return 6;
     int nl = 0;
     ProArgs pa{nullptr, 0, 0, 0};
-    if (correct) pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny};
+    if (correct) pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py};
     MgLevel &w = L[l];
     int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), &w.z, &w.z2, v.b,
                                    kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl, pa);
@@ -1340,7 +1451,7 @@ int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
     if (timed && record_begin()) return 6;
     int nl = 0;
     ProArgs pa{nullptr, 0, 0, 0};
-    if (correct) pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny};
+    if (correct) pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py};
     MgLevel &w = L[l];
     int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), &w.zf, &w.zf2, v.bf,
                                   kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl, pa);
@@ -1355,6 +1466,7 @@ int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
 
 int Multigrid::apply(const double *x, double *y) {
     const MgLevel &f = L[0];
+    if (int rc = wrap_fine(const_cast<double *>(x))) return rc;
     k_apply<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), fem_consts(dx_, dy_), x, f.fw, f.fs, f.kc, y);
     KCHECK();
     return 0;
@@ -1513,6 +1625,7 @@ template <class Cycle>
 int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) {
     const MgLevel &f = L[0];
     int rc;
+    if ((rc = wrap_fine(f.z))) return rc;  // the iterate may come from outside
     if (extrapolate) {
         // start from the linear extrapolation in time of the last two
         // solutions (the system is solved once per step): the start is

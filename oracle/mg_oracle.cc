@@ -22,13 +22,16 @@ namespace {
 struct level {
     int nx, ny;
     bool cell;               // cell-centred (MAC) or vertex (FEM)
+    bool px = false, py = false;  // periodic directions: neighbours wrap
     std::vector<double> c[9];
     std::vector<double> z, b, r;
     // coefficients that define the operator and are averaged to coarsen
     std::vector<double> fw, fs;  // MAC: face inverse densities (nx+1)*ny, nx*(ny+1)
-    std::vector<double> kc;      // FEM: cell inverse densities (nx-1)*(ny-1)
+    std::vector<double> kc;      // FEM: cell inverse densities cx()*cy()
     double kx, ky;               // MAC scalings 1/dx^2, 1/dy^2
     double dxdy, dydx;           // FEM ratios
+    int cx() const { return px ? nx : nx - 1; }  // FEM cells per row
+    int cy() const { return py ? ny : ny - 1; }
     void alloc() {
         const size_t N = (size_t)nx * ny;
         for (auto &v : c) v.assign(N, 0.0);
@@ -43,11 +46,13 @@ void mac_stencil(level &L) {
             const int p = j * nx + i;
             const double el = L.fw[j * (nx + 1) + i], er = L.fw[j * (nx + 1) + i + 1];
             const double ed = L.fs[j * nx + i], eu = L.fs[(j + 1) * nx + i];
-            const double cl = i > 0 ? L.kx * el : 0, cr = i < nx - 1 ? L.kx * er : 0;
-            const double dc = j > 0 ? L.ky * ed : 0, uc = j < ny - 1 ? L.ky * eu : 0;
+            // a periodic direction keeps its seam faces: face 0 on the left
+            // of cell 0, face nx (the right-edge entry) right of cell nx-1
+            const bool hl = L.px || i > 0, hr = L.px || i < nx - 1, hd = L.py || j > 0, hu = L.py || j < ny - 1;
+            const double cl = hl ? L.kx * el : 0, cr = hr ? L.kx * er : 0;
+            const double dc = hd ? L.ky * ed : 0, uc = hu ? L.ky * eu : 0;
             L.c[3][p] = cl; L.c[5][p] = cr; L.c[1][p] = dc; L.c[7][p] = uc;
-            L.c[4][p] = -L.kx * ((i > 0 ? el : 0) + (i < nx - 1 ? er : 0))
-                        - L.ky * ((j > 0 ? ed : 0) + (j < ny - 1 ? eu : 0));
+            L.c[4][p] = -L.kx * ((hl ? el : 0) + (hr ? er : 0)) - L.ky * ((hd ? ed : 0) + (hu ? eu : 0));
         }
 }
 
@@ -56,8 +61,11 @@ void fem_stencil(level &L) {
     const double fm = 1. / 3. * (L.dxdy + L.dydx), fey = 1. / 3. * (-2 * L.dxdy + L.dydx);
     const double hey = 0.5 * fey, fex = 1. / 3. * (-2 * L.dydx + L.dxdy), hex = 0.5 * fex;
     const double fc = -1. / 6. * (L.dxdy + L.dydx);
+    const int cx = L.cx(), cy = L.cy();
     auto K = [&](int ci, int cj) -> double {
-        return (ci >= 0 && ci < nx - 1 && cj >= 0 && cj < ny - 1) ? L.kc[cj * (nx - 1) + ci] : 0.0;
+        if (L.px) ci = (ci + nx) % nx;
+        if (L.py) cj = (cj + ny) % ny;
+        return (ci >= 0 && ci < cx && cj >= 0 && cj < cy) ? L.kc[cj * cx + ci] : 0.0;
     };
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
@@ -74,10 +82,12 @@ double stencil_at(const level &L, const std::vector<double> &x, int i, int j, bo
     const int nx = L.nx, ny = L.ny, p = j * nx + i;
     double s = 0;
     for (int dj = -1; dj <= 1; dj++) {
-        const int jj = j + dj;
+        int jj = j + dj;
+        if (L.py) jj = (jj + ny) % ny;
         if (jj < 0 || jj >= ny) continue;
         for (int di = -1; di <= 1; di++) {
-            const int ii = i + di;
+            int ii = i + di;
+            if (L.px) ii = (ii + nx) % nx;
             if (ii < 0 || ii >= nx) continue;
             if (offdiag_only && di == 0 && dj == 0) continue;
             s += L.c[(dj + 1) * 3 + di + 1][p] * x[jj * nx + ii];
@@ -116,6 +126,17 @@ void smooth(level &L, int sweeps, bool reverse) {
         }
 }
 
+// Lexicographic Gauss-Seidel, serial: the coarsest level of a periodic
+// direction may have an odd size, where the colours are not independent.
+void smooth_serial(level &L) {
+    for (int j = 0; j < L.ny; j++)
+        for (int i = 0; i < L.nx; i++) {
+            const int p = j * L.nx + i;
+            const double d = L.c[4][p];
+            if (d != 0) L.z[p] = (L.b[p] - stencil_at(L, L.z, i, j, true)) / d;
+        }
+}
+
 double mean(const std::vector<double> &v) {
     double s = 0;
     for (double x : v) s += x;
@@ -134,10 +155,12 @@ void restrict_to(const level &F, level &C) {
             for (int I = 0; I < C.nx; I++) {
                 double s = 0;
                 for (int dj = -1; dj <= 1; dj++) {
-                    const int j = 2 * J + dj;
+                    int j = 2 * J + dj;
+                    if (F.py) j = (j + F.ny) % F.ny;
                     if (j < 0 || j >= F.ny) continue;
                     for (int di = -1; di <= 1; di++) {
-                        const int i = 2 * I + di;
+                        int i = 2 * I + di;
+                        if (F.px) i = (i + F.nx) % F.nx;
                         if (i < 0 || i >= F.nx) continue;
                         s += (di ? 0.5 : 1.0) * (dj ? 0.5 : 1.0) * F.r[j * F.nx + i];
                     }
@@ -153,8 +176,8 @@ void prolong_add(level &F, const level &C) {
             for (int i = 0; i < F.nx; i++) {
                 const int I = i / 2, J = j / 2;
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-                if (I2 < 0 || I2 >= C.nx) I2 = I;
-                if (J2 < 0 || J2 >= C.ny) J2 = J;
+                if (I2 < 0 || I2 >= C.nx) I2 = F.px ? (I2 + C.nx) % C.nx : I;
+                if (J2 < 0 || J2 >= C.ny) J2 = F.py ? (J2 + C.ny) % C.ny : J;
                 F.z[j * F.nx + i] += 0.5625 * C.z[J * C.nx + I] + 0.1875 * C.z[J * C.nx + I2]
                                      + 0.1875 * C.z[J2 * C.nx + I] + 0.0625 * C.z[J2 * C.nx + I2];
             }
@@ -164,7 +187,7 @@ void prolong_add(level &F, const level &C) {
                 const int I = i / 2, J = j / 2;
                 const bool oi = i & 1, oj = j & 1;
                 double v;
-                auto e = [&](int a, int b) { return C.z[b * C.nx + a]; };
+                auto e = [&](int a, int b) { return C.z[(b % C.ny) * C.nx + a % C.nx]; };
                 if (!oi && !oj) v = e(I, J);
                 else if (oi && !oj) v = 0.5 * (e(I, J) + e(I + 1, J));
                 else if (!oi && oj) v = 0.5 * (e(I, J) + e(I, J + 1));
@@ -181,6 +204,7 @@ struct hierarchy {
             const level &F = L.back();
             level C;
             C.cell = F.cell;
+            C.px = F.px; C.py = F.py;
             if (F.cell) {
                 if ((F.nx & 1) || (F.ny & 1) || F.nx < 4 || F.ny < 4) break;
                 C.nx = F.nx / 2; C.ny = F.ny / 2;
@@ -196,14 +220,17 @@ struct hierarchy {
                 C.alloc();
                 mac_stencil(C);
             } else {
-                if (!(F.nx & 1) || !(F.ny & 1) || F.nx < 5 || F.ny < 5) break;
-                C.nx = (F.nx + 1) / 2; C.ny = (F.ny + 1) / 2;
+                // 2^k+1 node counts keep every other node; a periodic
+                // direction halves an even node count
+                auto more = [](int k, bool p) { return p ? (k % 2 == 0 && k >= 4) : (k % 2 == 1 && k >= 5); };
+                if (!more(F.nx, F.px) || !more(F.ny, F.py)) break;
+                C.nx = F.px ? F.nx / 2 : (F.nx + 1) / 2; C.ny = F.py ? F.ny / 2 : (F.ny + 1) / 2;
                 C.dxdy = F.dxdy; C.dydx = F.dydx;
-                C.kc.resize((C.nx - 1) * (C.ny - 1));
-                for (int J = 0; J < C.ny - 1; J++)
-                    for (int I = 0; I < C.nx - 1; I++) {
-                        const int w = F.nx - 1, p = 2 * J * w + 2 * I;
-                        C.kc[J * (C.nx - 1) + I] = 0.25 * (F.kc[p] + F.kc[p + 1] + F.kc[p + w] + F.kc[p + w + 1]);
+                C.kc.resize((size_t)C.cx() * C.cy());
+                for (int J = 0; J < C.cy(); J++)
+                    for (int I = 0; I < C.cx(); I++) {
+                        const int w = F.cx(), p = 2 * J * w + 2 * I;
+                        C.kc[J * C.cx() + I] = 0.25 * (F.kc[p] + F.kc[p + 1] + F.kc[p + w] + F.kc[p + w + 1]);
                     }
                 C.alloc();
                 fem_stencil(C);
@@ -219,7 +246,8 @@ struct hierarchy {
         for (double v : C.b) bn += v * v;
         if (bn == 0) return;
         for (int it = 0; it < 4000; it++) {
-            smooth(C, 1, false);
+            if (C.px || C.py) smooth_serial(C);
+            else smooth(C, 1, false);
             if ((it & 15) == 15) {
                 residual(C);
                 double rn = 0;
@@ -274,6 +302,7 @@ struct hierarchy {
 void fill_mac_fine(const mac_system &s, level &F) {
     const int m = s.m, n = s.n, mn = m * n;
     F.cell = true; F.nx = m; F.ny = n; F.kx = s.xxsp; F.ky = s.yysp;
+    F.px = s.px; F.py = s.py;
     F.fw.assign((m + 1) * n, 0.0); F.fs.assign(m * (n + 1), 0.0);
     // reference irho layout: [2ij] left edge, [2ij+1] down edge, then the
     // right column (n) and top row (m) (fluid_2d.cc:1140-1160)
@@ -292,12 +321,15 @@ void fill_mac_fine(const mac_system &s, level &F) {
 void fill_fem_fine(const fem_system &s, level &F) {
     const int m = s.m, n = s.n;
     F.cell = false; F.nx = m; F.ny = n; F.dxdy = s.dx / s.dy; F.dydx = s.dy / s.dx;
-    F.kc.assign((m - 1) * (n - 1), 1.0);
+    F.px = s.px; F.py = s.py;
+    const int cx = F.cx(), cy = F.cy();
+    F.kc.assign((size_t)cx * cy, 1.0);
     // reference corner layout: irho[j*m+i] = 1/rho of the cell below-left of
-    // node (i,j) (fluid_2d.cc:1163-1170)
+    // node (i,j) (fluid_2d.cc:1163-1170); across a periodic seam node m is
+    // node 0 (mgs_fem.hh:99-106)
     if (s.irho)
-        for (int cj = 0; cj < n - 1; cj++)
-            for (int ci = 0; ci < m - 1; ci++) F.kc[cj * (m - 1) + ci] = s.irho[(cj + 1) * m + ci + 1];
+        for (int cj = 0; cj < cy; cj++)
+            for (int ci = 0; ci < cx; ci++) F.kc[cj * cx + ci] = s.irho[((cj + 1) % n) * m + (ci + 1) % m];
     F.alloc();
     fem_stencil(F);
 }

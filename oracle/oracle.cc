@@ -60,26 +60,27 @@ double orc_bicubic(int m, int n, double ax, double ay, double dx, double dy,
 
 extern "C" {
 
-// Projection operators and solves (mgs_mac.hh:64-102, mgs_fem.hh:86-136).
-// irho may be null for the constant-density systems.
+// Projection operators and solves (mgs_mac.hh:64-102, mgs_fem.hh:86-136),
+// periodic in x and/or y when px / py are set (mgs_mac.hh:45-56,
+// mgs_fem.hh:47-52). irho may be null for the constant-density systems.
 void orc_mac_apply(int m, int n, double xxsp, double yysp, const double *irho,
-                   const double *x, double *y) {
-    mac_system s{m, n, xxsp, yysp, irho};
+                   const double *x, double *y, int px, int py) {
+    mac_system s{m, n, xxsp, yysp, irho, px, py};
     mac_apply(s, x, y);
 }
 void orc_fem_apply(int m, int n, double dx, double dy, const double *irho,
-                   const double *x, double *y) {
-    fem_system s{m, n, dx, dy, irho};
+                   const double *x, double *y, int px, int py) {
+    fem_system s{m, n, dx, dy, irho, px, py};
     fem_apply(s, x, y);
 }
 int orc_mac_solve(int m, int n, double xxsp, double yysp, const double *irho,
-                  const double *b, double *z, double tol, double *relres) {
-    mac_system s{m, n, xxsp, yysp, irho};
+                  const double *b, double *z, double tol, double *relres, int px, int py) {
+    mac_system s{m, n, xxsp, yysp, irho, px, py};
     return mac_solve(s, b, z, tol, relres);
 }
 int orc_fem_solve(int m, int n, double dx, double dy, const double *irho,
-                  const double *b, double *z, double tol, double *relres) {
-    fem_system s{m, n, dx, dy, irho};
+                  const double *b, double *z, double tol, double *relres, int px, int py) {
+    fem_system s{m, n, dx, dy, irho, px, py};
     return fem_solve(s, b, z, tol, relres);
 }
 

@@ -59,11 +59,13 @@ struct mac_system {
     int m, n;
     double xxsp, yysp;
     const double *irho;  // 2mn+n+m edge inverse densities, or null (const rho)
+    int px = 0, py = 0;  // periodic directions (mgs_mac.hh:45-56)
 };
 struct fem_system {
-    int m, n;            // nodes
+    int m, n;            // nodes (a periodic direction: m nodes, m cells)
     double dx, dy;
     const double *irho;  // m*n (+m padding row) corner inverse densities, or null
+    int px = 0, py = 0;  // periodic directions (mgs_fem.hh:47-52)
 };
 // y = A x (full operator, diagonal included)
 void mac_apply(const mac_system &s, const double *x, double *y);

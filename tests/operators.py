@@ -18,24 +18,36 @@ import numpy as np
 import scipy.sparse as sp
 
 
-def mac_matrix(m, n, dx, dy, irho=None):
+def mac_matrix(m, n, dx, dy, irho=None, px=False, py=False):
+    """Row c gets k_f (q_nb - q_c) / h^2 for each of its faces. Along a
+    periodic direction the wall faces become the seam, whose neighbour is
+    the opposite edge's cell; the right side of the seam uses the
+    right-edge coefficient irho[2mn + j] and the top side irho[2mn + n + i]
+    (mgs_mac.hh:81-96), so the matrix is symmetric only when those agree
+    with the left and bottom entries."""
     rows, cols, vals = [], [], []
     kx, ky = 1.0 / dx**2, 1.0 / dy**2
+    mn = m * n
 
-    def add(a, b, k):
-        rows.extend([a, a, b, b])
-        cols.extend([b, a, a, b])
-        vals.extend([k, -k, k, -k])
+    def coef(e):
+        return irho[e] if irho is not None else 1.0
+
+    def face(c, nb, k):
+        rows.extend([c, c])
+        cols.extend([nb, c])
+        vals.extend([k, -k])
 
     for j in range(n):
         for i in range(m):
             c = j * m + i
-            if i > 0:
-                k = irho[2 * c] if irho is not None else 1.0
-                add(c, c - 1, kx * k)
-            if j > 0:
-                k = irho[2 * c + 1] if irho is not None else 1.0
-                add(c, c - m, ky * k)
+            if i > 0 or px:
+                face(c, j * m + (i - 1) % m, kx * coef(2 * c))
+            if i < m - 1 or px:
+                face(c, j * m + (i + 1) % m, kx * coef(2 * (c + 1) if i < m - 1 else 2 * mn + j))
+            if j > 0 or py:
+                face(c, ((j - 1) % n) * m + i, ky * coef(2 * c + 1))
+            if j < n - 1 or py:
+                face(c, ((j + 1) % n) * m + i, ky * coef(2 * (c + m) + 1 if j < n - 1 else 2 * mn + n + i))
     return sp.csr_matrix((vals, (rows, cols)), shape=(m * n, m * n))
 
 
@@ -58,13 +70,17 @@ def bilinear_element(dx, dy):
     return K
 
 
-def fem_matrix(m, n, dx, dy, irho=None):
+def fem_matrix(m, n, dx, dy, irho=None, px=False, py=False):
+    """m x n nodes; along a periodic direction there are m cells, the last
+    one joining node m-1 to node 0 (mgs_fem.hh:99-106 wraps the corner
+    lookup the same way)."""
     Ke = bilinear_element(dx, dy)
     rows, cols, vals = [], [], []
-    for cj in range(n - 1):
-        for ci in range(m - 1):
-            k = irho[(cj + 1) * m + ci + 1] if irho is not None else 1.0
-            nodes = [cj * m + ci, cj * m + ci + 1, (cj + 1) * m + ci, (cj + 1) * m + ci + 1]
+    for cj in range(n if py else n - 1):
+        for ci in range(m if px else m - 1):
+            i1, j1 = (ci + 1) % m, (cj + 1) % n
+            k = irho[j1 * m + i1] if irho is not None else 1.0
+            nodes = [cj * m + ci, cj * m + i1, j1 * m + ci, j1 * m + i1]
             for p in range(4):
                 for q in range(4):
                     rows.append(nodes[p])
@@ -135,3 +151,22 @@ def blob_density(m, n, ratio=3.0, seed=None):
         rng = np.random.default_rng(seed)
         R = R * (1 + 0.2 * np.sin(2 * np.pi * (X * rng.uniform(1, 3) + Y * rng.uniform(1, 3))))
     return R
+
+
+def periodic_density(m, n, px, py, ratio=3.0, seed=1):
+    """A smooth cell density with one ghost layer that is periodic along the
+    periodic directions (ghost cells equal the opposite edge), so the MAC
+    seam faces agree on both sides."""
+    R = blob_density(m, n, ratio=ratio, seed=seed)
+    if px:
+        R[:, 0], R[:, m + 1] = R[:, m], R[:, 1]
+    if py:
+        R[0, :], R[n + 1, :] = R[n, :], R[1, :]
+    return R
+
+
+def fem_irho_periodic(R, m, n, px, py):
+    """Corner inverse densities of an m x n node grid with m (n) cells along
+    a periodic direction: node (i,j) holds 1/rho of cell (i-1,j-1), the
+    ghost cell -1 standing for the seam cell (fluid_2d.cc:1163-1170)."""
+    return (1.0 / R[0:n, 0:m]).ravel().copy()

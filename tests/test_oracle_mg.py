@@ -20,10 +20,10 @@ D, I = ctypes.c_double, ctypes.c_int
 
 def _lib():
     L = ol.oracle()
-    L.orc_mac_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp]
-    L.orc_fem_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp]
-    L.orc_mac_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp]
-    L.orc_fem_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp]
+    L.orc_mac_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I]
+    L.orc_fem_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I]
+    L.orc_mac_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I]
+    L.orc_fem_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I]
     return L
 
 
@@ -31,29 +31,31 @@ def _ptr(a):
     return None if a is None else ol.dptr(a)
 
 
-def mac_apply(m, n, dx, dy, irho, x):
+def mac_apply(m, n, dx, dy, irho, x, px=False, py=False):
     y = np.zeros(m * n)
-    _lib().orc_mac_apply(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(x), ol.dptr(y))
+    _lib().orc_mac_apply(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py))
     return y
 
 
-def fem_apply(m, n, dx, dy, irho, x):
+def fem_apply(m, n, dx, dy, irho, x, px=False, py=False):
     y = np.zeros(m * n)
-    _lib().orc_fem_apply(m, n, dx, dy, _ptr(irho), ol.dptr(x), ol.dptr(y))
+    _lib().orc_fem_apply(m, n, dx, dy, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py))
     return y
 
 
-def mac_solve(m, n, dx, dy, irho, b, tol=1e-13):
+def mac_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False):
     z = np.zeros(m * n)
     r = D(0)
-    c = _lib().orc_mac_solve(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r))
+    c = _lib().orc_mac_solve(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r),
+                             int(px), int(py))
     return z, c, r.value
 
 
-def fem_solve(m, n, dx, dy, irho, b, tol=1e-13):
+def fem_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False):
     z = np.zeros(m * n)
     r = D(0)
-    c = _lib().orc_fem_solve(m, n, dx, dy, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r))
+    c = _lib().orc_fem_solve(m, n, dx, dy, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r),
+                             int(px), int(py))
     return z, c, r.value
 
 
@@ -147,3 +149,70 @@ def test_manufactured_solution_second_order(kind):
     err = [(_mms_error_mac if kind == "mac" else _mms_error_fem)(m) for m in (16, 32, 64)]
     rates = [np.log2(err[k] / err[k + 1]) for k in range(2)]
     assert all(1.8 < r < 2.3 for r in rates), (err, rates)
+
+
+PERIODS = [(True, False), (False, True), (True, True)]
+
+
+@pytest.mark.parametrize("px,py", PERIODS)
+@pytest.mark.parametrize("m,n", [(16, 12), (24, 8)])
+@pytest.mark.parametrize("varying", [False, True])
+def test_periodic_operators_match_definitions(px, py, m, n, varying):
+    """Periodic MAC faces wrap across the seam (its right side the
+    right-edge irho entries); periodic FEM directions have m nodes and m
+    cells, the seam cell joining node m-1 to node 0."""
+    rng = np.random.default_rng(7)
+    dx, dy = 2.0 / m, 1.5 / n
+    R = ops.blob_density(m, n, seed=3)  # not periodic: the seam faces differ
+    irho = ops.mac_irho_from_rho(R, m, n) if varying else None
+    A = ops.mac_matrix(m, n, dx, dy, irho, px, py)
+    x = rng.standard_normal(m * n)
+    assert np.abs(mac_apply(m, n, dx, dy, irho, x, px, py) - A @ x).max() <= 1e-12 * np.abs(A @ x).max()
+    mf, nf = (m if px else m + 1), (n if py else n + 1)
+    hx, hy = 2.0 / m, 1.5 / n
+    irf = ops.fem_irho_periodic(R, mf, nf, px, py) if varying else None
+    F = ops.fem_matrix(mf, nf, hx, hy, irf, px, py)
+    x = rng.standard_normal(mf * nf)
+    assert np.abs(fem_apply(mf, nf, hx, hy, irf, x, px, py) - F @ x).max() <= 1e-12 * np.abs(F @ x).max()
+
+
+@pytest.mark.parametrize("px,py", PERIODS)
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+def test_periodic_solves_match_direct(px, py, kind):
+    m, n = 32, 16
+    R = ops.periodic_density(m, n, px, py, seed=5)
+    rng = np.random.default_rng(8)
+    if kind == "mac":
+        dx, dy = 2.0 / m, 1.5 / n
+        irho = ops.mac_irho_from_rho(R, m, n)
+        A = ops.mac_matrix(m, n, dx, dy, irho, px, py)
+        b = rng.standard_normal(m * n)
+        z, cyc, rr = mac_solve(m, n, dx, dy, irho, b - b.mean(), 1e-12, px, py)
+    else:
+        mf, nf = (m if px else m + 1), (n if py else n + 1)
+        dx, dy = 2.0 / m, 1.5 / n
+        irho = ops.fem_irho_periodic(R, mf, nf, px, py)
+        A = ops.fem_matrix(mf, nf, dx, dy, irho, px, py)
+        b = rng.standard_normal(mf * nf)
+        z, cyc, rr = fem_solve(mf, nf, dx, dy, irho, b - b.mean(), 1e-12, px, py)
+    assert cyc > 0 and rr <= 1e-12
+    zd = ops.neumann_solve(A, b)
+    assert np.abs((z - z.mean()) - zd).max() <= 1e-9 * np.abs(zd).max()
+
+
+def test_periodic_poisson_second_order():
+    """Doubly periodic MAC Poisson problem with the exact solution
+    sin(pi x) cos(2 pi y) on [-1,1]^2: the error falls by about 4 per
+    halving of h."""
+    errs = []
+    for m in (16, 32, 64):
+        h = 2.0 / m
+        x = -1 + (np.arange(m) + 0.5) * h
+        X, Y = np.meshgrid(x, x)
+        ue = np.sin(np.pi * X) * np.cos(2 * np.pi * Y)
+        f = -5 * np.pi**2 * ue
+        z, _, _ = mac_solve(m, m, h, h, None, f.ravel() - f.mean(), 1e-13, True, True)
+        z = z.reshape(m, m)
+        errs.append(np.abs((z - z.mean()) - (ue - ue.mean())).max())
+    r = [errs[k] / errs[k + 1] for k in range(2)]
+    assert all(3.6 < q < 4.4 for q in r), r
