@@ -62,7 +62,11 @@ int incrmt_copy_probe(long long bytes, int reps, double *ms_per_copy);
 /* ---- fluid_2d ------------------------------------------------------------
  * fluid_2d::fluid_2d(m,n,sim_type&,visc,rhof,tw,pinw,ntrace,fflags,dir)
  * (fluid_2d.cc:26-43) with sim_type(ax,bx,ay,by,x_prd,y_prd,w_scale,w_force)
- * (sim_type.hh:63-71). Only non-periodic boxes are implemented. */
+ * (sim_type.hh:63-71): no-slip walls, or periodic along x_prd / y_prd
+ * (fluid_2d.cc:689-872, 1193-1235, 1317-1348; a periodic direction needs an
+ * even cell count whose halvings end at 2 or 3, checked at initialize).
+ * The outflow (zpr) and periodic-solid (solid_prd_bc) variants are not
+ * implemented. */
 int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
                         double w_scale, double w_force, double visc, double rhof, double tw, double pinw,
                         incrmt_fluid **out);
@@ -73,6 +77,15 @@ int incrmt_fluid_add_spin(incrmt_fluid *f, double r_scale, double a1, double a2,
 /* fluid_2d::add_object(new obj_circle(cx,cy,cr,grav), mat_const) (object.hh:128-138). */
 int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
                             double ev_trans_mult);
+/* The sim_type's initial fluid velocity, which init_fields blends with the
+ * solids' momenta (fluid_2d.cc:210-238): at rest (sim_type.hh:67-70, the
+ * default), sim_four_rolls (sim_type.cc:30-35) or sim_velocity_pulses
+ * (sim_type.cc:43-47, sim_type.hh:99-114; conv_test's initial field).
+ * Set before initialize. */
+#define INCRMT_SIM_REST 0
+#define INCRMT_SIM_FOUR_ROLLS 1
+#define INCRMT_SIM_VELOCITY_PULSES 2
+int incrmt_fluid_set_sim_velocity(incrmt_fluid *f, int kind);
 /* fluid_2d::initialize(ev_mult,dt_pad,dt_ev_pad) (fluid_2d.cc:101-122). */
 int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, double dt_ev_pad);
 /* Projection stopping rule: relative residual ||b-Az||/||b|| (the

@@ -29,6 +29,7 @@ MG_MAC, MG_FEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_PREVIOUS, GUESS_EXTRAPOLATE = 0, 1
 LAUNCH_DIRECT, LAUNCH_GRAPH = 0, 1
+SIM_REST, SIM_FOUR_ROLLS, SIM_VELOCITY_PULSES = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -77,6 +78,7 @@ _PROTOS = [
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
     ("incrmt_fluid_set_guess", _I, [_V, _I]),
     ("incrmt_fluid_set_launch", _I, [_V, _I]),
+    ("incrmt_fluid_set_sim_velocity", _I, [_V, _I]),
     ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
@@ -147,7 +149,9 @@ def device_info():
 
 
 class Fluid2D:
-    """fluid_2d (fluid_2d.hh:31-272) on one GPU, for a non-periodic box."""
+    """fluid_2d (fluid_2d.hh:31-272) on one GPU: a box with no-slip walls or
+    periodic directions (x_prd, y_prd; even cell counts whose halvings end at
+    2 or 3)."""
 
     def __init__(self, m, n, visc, rhof, tw=2.5, pinw=2.5, ax=-1.0, bx=1.0, ay=-1.0, by=1.0,
                  x_prd=False, y_prd=False, w_scale=3.0, w_force=20.0):
@@ -162,6 +166,7 @@ class Fluid2D:
         dx, dy = (bx - ax) / m, (by - ay) / n
         self.eps = tw * (dx if dx < dy else dy)  # the transition width (fluid_2d.cc:35)
         self.ntrace = 0
+        self.x_prd, self.y_prd = bool(x_prd), bool(y_prd)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -190,6 +195,11 @@ class Fluid2D:
         """GUESS_PREVIOUS (the reference's warm start) or GUESS_EXTRAPOLATE
         (2 z_n - z_(n-1)) as the projections' starting iterate."""
         _check(lib().incrmt_fluid_set_guess(self._h, mode))
+
+    def set_sim_velocity(self, kind):
+        """SIM_REST, SIM_FOUR_ROLLS or SIM_VELOCITY_PULSES: the sim_type's
+        initial fluid velocity (sim_type.cc:30-47); before initialize."""
+        _check(lib().incrmt_fluid_set_sim_velocity(self._h, kind))
 
     def set_launch(self, mode):
         """LAUNCH_GRAPH (V-cycles captured once as HIP graphs and replayed;

@@ -5,6 +5,7 @@
     python -m incrmt_amd.configs multibody  2048^2 cells, 64 circles, mixed vs FP64
     python -m incrmt_amd.configs ensemble   256 rotor instances of uneven size
     python -m incrmt_amd.configs conv       resolution ladder of the rotor (conv_test)
+    python -m incrmt_amd.configs conv-test  conv_test itself: periodic pulses, type 0/1/4..7
 """
 import argparse
 import json
@@ -102,9 +103,34 @@ def run_conv(sizes=(128, 256, 512, 1024), duration=0.2):
     return out
 
 
+def run_conv_test(ctype=4, sizes=(1536, 768, 512, 384, 256), intervals=50, every=10):
+    """conv_test.cc:14-194 for one test type on a ladder of periodic sizes
+    that all divide the finest (the reference's ladder 2520..280 has odd
+    factors that this solver does not coarsen; 2^k and 3 * 2^k sizes do):
+    the error table every `every` intervals, and the observed orders of
+    the total velocity and pressure L2 errors at the last interval."""
+    from incrmt_amd import conv
+    t0 = time.perf_counter()
+
+    def cb(i, errs):
+        if i % every == 0 or i == intervals:
+            print(json.dumps(dict(interval=i, vel_l2={m: e[11] for m, e in errs.items()},
+                                  p_l2={m: e[15] for m, e in errs.items()})), file=sys.stderr, flush=True)
+
+    table, steps = conv.conv_test_run(ctype, sizes, intervals=intervals, callback=cb)
+    last = table[-1]
+    return dict(config="conv_test", test_type=ctype, sizes=sorted(sizes), intervals=intervals, interval=0.02,
+                steps_per_interval=steps, errors_last={m: [float(x) for x in e] for m, e in last.items()},
+                velocity_l2_orders=conv.orders(last, 11), pressure_l2_orders=conv.orders(last, 15),
+                seconds=time.perf_counter() - t0)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["fem", "multibody", "ensemble", "conv"])
+    ap.add_argument("which", choices=["fem", "multibody", "ensemble", "conv", "conv-test"])
+    ap.add_argument("--type", type=int, default=4)
+    ap.add_argument("--sizes", type=str, default="1536,768,512,384,256")
+    ap.add_argument("--intervals", type=int, default=50)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--threads", type=int, default=16)
@@ -115,6 +141,8 @@ def main():
         r = run_multibody(steps=a.steps or 200, every=a.every)
     elif a.which == "conv":
         r = run_conv()
+    elif a.which == "conv-test":
+        r = run_conv_test(a.type, tuple(int(x) for x in a.sizes.split(",")), a.intervals)
     else:
         r = run_ensemble(steps=a.steps or 10, threads=a.threads)
     r["device"] = ia.device_info()[0]

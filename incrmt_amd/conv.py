@@ -1,5 +1,6 @@
 """Resolution comparison of the conv_test driver (conv_test.cc:14-194) for
-non-periodic boxes: fluid_2d::l_comparison (fluid_2d_io.cc:186-281)
+boxes with walls or periodic directions: fluid_2d::l_comparison
+(fluid_2d_io.cc:186-281)
 restated with array operations, and a ladder of runs of one case at
 several resolutions, each compared with the finest, from which the
 convergence order follows.
@@ -26,9 +27,11 @@ _STE = np.array([1.0, -1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0])
 
 def _state(f):
     """The fields l_comparison reads, from an incrmt_amd.Fluid2D (or any
-    object with the same get/m/n/nobj/eps members)."""
+    object with the same get/m/n/nobj/eps members, and x_prd/y_prd when
+    periodic)."""
     phis = [f.get("phi", k) for k in range(f.nobj)]
-    return dict(m=f.m, n=f.n, u=f.get("u"), v=f.get("v"), p=f.get("p"), phi=phis, eps=f.eps)
+    return dict(m=f.m, n=f.n, u=f.get("u"), v=f.get("v"), p=f.get("p"), phi=phis, eps=f.eps,
+                px=bool(getattr(f, "x_prd", False)), py=bool(getattr(f, "y_prd", False)))
 
 
 def _min_phi(st):
@@ -79,16 +82,18 @@ def l_comparison(coarse, fine):
         o[12 + c] += dl[sel].sum()
         if sel.any():
             o[18 + c] = max(o[18 + c], dl[sel].max())
-    # pressure at the coarse corners (i <= m, j <= n)
-    J, I = np.mgrid[0:n + 1, 0:m + 1]
-    tf = np.where((J > 0) & (J < n), 1.0, 0.5)
-    tf2 = np.where((I > 0) & (I < m), tf, 0.5 * tf)
+    # pressure at the coarse corners (i <= m, j <= n; i < m, j < n along a
+    # periodic direction, whose corners all weigh 1)
+    px, py = coarse.get("px", False), coarse.get("py", False)
+    J, I = np.mgrid[0:(n if py else n + 1), 0:(m if px else m + 1)]
+    tf = np.where(py | ((J > 0) & (J < n)), 1.0, 0.5)
+    tf2 = np.where(px | ((I > 0) & (I < m)), tf, 0.5 * tf)
     ml = m + 4
     # the stencil of the top row and right column of corners reaches one row
     # and column past the ghosted grid (the reference reads past its level-set
     # array there); those points count as far from every solid
     mpf = np.concatenate([mp.ravel(), np.full(3 * ml, np.finfo(np.float64).max)])
-    mph = np.zeros((n + 1, m + 1))
+    mph = np.zeros(J.shape)
     base = ml * (2 + J) + (I + 2)
     with np.errstate(over="ignore", invalid="ignore"):
         for jj in range(-1, 3):
@@ -140,3 +145,56 @@ def orders(errs, index=11):
     total velocity L2, err[11]) between successive sizes."""
     ms = sorted(errs)
     return [math.log(errs[a][index] / errs[b][index]) / math.log(b / a) for a, b in zip(ms[:-1], ms[1:])]
+
+
+# conv_test's test types (conv_test.cc:59-88): solid density and fluid
+# viscosity per type 0..7; types 0 and 1 are fluid only, 4..7 carry
+# obj_circle(-0.1,0,0.6) with G = 1. Types 2 (obj_full with periodic solid
+# boundaries) and 3 (obj_square) are not built here.
+CONV_RHOS = (1, 1, 1, 1, 3, 1, 1, 1)
+CONV_VISC = (1e-3, 4e-3, 1e-20, 1e-3, 1e-3, 1e-3, 4e-3, 16e-3)
+
+
+def conv_test_case(m, ctype, coarsest=280, rel_tol=1e-10, precision=0):
+    """One of conv_test's simulations (conv_test.cc:98-128) on the GPU: the
+    periodic [-1,1]^2 box with sim_velocity_pulses, initialized with
+    ev_mult 0.4 and the timestep paddings 0.4 and 0.8 scaled by
+    coarsest / m (the reference scales by its coarsest grid, 280), so the
+    timestep falls as h^2."""
+    import incrmt_amd as ia
+    if ctype not in (0, 1, 4, 5, 6, 7):
+        raise ValueError("conv_test types 0, 1 and 4..7 are built")
+    f = ia.Fluid2D(m, m, visc=CONV_VISC[ctype], rhof=1.0, tw=2.5, pinw=2.5, x_prd=True, y_prd=True)
+    if ctype >= 4:
+        f.add_circle(-0.1, 0.0, 0.6, 0.0, G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
+    f.set_sim_velocity(ia.SIM_VELOCITY_PULSES)
+    f.set_solver(rel_tol=rel_tol, precision=precision)
+    f.initialize(0.4, 0.4 * coarsest / m, 0.8 * coarsest / m)
+    return f
+
+
+def conv_test_run(ctype, sizes, intervals=50, interval=0.02, coarsest=280, rel_tol=1e-10, callback=None):
+    """conv_test's main loop (conv_test.cc:130-175): every size advances
+    interval by interval (timestep_select(interval) steps each), and at
+    time 0 and after each interval every size but the finest is compared
+    with the finest by l_comparison. Returns ([{m: 32 errors}] for
+    intervals 0..intervals, {m: steps per interval})."""
+    sims = {m: conv_test_case(m, ctype, coarsest, rel_tol) for m in sorted(sizes)}
+    finest = max(sizes)
+    sel = {m: f.timestep_select(interval) for m, f in sims.items()}
+    out = []
+    try:
+        for i in range(intervals + 1):
+            fs = _state(sims[finest])
+            out.append({m: l_comparison(_state(f), fs) for m, f in sims.items() if m != finest})
+            if callback is not None:
+                callback(i, out[-1])
+            if i == intervals:
+                break
+            for m, f in sims.items():
+                l, adt = sel[m]
+                f.steps(adt, l)
+    finally:
+        for f in sims.values():
+            f.close()
+    return out, {m: s[0] for m, s in sel.items()}

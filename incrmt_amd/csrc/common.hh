@@ -27,6 +27,8 @@ struct Grid {
     long long N; // allocation length (doubles)
     double ax, bx, ay, by, dx, dy, xsp, ysp, xxsp, yysp;
     double lax, lay;  // origin of the level-set grid: ax-1.5dx, ay-1.5dy (obj_field.cc:18)
+    int px, py;       // periodic directions (sim_type::x_prd, y_prd)
+    int vel;          // initial fluid velocity: one of the SimVelocity kinds
     // cell (i,j), i in [-2,m+1], j in [-2,n+1]
     __host__ __device__ inline int at(int i, int j) const { return kOrigin + (j + 2) * P + i; }
     // level-set point (I,J) = cell (I-2,J-2)
@@ -34,6 +36,25 @@ struct Grid {
 };
 
 enum ObjKind : int { kSpin = 0, kCircle = 1 };
+
+// sim_type::velocity of the reference's simulation types: at rest (the
+// default, sim_type.hh:67-70), four_rolls (sim_type.cc:30-35) and
+// velocity_pulses (sim_type.cc:43-47, sim_type.hh:109-113).
+enum SimVelocity : int { kSimRest = 0, kSimFourRolls = 1, kSimPulses = 2 };
+__host__ __device__ inline void sim_velocity(int kind, double x, double y, double &u, double &v) {
+    u = v = 0;
+    if (kind == kSimFourRolls) {
+        u = sin(M_PI * x) * cos(M_PI * y);
+        v = -cos(M_PI * x) * sin(M_PI * y);
+    } else if (kind == kSimPulses) {
+        for (double k = 2, s = 1, z = -5 / 6.; z < 1; z += 1 / 3., k += 2, s = -s) {
+            const double xx = x - z, yy = y - z;
+            const double f = s * exp(-k * (2 - cos(xx * M_PI) - cos(yy * M_PI)));
+            u -= sin(yy * M_PI) * f;
+            v += sin(xx * M_PI) * f;
+        }
+    }
+}
 
 // Constants of one solid (obj_field.hh:32-135, object.hh:128-138, 422-467).
 struct ObjConst {

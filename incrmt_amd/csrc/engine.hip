@@ -41,15 +41,15 @@ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 // Trapezoid-weighted mean of the nodal pressure (fluid_2d.cc:1332-1348), as
 // a deterministic two-pass reduction: weights 1/2 on the boundary rows and
-// columns, the sum divided by m*n.
-__global__ void k_pavg_part(int mf, int nf, const double *z, int zp, double *part) {
+// columns of non-periodic directions, the sum divided by m*n.
+__global__ void k_pavg_part(int mf, int nf, int px, int py, const double *z, int zp, double *part) {
     __shared__ double s[256];
     double a = 0;
     const int N = mf * nf;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         const int i = p % mf, j = p / mf;
-        double w = (i == 0 || i == mf - 1) ? 0.5 : 1.0;
-        if (j == 0 || j == nf - 1) w *= 0.5;
+        double w = (!px && (i == 0 || i == mf - 1)) ? 0.5 : 1.0;
+        if (!py && (j == 0 || j == nf - 1)) w *= 0.5;
         a += w * z[j * zp + i];
     }
     s[threadIdx.x] = a;
@@ -232,7 +232,6 @@ int FluidEngine::init_tracers() {
 }
 
 int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
-    if (obj.empty()) return fail(INCRMT_ERR_ARG, "no solid added");
     if ((int)obj.size() > kMaxObj) return fail(INCRMT_ERR_ARG, "too many solids");
     choose_ev_and_dt(ev_mult, dt_pad, dt_ev_pad);
     const_rho = true;
@@ -299,18 +298,25 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
         HIP_CHECK(hipMemcpy(o.a.nY, hY.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(o.a.phi, hphi.data(), g.N * sizeof(double), hipMemcpyHostToDevice));
     }
-    HIP_CHECK(hipMalloc((void **)&d_oc, obj.size() * sizeof(ObjConst)));
-    HIP_CHECK(hipMalloc((void **)&d_oa, obj.size() * sizeof(ObjArrays)));
-    allocs.push_back(d_oc);
-    allocs.push_back(d_oa);
-    std::vector<ObjConst> hc;
-    std::vector<ObjArrays> ha;
-    for (auto &o : obj) { hc.push_back(o.c); ha.push_back(o.a); }
-    HIP_CHECK(hipMemcpy(d_oc, hc.data(), hc.size() * sizeof(ObjConst), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(d_oa, ha.data(), ha.size() * sizeof(ObjArrays), hipMemcpyHostToDevice));
+    // (a fluid-only simulation, such as conv_test's types 0 and 1, has none)
+    if (!obj.empty()) {
+        HIP_CHECK(hipMalloc((void **)&d_oc, obj.size() * sizeof(ObjConst)));
+        HIP_CHECK(hipMalloc((void **)&d_oa, obj.size() * sizeof(ObjArrays)));
+        allocs.push_back(d_oc);
+        allocs.push_back(d_oa);
+        std::vector<ObjConst> hc;
+        std::vector<ObjArrays> ha;
+        for (auto &o : obj) { hc.push_back(o.c); ha.push_back(o.a); }
+        HIP_CHECK(hipMemcpy(d_oc, hc.data(), hc.size() * sizeof(ObjConst), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_oa, ha.data(), ha.size() * sizeof(ObjArrays), hipMemcpyHostToDevice));
+    }
     // projection solvers (fluid_2d::setup_multigrid, fluid_2d.cc:126-142)
-    if (int rc = mac.init(kMAC, g.m, g.n, g.dx, g.dy, s)) return rc;
-    if (int rc = fem.init(kFEM, g.m + 1, g.n + 1, g.dx, g.dy, s)) return rc;
+    // m_fem = m nodes along a periodic direction (fluid_2d.cc:30-31)
+    int rc0 = mac.init(kMAC, g.m, g.n, g.dx, g.dy, s, g.px, g.py);
+    if (!rc0) rc0 = fem.init(kFEM, g.px ? g.m : g.m + 1, g.py ? g.n : g.n + 1, g.dx, g.dy, s, g.px, g.py);
+    if (rc0 == INCRMT_ERR_UNSUPPORTED)
+        return fail(rc0, "a periodic direction needs an even cell count whose halvings end at most 16 points");
+    if (rc0) return rc0;
     // fluid_2d::init_fields (fluid_2d.cc:195-254)
     for (auto &o : obj) {
         int passes;
@@ -381,7 +387,7 @@ int FluidEngine::step(double dt) {
     if (rc) return rc;
     st.fem_cycles = ms.cycles; st.fem_rel_res = ms.rel_res; st.total_fem_cycles += ms.cycles;
     // copy_pressure + update_velocity (fluid_2d.cc:1317-1374)
-    k_pavg_part<<<256, 256, 0, s>>>(g.m + 1, g.n + 1, fl.z, fl.pitch, d_part);
+    k_pavg_part<<<256, 256, 0, s>>>(fl.nx, fl.ny, g.px, g.py, fl.z, fl.pitch, d_part);
     HIP_CHECK(hipGetLastError());
     k_pavg_final<<<1, 256, 0, s>>>(d_part, 256, 1. / (double(g.m) * g.n), d_pavg);
     HIP_CHECK(hipGetLastError());
@@ -534,7 +540,6 @@ int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by
     if (!out) return fail(INCRMT_ERR_ARG, "null output handle");
     *out = nullptr;
     if (m < 8 || n < 8 || m > 65536 || n > 65536) return fail(INCRMT_ERR_ARG, "grid size out of range");
-    if (x_prd || y_prd) return fail(INCRMT_ERR_UNSUPPORTED, "periodic boundaries are not implemented");
     if (!(bx > ax) || !(by > ay) || !(visc > 0) || !(rhof > 0) || !(tw > 0)) return fail(INCRMT_ERR_ARG, "bad geometry or material");
     incrmt_fluid *f = new incrmt_fluid;
     FluidEngine &e = f->e;
@@ -546,6 +551,8 @@ int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by
     g.dx = (bx - ax) / m; g.dy = (by - ay) / n;
     g.xsp = 1 / g.dx; g.ysp = 1 / g.dy; g.xxsp = g.xsp * g.xsp; g.yysp = g.ysp * g.ysp;
     g.lax = ax - 1.5 * g.dx; g.lay = ay - 1.5 * g.dy;
+    g.px = x_prd != 0; g.py = y_prd != 0;
+    g.vel = kSimRest;
     e.visc = visc; e.rhof = rhof; e.tw = tw; e.pinw = pinw; e.w_scale = w_scale; e.w_force = w_force;
     e.eps = tw * (g.dx < g.dy ? g.dx : g.dy);
     hipError_t er = hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking);
@@ -890,6 +897,15 @@ int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE) return fail(INCRMT_ERR_ARG, "bad guess mode");
     f->e.mac.extrapolate = f->e.fem.extrapolate = mode == INCRMT_GUESS_EXTRAPOLATE;
+    return 0;
+}
+
+int incrmt_fluid_set_sim_velocity(incrmt_fluid *f, int kind) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "the initial velocity is set by initialize: set it before");
+    if (kind != INCRMT_SIM_REST && kind != INCRMT_SIM_FOUR_ROLLS && kind != INCRMT_SIM_VELOCITY_PULSES)
+        return fail(INCRMT_ERR_ARG, "bad simulation velocity");
+    f->e.g.vel = kind;
     return 0;
 }
 

@@ -65,7 +65,7 @@ __global__ void k_normal_derivs(Grid g, Fields f, Solids o, double dt) {
 }
 
 // ---- fluid_2d::godunov_upwinding_set<tang> + edge_boundary_conditions
-// (fluid_2d.cc:642-686, 729-779, 1112-1121), no-slip walls. Thread (i,j)
+// (fluid_2d.cc:642-686, 729-779, 1112-1121), walls or seams. Thread (i,j)
 // owns the face to its right and the face above it; every value it reads or
 // writes belongs to those two faces, so the faces are independent. The wall
 // faces that the boundary conditions zero are disjoint from the faces the
@@ -126,10 +126,76 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         }
     }
     // no-slip wall faces (fluid_2d.cc:754-776)
-    if (i == 0) { e[UL][ij] = 0; e[VL][ij] = 0; e[UR][ij - 1] = 0; }
-    if (i == g.m - 1) { e[UL][ij + 1] = 0; e[VL][ij + 1] = 0; e[UR][ij] = 0; }
-    if (j == 0) { e[UD][ij] = 0; e[VD][ij] = 0; e[VU][ij - P] = 0; }
-    if (j == g.n - 1) { e[UD][ij + P] = 0; e[VD][ij + P] = 0; e[VU][ij] = 0; }
+    if (!g.px) {
+        if (i == 0) { e[UL][ij] = 0; e[VL][ij] = 0; e[UR][ij - 1] = 0; }
+        if (i == g.m - 1) { e[UL][ij + 1] = 0; e[VL][ij + 1] = 0; e[UR][ij] = 0; }
+    } else if (i == g.m - 1) {
+        // periodic seam (fluid_2d.cc:733-744): ghost cell -1 takes cell
+        // m-1's right state, godunov_set_lr<true> runs on the face between
+        // it and cell 0, ghost cell m takes cell 0's new left state and cell
+        // m-1 the ghost's right state. Nothing else in the sweep reads or
+        // writes these values, so the seam of row j is this thread's.
+        const int r0 = g.at(0, j), gl = r0 - 1, gr = ij + 1;
+        double ur = e[UR][ij], vr = e[VR][ij], ul0 = e[UL][r0], vl0 = e[VL][r0];
+        if (ur + ul0 > 0 && ur > 0) {
+            ul0 = ur;
+            vl0 = vr;
+            if (!TANG)
+                for (int k = 0; k < o.n; k++) {
+                    const ObjArrays &a = o.oa[k];
+                    a.Xl[r0] = a.Xr[gl]; a.Yl[r0] = a.Yr[gl];
+                }
+        } else if (ur + ul0 < 0 && ul0 < 0) {
+            ur = ul0;
+        } else {
+            ur = 0.5 * (ur + ul0);
+            ul0 = 0;
+            vl0 = 0.5 * (vr + vl0);
+            if (!TANG)
+                for (int k = 0; k < o.n; k++) {
+                    const ObjArrays &a = o.oa[k];
+                    a.Xl[r0] = 0.5 * (a.Xr[gl] + a.Xl[r0]);
+                    a.Yl[r0] = 0.5 * (a.Yr[gl] + a.Yl[r0]);
+                }
+        }
+        e[UR][gl] = ur; e[VR][gl] = vr;
+        e[UL][r0] = ul0; e[VL][r0] = vl0;
+        e[UL][gr] = ul0; e[VL][gr] = vl0;
+        e[UR][ij] = ur;
+    }
+    if (!g.py) {
+        if (j == 0) { e[UD][ij] = 0; e[VD][ij] = 0; e[VU][ij - P] = 0; }
+        if (j == g.n - 1) { e[UD][ij + P] = 0; e[VD][ij + P] = 0; e[VU][ij] = 0; }
+    } else if (j == g.n - 1) {
+        // periodic seam of column i (fluid_2d.cc:760-769), as above
+        const int r0 = g.at(i, 0), gd = r0 - P, gu = ij + P;
+        double vu = e[VU][ij], uu = e[UU][ij], vd0 = e[VD][r0], ud0 = e[UD][r0];
+        if (vu + vd0 > 0 && vu > 0) {
+            vd0 = vu;
+            ud0 = uu;
+            if (!TANG)
+                for (int k = 0; k < o.n; k++) {
+                    const ObjArrays &a = o.oa[k];
+                    a.Xd[r0] = a.Xu[gd]; a.Yd[r0] = a.Yu[gd];
+                }
+        } else if (vu + vd0 < 0 && vd0 < 0) {
+            vu = vd0;
+        } else {
+            vu = 0.5 * (vu + vd0);
+            vd0 = 0;
+            ud0 = 0.5 * (uu + ud0);
+            if (!TANG)
+                for (int k = 0; k < o.n; k++) {
+                    const ObjArrays &a = o.oa[k];
+                    a.Xd[r0] = 0.5 * (a.Xu[gd] + a.Xd[r0]);
+                    a.Yd[r0] = 0.5 * (a.Yu[gd] + a.Yd[r0]);
+                }
+        }
+        e[VU][gd] = vu; e[UU][gd] = uu;
+        e[VD][r0] = vd0; e[UD][r0] = ud0;
+        e[VD][gu] = vd0; e[UD][gu] = ud0;
+        e[VU][ij] = vu;
+    }
 }
 
 // ---- fluid_2d::acceleration + wall_force + object::accel
@@ -251,16 +317,22 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j), P = g.P, q = j * zp + i;
     double *ul = f.eB[UL], *vd = f.eB[VD];
-    auto corr_l = [&](int ii, int jj) -> double {  // corrected left face of cell (ii,jj)
+    // corrected left face of cell (ii,jj); a periodic seam is corrected on
+    // cell 0 with the opposite edge's q, and ghost cell m repeats it
+    auto corr_l = [&](int ii, int jj) -> double {
+        if (g.px && ii == g.m) ii = 0;
         const int c = g.at(ii, jj);
-        if (ii <= 0 || ii >= g.m) return ul[c];
-        const double dz = z[jj * zp + ii] - z[jj * zp + ii - 1];
+        if (!g.px && (ii <= 0 || ii >= g.m)) return ul[c];
+        const int im = ii == 0 ? g.m - 1 : ii - 1;
+        const double dz = z[jj * zp + ii] - z[jj * zp + im];
         return const_rho ? ul[c] - dz * g.xsp : ul[c] - fw[jj * cp + ii] * dz * g.xsp;
     };
     auto corr_d = [&](int ii, int jj) -> double {
+        if (g.py && jj == g.n) jj = 0;
         const int c = g.at(ii, jj);
-        if (jj <= 0 || jj >= g.n) return vd[c];
-        const double dz = z[jj * zp + ii] - z[(jj - 1) * zp + ii];
+        if (!g.py && (jj <= 0 || jj >= g.n)) return vd[c];
+        const int jm = jj == 0 ? g.n - 1 : jj - 1;
+        const double dz = z[jj * zp + ii] - z[jm * zp + ii];
         return const_rho ? vd[c] - dz * g.ysp : vd[c] - fs[jj * cp + ii] * dz * g.ysp;
     };
     const double ulc = corr_l(i, j), vdc = corr_d(i, j);
@@ -277,8 +349,8 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
     }
     f.ulc[ij] = ulc;
     f.vdc[ij] = vdc;
-    if (i == g.m - 1) f.ulc[ij + 1] = ul[ij + 1];  // right wall face
-    if (j == g.n - 1) f.vdc[ij + P] = vd[ij + P];  // top wall face
+    if (i == g.m - 1) f.ulc[ij + 1] = g.px ? ulr : ul[ij + 1];  // right wall face, or the seam again
+    if (j == g.n - 1) f.vdc[ij + P] = g.py ? vdu : vd[ij + P];  // top wall face, or the seam again
 }
 
 // ---- update_reference_map's last loop (fluid_2d.cc:1265-1272)
@@ -447,9 +519,13 @@ __global__ void k_ustar(Grid g, Fields f, Solids o, Phys ph, double dt) {
 // (fluid_2d.cc:839-872) applied as predicates.
 __global__ void k_fem_src(Grid g, Fields f, double dt, double *b, int bp) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i > g.m || j > g.n) return;
+    // m_fem = m nodes along a periodic direction, m+1 otherwise
+    if (i >= g.m + (g.px ? 0 : 1) || j >= g.n + (g.py ? 0 : 1)) return;
     const int ij = g.at(i, j), P = g.P;
+    // ghost cells: zero beyond a wall, the opposite edge across a seam
     auto c = [&](const double *a, int ii, int jj, int q) -> double {
+        if (g.px && ii < 0) { ii += g.m; q += g.m; }
+        if (g.py && jj < 0) { jj += g.n; q += g.n * P; }
         return (ii < 0 || ii >= g.m || jj < 0 || jj >= g.n) ? 0.0 : a[q];
     };
     const double hx = 0.5 * g.dy / dt, hy = 0.5 * g.dx / dt;
@@ -462,6 +538,8 @@ __global__ void k_fem_src(Grid g, Fields f, double dt, double *b, int bp) {
 // (fluid_2d.cc:1317-1374). Thread (i,j) writes the corner pressure p(i,j)
 // and updates cell (i,j) from its four corners, each formed as z - pavg
 // exactly as copy_pressure stores it.
+// Along a periodic direction node m is node 0: the solver's ghost ring
+// mirrors it, so reading z there is copy_pressure's fp[-m].p.
 __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z, int zp,
                                     const double *pavg_p, double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -483,7 +561,8 @@ __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z,
     }
 }
 
-// ---- fluid_2d::set_boundaries (fluid_2d.cc:689-726), no-slip walls.
+// ---- fluid_2d::set_boundaries (fluid_2d.cc:689-726), no-slip walls or
+// periodic copies.
 // Phase 0 sets the left/right ghost columns of the interior rows, phase 1
 // the bottom/top ghost rows over the full ghosted width (which reads the
 // ghost columns phase 0 wrote, as in the reference).
@@ -492,6 +571,13 @@ __global__ void k_bc_cols(Grid g, Fields f) {
     if (j >= g.n) return;
     double *u = f.u, *v = f.v;
     const int r = g.at(0, j), m = g.m;
+    if (g.px) {  // field::prd_bc (fields.hh:39-41)
+        u[r - 2] = u[r + m - 2]; v[r - 2] = v[r + m - 2];
+        u[r - 1] = u[r + m - 1]; v[r - 1] = v[r + m - 1];
+        u[r + m] = u[r]; v[r + m] = v[r];
+        u[r + m + 1] = u[r + 1]; v[r + m + 1] = v[r + 1];
+        return;
+    }
     u[r - 2] = -u[r + 1]; v[r - 2] = -v[r + 1];
     u[r - 1] = -u[r]; v[r - 1] = -v[r];
     u[r + m] = -u[r + m - 1]; v[r + m] = -v[r + m - 1];
@@ -502,6 +588,13 @@ __global__ void k_bc_rows(Grid g, Fields f) {
     if (i >= g.m + 2) return;
     double *u = f.u, *v = f.v;
     const int P = g.P, b = g.at(i, 0), t = g.at(i, g.n - 1);
+    if (g.py) {
+        u[b - 2 * P] = u[t - P]; v[b - 2 * P] = v[t - P];
+        u[b - P] = u[t]; v[b - P] = v[t];
+        u[t + P] = u[b]; v[t + P] = v[b];
+        u[t + 2 * P] = u[b + P]; v[t + 2 * P] = v[b + P];
+        return;
+    }
     u[b - 2 * P] = -u[b + P]; v[b - 2 * P] = -v[b + P];
     u[b - P] = -u[b]; v[b - P] = -v[b];
     u[t + P] = -u[t]; v[t + P] = -v[t];
@@ -534,7 +627,8 @@ __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
         f.u[ij] = mu / rho;
         f.v[ij] = mv / rho;
     } else {
-        double uu = 0, vv = 0, irho = 1 / ((1 - sfrac) * rhof + rho);
+        double uu, vv, irho = 1 / ((1 - sfrac) * rhof + rho);
+        sim_velocity(g.vel, g.ax + g.dx * (0.5 + i), g.ay + g.dy * (0.5 + j), uu, vv);
         f.u[ij] = ((1 - sfrac) * rhof * uu + mu) * irho;
         f.v[ij] = ((1 - sfrac) * rhof * vv + mv) * irho;
     }

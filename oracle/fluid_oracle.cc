@@ -259,6 +259,9 @@ class fluid {
     void normal_derivatives(double dt);
     void godunov(bool tang);
     void edge_bc();
+    void set_lr_tang(int ij);
+    void set_du_tang(int ij);
+    void sim_velocity(double x, double y, double &u, double &v) const;
     void tangential();
     void half_time_edges(double dt);
     void acceleration(int i, int j, double &accx, double &accy, double x, double y, bool pressure);
@@ -285,7 +288,8 @@ class fluid {
 };
 
 fluid::fluid(const sim_spec &s, const std::vector<object_spec> &os)
-    : sp(s), m(s.m), n(s.n), mn(s.m * s.n), m_fem(s.m + 1), n_fem(s.n + 1), ml(s.m + 4), nl(s.n + 4),
+    : sp(s), m(s.m), n(s.n), mn(s.m * s.n), m_fem(s.x_prd ? s.m : s.m + 1), n_fem(s.y_prd ? s.n : s.n + 1),
+      ml(s.m + 4), nl(s.n + 4),
       ax(s.ax), bx(s.bx), ay(s.ay), by(s.by), dx((s.bx - s.ax) / s.m), dy((s.by - s.ay) / s.n),
       xsp(1 / dx), ysp(1 / dy), xxsp(xsp * xsp), yysp(ysp * ysp), visc(s.visc), rhof(s.rhof),
       eps(s.tw * (dx < dy ? dx : dy)), epsinv(1 / eps), w_scale(s.w_scale), w_force(s.w_force),
@@ -386,6 +390,7 @@ void fluid::init_fields() {  // fluid_2d.cc:195-254
                 f.v = mv / rho;
             } else {
                 double uu = 0, vv = 0, irho = 1 / ((1 - sfrac) * rhof + rho);
+                sim_velocity(ax + dx * (0.5 + i), ay + dy * (0.5 + j), uu, vv);
                 f.u = ((1 - sfrac) * rhof * uu + mu) * irho;
                 f.v = ((1 - sfrac) * rhof * vv + mv) * irho;
             }
@@ -399,14 +404,45 @@ void fluid::init_fields() {  // fluid_2d.cc:195-254
     set_rho();
 }
 
-void fluid::set_boundaries() {  // fluid_2d.cc:689-726, no-slip walls
+// sim_type::velocity (sim_type.hh:67-70; sim_four_rolls, sim_type.cc:30-35;
+// sim_velocity_pulses, sim_type.cc:43-47 with pulse(), sim_type.hh:109-113)
+void fluid::sim_velocity(double x, double y, double &u, double &v) const {
+    u = v = 0;
+    if (sp.vel == 1) {
+        u = std::sin(M_PI * x) * std::cos(M_PI * y);
+        v = -std::cos(M_PI * x) * std::sin(M_PI * y);
+    } else if (sp.vel == 2) {
+        for (double k = 2, s = 1, z = -5 / 6.; z < 1; z += 1 / 3., k += 2, s = -s) {
+            const double xx = x - z, yy = y - z;
+            const double f = s * std::exp(-k * (2 - std::cos(xx * M_PI) - std::cos(yy * M_PI)));
+            u -= std::sin(yy * M_PI) * f;
+            v += std::sin(xx * M_PI) * f;
+        }
+    }
+}
+
+void fluid::set_boundaries() {  // fluid_2d.cc:689-726: no-slip walls or periodic copies
     for (int j = 0; j < n; j++) {
+        if (sp.x_prd) {  // field::prd_bc (fields.hh:39-41)
+            F(-2, j).u = F(m - 2, j).u; F(-2, j).v = F(m - 2, j).v;
+            F(-1, j).u = F(m - 1, j).u; F(-1, j).v = F(m - 1, j).v;
+            F(m, j).u = F(0, j).u; F(m, j).v = F(0, j).v;
+            F(m + 1, j).u = F(1, j).u; F(m + 1, j).v = F(1, j).v;
+            continue;
+        }
         F(-2, j).u = -F(1, j).u; F(-2, j).v = -F(1, j).v;
         F(-1, j).u = -F(0, j).u; F(-1, j).v = -F(0, j).v;
         F(m, j).u = -F(m - 1, j).u; F(m, j).v = -F(m - 1, j).v;
         F(m + 1, j).u = -F(m - 2, j).u; F(m + 1, j).v = -F(m - 2, j).v;
     }
     for (int i = -2; i < m + 2; i++) {
+        if (sp.y_prd) {
+            F(i, -2).u = F(i, n - 2).u; F(i, -2).v = F(i, n - 2).v;
+            F(i, -1).u = F(i, n - 1).u; F(i, -1).v = F(i, n - 1).v;
+            F(i, n).u = F(i, 0).u; F(i, n).v = F(i, 0).v;
+            F(i, n + 1).u = F(i, 1).u; F(i, n + 1).v = F(i, 1).v;
+            continue;
+        }
         F(i, -2).u = -F(i, 1).u; F(i, -2).v = -F(i, 1).v;
         F(i, -1).u = -F(i, 0).u; F(i, -1).v = -F(i, 0).v;
         F(i, n).u = -F(i, n - 1).u; F(i, n).v = -F(i, n - 1).v;
@@ -650,14 +686,68 @@ void fluid::godunov(bool tang) {
         }
 }
 
-void fluid::edge_bc() {  // fluid_2d.cc:729-779, no-slip walls
+// godunov_set_lr<true> / godunov_set_du<true> on one face (fluid_2d.cc:642-686)
+void fluid::set_lr_tang(int ij) {
+    cellf &f = fb[ij], &fr = fb[ij + 1];
+    if (f.ur + fr.ul > 0 && f.ur > 0) {
+        fr.ul = f.ur; fr.vl = f.vr;
+        for (auto &o : obj) { o.sb[ij + 1].Xl = o.sb[ij].Xr; o.sb[ij + 1].Yl = o.sb[ij].Yr; }
+    } else if (f.ur + fr.ul < 0 && fr.ul < 0) {
+        f.ur = fr.ul;
+    } else {
+        f.ur = 0.5 * (f.ur + fr.ul);
+        fr.ul = 0;
+        fr.vl = 0.5 * (f.vr + fr.vl);
+        for (auto &o : obj) {
+            o.sb[ij + 1].Xl = 0.5 * (o.sb[ij].Xr + o.sb[ij + 1].Xl);
+            o.sb[ij + 1].Yl = 0.5 * (o.sb[ij].Yr + o.sb[ij + 1].Yl);
+        }
+    }
+}
+void fluid::set_du_tang(int ij) {
+    cellf &f = fb[ij], &fu = fb[ij + ml];
+    if (f.vu + fu.vd > 0 && f.vu > 0) {
+        fu.vd = f.vu; fu.ud = f.uu;
+        for (auto &o : obj) { o.sb[ij + ml].Xd = o.sb[ij].Xu; o.sb[ij + ml].Yd = o.sb[ij].Yu; }
+    } else if (f.vu + fu.vd < 0 && fu.vd < 0) {
+        f.vu = fu.vd;
+    } else {
+        f.vu = 0.5 * (f.vu + fu.vd);
+        fu.vd = 0;
+        fu.ud = 0.5 * (f.uu + fu.ud);
+        for (auto &o : obj) {
+            o.sb[ij + ml].Xd = 0.5 * (o.sb[ij].Xu + o.sb[ij + ml].Xd);
+            o.sb[ij + ml].Yd = 0.5 * (o.sb[ij].Yu + o.sb[ij + ml].Yd);
+        }
+    }
+}
+
+void fluid::edge_bc() {  // fluid_2d.cc:729-779: no-slip walls or periodic seams
     for (int j = 0; j < n; j++) {
+        if (sp.x_prd) {
+            F(-1, j).ur = F(m - 1, j).ur;
+            F(-1, j).vr = F(m - 1, j).vr;
+            set_lr_tang(at(-1, j));
+            F(m, j).ul = F(0, j).ul;
+            F(m, j).vl = F(0, j).vl;
+            F(m - 1, j).ur = F(-1, j).ur;
+            continue;
+        }
         F(0, j).ul = F(0, j).vl = 0;
         F(-1, j).ur = 0;
         F(m, j).ul = F(m, j).vl = 0;
         F(m - 1, j).ur = 0;
     }
     for (int i = 0; i < m; i++) {
+        if (sp.y_prd) {
+            F(i, -1).vu = F(i, n - 1).vu;
+            F(i, -1).uu = F(i, n - 1).uu;
+            set_du_tang(at(i, -1));
+            F(i, n).vd = F(i, 0).vd;
+            F(i, n).ud = F(i, 0).ud;
+            F(i, n - 1).vu = F(i, -1).vu;
+            continue;
+        }
         F(i, 0).ud = F(i, 0).vd = 0;
         F(i, -1).vu = 0;
         F(i, n).ud = F(i, n).vd = 0;
@@ -774,7 +864,7 @@ int fluid::mac_projection() {  // fluid_2d.cc:1175-1236, mgs_mac.cc:58-71
         for (int i = 0; i < m; i++) irho_mac[q++] = 0.5 / F(i, n - 1).rho + 0.5 / F(i, n).rho;
         ir = irho_mac.data();
     }
-    mac_system s{m, n, xxsp, yysp, ir};
+    mac_system s{m, n, xxsp, yysp, ir, sp.x_prd, sp.y_prd};
     mac_cycles = mac_solve(s, src.data(), zmac.data(), sp.mg_tol, &mac_res);
     if (mac_cycles < 0) return 2;
     for (int j = 0; j < n; j++)
@@ -782,12 +872,18 @@ int fluid::mac_projection() {  // fluid_2d.cc:1175-1236, mgs_mac.cc:58-71
             cellf &f = F(i, j);
             const double *so = &zmac[j * m + i];
             const double ed = const_rho ? 1 : irho_mac[2 * (j * m + i) + 1], el = const_rho ? 1 : irho_mac[2 * (j * m + i)];
+            // a periodic seam is corrected on row 0 / column 0 with the
+            // opposite edge's q, and the ghost row n / column m repeats it
             if (const_rho) {
                 if (j > 0) f.vd -= (*so - so[-m]) * ysp;
+                else if (sp.y_prd) { f.vd -= (*so - so[mn - m]) * ysp; F(i, n).vd = f.vd; }
                 if (i > 0) f.ul -= (*so - so[-1]) * xsp;
+                else if (sp.x_prd) { f.ul -= (*so - so[m - 1]) * xsp; F(m, j).ul = f.ul; }
             } else {
                 if (j > 0) f.vd -= ed * (*so - so[-m]) * ysp;
+                else if (sp.y_prd) { f.vd -= ed * (*so - so[mn - m]) * ysp; F(i, n).vd = f.vd; }
                 if (i > 0) f.ul -= el * (*so - so[-1]) * xsp;
+                else if (sp.x_prd) { f.ul -= el * (*so - so[m - 1]) * xsp; F(m, j).ul = f.ul; }
             }
         }
     return 0;
@@ -856,12 +952,23 @@ void fluid::compute_ustar(double dt) {  // fluid_2d.cc:1278-1291
 
 int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_fem.cc:104-121
     for (int j = 0; j < n; j++) {
-        F(-1, j).c0 = F(m, j).c0 = 0;
-        F(-1, j).c1 = F(m, j).c1 = 0;
+        if (sp.x_prd) {
+            F(-1, j).c0 = F(m - 1, j).c0;
+            F(-1, j).c1 = F(m - 1, j).c1;
+        } else {
+            F(-1, j).c0 = F(m, j).c0 = 0;
+            F(-1, j).c1 = F(m, j).c1 = 0;
+        }
     }
-    for (int i = -1; i < m + 1; i++) {
-        F(i, -1).c0 = F(i, n).c0 = 0;
-        F(i, -1).c1 = F(i, n).c1 = 0;
+    const int xl = sp.x_prd ? m : m + 1;
+    for (int i = -1; i < xl; i++) {
+        if (sp.y_prd) {
+            F(i, -1).c0 = F(i, n - 1).c0;
+            F(i, -1).c1 = F(i, n - 1).c1;
+        } else {
+            F(i, -1).c0 = F(i, n).c0 = 0;
+            F(i, -1).c1 = F(i, n).c1 = 0;
+        }
     }
     double hx = 0.5 * dy / dt, hy = 0.5 * dx / dt;
     for (int j = 0; j < n_fem; j++)
@@ -876,7 +983,7 @@ int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_f
             for (int i = 0; i < m_fem; i++) irho_fem[j * m_fem + i] = 1. / F(i - 1, j - 1).rho;
         ir = irho_fem.data();
     }
-    fem_system s{m_fem, n_fem, dx, dy, ir};
+    fem_system s{m_fem, n_fem, dx, dy, ir, sp.x_prd, sp.y_prd};
     fem_cycles = fem_solve(s, src.data(), zfem.data(), sp.mg_tol, &fem_res);
     return fem_cycles < 0 ? 3 : 0;
 }
@@ -885,15 +992,19 @@ void fluid::copy_pressure() {  // fluid_2d.cc:1317-1348
     double pavg = 0;
     for (int j = 0; j < n_fem; j++) {
         const double *so = &zfem[j * m_fem];
-        double prow = so[0] * 0.5;
+        double prow = so[0] * (sp.x_prd ? 1 : 0.5);
         for (int i = 1; i < m_fem - 1; i++) prow += so[i];
-        prow += so[m_fem - 1] * 0.5;
-        if (j == 0 || j == n) prow *= 0.5;
+        prow += so[m_fem - 1] * (sp.x_prd ? 1 : 0.5);
+        if (!sp.y_prd && (j == 0 || j == n)) prow *= 0.5;
         pavg += prow;
     }
     pavg = pavg * (1. / mn);
-    for (int j = 0; j < n + 1; j++)
-        for (int i = 0; i < m + 1; i++) F(i, j).p = zfem[j * m_fem + i] - pavg;
+    // node m (row n) of a periodic direction is node 0 (row 0)
+    for (int j = 0; j < n + 1; j++) {
+        const int jr = sp.y_prd && j == n ? 0 : j;
+        for (int i = 0; i < m; i++) F(i, j).p = zfem[jr * m_fem + i] - pavg;
+        F(m, j).p = sp.x_prd ? F(0, j).p : zfem[jr * m_fem + m] - pavg;
+    }
 }
 
 void fluid::update_velocity(double dt) {  // fluid_2d.cc:1354-1374

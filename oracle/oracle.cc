@@ -88,15 +88,16 @@ int orc_fem_solve(int m, int n, double dx, double dy, const double *irho,
 
 extern "C" {
 
-// fluid_2d construction for a no-slip box (fluid_2d.cc:26-43, 101-122).
+// fluid_2d construction (fluid_2d.cc:26-43, 101-122) for a box with no-slip
+// walls or periodic directions, at rest or with a sim_type's velocity.
 // params: 9 doubles per object: 6 shape parameters, G, rhos, ev_trans_mult.
 void *orc_fluid_create(int m, int n, double ax, double bx, double ay, double by,
                        double visc, double rhof, double tw, double pinw,
                        double w_scale, double w_force, double ev_mult,
                        double dt_pad, double dt_ev_pad, double mg_tol, int nobj,
-                       const int *kinds, const double *params) {
+                       const int *kinds, const double *params, int x_prd, int y_prd, int vel) {
     sim_spec sp{m, n, ax, bx, ay, by, visc, rhof, tw, pinw, w_scale, w_force,
-                ev_mult, dt_pad, dt_ev_pad, mg_tol};
+                ev_mult, dt_pad, dt_ev_pad, mg_tol, x_prd, y_prd, vel};
     std::vector<object_spec> os(nobj);
     for (int k = 0; k < nobj; k++) {
         os[k].kind = kinds[k];

@@ -95,6 +95,8 @@ struct sim_spec {
     double w_scale, w_force;
     double ev_mult, dt_pad, dt_ev_pad;
     double mg_tol;  // relative residual for both projections
+    int x_prd = 0, y_prd = 0;  // periodic directions (sim_type.hh:63-71)
+    int vel = 0;    // sim_type::velocity: 0 rest, 1 four_rolls, 2 velocity_pulses
 };
 
 class fluid;  // opaque

@@ -96,6 +96,21 @@ def simple_spin_spec(m, omega=1.0):
                 objects=[(SPIN, [0.8, 0.5, 0.125, 0.2, math.pi, omega], 48.0, 3.0, 1.0)])
 
 
+def conv_spec(m, ctype, coarsest):
+    """conv_test's simulations (conv_test.cc:37-128) for test type ctype
+    0..7 without the option bits: sim_velocity_pulses in the periodic
+    [-1,1]^2 box, 0/1 fluid only, 4..7 obj_circle(-0.1,0,0.6) with the
+    listed densities and viscosities, G=1, initialize(0.4,
+    0.4 coarsest/m, 0.8 coarsest/m). Types 2 (obj_full with periodic
+    solid boundaries) and 3 (obj_square) are not built."""
+    rhos = [1, 1, 1, 1, 3, 1, 1, 1][ctype]
+    visc = [1e-3, 4e-3, 1e-20, 1e-3, 1e-3, 1e-3, 4e-3, 16e-3][ctype]
+    objs = [] if ctype in (0, 1) else [(CIRCLE, [-0.1, 0.0, 0.6, 0.0, 0.0, 0.0], 1.0, float(rhos), 1.0)]
+    return dict(m=m, n=m, ax=-1.0, bx=1.0, ay=-1.0, by=1.0, visc=visc, rhof=1.0, tw=2.5, pinw=2.5,
+                w_scale=3.0, w_force=20.0, ev_mult=0.4, dt_pad=0.4 * coarsest / m,
+                dt_ev_pad=0.8 * coarsest / m, x_prd=1, y_prd=1, vel=2, objects=objs)
+
+
 class OracleFluid:
     def __init__(self, spec, mg_tol=1e-13):
         lib = oracle()
@@ -107,7 +122,7 @@ class OracleFluid:
         params = np.array([list(o[1]) + [o[2], o[3], o[4]] for o in objs], dtype=np.float64).ravel()
         D, I = ctypes.c_double, ctypes.c_int
         lib.orc_fluid_create.restype = ctypes.c_void_p
-        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp]
+        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp, I, I, I]
         lib.orc_fluid_destroy.argtypes = [ctypes.c_void_p]
         lib.orc_fluid_step.argtypes = [ctypes.c_void_p, D]
         lib.orc_fluid_dt_reg.argtypes = [ctypes.c_void_p]
@@ -121,7 +136,8 @@ class OracleFluid:
             spec["m"], spec["n"], spec["ax"], spec["bx"], spec["ay"], spec["by"], spec["visc"],
             spec["rhof"], spec["tw"], spec["pinw"], spec["w_scale"], spec["w_force"],
             spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"], mg_tol, len(objs),
-            iptr(kinds), dptr(params))
+            iptr(kinds), dptr(params), int(spec.get("x_prd", 0)), int(spec.get("y_prd", 0)),
+            int(spec.get("vel", 0)))
         self.nobj = len(objs)
         dx, dy = (spec["bx"] - spec["ax"]) / self.m, (spec["by"] - spec["ay"]) / self.n
         self.eps = spec["tw"] * (dx if dx < dy else dy)

@@ -69,3 +69,19 @@ def test_odd_ratio_uses_the_centre_value():
     b = _fields(48, f, f, lambda x, y: 0 * x)
     # ratio 3: the fine cell at the coarse centre coincides with it
     assert conv.l_comparison(a, b)[11] < 1e-14
+
+
+def test_periodic_corners_and_weights():
+    """Along a periodic direction the pressure corners run to m-1 (n-1) and
+    all weigh 1 (fluid_2d_io.cc:210-214)."""
+    for px, py in ((True, False), (False, True), (True, True)):
+        a = _fields(32, SIN, COS, lambda x, y: 1 + 0 * x)
+        b = _fields(64, SIN, COS, lambda x, y: 3 + 0 * x)
+        a.update(px=px, py=py)
+        e = conv.l_comparison(a, b)
+        cx, cy = (32 if px else 33), (32 if py else 33)
+        assert math.isclose(e[7], cx * cy / (32 * 32)), (px, py)
+        # the corner weights sum to m n either way (m+1 corners with halved
+        # ends, or m whole ones): a constant difference 2 has L1 norm 2
+        assert math.isclose(e[23], 2.0), (px, py, e[23])
+        assert e[31] == 2
