@@ -77,6 +77,10 @@ int incrmt_fluid_add_spin(incrmt_fluid *f, double r_scale, double a1, double a2,
 /* fluid_2d::add_object(new obj_circle(cx,cy,cr,grav), mat_const) (object.hh:128-138). */
 int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
                             double ev_trans_mult);
+/* fluid_2d::add_object(new obj_square(cx,cy,cr,grav), mat_const) (object.hh:140-150,
+ * object.cc:41-44, 83-87): the square of half-width cr, at rest. */
+int incrmt_fluid_add_square(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
+                            double ev_trans_mult);
 /* The sim_type's initial fluid velocity, which init_fields blends with the
  * solids' momenta (fluid_2d.cc:210-238): at rest (sim_type.hh:67-70, the
  * default), sim_four_rolls (sim_type.cc:30-35) or sim_velocity_pulses

@@ -60,6 +60,7 @@ _PROTOS = [
     ("incrmt_fluid_create", _I, [_I, _I, _D, _D, _D, _D, _I, _I, _D, _D, _D, _D, _D, _D, ctypes.POINTER(_V)]),
     ("incrmt_fluid_add_spin", _I, [_V, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("incrmt_fluid_add_circle", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
+    ("incrmt_fluid_add_square", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
     ("incrmt_fluid_initialize", _I, [_V, _D, _D, _D]),
     ("incrmt_fluid_set_solver", _I, [_V, _D, _I, _I]),
     ("incrmt_fluid_step", _I, [_V, _D]),
@@ -176,13 +177,17 @@ class Fluid2D:
     def __del__(self):
         self.close()
 
-    # obj_simple_spin / obj_circle with mat_const (object.hh:128-138, 422-467)
+    # obj_simple_spin / obj_circle / obj_square with mat_const (object.hh:128-150, 422-467)
     def add_spin(self, r_scale, a1, a2, piv_r, th_max, omega, G, rhos, ev_trans_mult=1.0):
         _check(lib().incrmt_fluid_add_spin(self._h, r_scale, a1, a2, piv_r, th_max, omega, G, rhos, ev_trans_mult))
         self.nobj += 1
 
     def add_circle(self, cx, cy, cr, grav, G, rhos, ev_trans_mult=1.0):
         _check(lib().incrmt_fluid_add_circle(self._h, cx, cy, cr, grav, G, rhos, ev_trans_mult))
+        self.nobj += 1
+
+    def add_square(self, cx, cy, cr, grav, G, rhos, ev_trans_mult=1.0):
+        _check(lib().incrmt_fluid_add_square(self._h, cx, cy, cr, grav, G, rhos, ev_trans_mult))
         self.nobj += 1
 
     def initialize(self, ev_mult=0.8, dt_pad=0.4, dt_ev_pad=0.8):

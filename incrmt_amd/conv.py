@@ -148,9 +148,9 @@ def orders(errs, index=11):
 
 
 # conv_test's test types (conv_test.cc:59-88): solid density and fluid
-# viscosity per type 0..7; types 0 and 1 are fluid only, 4..7 carry
-# obj_circle(-0.1,0,0.6) with G = 1. Types 2 (obj_full with periodic solid
-# boundaries) and 3 (obj_square) are not built here.
+# viscosity per type 0..7; types 0 and 1 are fluid only, 3 carries
+# obj_square(-0.1,0,0.6) and 4..7 obj_circle(-0.1,0,0.6), with G = 1.
+# Type 2 (obj_full with periodic solid boundaries) is not built here.
 CONV_RHOS = (1, 1, 1, 1, 3, 1, 1, 1)
 CONV_VISC = (1e-3, 4e-3, 1e-20, 1e-3, 1e-3, 1e-3, 4e-3, 16e-3)
 
@@ -162,10 +162,12 @@ def conv_test_case(m, ctype, coarsest=280, rel_tol=1e-10, precision=0):
     coarsest / m (the reference scales by its coarsest grid, 280), so the
     timestep falls as h^2."""
     import incrmt_amd as ia
-    if ctype not in (0, 1, 4, 5, 6, 7):
-        raise ValueError("conv_test types 0, 1 and 4..7 are built")
+    if ctype not in (0, 1, 3, 4, 5, 6, 7):
+        raise ValueError("conv_test types 0, 1 and 3..7 are built")
     f = ia.Fluid2D(m, m, visc=CONV_VISC[ctype], rhof=1.0, tw=2.5, pinw=2.5, x_prd=True, y_prd=True)
-    if ctype >= 4:
+    if ctype == 3:
+        f.add_square(-0.1, 0.0, 0.6, 0.0, G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
+    elif ctype >= 4:
         f.add_circle(-0.1, 0.0, 0.6, 0.0, G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
     f.set_sim_velocity(ia.SIM_VELOCITY_PULSES)
     f.set_solver(rel_tol=rel_tol, precision=precision)

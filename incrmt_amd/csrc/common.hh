@@ -35,7 +35,7 @@ struct Grid {
     __host__ __device__ inline int ls(int I, int J) const { return kOrigin - 2 + J * P + I; }
 };
 
-enum ObjKind : int { kSpin = 0, kCircle = 1 };
+enum ObjKind : int { kSpin = 0, kCircle = 1, kSquare = 2 };
 
 // sim_type::velocity of the reference's simulation types: at rest (the
 // default, sim_type.hh:67-70), four_rolls (sim_type.cc:30-35) and
@@ -88,11 +88,16 @@ __device__ inline double tderiv_func_in(const ObjConst &o, double x) {
 __device__ inline double coll_func(const ObjConst &o, double x) {
     return x > o.eps ? 0 : 0.5 - x * o.tf1;  // obj_field.hh:97-99
 }
-// object::phi for the supported solids (object.cc:22-24, 167-171)
+// object::phi for the supported solids (object.cc:22-24, 83-87, 167-171)
 __host__ __device__ inline double obj_phi(const ObjConst &o, double X, double Y) {
     if (o.kind == kSpin) {
         double th = atan2(Y, X), c = cos(3 * th);
         return sqrt(X * X + Y * Y) - o.c0 - c * (o.c1 + o.c2 * c);
+    }
+    if (o.kind == kSquare) {
+        X = fabs(X) - o.cr;
+        Y = fabs(Y) - o.cr;
+        return X < 0 || Y < 0 ? fmax(X, Y) : sqrt(X * X + Y * Y);
     }
     return sqrt(X * X + Y * Y) - o.cr;
 }

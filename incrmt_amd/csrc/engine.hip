@@ -286,7 +286,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
             for (int i = -2; i < g.m + 2; i++) {
                 const int ij = g.at(i, j);
                 double xx = g.ax + g.dx * (0.5 + i), X, Y;
-                if (o.c.kind == kCircle) { X = xx - o.c.cx; Y = y - o.c.cy; }
+                if (o.c.kind != kSpin) { X = xx - o.c.cx; Y = y - o.c.cy; }  // obj_standard::transform
                 else { X = xx; Y = y; }
                 hX[ij] = X; hY[ij] = Y;
                 hphi[ij] = obj_phi(o.c, X, Y);
@@ -581,6 +581,15 @@ int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, do
                             double ev_trans_mult) {
     ObjConst c{};
     c.kind = kCircle;
+    c.cx = cx; c.cy = cy; c.cr = cr; c.grav = grav;
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_add_square(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
+                            double ev_trans_mult) {
+    ObjConst c{};
+    c.kind = kSquare;
     c.cx = cx; c.cy = cy; c.cr = cr; c.grav = grav;
     c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
     return add_object(f, c);

@@ -612,7 +612,7 @@ __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
     for (int k = 0; k < o.n; k++) {
         const ObjConst &c = o.oc[k];
         double tf = trans_func(c, o.oa[k].phi[ij]), uu, vv;
-        if (c.kind == kCircle) {  // obj_standard::velocity with zero speed
+        if (c.kind != kSpin) {  // obj_standard::velocity with zero speed
             uu = 0 - 0 * o.oa[k].Y[ij];
             vv = 0 + 0 * o.oa[k].X[ij];
         } else {

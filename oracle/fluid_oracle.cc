@@ -188,14 +188,19 @@ struct solid {
             double th = std::atan2(Y, X), c = std::cos(3 * th);
             return std::sqrt(X * X + Y * Y) - c0 - c * (c1 + c2 * c);
         }
+        if (sp.kind == OBJ_SQUARE) {  // object.cc:83-87
+            X = std::fabs(X) - cr;
+            Y = std::fabs(Y) - cr;
+            return X < 0 || Y < 0 ? std::max(X, Y) : std::sqrt(X * X + Y * Y);
+        }
         return std::sqrt(X * X + Y * Y) - cr;  // object.cc:22-24
     }
     void transform(double x, double y, double &X, double &Y) const {
-        if (sp.kind == OBJ_CIRCLE) { X = x - cx; Y = y - cy; }  // object.hh:116-118
-        else { X = x; Y = y; }                                 // object.hh:26-28
+        if (sp.kind != OBJ_SIMPLE_SPIN) { X = x - cx; Y = y - cy; }  // obj_standard, object.hh:116-118
+        else { X = x; Y = y; }                                      // object.hh:26-28
     }
     void velocity(double X, double Y, double &u, double &v) const {
-        if (sp.kind == OBJ_CIRCLE) { u = 0 - 0 * Y; v = 0 + 0 * X; }  // object.cc:13-16
+        if (sp.kind != OBJ_SIMPLE_SPIN) { u = 0 - 0 * Y; v = 0 + 0 * X; }  // obj_standard, object.cc:13-16
         else { u = v = 0.; }
     }
     // smoothed Heaviside and helpers (obj_field.hh:97-121)

@@ -78,7 +78,7 @@ int fem_solve(const fem_system &s, const double *b, double *z, double tol,
               double *relres);
 
 // ------------------------------------------------------------ fluid step
-enum obj_kind { OBJ_SIMPLE_SPIN = 0, OBJ_CIRCLE = 1 };
+enum obj_kind { OBJ_SIMPLE_SPIN = 0, OBJ_CIRCLE = 1, OBJ_SQUARE = 2 };
 
 struct object_spec {
     int kind;

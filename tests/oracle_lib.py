@@ -84,6 +84,7 @@ def build_band_extrapolate(which, m, n, ax, ay, dx, dy, pd, pu, phi, fx, fy):
 # ------------------------------------------------------------------ fluid
 SPIN = 0
 CIRCLE = 1
+SQUARE = 2
 
 
 def simple_spin_spec(m, omega=1.0):
@@ -99,13 +100,14 @@ def simple_spin_spec(m, omega=1.0):
 def conv_spec(m, ctype, coarsest):
     """conv_test's simulations (conv_test.cc:37-128) for test type ctype
     0..7 without the option bits: sim_velocity_pulses in the periodic
-    [-1,1]^2 box, 0/1 fluid only, 4..7 obj_circle(-0.1,0,0.6) with the
-    listed densities and viscosities, G=1, initialize(0.4,
-    0.4 coarsest/m, 0.8 coarsest/m). Types 2 (obj_full with periodic
-    solid boundaries) and 3 (obj_square) are not built."""
+    [-1,1]^2 box, 0/1 fluid only, 3 obj_square(-0.1,0,0.6), 4..7
+    obj_circle(-0.1,0,0.6) with the listed densities and viscosities, G=1,
+    initialize(0.4, 0.4 coarsest/m, 0.8 coarsest/m). Type 2 (obj_full with
+    periodic solid boundaries) is not built."""
     rhos = [1, 1, 1, 1, 3, 1, 1, 1][ctype]
     visc = [1e-3, 4e-3, 1e-20, 1e-3, 1e-3, 1e-3, 4e-3, 16e-3][ctype]
-    objs = [] if ctype in (0, 1) else [(CIRCLE, [-0.1, 0.0, 0.6, 0.0, 0.0, 0.0], 1.0, float(rhos), 1.0)]
+    kind = SQUARE if ctype == 3 else CIRCLE
+    objs = [] if ctype in (0, 1) else [(kind, [-0.1, 0.0, 0.6, 0.0, 0.0, 0.0], 1.0, float(rhos), 1.0)]
     return dict(m=m, n=m, ax=-1.0, bx=1.0, ay=-1.0, by=1.0, visc=visc, rhof=1.0, tw=2.5, pinw=2.5,
                 w_scale=3.0, w_force=20.0, ev_mult=0.4, dt_pad=0.4 * coarsest / m,
                 dt_ev_pad=0.8 * coarsest / m, x_prd=1, y_prd=1, vel=2, objects=objs)

@@ -27,6 +27,8 @@ def gpu_from_spec(spec, tol=1e-12):
     for kind, p, G, rhos, evm in spec["objects"]:
         if kind == ol.SPIN:
             f.add_spin(*p, G=G, rhos=rhos, ev_trans_mult=evm)
+        elif kind == ol.SQUARE:
+            f.add_square(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
         else:
             f.add_circle(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
     f.set_sim_velocity(spec.get("vel", 0))
@@ -58,10 +60,11 @@ def off_axis(spec):
     return spec
 
 
-@pytest.mark.parametrize("ctype", [0, 4, 5])
+@pytest.mark.parametrize("ctype", [0, 3, 4, 5])
 def test_conv_cases_per_step(ctype):
-    """conv_test types 0 (fluid only), 4 (circle, density 3) and 5 (circle,
-    density 1) in the doubly periodic box, 64^2, 12 steps at a stable dt."""
+    """conv_test types 0 (fluid only), 3 (square, obj_square), 4 (circle,
+    density 3) and 5 (circle, density 1) in the doubly periodic box, 64^2,
+    12 steps at a stable dt."""
     spec = off_axis(ol.conv_spec(64, ctype, 280))
     g, o = run_pair(spec, 12, 0.004)
     # the seam: ghost columns and rows repeat the opposite edge
@@ -118,3 +121,19 @@ def test_unsupported_periodic_size():
         f = ia.Fluid2D(80, 80, visc=1e-3, rhof=1.0, x_prd=True, y_prd=True)
         f.initialize(0.4, 0.4, 0.8)
     assert e.value.code == ia.ERR_UNSUPPORTED
+
+
+def test_square_in_walled_box_per_step():
+    """obj_square (object.cc:41-44, 83-87) sedimenting in the no-slip box:
+    its corners exercise the band rebuild's Newton on a non-smooth level
+    set."""
+    spec = ol.conv_spec(64, 3, 280)
+    spec.update(x_prd=0, y_prd=0, vel=0, visc=0.01, dt_pad=0.4, dt_ev_pad=0.8, ev_mult=0.8)
+    spec["objects"] = [(ol.SQUARE, [0.07, 0.11, 0.3, 1.0, 0.0, 0.0], 1.0, 2.0, 1.0)]
+    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    dx = 2.0 / 64
+    for k in range(1, 11):
+        g.step_forward(g.dt_reg)
+        o.step(o.dt_reg)
+        compare(g, o, 64, dx, f"step {k}")
+    g.close()
