@@ -146,11 +146,14 @@ __device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
 template <int SX, int SY>
 constexpr int smooth_threads() { return SX * SY < 1024 ? SX * SY : 1024; }
 
-template <class T, int KIND, int H, int kSX, int kSY>
+// PER: the level has a periodic direction (compiled separately, so a
+// walled level runs none of the wrapping).
+template <class T, int KIND, int H, int kSX, int kSY, bool PER>
 __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(LevelArgs a, FemC k, const T *z, T *zout,
                                                                             const T *b, const T *c1, const T *c2,
                                                                             int phases, int first_color, int dir,
                                                                             ProArgs pa) {
+    if (!PER) a.px = a.py = 0;
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
     __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
     const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
@@ -208,7 +211,10 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
     }
     for (int e = tid; e < kSX * kSY; e += nt) {
         const int li = e % kSX + H, lj = e / kSX + H, i = i0 + li, j = j0 + lj;
-        if (i < a.nx && j < a.ny) put_z(a, zout, i, j, sz[lj * PW + li]);
+        if (i < a.nx && j < a.ny) {
+            if (PER) put_z(a, zout, i, j, sz[lj * PW + li]);
+            else zout[j * a.zp + i] = sz[lj * PW + li];
+        }
     }
 }
 
@@ -475,8 +481,9 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <class T>
+template <class T, bool PER>
 __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
+    const int px = PER ? A.px : 0, py = PER ? A.py : 0;
     extern __shared__ __align__(16) unsigned char smem[];
     T *pool = (T *)smem;
     __shared__ double red[16];
@@ -486,7 +493,12 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     // a level's LDS arrays as a LevelArgs, for put_z (mirrored periodic rings)
     auto lds = [&](int l) -> LevelArgs {
         const TailLev &v = A.lv[l];
-        return LevelArgs{v.nx, v.ny, v.zp, v.zp, v.kx, v.ky, A.px, A.py};
+        return LevelArgs{v.nx, v.ny, v.zp, v.zp, v.kx, v.ky, px, py};
+    };
+    // an iterate write, with the periodic mirrors when the levels have them
+    auto tput = [&](const LevelArgs &la, T *arr_, int i, int j, T v) {
+        if (PER) put_z(la, arr_, i, j, v);
+        else arr_[j * la.zp + i] = v;
     };
     for (int p = tid; p < kTailArrays * A.total; p += nt) pool[p] = 0;
     __syncthreads();
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             const int i = p % v.nx, j = p / v.nx;
             const T bb = gb[j * A.top_pitch + i], zz = gz[j * A.top_pitch + i];
             arr(1, 0)[j * v.zp + i] = bb;
-            put_z(lds(0), arr(0, 0), i, j, zz);
+            tput(lds(0), arr(0, 0), i, j, zz);
         }
     }
     __syncthreads();
@@ -558,7 +570,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                     if (i >= v.nx || j >= v.ny) continue;
                     T diag;
                     const T off = offd(l, i, j, diag);
-                    if (diag != T(0)) put_z(lds(l), arr(0, l), i, j, (arr(1, l)[j * v.zp + i] - off) / diag);
+                    if (diag != T(0)) tput(lds(l), arr(0, l), i, j, (arr(1, l)[j * v.zp + i] - off) / diag);
                 }
                 sync(w);
             }
@@ -582,12 +594,12 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 for (int dj = 0; dj < 2; dj++)
                     for (int di = 0; di < 2; di++) s += residual(l, 2 * I + di, 2 * J + dj);
                 arr(1, l + 1)[J * c.zp + I] = T(0.25) * s;
-                put_z(lds(l + 1), arr(0, l + 1), I, J, T(0));
+                tput(lds(l + 1), arr(0, l + 1), I, J, T(0));
             }
         } else {
             for (int p = t0; p < f.nx * f.ny; p += st) {
                 const int i = p % f.nx, j = p / f.nx;
-                put_z(lds(l), arr(3, l), i, j, residual(l, i, j));
+                tput(lds(l), arr(3, l), i, j, residual(l, i, j));
             }
             sync(w);
             const T *r = arr(3, l);
@@ -596,7 +608,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 const int I = p % c.nx, J = p / c.nx, q = 2 * J * pz + 2 * I;
                 arr(1, l + 1)[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
                                               + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
-                put_z(lds(l + 1), arr(0, l + 1), I, J, T(0));
+                tput(lds(l + 1), arr(0, l + 1), I, J, T(0));
             }
         }
         sync(w);
@@ -611,8 +623,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             T add;
             if (A.kind == kMAC) {
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-                if (I2 < 0 || I2 >= c.nx) I2 = A.px ? wrap_idx(I2, c.nx) : I;
-                if (J2 < 0 || J2 >= c.ny) J2 = A.py ? wrap_idx(J2, c.ny) : J;
+                if (I2 < 0 || I2 >= c.nx) I2 = px ? wrap_idx(I2, c.nx) : I;
+                if (J2 < 0 || J2 >= c.ny) J2 = py ? wrap_idx(J2, c.ny) : J;
                 add = T(0.5625) * e[J * p2 + I] + T(0.1875) * e[J * p2 + I2] + T(0.1875) * e[J2 * p2 + I]
                       + T(0.0625) * e[J2 * p2 + I2];
             } else {
@@ -622,7 +634,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 else if (!(i & 1)) add = T(0.5) * (e[q] + e[q + p2]);
                 else add = T(0.25) * (e[q] + e[q + 1] + e[q + p2] + e[q + p2 + 1]);
             }
-            put_z(lds(l), arr(0, l), i, j, arr(0, l)[j * f.zp + i] + add);
+            tput(lds(l), arr(0, l), i, j, arr(0, l)[j * f.zp + i] + add);
         }
         sync(w);
         smooth(l, A.nu2, true, w);
@@ -642,8 +654,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             for (int c = 0; c < kTailDirect; c++) row[c] = 0;
             auto put = [&](int di, int dj, T a) {
                 int ii = i + di, jj = j + dj;
-                if (A.px) ii = wrap_idx(ii, v.nx);
-                if (A.py) jj = wrap_idx(jj, v.ny);
+                if (px) ii = wrap_idx(ii, v.nx);
+                if (py) jj = wrap_idx(jj, v.ny);
                 if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.ny) return;
                 const int pn = jj * v.nx + ii;
                 if (pn > 0) row[pn - 1] += a;
@@ -706,10 +718,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         }
         if (lane < n) {
             const int p = lane + 1;
-            put_z(lds(l), arr(0, l), p % v.nx, p / v.nx, x);
+            tput(lds(l), arr(0, l), p % v.nx, p / v.nx, x);
         }
         // point 0 (a warm-started single-level tail holds an iterate there)
-        if (lane == 0) put_z(lds(l), arr(0, l), 0, 0, T(0));
+        if (lane == 0) tput(lds(l), arr(0, l), 0, 0, T(0));
         wave_sync();
         return true;
     };
@@ -747,7 +759,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         // (total() synchronises: every lane has read z before it changes)
         for (int p = t0; p < N; p += st) {
             const int i = p % v.nx, j = p / v.nx;
-            put_z(lds(l), arr(0, l), i, j, arr(0, l)[j * v.zp + i] - mz);
+            tput(lds(l), arr(0, l), i, j, arr(0, l)[j * v.zp + i] - mz);
         }
         sync(w);
     };
@@ -768,10 +780,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         // the top level's iterate back, with its periodic ring mirrors
         const TailLev &v = A.lv[0];
         T *gz = (T *)A.z_top;
-        const LevelArgs ga{v.nx, v.ny, A.top_pitch, A.top_pitch, v.kx, v.ky, A.px, A.py};
+        const LevelArgs ga{v.nx, v.ny, A.top_pitch, A.top_pitch, v.kx, v.ky, px, py};
         for (int p = tid; p < v.nx * v.ny; p += nt) {
             const int i = p % v.nx, j = p / v.nx;
-            put_z(ga, gz, i, j, arr(0, 0)[j * v.zp + i]);
+            tput(ga, gz, i, j, arr(0, 0)[j * v.zp + i]);
         }
     }
 }
@@ -1287,8 +1299,12 @@ static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, c
     for (int sw = 0; sw < sweeps; sw += per) {
         const int nsw = sweeps - sw < per ? sweeps - sw : per;
         const ProArgs p = sw == 0 ? pa : ProArgs{nullptr, 0, 0, 0};  // the correction enters once
-        k_smooth_fused<T, KIND, H, SX, SY><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
-                                                                  reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
+        if (a.px || a.py)
+            k_smooth_fused<T, KIND, H, SX, SY, true><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
+                                                                            reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
+        else
+            k_smooth_fused<T, KIND, H, SX, SY, false><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
+                                                                             reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
         T *t = *z;  // the result is the level's iterate from now on
         *z = *z2;
         *z2 = t;
@@ -1500,7 +1516,8 @@ int Multigrid::vcycle(size_t l) {
     if (l == tail_) {
         size_t sh;
         const TailArgs A = tail_args<double>(L, l, kind_, k, nu1, nu2, false, &sh);
-        k_tail<double><<<1, kTailThreads, sh, s_>>>(A);
+        if (A.px || A.py) k_tail<double, true><<<1, kTailThreads, sh, s_>>>(A);
+        else k_tail<double, false><<<1, kTailThreads, sh, s_>>>(A);
         KCHECK();
         return 0;
     }
@@ -1551,7 +1568,8 @@ int Multigrid::vcycle_f(size_t l) {
     if (l == tail_) {
         size_t sh;
         const TailArgs A = tail_args<float>(L, l, kind_, k, nu1, nu2, true, &sh);
-        k_tail<float><<<1, kTailThreads, sh, s_>>>(A);
+        if (A.px || A.py) k_tail<float, true><<<1, kTailThreads, sh, s_>>>(A);
+        else k_tail<float, false><<<1, kTailThreads, sh, s_>>>(A);
         KCHECK();
         return 0;
     }
