@@ -187,7 +187,8 @@ def main():
     # plain FP64 multigrid warm-started from the previous step's solution as
     # the reference does (mgs_common.hh:48-50), against FP32 V-cycles in FP64
     # refinement started from the linear extrapolation in time of the last
-    # two solutions (the predicted initial guess; no trained network)
+    # two solutions (the predicted initial guess: mgs_base::initial_guess
+    # with mg_lin_t_interp, mgs_common.cc:33-66; no trained network)
     fp64 = gpu_run(m, a.steps, a.warmup, ia.PREC_FP64, timing=True, guess=ia.GUESS_PREVIOUS)
     mixed = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_PREVIOUS)
     pred = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_EXTRAPOLATE)

@@ -90,6 +90,18 @@ int incrmt_fluid_add_square(incrmt_fluid *f, double cx, double cy, double cr, do
 #define INCRMT_SIM_FOUR_ROLLS 1
 #define INCRMT_SIM_VELOCITY_PULSES 2
 int incrmt_fluid_set_sim_velocity(incrmt_fluid *f, int kind);
+/* sim_horiz_flow(U,ax,bx,ay,by) (sim_type.hh:116-145, sim_type.cc:51-65):
+ * initial velocity (U,0) blended with the solids' momenta; inflow (U,0) in
+ * the two left ghost columns of every row after each set_boundaries; zero
+ * pressure on the right edge (fluid_2d::zpr): linearly extrapolated right
+ * ghost columns (fluid_2d.cc:701-704), the edge conditions of
+ * fluid_2d.cc:745-749 and 847-851, the MAC face past the last cell held at
+ * zero q (mgs_mac.hh:50, 95), the FEM node column m fixed at zero
+ * (mgs_fem.cc:21, 85, 112) and no pressure average removed
+ * (fluid_2d.cc:1318). The fluid must be non-periodic in x (the reference's
+ * sim_horiz_flow is periodic in y; walls in y are accepted as well). Set
+ * before initialize. */
+int incrmt_fluid_set_horizontal_flow(incrmt_fluid *f, double U);
 /* fluid_2d::initialize(ev_mult,dt_pad,dt_ev_pad) (fluid_2d.cc:101-122). */
 int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, double dt_ev_pad);
 /* Projection stopping rule: relative residual ||b-Az||/||b|| (the
@@ -97,9 +109,11 @@ int incrmt_fluid_initialize(incrmt_fluid *f, double ev_mult, double dt_pad, doub
 int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int precision);
 /* Starting iterate of the two projections: the previous step's solution
  * (INCRMT_GUESS_PREVIOUS, the reference's warm start, mgs_common.hh:48-50;
- * the default) or the linear extrapolation in time of the last two
- * (INCRMT_GUESS_EXTRAPOLATE: 2 z_n - z_(n-1); no reference counterpart).
- * Either way each solve stops at the same relative residual. */
+ * the default) or the linear extrapolation in time of the last two at the
+ * solve times t + dt/2 (MAC) and t + dt (FEM) (INCRMT_GUESS_EXTRAPOLATE:
+ * mgs_base::initial_guess with mg_lin_t_interp, mgs_common.cc:33-66, an
+ * option the reference compiles off, mgs_common.hh:12). Either way each
+ * solve stops at the same relative residual. */
 #define INCRMT_GUESS_PREVIOUS 0
 #define INCRMT_GUESS_EXTRAPOLATE 1
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
@@ -182,12 +196,22 @@ int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec
  * repeated halving ends on a coarsest level of at most 16 points (e.g.
  * 2^k or 3 * 2^k); others return INCRMT_ERR_UNSUPPORTED. */
 int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out);
+/* The same with the zero-pressure right edge of mgs_base(m,n,x_prd,y_prd,zpr)
+ * (mgs_common.cc:15-17; the reference sets it through fluid_2d::zpr): MAC
+ * a_cc of the last column takes twice the right-edge inverse density
+ * (mgs_mac.hh:50, 95); FEM node column m-1 is fixed at zero (pfix,
+ * mgs_fem.cc:21): arrays keep it, its b and x entries are read as zero and
+ * its z and y entries are written as zero. x must not be periodic. The
+ * system is definite: b is used as it is. */
+int incrmt_mg_create_bc(int kind, int m, int n, int x_prd, int y_prd, int zpr, double dx, double dy,
+                        incrmt_mg **out);
 /* Varying density: the reference's irho layouts (mgs_mac_varying_rho::irho,
  * 2mn+n+m edge values, fluid_2d.cc:1140-1160; mgs_fem_varying_rho::irho, m*n
  * corner values, fluid_2d.cc:1163-1170). NULL restores constant density. */
 int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
 /* mgs_base::solve (mgs_common.hh:72-77): solve A z = b, z warm-started from
- * its input, to ||b-Az|| <= rel_tol ||b|| (b's mean is projected out first). */
+ * its input, to ||b-Az|| <= rel_tol ||b|| (b's mean is projected out first,
+ * except with a zero-pressure edge). */
 int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
                     int *cycles, double *rel_res);
 /* V-cycle issue mode of incrmt_mg_solve (INCRMT_LAUNCH_GRAPH, the default,

@@ -80,6 +80,7 @@ _PROTOS = [
     ("incrmt_fluid_set_guess", _I, [_V, _I]),
     ("incrmt_fluid_set_launch", _I, [_V, _I]),
     ("incrmt_fluid_set_sim_velocity", _I, [_V, _I]),
+    ("incrmt_fluid_set_horizontal_flow", _I, [_V, _D]),
     ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
@@ -89,6 +90,7 @@ _PROTOS = [
     ("incrmt_fluid_frame_output", _I, [_V]),
     ("incrmt_fluid_track", _I, [_V, _I, _dp, _I, _ip]),
     ("incrmt_mg_create", _I, [_I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_V)]),
+    ("incrmt_mg_create_bc", _I, [_I, _I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_V)]),
     ("incrmt_mg_set_irho", _I, [_V, _dp]),
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
     ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
@@ -198,13 +200,22 @@ class Fluid2D:
 
     def set_guess(self, mode):
         """GUESS_PREVIOUS (the reference's warm start) or GUESS_EXTRAPOLATE
-        (2 z_n - z_(n-1)) as the projections' starting iterate."""
+        (the linear extrapolation in time of the last two solutions,
+        mgs_base::initial_guess with mg_lin_t_interp, mgs_common.cc:33-66)
+        as the projections' starting iterate."""
         _check(lib().incrmt_fluid_set_guess(self._h, mode))
 
     def set_sim_velocity(self, kind):
         """SIM_REST, SIM_FOUR_ROLLS or SIM_VELOCITY_PULSES: the sim_type's
         initial fluid velocity (sim_type.cc:30-47); before initialize."""
         _check(lib().incrmt_fluid_set_sim_velocity(self._h, kind))
+
+    def set_horizontal_flow(self, U):
+        """sim_horiz_flow (sim_type.hh:116-145): initial velocity (U, 0),
+        inflow (U, 0) through the left ghost columns and zero pressure on the
+        right edge (fluid_2d::zpr); x must not be periodic. Before
+        initialize."""
+        _check(lib().incrmt_fluid_set_horizontal_flow(self._h, float(U)))
 
     def set_launch(self, mode):
         """LAUNCH_GRAPH (V-cycles captured once as HIP graphs and replayed;
@@ -340,11 +351,14 @@ def simple_spin(m, omega=1.0, rel_tol=1e-8, precision=PREC_FP64, guess=GUESS_PRE
 
 class MgSolver:
     """mgs_mac_* / mgs_fem_* with the standalone constructor
-    (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM)."""
+    (mgs_mac.cc:20-25, mgs_fem.cc:39-43): m x n cells (MAC) or nodes (FEM).
+    zpr: zero pressure on the right edge (mgs_base's zpr,
+    mgs_common.cc:15-17); a FEM system then keeps its fixed node column
+    m-1 in every array, read and written as zero."""
 
-    def __init__(self, kind, m, n, dx, dy, x_prd=False, y_prd=False):
+    def __init__(self, kind, m, n, dx, dy, x_prd=False, y_prd=False, zpr=False):
         h = _V()
-        _check(lib().incrmt_mg_create(kind, m, n, int(x_prd), int(y_prd), dx, dy, ctypes.byref(h)))
+        _check(lib().incrmt_mg_create_bc(kind, m, n, int(x_prd), int(y_prd), int(zpr), dx, dy, ctypes.byref(h)))
         self._h = h
         self.kind, self.m, self.n = kind, m, n
         self.z = np.zeros(m * n)

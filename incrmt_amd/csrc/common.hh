@@ -29,6 +29,11 @@ struct Grid {
     double lax, lay;  // origin of the level-set grid: ax-1.5dx, ay-1.5dy (obj_field.cc:18)
     int px, py;       // periodic directions (sim_type::x_prd, y_prd)
     int vel;          // initial fluid velocity: one of the SimVelocity kinds
+    // horizontal flow (sim_horiz_flow, sim_type.hh:116-145): zero pressure on
+    // the right edge (fluid_2d::zpr) and inflow U through the left ghost
+    // columns (sim_type.cc:51-55)
+    int zpr;
+    double U;
     // cell (i,j), i in [-2,m+1], j in [-2,n+1]
     __host__ __device__ inline int at(int i, int j) const { return kOrigin + (j + 2) * P + i; }
     // level-set point (I,J) = cell (I-2,J-2)
@@ -38,12 +43,15 @@ struct Grid {
 enum ObjKind : int { kSpin = 0, kCircle = 1, kSquare = 2 };
 
 // sim_type::velocity of the reference's simulation types: at rest (the
-// default, sim_type.hh:67-70), four_rolls (sim_type.cc:30-35) and
-// velocity_pulses (sim_type.cc:43-47, sim_type.hh:109-113).
-enum SimVelocity : int { kSimRest = 0, kSimFourRolls = 1, kSimPulses = 2 };
-__host__ __device__ inline void sim_velocity(int kind, double x, double y, double &u, double &v) {
+// default, sim_type.hh:67-70), four_rolls (sim_type.cc:30-35),
+// velocity_pulses (sim_type.cc:43-47, sim_type.hh:109-113) and horiz_flow
+// (u = U, sim_type.hh:129-135).
+enum SimVelocity : int { kSimRest = 0, kSimFourRolls = 1, kSimPulses = 2, kSimHorizFlow = 3 };
+__host__ __device__ inline void sim_velocity(int kind, double x, double y, double &u, double &v, double U) {
     u = v = 0;
-    if (kind == kSimFourRolls) {
+    if (kind == kSimHorizFlow) {
+        u = U;
+    } else if (kind == kSimFourRolls) {
         u = sin(M_PI * x) * cos(M_PI * y);
         v = -cos(M_PI * x) * sin(M_PI * y);
     } else if (kind == kSimPulses) {
@@ -112,6 +120,13 @@ __host__ __device__ inline double obj_phi(const ObjConst &o, double X, double Y)
 
 namespace incrmt {
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
+// Clear device memory and wait for it. A plain hipMemset runs on the null
+// stream and may return before it is done; the library's streams are
+// non-blocking, so their next kernels are not ordered after it.
+inline hipError_t memset_sync(void *p, int v, size_t bytes) {
+    const hipError_t e = hipMemset(p, v, bytes);
+    return e != hipSuccess ? e : hipDeviceSynchronize();
+}
 }
 
 #endif

@@ -110,7 +110,7 @@ struct FluidEngine {
     int dalloc(double *&p) {
         void *q;
         HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(double)));
-        HIP_CHECK(hipMemset(q, 0, (size_t)g.N * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(double)));
         allocs.push_back(q);
         p = (double *)q;
         return 0;
@@ -212,7 +212,7 @@ int FluidEngine::init_tracers() {
     tr.rng = (GlibcRand *)q;
     tr.nout = (int *)((char *)q + ((sizeof(GlibcRand) + 15) / 16) * 16);
     HIP_CHECK(hipMemcpy(tr.rng, &rng, sizeof(GlibcRand), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemset(tr.nout, 0, sizeof(int)));
+    HIP_CHECK(memset_sync(tr.nout, 0, sizeof(int)));
     if (nt > 0) {
         HIP_CHECK(hipMalloc(&q, tm.size() * sizeof(double)));
         allocs.push_back(q);
@@ -264,6 +264,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     ph.xrow = d_xrow;
     HIP_CHECK(hipMalloc((void **)&d_part, 1024 * sizeof(double)));
     allocs.push_back(d_part);
+    HIP_CHECK(memset_sync(d_part, 0, 1024 * sizeof(double)));
     d_pavg = d_part + 1000;
     ph.visc = visc; ph.rhof = rhof; ph.eps = eps;
     ph.w = w_scale * eps; ph.winv = 1 / ph.w; ph.k_rep = w_force * ph.winv;
@@ -277,7 +278,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
             if (int rc = dalloc(*p)) return rc;
         void *q;
         HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(int)));
-        HIP_CHECK(hipMemset(q, 0, (size_t)g.N * sizeof(int)));
+        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(int)));
         allocs.push_back(q);
         o.a.st = (int *)q;
         if (int rc = ls_alloc(g, o.w)) return rc;
@@ -312,10 +313,15 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     }
     // projection solvers (fluid_2d::setup_multigrid, fluid_2d.cc:126-142)
     // m_fem = m nodes along a periodic direction (fluid_2d.cc:30-31)
-    int rc0 = mac.init(kMAC, g.m, g.n, g.dx, g.dy, s, g.px, g.py);
-    if (!rc0) rc0 = fem.init(kFEM, g.px ? g.m : g.m + 1, g.py ? g.n : g.n + 1, g.dx, g.dy, s, g.px, g.py);
+    // m_fem = m nodes along a periodic direction (fluid_2d.cc:30-31); with
+    // zero pressure on the right edge the solver's unknowns stop before the
+    // fixed node column m (mgs_fem.cc:21)
+    int rc0 = mac.init(kMAC, g.m, g.n, g.dx, g.dy, s, g.px, g.py, g.zpr);
+    if (!rc0)
+        rc0 = fem.init(kFEM, (g.px || g.zpr) ? g.m : g.m + 1, g.py ? g.n : g.n + 1, g.dx, g.dy, s, g.px, g.py, g.zpr);
     if (rc0 == INCRMT_ERR_UNSUPPORTED)
-        return fail(rc0, "a periodic direction needs an even cell count whose halvings end at most 16 points");
+        return fail(rc0, "a periodic direction or a zero-pressure edge needs an even cell count whose halvings end "
+                         "at most 16 points");
     if (rc0) return rc0;
     // fluid_2d::init_fields (fluid_2d.cc:195-254)
     for (auto &o : obj) {
@@ -358,6 +364,7 @@ int FluidEngine::step(double dt) {
     if ((rc = launch_mac_src(s, g, f, ml.b, ml.pitch))) return rc;
     if (!const_rho && (rc = mac.set_density(g, f.rho))) return rc;
     MgStats ms{};
+    mac.t_solve = time;  // the MAC solve's time, t + dt/2 (initial_guess(f.time))
     rc = precision == INCRMT_PREC_MIXED ? mac.solve_mixed(mg_tol, max_cycles, &ms) : mac.solve(mg_tol, max_cycles, &ms);
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in MAC problem");
     if (rc) return rc;
@@ -382,15 +389,20 @@ int FluidEngine::step(double dt) {
     MgLevel &fl = fem.fine();
     if ((rc = launch_fem_src(s, g, f, dt, fl.b, fl.pitch))) return rc;
     if (!const_rho && (rc = fem.set_density(g, f.rho))) return rc;
+    fem.t_solve = time;  // t + dt
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
     if (rc) return rc;
     st.fem_cycles = ms.cycles; st.fem_rel_res = ms.rel_res; st.total_fem_cycles += ms.cycles;
     // copy_pressure + update_velocity (fluid_2d.cc:1317-1374)
-    k_pavg_part<<<256, 256, 0, s>>>(fl.nx, fl.ny, g.px, g.py, fl.z, fl.pitch, d_part);
-    HIP_CHECK(hipGetLastError());
-    k_pavg_final<<<1, 256, 0, s>>>(d_part, 256, 1. / (double(g.m) * g.n), d_pavg);
-    HIP_CHECK(hipGetLastError());
+    // (no average is subtracted with zero pressure on the right edge,
+    // fluid_2d.cc:1318: d_pavg stays zero)
+    if (!g.zpr) {
+        k_pavg_part<<<256, 256, 0, s>>>(fl.nx, fl.ny, g.px, g.py, fl.z, fl.pitch, d_part);
+        HIP_CHECK(hipGetLastError());
+        k_pavg_final<<<1, 256, 0, s>>>(d_part, 256, 1. / (double(g.m) * g.n), d_pavg);
+        HIP_CHECK(hipGetLastError());
+    }
     if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return rc;
     if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
     if ((rc = launch_set_boundaries(s, g, f))) return rc;
@@ -411,6 +423,10 @@ struct incrmt_mg {
     Multigrid mg;
     hipStream_t s = nullptr;
     double *x = nullptr, *y = nullptr;
+    // host row length: the solver's nx, plus the fixed node column of a FEM
+    // system with zero pressure on the right edge (read as zero, written
+    // as zero)
+    int hm = 0;
     ~incrmt_mg() {
         if (x) (void)hipFree(x);
         if (y) (void)hipFree(y);
@@ -553,6 +569,7 @@ int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by
     g.lax = ax - 1.5 * g.dx; g.lay = ay - 1.5 * g.dy;
     g.px = x_prd != 0; g.py = y_prd != 0;
     g.vel = kSimRest;
+    g.zpr = 0; g.U = 0;
     e.visc = visc; e.rhof = rhof; e.tw = tw; e.pinw = pinw; e.w_scale = w_scale; e.w_force = w_force;
     e.eps = tw * (g.dx < g.dy ? g.dx : g.dy);
     hipError_t er = hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking);
@@ -918,6 +935,16 @@ int incrmt_fluid_set_sim_velocity(incrmt_fluid *f, int kind) {
     return 0;
 }
 
+int incrmt_fluid_set_horizontal_flow(incrmt_fluid *f, double U) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "the flow is set up by initialize: set it before");
+    if (f->e.g.px) return fail(INCRMT_ERR_ARG, "horizontal flow needs a non-periodic x direction");
+    if (!std::isfinite(U)) return fail(INCRMT_ERR_ARG, "bad flow velocity");
+    Grid &g = f->e.g;
+    g.zpr = 1; g.U = U; g.vel = kSimHorizFlow;
+    return 0;
+}
+
 int incrmt_fluid_set_launch(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_LAUNCH_DIRECT && mode != INCRMT_LAUNCH_GRAPH) return fail(INCRMT_ERR_ARG, "bad launch mode");
@@ -1025,26 +1052,36 @@ void incrmt_fluid_destroy(incrmt_fluid *f) { delete f; }
 
 // ------------------------------------------------------------ standalone MG
 int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out) {
+    return incrmt_mg_create_bc(kind, m, n, x_prd, y_prd, 0, dx, dy, out);
+}
+
+int incrmt_mg_create_bc(int kind, int m, int n, int x_prd, int y_prd, int zpr, double dx, double dy,
+                        incrmt_mg **out) {
     if (!out) return fail(INCRMT_ERR_ARG, "null output handle");
     *out = nullptr;
     if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
     if (m < 2 || n < 2 || !(dx > 0) || !(dy > 0)) return fail(INCRMT_ERR_ARG, "bad grid");
+    if (zpr && x_prd) return fail(INCRMT_ERR_ARG, "a zero-pressure right edge needs a non-periodic x direction");
+    const bool fixcol = zpr && kind == INCRMT_MG_FEM;
     incrmt_mg *h = new incrmt_mg;
     hipError_t er = hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking);
     if (er != hipSuccess) {
         delete h;
         return hip_fail(er, "hipStreamCreate", __FILE__, __LINE__);
     }
-    int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, m, n, dx, dy, h->s, x_prd != 0, y_prd != 0);
+    h->hm = m;
+    int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, fixcol ? m - 1 : m, n, dx, dy, h->s, x_prd != 0,
+                        y_prd != 0, zpr != 0);
     if (rc == INCRMT_ERR_UNSUPPORTED)
-        rc = fail(rc, "a periodic direction needs an even size whose halvings end at most 16 points");
+        rc = fail(rc, "a periodic direction or a zero-pressure edge needs an even size whose halvings end at most "
+                      "16 points");
     if (!rc) {
         const MgLevel &l = h->mg.fine();
         const size_t len = (size_t)(l.ny + 2) * l.pitch + 64;
         if (hipMalloc((void **)&h->x, len * sizeof(double)) != hipSuccess ||
             hipMalloc((void **)&h->y, len * sizeof(double)) != hipSuccess ||
-            hipMemset(h->x, 0, len * sizeof(double)) != hipSuccess ||
-            hipMemset(h->y, 0, len * sizeof(double)) != hipSuccess)
+            memset_sync(h->x, 0, len * sizeof(double)) != hipSuccess ||
+            memset_sync(h->y, 0, len * sizeof(double)) != hipSuccess)
             rc = fail(INCRMT_ERR_HIP, "allocation failed");
     }
     if (rc) {
@@ -1065,13 +1102,17 @@ int incrmt_mg_set_irho(incrmt_mg *h, const double *irho) {
 
 static int mg_copy(incrmt_mg *h, double *dev, const double *host_in, double *host_out) {
     const MgLevel &l = h->mg.fine();
+    const size_t hp = (size_t)h->hm * sizeof(double);
     if (host_in)
-        HIP_CHECK(hipMemcpy2DAsync(dev, l.pitch * sizeof(double), host_in, l.nx * sizeof(double), l.nx * sizeof(double),
-                                   l.ny, hipMemcpyHostToDevice, h->s));
+        HIP_CHECK(hipMemcpy2DAsync(dev, l.pitch * sizeof(double), host_in, hp, l.nx * sizeof(double), l.ny,
+                                   hipMemcpyHostToDevice, h->s));
     else
-        HIP_CHECK(hipMemcpy2DAsync(host_out, l.nx * sizeof(double), dev, l.pitch * sizeof(double), l.nx * sizeof(double),
-                                   l.ny, hipMemcpyDeviceToHost, h->s));
+        HIP_CHECK(hipMemcpy2DAsync(host_out, hp, dev, l.pitch * sizeof(double), l.nx * sizeof(double), l.ny,
+                                   hipMemcpyDeviceToHost, h->s));
     HIP_CHECK(hipStreamSynchronize(h->s));
+    if (host_out)
+        for (int j = 0; j < l.ny; j++)
+            for (int i = l.nx; i < h->hm; i++) host_out[(size_t)j * h->hm + i] = 0;  // the fixed column
     return 0;
 }
 
@@ -1155,13 +1196,13 @@ struct LsStandalone {
         for (auto p : al) {
             void *q;
             HIP_CHECK(hipMalloc(&q, g.N * sizeof(double)));
-            HIP_CHECK(hipMemset(q, 0, g.N * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, g.N * sizeof(double)));
             allocs.push_back(q);
             *p = (double *)q;
         }
         void *q;
         HIP_CHECK(hipMalloc(&q, g.N * sizeof(int)));
-        HIP_CHECK(hipMemset(q, 0, g.N * sizeof(int)));
+        HIP_CHECK(memset_sync(q, 0, g.N * sizeof(int)));
         allocs.push_back(q);
         a.st = (int *)q;
         return ls_alloc(g, w);

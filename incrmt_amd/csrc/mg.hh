@@ -2,8 +2,9 @@
 //   MAC: cell-centred 5-point  div((1/rho) grad q) (mgs_mac.hh:14-102)
 //   FEM: vertex-centred 9-point bilinear-FEM stiffness with cell 1/rho
 //        (mgs_fem.hh:15-136)
-// on pure-Neumann boxes, periodic in x and/or y or not. The reference's engine (TGMG) is an
-// external library absent from the reference tree; this solver honours its
+// on pure-Neumann boxes, periodic in x and/or y or not, or with the
+// zero-pressure right edge of an outflow (zpr, mgs_common.hh:33). The
+// reference's engine (TGMG) is an external library absent from the reference tree; this solver honours its
 // contract (solve A z = b in place, z warm-started from the previous solve)
 // with its own algorithm: colored Gauss-Seidel V-cycles over rediscretized
 // coarse levels, stopped on the relative residual ||b - A z||_2 / ||b||_2.
@@ -15,6 +16,13 @@
 #include <string>
 #include <vector>
 #include "common.hh"
+
+#ifndef INCRMT_MG_NU1
+#define INCRMT_MG_NU1 2
+#endif
+#ifndef INCRMT_MG_NU2
+#define INCRMT_MG_NU2 2
+#endif
 
 namespace incrmt {
 
@@ -43,6 +51,12 @@ struct MgLevel {
     // MAC faces 0 and nx are the two sides of the seam face, FEM has nx
     // cells with the ring cell mirroring cell nx-1
     int px = 0, py = 0;
+    // zero-pressure right edge (the reference's zpr): the point column nx
+    // past the last unknown is held at zero (the ring column). MAC: face nx
+    // carries twice its inverse density (mgs_mac.hh:50, 95); FEM: the fixed
+    // node column of mgs_fem_base::pfix is that ring column, so the level
+    // has nx cells per row. The system is then definite, not singular.
+    int dr = 0;
 };
 
 // Transfer tables between a FEM level and a uniform coarse level whose
@@ -68,16 +82,20 @@ class Multigrid {
     ~Multigrid();
     // Allocate a hierarchy for an m x n system (cells for MAC, nodes for FEM),
     // periodic in x and/or y (mgs_mac.hh:45-56, mgs_fem.hh:47-52: a periodic
-    // FEM direction has m nodes, node m being node 0).
-    int init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px = false, bool py = false);
+    // FEM direction has m nodes, node m being node 0). dr: zero-pressure
+    // right edge (x not periodic); for FEM, m counts the unknown nodes, the
+    // fixed column not included.
+    int init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px = false, bool py = false,
+             bool dr = false);
     // Refresh the periodic ghost ring of a level-0 array (after a host upload).
     int wrap_fine(double *z);
     // Fine-level coefficients from the ghosted cell densities (fluid_2d.cc:
     // 1140-1170 build the same values), then coarsen.
     int set_density(const Grid &g, const double *rho);
     // Fine-level coefficients from the reference's irho layouts on the host
-    // (mgs_mac_varying_rho::irho 2mn+n+m; mgs_fem_varying_rho::irho m*n), or
-    // constant density when irho is null.
+    // (mgs_mac_varying_rho::irho 2mn+n+m; mgs_fem_varying_rho::irho m*n, m
+    // counting the fixed column with dr), or constant density when irho is
+    // null.
     int set_irho_host(const double *irho);
     // Solve with the right-hand side in level 0's b and the warm start in z.
     int solve(double tol, int max_cycles, MgStats *st);
@@ -97,12 +115,19 @@ class Multigrid {
     MgLevel &fine() { return L[0]; }
     const std::vector<MgLevel> &levels() const { return L; }
     MgKind kind() const { return kind_; }
-    int nu1 = 2, nu2 = 2;
+    // a pure-Neumann system (constants in the null space): right-hand sides
+    // and coarsest solutions are made mean-free
+    bool singular() const { return !dr_; }
+    // pre- and post-smoothing sweeps per level and cycle (a build parameter)
+    int nu1 = INCRMT_MG_NU1, nu2 = INCRMT_MG_NU2;
     size_t tail_level() const { return tail_; }
     // timing of fine-level smoothing launches (HIP events on the stream)
     bool time_smoother = false;
-    // start each solve from 2 z_n - z_(n-1) (set for the step's projections)
+    // start each solve from the linear extrapolation in time of the last two
+    // solutions (mgs_base::initial_guess with mg_lin_t_interp), the solve's
+    // time being t_solve (the step's projections set both)
     bool extrapolate = false;
+    double t_solve = 0;
     // replay captured V-cycles (HIP graphs) instead of launching them
     bool use_graphs = true;
     double smoother_ms = 0;
@@ -128,7 +153,8 @@ class Multigrid {
     std::vector<uintptr_t> launch_state() const;
     void restore_buffers(const std::vector<uintptr_t> &key);
     int run_cycle(int which, const std::function<int()> &cycle);
-    long nsolved_ = 0;
+    int scount_ = 0;                 // solves recorded for the extrapolation (0, 1, 2)
+    double t0_ = 0, tprev_ = 0;      // times of the last two solves
     bool resampled_ = false;         // level 1 is a uniform resampling of level 0
     Resample rx_{}, ry_{};
     int make_resample(int nf, double hf, int nc, Resample &r);
@@ -144,6 +170,7 @@ class Multigrid {
     int smooth(size_t l, int sweeps, bool reverse, bool correct = false);
     int smooth_f(size_t l, int sweeps, bool reverse, bool correct = false);
     MgKind kind_ = kMAC;
+    bool dr_ = false;
     std::vector<MgLevel> L;
     size_t tail_ = 0;  // first level handled by the single-workgroup tail cycle
     std::vector<void *> allocs;

@@ -11,6 +11,8 @@
 #include <functional>
 #include <cstring>
 #include <cstdio>
+#include <cmath>
+#include <limits>
 #include <type_traits>
 #include "mg.hh"
 
@@ -55,6 +57,7 @@ struct LevelArgs {
     int nx, ny, zp, cp;
     double kx, ky;
     int px, py;  // periodic directions
+    int dr;      // zero-pressure right edge (ring column nx held at zero)
 };
 
 // Periodic directions keep the ghost ring of an iterate equal to the
@@ -119,9 +122,12 @@ struct ProArgs {
     const void *e;   // coarse correction at (0,0), or null for none
     int ezp, cnx, cny;
     int px, py;      // periodic directions
+    int dr;          // zero-pressure right edge
 };
 // MAC: bilinear cell-centred interpolation, the neighbour clamped at walls
-// (constant extrapolation) and wrapped across periodic seams. FEM: bilinear
+// (constant extrapolation), wrapped across periodic seams and mirrored with
+// the opposite sign past a zero-pressure edge (the correction vanishes on
+// that face). FEM: bilinear
 // vertex interpolation (the next coarse node of a periodic direction's last
 // interval is node 0, which the coarse ring mirrors).
 template <class T, int KIND>
@@ -130,10 +136,12 @@ __device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
     const int I = i >> 1, J = j >> 1, p = pa.ezp;
     if (KIND == kMAC) {
         int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+        const bool flip = pa.dr && I2 >= pa.cnx;
         if (I2 < 0 || I2 >= pa.cnx) I2 = pa.px ? wrap_idx(I2, pa.cnx) : I;
         if (J2 < 0 || J2 >= pa.cny) J2 = pa.py ? wrap_idx(J2, pa.cny) : J;
-        return T(0.5625) * e[J * p + I] + T(0.1875) * e[J * p + I2] + T(0.1875) * e[J2 * p + I]
-               + T(0.0625) * e[J2 * p + I2];
+        T e2 = e[J * p + I2], e22 = e[J2 * p + I2];
+        if (flip) { e2 = -e2; e22 = -e22; }
+        return T(0.5625) * e[J * p + I] + T(0.1875) * e2 + T(0.1875) * e[J2 * p + I] + T(0.0625) * e22;
     }
     const int q = J * p + I;
     if (!(i & 1) && !(j & 1)) return e[q];
@@ -146,14 +154,14 @@ __device__ inline T prolong_at(const ProArgs &pa, int i, int j) {
 template <int SX, int SY>
 constexpr int smooth_threads() { return SX * SY < 1024 ? SX * SY : 1024; }
 
-// PER: the level has a periodic direction (compiled separately, so a
-// walled level runs none of the wrapping).
+// PER: the level has a periodic direction or a zero-pressure edge (compiled
+// separately, so a walled level runs none of the wrapping).
 template <class T, int KIND, int H, int kSX, int kSY, bool PER>
 __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(LevelArgs a, FemC k, const T *z, T *zout,
                                                                             const T *b, const T *c1, const T *c2,
                                                                             int phases, int first_color, int dir,
                                                                             ProArgs pa) {
-    if (!PER) a.px = a.py = 0;
+    if (!PER) { a.px = a.py = 0; pa.dr = 0; }
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
     __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
     const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
@@ -266,7 +274,7 @@ template <class T>
 __global__ void k_mac_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= f.nx || j >= f.ny) return;
-    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py};
+    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py, cl.dr};
     put_z(f, z, i, j, z[j * f.zp + i] + prolong_at<T, kMAC>(pa, i, j));
 }
 
@@ -275,7 +283,7 @@ template <class T>
 __global__ void k_fem_prolong(LevelArgs f, LevelArgs cl, T *z, const T *e) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= f.nx || j >= f.ny) return;
-    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py};
+    const ProArgs pa{e, cl.zp, cl.nx, cl.ny, cl.px, cl.py, cl.dr};
     put_z(f, z, i, j, z[j * f.zp + i] + prolong_at<T, kFEM>(pa, i, j));
 }
 
@@ -375,13 +383,15 @@ __device__ double block_sum(double v, double *red) {
 }
 template <class T>
 __global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const T *fw, const T *fs, const T *kc,
-                               double reduce, int max_sweeps) {
+                               double reduce, int max_sweeps, int sing) {
     __shared__ double red[256];
     const int tid = threadIdx.x, N = a.nx * a.ny;
     auto Q = [&](int p) { return (p / a.nx) * a.zp + p % a.nx; };
     double s = 0;
     for (int p = tid; p < N; p += blockDim.x) s += b[Q(p)];
-    const T mb = T(block_sum<T>(s, red) / N);
+    // a definite system (zero-pressure edge) keeps its right-hand side and
+    // solution as they are
+    const T mb = sing ? T(block_sum<T>(s, red) / N) : T(0);
     // z is the correction the restriction zeroed, or, when the finest level
     // is also the coarsest, the current iterate: it is not reset here
     for (int p = tid; p < N; p += blockDim.x) b[Q(p)] -= mb;
@@ -418,7 +428,7 @@ __global__ void k_coarse_solve(int kind, LevelArgs a, FemC k, T *z, T *b, const 
     }
     s = 0;
     for (int p = tid; p < N; p += blockDim.x) s += z[Q(p)];
-    const T mz = T(block_sum<T>(s, red) / N);
+    const T mz = sing ? T(block_sum<T>(s, red) / N) : T(0);
     for (int p = tid; p < N; p += blockDim.x) z[Q(p)] -= mz;
 }
 
@@ -448,6 +458,8 @@ struct TailArgs {
     int kind, nlev, nu1, nu2, total;
     int wave_from;         // first level run by the first wavefront alone (nlev: none)
     int px, py;            // periodic directions (the LDS iterates keep mirrored rings)
+    int dr;                // zero-pressure right edge (MAC prolongation mirrors with a sign)
+    int sing;              // pure-Neumann system: mean-free coarsest level, one point pinned
     FemC k;
     TailLev lv[kTailMaxLev];
     void *z_top;
@@ -623,10 +635,12 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             T add;
             if (A.kind == kMAC) {
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+                const bool flip = A.dr && I2 >= c.nx;
                 if (I2 < 0 || I2 >= c.nx) I2 = px ? wrap_idx(I2, c.nx) : I;
                 if (J2 < 0 || J2 >= c.ny) J2 = py ? wrap_idx(J2, c.ny) : J;
-                add = T(0.5625) * e[J * p2 + I] + T(0.1875) * e[J * p2 + I2] + T(0.1875) * e[J2 * p2 + I]
-                      + T(0.0625) * e[J2 * p2 + I2];
+                T e2 = e[J * p2 + I2], e22 = e[J2 * p2 + I2];
+                if (flip) { e2 = -e2; e22 = -e22; }
+                add = T(0.5625) * e[J * p2 + I] + T(0.1875) * e2 + T(0.1875) * e[J2 * p2 + I] + T(0.0625) * e22;
             } else {
                 const int q = J * p2 + I;
                 if (!(i & 1) && !(j & 1)) add = e[q];
@@ -640,16 +654,17 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         smooth(l, A.nu2, true, w);
     };
     // Exact solve of a coarsest level of N <= kTailDirect points by the first
-    // wavefront: the right-hand side is mean-free, hence compatible, and the
-    // operator is symmetric with the constants as its null space, so fixing
-    // point 0 at zero leaves a definite system in points 1..N-1, eliminated
-    // without pivoting (row r of dm is point r+1). False (nothing written)
+    // wavefront. A pure-Neumann level has a mean-free, hence compatible,
+    // right-hand side and a symmetric operator with the constants as its null
+    // space, so fixing point 0 at zero leaves a definite system in points
+    // 1..N-1 (o = 1); a definite level keeps all N points (o = 0). Eliminated
+    // without pivoting (row r of dm is point r+o). False (nothing written)
     // when a pivot is zero, not finite or of the other sign.
     auto direct = [&](int l) -> bool {
         const TailLev &v = A.lv[l];
-        const int N = v.nx * v.ny, n = N - 1;
+        const int o = A.sing ? 1 : 0, N = v.nx * v.ny, n = N - o;
         if (lane < n) {
-            const int p = lane + 1, i = p % v.nx, j = p / v.nx, q = j * v.zp + i;
+            const int p = lane + o, i = p % v.nx, j = p / v.nx, q = j * v.zp + i;
             T *row = dm + lane * kTailDirect;
             for (int c = 0; c < kTailDirect; c++) row[c] = 0;
             auto put = [&](int di, int dj, T a) {
@@ -658,7 +673,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 if (py) jj = wrap_idx(jj, v.ny);
                 if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.ny) return;
                 const int pn = jj * v.nx + ii;
-                if (pn > 0) row[pn - 1] += a;
+                if (pn >= o) row[pn - o] += a;
             };
             if (A.kind == kMAC) {
                 const T *fw = arr(2, l), *fs = arr(3, l), kx = T(v.kx), ky = T(v.ky);
@@ -685,7 +700,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         // lane r keeps row r's right-hand side in a register
         T rhs = 0;
         if (lane < n) {
-            const int p = lane + 1;
+            const int p = lane + o;
             rhs = arr(1, l)[(p / v.nx) * v.zp + p % v.nx];
         }
         wave_sync();
@@ -717,16 +732,18 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             if (lane == k) x = xk;
         }
         if (lane < n) {
-            const int p = lane + 1;
+            const int p = lane + o;
             tput(lds(l), arr(0, l), p % v.nx, p / v.nx, x);
         }
-        // point 0 (a warm-started single-level tail holds an iterate there)
-        if (lane == 0) tput(lds(l), arr(0, l), 0, 0, T(0));
+        // the pinned point 0 (a warm-started single-level tail holds an
+        // iterate there)
+        if (o && lane == 0) tput(lds(l), arr(0, l), 0, 0, T(0));
         wave_sync();
         return true;
     };
     // coarsest: mean-free right-hand side, then the direct solve or sweeps to
-    // a residual reduction, then a mean-free iterate
+    // a residual reduction, then a mean-free iterate (pure Neumann; a
+    // definite level skips both projections)
     auto coarsest = [&](bool w) {
         const int l = A.nlev - 1;
         const TailLev &v = A.lv[l];
@@ -734,7 +751,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         auto total = [&](double a) -> double { return w ? tail_wave_sum<T>(a) : tail_block_sum<T>(a, red); };
         double sb = 0;
         for (int p = t0; p < N; p += st) sb += double(arr(1, l)[(p / v.nx) * v.zp + p % v.nx]);
-        const T mb = T(total(sb) / N);
+        const T mb = A.sing ? T(total(sb) / N) : T(0);
         for (int p = t0; p < N; p += st) arr(1, l)[(p / v.nx) * v.zp + p % v.nx] -= mb;
         sync(w);
         const bool solved = w && N <= kTailDirect && N > 1 && direct(l);
@@ -755,7 +772,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         }
         double sz = 0;
         for (int p = t0; p < N; p += st) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
-        const T mz = T(total(sz) / N);
+        const T mz = A.sing ? T(total(sz) / N) : T(0);
         // (total() synchronises: every lane has read z before it changes)
         for (int p = t0; p < N; p += st) {
             const int i = p % v.nx, j = p / v.nx;
@@ -799,7 +816,9 @@ __global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *
     const int c = j * a.cp + i;
     if (j < a.ny) {
         const int ij = g.at(i, j);
-        fw[c] = (!a.px && (i == 0 || i == a.nx)) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
+        // a zero-pressure right edge doubles its face (mgs_mac.hh:95)
+        fw[c] = (!a.px && (i == 0 || i == a.nx)) ? (a.dr && i == a.nx ? 2 * (0.5 / rho[ij] + 0.5 / rho[ij - 1]) : 0.0)
+                                                 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
     }
     if (i < a.nx) {
         const int ij = g.at(i, j);
@@ -808,10 +827,12 @@ __global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *
 }
 // FEM fine cell weights 1/rho (fluid_2d.cc:1163-1170). A periodic
 // direction has nx cells; the seam cell's weight is the ghost cell -1's,
-// as mgs_fem_varying_rho wraps its corner lookup (mgs_fem.hh:99-106).
+// as mgs_fem_varying_rho wraps its corner lookup (mgs_fem.hh:99-106). A
+// zero-pressure edge also has nx cells, the last one between the last
+// unknown node and the fixed column.
 __global__ void k_fem_coef_fine(Grid g, const double *rho, LevelArgs a, double *kc) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.nx - (a.px ? 0 : 1) || j >= a.ny - (a.py ? 0 : 1)) return;
+    if (i >= a.nx - ((a.px || a.dr) ? 0 : 1) || j >= a.ny - (a.py ? 0 : 1)) return;
     const int ci = a.px && i == a.nx - 1 ? -1 : i, cj = a.py && j == a.ny - 1 ? -1 : j;
     kc[j * a.cp + i] = 1. / rho[g.at(ci, cj)];
 }
@@ -824,7 +845,7 @@ __global__ void k_mac_coef_coarsen(LevelArgs f, LevelArgs c, const double *fwf, 
 }
 __global__ void k_fem_coef_coarsen(LevelArgs f, LevelArgs c, const double *kf, double *kcc) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= c.nx - (c.px ? 0 : 1) || J >= c.ny - (c.py ? 0 : 1)) return;
+    if (I >= c.nx - ((c.px || c.dr) ? 0 : 1) || J >= c.ny - (c.py ? 0 : 1)) return;
     const int q = 2 * J * f.cp + 2 * I;
     kcc[J * c.cp + I] = 0.25 * (kf[q] + kf[q + 1] + kf[q + f.cp] + kf[q + f.cp + 1]);
 }
@@ -907,14 +928,14 @@ __global__ void k_residual_f32(int kind, LevelArgs a, FemC k, const double *z, c
     bf[q] = float(b[q] - off - diag * z[q]);
     put_z(a, zf, i, j, 0.0f);
 }
-// Starting iterate from the two previous solutions: z <- 2 z - zp, zp <- z.
-__global__ void k_extrapolate(LevelArgs a, double *z, double *zp) {
+// Starting iterate from the two previous solutions: z <- cb z - cc zp, zp <- z.
+__global__ void k_extrapolate(LevelArgs a, double *z, double *zp, double cb, double cc) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
     const int q = j * a.zp + i;
-    const double t = z[q];
-    put_z(a, z, i, j, 2 * t - zp[q]);
-    zp[q] = t;
+    const double zz = z[q];
+    put_z(a, z, i, j, cb * zz - cc * zp[q]);
+    zp[q] = zz;
 }
 __global__ void k_copy_level(LevelArgs a, const double *z, double *zp) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -959,7 +980,7 @@ FemC fem_consts(double dx, double dy) {
     return k;
 }
 
-LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky, l.px, l.py}; }
+LevelArgs args_of(const MgLevel &l) { return LevelArgs{l.nx, l.ny, l.pitch, l.cp, l.kx, l.ky, l.px, l.py, l.dr}; }
 RsArgs rs_args(const Resample &r) { return RsArgs{r.i0, r.lo, r.hi, r.t}; }
 
 }  // namespace
@@ -985,7 +1006,7 @@ static TailArgs tail_args(const std::vector<MgLevel> &L, size_t t, MgKind kind, 
                           size_t *shmem) {
     TailArgs A{};
     A.kind = kind; A.nlev = (int)(L.size() - t); A.nu1 = nu1; A.nu2 = nu2; A.k = k;
-    A.px = L[t].px; A.py = L[t].py;
+    A.px = L[t].px; A.py = L[t].py; A.dr = L[t].dr; A.sing = L[t].dr ? 0 : 1;
     int off = 0;
     A.wave_from = A.nlev;
     for (int q = 0; q < A.nlev; q++) {
@@ -1060,17 +1081,18 @@ int Multigrid::make_resample(int nf, double hf, int nc, Resample &r) {
     return 0;
 }
 
-int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px, bool py) {
-    kind_ = kind; s_ = s; dx_ = dx; dy_ = dy;
+int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px, bool py, bool dr) {
+    kind_ = kind; s_ = s; dx_ = dx; dy_ = dy; dr_ = dr;
     int nx = m, ny = n;
     double kx = 1. / (dx * dx), ky = 1. / (dy * dy), hx = dx, hy = dy;
     // a periodic direction is coloured across its seam: it needs an even
-    // size on every level that is smoothed
-    if ((px && nx % 2) || (py && ny % 2)) return INCRMT_ERR_GRID;
+    // size on every level that is smoothed; a zero-pressure edge closes a
+    // non-periodic x direction
+    if ((px && nx % 2) || (py && ny % 2) || (dr && px)) return INCRMT_ERR_GRID;
     while (true) {
         MgLevel l{};
         l.nx = nx; l.ny = ny;
-        l.px = px; l.py = py;
+        l.px = px; l.py = py; l.dr = dr;
         l.pitch = round_up(nx + 2, 32);
         l.cp = l.pitch;
         l.kx = kx; l.ky = ky;
@@ -1078,7 +1100,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         auto dalloc = [&](double *&p) -> int {
             void *q;
             HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-            HIP_CHECK(hipMemset(q, 0, len * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
             allocs.push_back(q);
             p = (double *)q + l.pitch + 1;
             return 0;
@@ -1086,7 +1108,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         auto falloc = [&](float *&p) -> int {
             void *q;
             HIP_CHECK(hipMalloc(&q, len * sizeof(float)));
-            HIP_CHECK(hipMemset(q, 0, len * sizeof(float)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(float)));
             allocs.push_back(q);
             p = (float *)q + l.pitch + 1;
             return 0;
@@ -1103,7 +1125,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         }
         l.hx = hx; l.hy = hy;
         L.push_back(l);
-        if (kind == kFEM && !px && !py && L.size() == 1 && nx >= 5 && ny >= 5 && (nx % 2 == 0 || ny % 2 == 0)) {
+        if (kind == kFEM && !px && !py && !dr && L.size() == 1 && nx >= 5 && ny >= 5 && (nx % 2 == 0 || ny % 2 == 0)) {
             // node counts that vertex coarsening cannot halve: the next level
             // is the largest uniform 2^k+1 grid on the same box
             auto pow2p1 = [](int n) { int c = 2; while (2 * (c - 1) + 1 <= n - 1) c = 2 * (c - 1) + 1; return c; };
@@ -1115,17 +1137,18 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
             continue;
         }
         // MAC halves even cell counts; FEM halves 2^k+1 node counts, or even
-        // node counts along a periodic direction (nodes 2I are kept)
+        // node counts along a periodic direction or up to a zero-pressure
+        // edge (nodes 2I are kept, the fixed column stays the ring)
         auto fem_more = [](int k, bool p) { return p ? k % 2 == 0 && k >= 4 : k % 2 == 1 && k >= 5; };
         bool more = kind == kMAC ? (nx % 2 == 0 && ny % 2 == 0 && nx >= 4 && ny >= 4)
-                                 : (fem_more(nx, px) && fem_more(ny, py));
+                                 : (fem_more(nx, px || dr) && fem_more(ny, py));
         if (!more) break;
         if (kind == kMAC) { nx /= 2; ny /= 2; kx *= 0.25; ky *= 0.25; }
-        else { nx = px ? nx / 2 : (nx + 1) / 2; ny = py ? ny / 2 : (ny + 1) / 2; hx *= 2; hy *= 2; }
+        else { nx = (px || dr) ? nx / 2 : (nx + 1) / 2; ny = py ? ny / 2 : (ny + 1) / 2; hx *= 2; hy *= 2; }
     }
-    // a periodic coarsest level is solved exactly (dense elimination in the
-    // tail cycle), so it must be tiny
-    if ((px || py) && (long long)L.back().nx * L.back().ny > 16) return INCRMT_ERR_GRID;
+    // a periodic or zero-pressure coarsest level is solved exactly (dense
+    // elimination in the tail cycle), so it must be tiny
+    if ((px || py || dr) && (long long)L.back().nx * L.back().ny > 16) return INCRMT_ERR_GRID;
     void *q;
     HIP_CHECK(hipMalloc(&q, (3 * kRedBlocks + 8) * sizeof(double)));
     allocs.push_back(q);
@@ -1221,11 +1244,13 @@ int Multigrid::set_irho_host(const double *irho) {
     const bool px = f.px, py = f.py;
     if (kind_ == kMAC) {
         // fw from the left-edge entries, walls zero (mgs_mac.hh:64-102); a
-        // periodic seam's right side is the right-edge entry irhor
+        // periodic seam's right side is the right-edge entry irhor, a
+        // zero-pressure edge twice that entry (mgs_mac.hh:50, 95)
         const long long mn = (long long)nx * ny;
         for (int j = 0; j < ny; j++) {
             for (int i = px ? 0 : 1; i < nx; i++) at(i, j) = irho ? irho[2 * (j * nx + i)] : 1.0;
             if (px) at(nx, j) = irho ? irho[2 * mn + j] : 1.0;
+            if (f.dr) at(nx, j) = 2 * (irho ? irho[2 * mn + j] : 1.0);
         }
         HIP_CHECK(hipMemcpy(f.fw - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         std::fill(h.begin(), h.end(), 0.0);
@@ -1236,10 +1261,12 @@ int Multigrid::set_irho_host(const double *irho) {
         HIP_CHECK(hipMemcpy(f.fs - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         // cell (i,j) weight = irho of its upper-right node (mgs_fem.hh:99-106),
-        // node nx being node 0 along a periodic direction
+        // node nx being node 0 along a periodic direction; with a
+        // zero-pressure edge the reference's rows hold the fixed node too
+        const int rl = f.dr ? nx + 1 : nx;
         for (int j = 0; j < ny - (py ? 0 : 1); j++)
-            for (int i = 0; i < nx - (px ? 0 : 1); i++)
-                at(i, j) = irho ? irho[(size_t)((j + 1) % ny) * nx + (i + 1) % nx] : 1.0;
+            for (int i = 0; i < nx - ((px || f.dr) ? 0 : 1); i++)
+                at(i, j) = irho ? irho[(size_t)((j + 1) % ny) * rl + (i + 1) % rl] : 1.0;
         HIP_CHECK(hipMemcpy(f.kc - f.pitch - 1, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         if (int rc = wrap_coefs(0)) return rc;
     }
@@ -1299,7 +1326,7 @@ static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, c
     for (int sw = 0; sw < sweeps; sw += per) {
         const int nsw = sweeps - sw < per ? sweeps - sw : per;
         const ProArgs p = sw == 0 ? pa : ProArgs{nullptr, 0, 0, 0};  // the correction enters once
-        if (a.px || a.py)
+        if (a.px || a.py || a.dr)
             k_smooth_fused<T, KIND, H, SX, SY, true><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
                                                                             reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
         else
@@ -1446,7 +1473,8 @@ int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
     if (timed && record_begin()) return 6;
     int nl = 0;
     ProArgs pa{nullptr, 0, 0, 0};
-    if (correct) pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py};
+    if (correct)
+        pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py, L[l + 1].dr};
     MgLevel &w = L[l];
     int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), &w.z, &w.z2, v.b,
                                    kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl, pa);
@@ -1467,7 +1495,8 @@ int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
     if (timed && record_begin()) return 6;
     int nl = 0;
     ProArgs pa{nullptr, 0, 0, 0};
-    if (correct) pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py};
+    if (correct)
+        pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py, L[l + 1].dr};
     MgLevel &w = L[l];
     int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), &w.zf, &w.zf2, v.bf,
                                   kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl, pa);
@@ -1522,7 +1551,8 @@ int Multigrid::vcycle(size_t l) {
         return 0;
     }
     if (l + 1 == L.size()) {
-        k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 1e-6, 20000);
+        k_coarse_solve<double><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, 1e-6, 20000,
+                                                  singular());
         KCHECK();
         return 0;
     }
@@ -1574,7 +1604,8 @@ int Multigrid::vcycle_f(size_t l) {
         return 0;
     }
     if (l + 1 == L.size()) {
-        k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 1e-5, 20000);
+        k_coarse_solve<float><<<1, 256, 0, s_>>>(kind_, args_of(f), k, f.zf, f.bf, f.fwf, f.fsf, f.kcf, 1e-5, 20000,
+                                                 singular());
         KCHECK();
         return 0;
     }
@@ -1645,27 +1676,47 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     int rc;
     if ((rc = wrap_fine(f.z))) return rc;  // the iterate may come from outside
     if (extrapolate) {
-        // start from the linear extrapolation in time of the last two
-        // solutions (the system is solved once per step): the start is
-        // O(dt^2) from the new solution instead of O(dt)
-        if (!zprev_) {
-            void *q;
-            const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
-            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-            HIP_CHECK(hipMemset(q, 0, len * sizeof(double)));
-            allocs.push_back(q);
-            zprev_ = (double *)q + f.pitch + 1;
+        // mgs_base::initial_guess (mgs_common.cc:33-66): start from the
+        // linear interpolation in time of the last two solutions, at the
+        // solve time t_solve: the first call records its time, the second
+        // keeps its starting iterate (the first solution) and time, later
+        // calls extrapolate and shift. The start is O(dt^2) from the new
+        // solution instead of O(dt).
+        const double t = t_solve;
+        if (scount_ < 2) {
+            if (scount_ == 1) {
+                if (!zprev_) {
+                    void *q;
+                    const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+                    HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+                    HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+                    allocs.push_back(q);
+                    zprev_ = (double *)q + f.pitch + 1;
+                }
+                k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_);
+                KCHECK();
+                t0_ = t;
+            } else {
+                tprev_ = t;
+            }
+            scount_++;
+        } else if (std::fabs(t0_ - tprev_) < 100. * std::numeric_limits<double>::epsilon()) {
+            // "Multigrid routine called for duplicate times": the start is
+            // left as it is
+        } else {
+            const double ia = 1. / (t0_ - tprev_), cb = (t - tprev_) * ia, cc = (t - t0_) * ia;
+            k_extrapolate<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_, cb, cc);
+            KCHECK();
+            tprev_ = t0_;
+            t0_ = t;
         }
-        if (nsolved_ >= 2)
-            k_extrapolate<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_);
-        else
-            k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zprev_);
+    }
+    if (singular()) {
+        // pure Neumann: only the mean-free part of b is solvable
+        if ((rc = reduce_dev())) return rc;
+        k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
         KCHECK();
     }
-    nsolved_++;
-    if ((rc = reduce_dev())) return rc;
-    k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
-    KCHECK();
     double nr[3];
     if ((rc = reduce_norms(nr))) return rc;
     const double bn = std::sqrt(nr[0]);

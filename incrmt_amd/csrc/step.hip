@@ -126,7 +126,14 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         }
     }
     // no-slip wall faces (fluid_2d.cc:754-776)
-    if (!g.px) {
+    if (g.zpr) {
+        // horizontal flow (fluid_2d.cc:745-749): the inflow face keeps cell
+        // 0's own left state and the ghost cell takes it as its right state;
+        // the outflow ghost cell takes cell m-1's right state. No thread of
+        // the upwinding writes cell 0's left or cell m-1's right state.
+        if (i == 0) e[UR][ij - 1] = e[UL][ij];
+        if (i == g.m - 1) { e[UL][ij + 1] = e[UR][ij]; e[VL][ij + 1] = e[VR][ij]; }
+    } else if (!g.px) {
         if (i == 0) { e[UL][ij] = 0; e[VL][ij] = 0; e[UR][ij - 1] = 0; }
         if (i == g.m - 1) { e[UL][ij + 1] = 0; e[VL][ij + 1] = 0; e[UR][ij] = 0; }
     } else if (i == g.m - 1) {
@@ -519,19 +526,26 @@ __global__ void k_ustar(Grid g, Fields f, Solids o, Phys ph, double dt) {
 // (fluid_2d.cc:839-872) applied as predicates.
 __global__ void k_fem_src(Grid g, Fields f, double dt, double *b, int bp) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    // m_fem = m nodes along a periodic direction, m+1 otherwise
-    if (i >= g.m + (g.px ? 0 : 1) || j >= g.n + (g.py ? 0 : 1)) return;
+    // m_fem = m nodes along a periodic direction, m+1 otherwise; with zero
+    // pressure on the right edge node m is fixed (its source is zeroed,
+    // mgs_fem.cc:85, 112) and not an unknown of the solver
+    if (i >= g.m + ((g.px || g.zpr) ? 0 : 1) || j >= g.n + (g.py ? 0 : 1)) return;
     const int ij = g.at(i, j), P = g.P;
-    // ghost cells: zero beyond a wall, the opposite edge across a seam
-    auto c = [&](const double *a, int ii, int jj, int q) -> double {
+    // ghost cells: zero beyond a wall, the opposite edge across a seam, the
+    // ghost velocity left of an inflow (fluid_2d.cc:847-851)
+    auto c = [&](const double *a, const double *vel, int ii, int jj, int q) -> double {
         if (g.px && ii < 0) { ii += g.m; q += g.m; }
         if (g.py && jj < 0) { jj += g.n; q += g.n * P; }
-        return (ii < 0 || ii >= g.m || jj < 0 || jj >= g.n) ? 0.0 : a[q];
+        if (jj < 0 || jj >= g.n || ii >= g.m) return 0.0;
+        if (ii < 0) return g.zpr ? vel[q] : 0.0;
+        return a[q];
     };
     const double hx = 0.5 * g.dy / dt, hy = 0.5 * g.dx / dt;
-    const double *c0 = f.c0, *c1 = f.c1;
-    b[j * bp + i] = hx * (c(c0, i - 1, j - 1, ij - P - 1) + c(c0, i - 1, j, ij - 1) - c(c0, i, j - 1, ij - P) - c(c0, i, j, ij))
-                    + hy * (c(c1, i - 1, j - 1, ij - P - 1) - c(c1, i - 1, j, ij - 1) + c(c1, i, j - 1, ij - P) - c(c1, i, j, ij));
+    const double *c0 = f.c0, *c1 = f.c1, *u = f.u, *v = f.v;
+    b[j * bp + i] = hx * (c(c0, u, i - 1, j - 1, ij - P - 1) + c(c0, u, i - 1, j, ij - 1) - c(c0, u, i, j - 1, ij - P)
+                          - c(c0, u, i, j, ij))
+                    + hy * (c(c1, v, i - 1, j - 1, ij - P - 1) - c(c1, v, i - 1, j, ij - 1)
+                            + c(c1, v, i, j - 1, ij - P) - c(c1, v, i, j, ij));
 }
 
 // ---- copy_pressure + update_velocity + set_full_timestep
@@ -539,7 +553,8 @@ __global__ void k_fem_src(Grid g, Fields f, double dt, double *b, int bp) {
 // and updates cell (i,j) from its four corners, each formed as z - pavg
 // exactly as copy_pressure stores it.
 // Along a periodic direction node m is node 0: the solver's ghost ring
-// mirrors it, so reading z there is copy_pressure's fp[-m].p.
+// mirrors it, so reading z there is copy_pressure's fp[-m].p. With zero
+// pressure on the right edge node m is the solver's zero ring column.
 __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z, int zp,
                                     const double *pavg_p, double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -562,7 +577,10 @@ __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z,
 }
 
 // ---- fluid_2d::set_boundaries (fluid_2d.cc:689-726), no-slip walls or
-// periodic copies.
+// periodic copies, and sim_horiz_flow::boundary (sim_type.cc:51-55): the
+// left ghost columns of every row, ghost rows included, carry the inflow
+// (U, 0); the right ghost columns extrapolate linearly (field::extrap,
+// fields.hh:45-48).
 // Phase 0 sets the left/right ghost columns of the interior rows, phase 1
 // the bottom/top ghost rows over the full ghosted width (which reads the
 // ghost columns phase 0 wrote, as in the reference).
@@ -576,6 +594,13 @@ __global__ void k_bc_cols(Grid g, Fields f) {
         u[r - 1] = u[r + m - 1]; v[r - 1] = v[r + m - 1];
         u[r + m] = u[r]; v[r + m] = v[r];
         u[r + m + 1] = u[r + 1]; v[r + m + 1] = v[r + 1];
+        return;
+    }
+    if (g.zpr) {
+        u[r - 2] = g.U; v[r - 2] = 0;
+        u[r - 1] = g.U; v[r - 1] = 0;
+        u[r + m] = 2 * u[r + m - 1] - u[r + m - 2]; v[r + m] = 2 * v[r + m - 1] - v[r + m - 2];
+        u[r + m + 1] = 2 * u[r + m] - u[r + m - 1]; v[r + m + 1] = 2 * v[r + m] - v[r + m - 1];
         return;
     }
     u[r - 2] = -u[r + 1]; v[r - 2] = -v[r + 1];
@@ -593,6 +618,11 @@ __global__ void k_bc_rows(Grid g, Fields f) {
         u[b - P] = u[t]; v[b - P] = v[t];
         u[t + P] = u[b]; v[t + P] = v[b];
         u[t + 2 * P] = u[b + P]; v[t + 2 * P] = v[b + P];
+        return;
+    }
+    if (g.zpr && i < 0) {  // the inflow columns of the ghost rows
+        u[b - 2 * P] = u[b - P] = u[t + P] = u[t + 2 * P] = g.U;
+        v[b - 2 * P] = v[b - P] = v[t + P] = v[t + 2 * P] = 0;
         return;
     }
     u[b - 2 * P] = -u[b + P]; v[b - 2 * P] = -v[b + P];
@@ -628,7 +658,7 @@ __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
         f.v[ij] = mv / rho;
     } else {
         double uu, vv, irho = 1 / ((1 - sfrac) * rhof + rho);
-        sim_velocity(g.vel, g.ax + g.dx * (0.5 + i), g.ay + g.dy * (0.5 + j), uu, vv);
+        sim_velocity(g.vel, g.ax + g.dx * (0.5 + i), g.ay + g.dy * (0.5 + j), uu, vv, g.U);
         f.u[ij] = ((1 - sfrac) * rhof * uu + mu) * irho;
         f.v[ij] = ((1 - sfrac) * rhof * vv + mv) * irho;
     }

@@ -410,9 +410,14 @@ void fluid::init_fields() {  // fluid_2d.cc:195-254
 }
 
 // sim_type::velocity (sim_type.hh:67-70; sim_four_rolls, sim_type.cc:30-35;
-// sim_velocity_pulses, sim_type.cc:43-47 with pulse(), sim_type.hh:109-113)
+// sim_velocity_pulses, sim_type.cc:43-47 with pulse(), sim_type.hh:109-113;
+// sim_horiz_flow, sim_type.hh:129-135)
 void fluid::sim_velocity(double x, double y, double &u, double &v) const {
     u = v = 0;
+    if (sp.zpr) {
+        u = sp.U;
+        return;
+    }
     if (sp.vel == 1) {
         u = std::sin(M_PI * x) * std::cos(M_PI * y);
         v = -std::cos(M_PI * x) * std::sin(M_PI * y);
@@ -437,6 +442,11 @@ void fluid::set_boundaries() {  // fluid_2d.cc:689-726: no-slip walls or periodi
         }
         F(-2, j).u = -F(1, j).u; F(-2, j).v = -F(1, j).v;
         F(-1, j).u = -F(0, j).u; F(-1, j).v = -F(0, j).v;
+        if (sp.zpr) {  // field::extrap (fields.hh:45-48)
+            F(m, j).u = 2 * F(m - 1, j).u - F(m - 2, j).u; F(m, j).v = 2 * F(m - 1, j).v - F(m - 2, j).v;
+            F(m + 1, j).u = 2 * F(m, j).u - F(m - 1, j).u; F(m + 1, j).v = 2 * F(m, j).v - F(m - 1, j).v;
+            continue;
+        }
         F(m, j).u = -F(m - 1, j).u; F(m, j).v = -F(m - 1, j).v;
         F(m + 1, j).u = -F(m - 2, j).u; F(m + 1, j).v = -F(m - 2, j).v;
     }
@@ -453,6 +463,11 @@ void fluid::set_boundaries() {  // fluid_2d.cc:689-726: no-slip walls or periodi
         F(i, n).u = -F(i, n - 1).u; F(i, n).v = -F(i, n - 1).v;
         F(i, n + 1).u = -F(i, n - 2).u; F(i, n + 1).v = -F(i, n - 2).v;
     }
+    if (sp.zpr)  // sim_horiz_flow::boundary (sim_type.cc:51-55)
+        for (int j = -2; j < n + 2; j++) {
+            F(-1, j).u = F(-2, j).u = sp.U;
+            F(-1, j).v = F(-2, j).v = 0;
+        }
 }
 
 void fluid::set_rho() {  // fluid_2d.cc:1124-1137, obj_field.cc:85-90
@@ -738,6 +753,12 @@ void fluid::edge_bc() {  // fluid_2d.cc:729-779: no-slip walls or periodic seams
             F(m - 1, j).ur = F(-1, j).ur;
             continue;
         }
+        if (sp.zpr) {  // fluid_2d.cc:745-749
+            F(-1, j).ur = F(0, j).ul;
+            F(m, j).ul = F(m - 1, j).ur;
+            F(m, j).vl = F(m - 1, j).vr;
+            continue;
+        }
         F(0, j).ul = F(0, j).vl = 0;
         F(-1, j).ur = 0;
         F(m, j).ul = F(m, j).vl = 0;
@@ -869,7 +890,7 @@ int fluid::mac_projection() {  // fluid_2d.cc:1175-1236, mgs_mac.cc:58-71
         for (int i = 0; i < m; i++) irho_mac[q++] = 0.5 / F(i, n - 1).rho + 0.5 / F(i, n).rho;
         ir = irho_mac.data();
     }
-    mac_system s{m, n, xxsp, yysp, ir, sp.x_prd, sp.y_prd};
+    mac_system s{m, n, xxsp, yysp, ir, sp.x_prd, sp.y_prd, sp.zpr};
     mac_cycles = mac_solve(s, src.data(), zmac.data(), sp.mg_tol, &mac_res);
     if (mac_cycles < 0) return 2;
     for (int j = 0; j < n; j++)
@@ -960,6 +981,9 @@ int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_f
         if (sp.x_prd) {
             F(-1, j).c0 = F(m - 1, j).c0;
             F(-1, j).c1 = F(m - 1, j).c1;
+        } else if (sp.zpr) {  // fluid_2d.cc:847-851
+            F(-1, j).c0 = F(-1, j).u; F(m, j).c0 = 0;
+            F(-1, j).c1 = F(-1, j).v; F(m, j).c1 = 0;
         } else {
             F(-1, j).c0 = F(m, j).c0 = 0;
             F(-1, j).c1 = F(m, j).c1 = 0;
@@ -988,7 +1012,7 @@ int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_f
             for (int i = 0; i < m_fem; i++) irho_fem[j * m_fem + i] = 1. / F(i - 1, j - 1).rho;
         ir = irho_fem.data();
     }
-    fem_system s{m_fem, n_fem, dx, dy, ir, sp.x_prd, sp.y_prd};
+    fem_system s{m_fem, n_fem, dx, dy, ir, sp.x_prd, sp.y_prd, sp.zpr};
     fem_cycles = fem_solve(s, src.data(), zfem.data(), sp.mg_tol, &fem_res);
     return fem_cycles < 0 ? 3 : 0;
 }
@@ -1003,7 +1027,7 @@ void fluid::copy_pressure() {  // fluid_2d.cc:1317-1348
         if (!sp.y_prd && (j == 0 || j == n)) prow *= 0.5;
         pavg += prow;
     }
-    pavg = pavg * (1. / mn);
+    pavg = sp.zpr ? 0. : pavg * (1. / mn);  // fluid_2d.cc:1318
     // node m (row n) of a periodic direction is node 0 (row 0)
     for (int j = 0; j < n + 1; j++) {
         const int jr = sp.y_prd && j == n ? 0 : j;

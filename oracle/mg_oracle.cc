@@ -23,6 +23,10 @@ struct level {
     int nx, ny;
     bool cell;               // cell-centred (MAC) or vertex (FEM)
     bool px = false, py = false;  // periodic directions: neighbours wrap
+    // zero pressure on the right edge: MAC doubles the face past the last
+    // cell into the diagonal; FEM fixes node column nx-1 (pfix rows hold
+    // only their diagonal and a zero right-hand side)
+    bool zpr = false;
     std::vector<double> c[9];
     std::vector<double> z, b, r;
     // coefficients that define the operator and are averaged to coarsen
@@ -52,7 +56,8 @@ void mac_stencil(level &L) {
             const double cl = hl ? L.kx * el : 0, cr = hr ? L.kx * er : 0;
             const double dc = hd ? L.ky * ed : 0, uc = hu ? L.ky * eu : 0;
             L.c[3][p] = cl; L.c[5][p] = cr; L.c[1][p] = dc; L.c[7][p] = uc;
-            L.c[4][p] = -L.kx * ((hl ? el : 0) + (hr ? er : 0)) - L.ky * ((hd ? ed : 0) + (hu ? eu : 0));
+            // mgs_mac.hh:50 (const rho, -3 xxsp) and 95 (2 er) at a zero-pressure edge
+            L.c[4][p] = -L.kx * ((hl ? el : 0) + (hr ? er : (L.zpr ? 2 * er : 0))) - L.ky * ((hd ? ed : 0) + (hu ? eu : 0));
         }
 }
 
@@ -75,6 +80,9 @@ void fem_stencil(level &L) {
             L.c[1][p] = hey * (kdl + kdr); L.c[7][p] = hey * (kul + kur);
             L.c[3][p] = hex * (kdl + kul); L.c[5][p] = hex * (kdr + kur);
             L.c[4][p] = fm * (kdl + kdr + kul + kur);
+            if (L.zpr && i == nx - 1)  // pfix row (mgs_fem.hh:65-72, 107-126)
+                for (int k = 0; k < 9; k++)
+                    if (k != 4) L.c[k][p] = 0;
         }
 }
 
@@ -167,6 +175,8 @@ void restrict_to(const level &F, level &C) {
                 }
                 C.b[J * C.nx + I] = s;
             }
+        if (F.zpr)  // the fixed column stays at zero
+            for (int J = 0; J < C.ny; J++) C.b[J * C.nx + C.nx - 1] = 0;
     }
 }
 
@@ -176,10 +186,13 @@ void prolong_add(level &F, const level &C) {
             for (int i = 0; i < F.nx; i++) {
                 const int I = i / 2, J = j / 2;
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
+                // past a zero-pressure edge the correction is mirrored with
+                // the opposite sign
+                const double sx = F.zpr && I2 >= C.nx ? -1.0 : 1.0;
                 if (I2 < 0 || I2 >= C.nx) I2 = F.px ? (I2 + C.nx) % C.nx : I;
                 if (J2 < 0 || J2 >= C.ny) J2 = F.py ? (J2 + C.ny) % C.ny : J;
-                F.z[j * F.nx + i] += 0.5625 * C.z[J * C.nx + I] + 0.1875 * C.z[J * C.nx + I2]
-                                     + 0.1875 * C.z[J2 * C.nx + I] + 0.0625 * C.z[J2 * C.nx + I2];
+                F.z[j * F.nx + i] += 0.5625 * C.z[J * C.nx + I] + sx * 0.1875 * C.z[J * C.nx + I2]
+                                     + 0.1875 * C.z[J2 * C.nx + I] + sx * 0.0625 * C.z[J2 * C.nx + I2];
             }
     } else {
         for (int j = 0; j < F.ny; j++)
@@ -204,7 +217,7 @@ struct hierarchy {
             const level &F = L.back();
             level C;
             C.cell = F.cell;
-            C.px = F.px; C.py = F.py;
+            C.px = F.px; C.py = F.py; C.zpr = F.zpr;
             if (F.cell) {
                 if ((F.nx & 1) || (F.ny & 1) || F.nx < 4 || F.ny < 4) break;
                 C.nx = F.nx / 2; C.ny = F.ny / 2;
@@ -239,7 +252,7 @@ struct hierarchy {
         }
     }
     void coarse_solve(level &C) {
-        const double mb = mean(C.b);
+        const double mb = C.zpr ? 0 : mean(C.b);  // a definite system keeps b
         for (double &v : C.b) v -= mb;
         std::fill(C.z.begin(), C.z.end(), 0.0);
         double bn = 0;
@@ -255,7 +268,7 @@ struct hierarchy {
                 if (rn <= 1e-30 * bn) break;
             }
         }
-        const double mz = mean(C.z);
+        const double mz = C.zpr ? 0 : mean(C.z);
         for (double &v : C.z) v -= mz;
     }
     void vcycle(size_t l) {
@@ -275,8 +288,10 @@ struct hierarchy {
         level &F = L[0];
         const size_t N = (size_t)F.nx * F.ny;
         F.b.assign(b, b + N);
-        const double mb = mean(F.b);
+        const double mb = F.zpr ? 0 : mean(F.b);
         for (double &v : F.b) v -= mb;
+        if (F.zpr && !F.cell)  // mgs_fem.cc:85, 112
+            for (int j = 0; j < F.ny; j++) F.b[(size_t)j * F.nx + F.nx - 1] = 0;
         F.z.assign(z, z + N);
         double bn = 0;
         for (double v : F.b) bn += v * v;
@@ -302,7 +317,7 @@ struct hierarchy {
 void fill_mac_fine(const mac_system &s, level &F) {
     const int m = s.m, n = s.n, mn = m * n;
     F.cell = true; F.nx = m; F.ny = n; F.kx = s.xxsp; F.ky = s.yysp;
-    F.px = s.px; F.py = s.py;
+    F.px = s.px; F.py = s.py; F.zpr = s.zpr;
     F.fw.assign((m + 1) * n, 0.0); F.fs.assign(m * (n + 1), 0.0);
     // reference irho layout: [2ij] left edge, [2ij+1] down edge, then the
     // right column (n) and top row (m) (fluid_2d.cc:1140-1160)
@@ -321,7 +336,7 @@ void fill_mac_fine(const mac_system &s, level &F) {
 void fill_fem_fine(const fem_system &s, level &F) {
     const int m = s.m, n = s.n;
     F.cell = false; F.nx = m; F.ny = n; F.dxdy = s.dx / s.dy; F.dydx = s.dy / s.dx;
-    F.px = s.px; F.py = s.py;
+    F.px = s.px; F.py = s.py; F.zpr = s.zpr;
     const int cx = F.cx(), cy = F.cy();
     F.kc.assign((size_t)cx * cy, 1.0);
     // reference corner layout: irho[j*m+i] = 1/rho of the cell below-left of

@@ -64,23 +64,23 @@ extern "C" {
 // periodic in x and/or y when px / py are set (mgs_mac.hh:45-56,
 // mgs_fem.hh:47-52). irho may be null for the constant-density systems.
 void orc_mac_apply(int m, int n, double xxsp, double yysp, const double *irho,
-                   const double *x, double *y, int px, int py) {
-    mac_system s{m, n, xxsp, yysp, irho, px, py};
+                   const double *x, double *y, int px, int py, int zpr) {
+    mac_system s{m, n, xxsp, yysp, irho, px, py, zpr};
     mac_apply(s, x, y);
 }
 void orc_fem_apply(int m, int n, double dx, double dy, const double *irho,
-                   const double *x, double *y, int px, int py) {
-    fem_system s{m, n, dx, dy, irho, px, py};
+                   const double *x, double *y, int px, int py, int zpr) {
+    fem_system s{m, n, dx, dy, irho, px, py, zpr};
     fem_apply(s, x, y);
 }
 int orc_mac_solve(int m, int n, double xxsp, double yysp, const double *irho,
-                  const double *b, double *z, double tol, double *relres, int px, int py) {
-    mac_system s{m, n, xxsp, yysp, irho, px, py};
+                  const double *b, double *z, double tol, double *relres, int px, int py, int zpr) {
+    mac_system s{m, n, xxsp, yysp, irho, px, py, zpr};
     return mac_solve(s, b, z, tol, relres);
 }
 int orc_fem_solve(int m, int n, double dx, double dy, const double *irho,
-                  const double *b, double *z, double tol, double *relres, int px, int py) {
-    fem_system s{m, n, dx, dy, irho, px, py};
+                  const double *b, double *z, double tol, double *relres, int px, int py, int zpr) {
+    fem_system s{m, n, dx, dy, irho, px, py, zpr};
     return fem_solve(s, b, z, tol, relres);
 }
 
@@ -89,15 +89,17 @@ int orc_fem_solve(int m, int n, double dx, double dy, const double *irho,
 extern "C" {
 
 // fluid_2d construction (fluid_2d.cc:26-43, 101-122) for a box with no-slip
-// walls or periodic directions, at rest or with a sim_type's velocity.
+// walls or periodic directions, at rest or with a sim_type's velocity, or
+// sim_horiz_flow's inflow and zero-pressure outflow (zpr, U).
 // params: 9 doubles per object: 6 shape parameters, G, rhos, ev_trans_mult.
 void *orc_fluid_create(int m, int n, double ax, double bx, double ay, double by,
                        double visc, double rhof, double tw, double pinw,
                        double w_scale, double w_force, double ev_mult,
                        double dt_pad, double dt_ev_pad, double mg_tol, int nobj,
-                       const int *kinds, const double *params, int x_prd, int y_prd, int vel) {
+                       const int *kinds, const double *params, int x_prd, int y_prd, int vel, int zpr,
+                       double U) {
     sim_spec sp{m, n, ax, bx, ay, by, visc, rhof, tw, pinw, w_scale, w_force,
-                ev_mult, dt_pad, dt_ev_pad, mg_tol, x_prd, y_prd, vel};
+                ev_mult, dt_pad, dt_ev_pad, mg_tol, x_prd, y_prd, vel, zpr, U};
     std::vector<object_spec> os(nobj);
     for (int k = 0; k < nobj; k++) {
         os[k].kind = kinds[k];

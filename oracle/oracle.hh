@@ -52,26 +52,30 @@ bool extrapolate2(const ls_grid &g, const double *phi, const int *s,
                   const std::vector<int> &band, double *fx, double *fy);
 
 // --------------------------------------------------------------- multigrid
-// Projection systems (mgs_mac.hh:64-102, mgs_fem.hh:86-136), non-periodic,
-// no zero-pressure column. irho layouts are the reference's own
-// (fluid_2d.cc:1140-1170).
+// Projection systems (mgs_mac.hh:64-102, mgs_fem.hh:86-136), walls,
+// periodic directions or a zero-pressure right edge. irho layouts are the
+// reference's own (fluid_2d.cc:1140-1170).
 struct mac_system {
     int m, n;
     double xxsp, yysp;
     const double *irho;  // 2mn+n+m edge inverse densities, or null (const rho)
     int px = 0, py = 0;  // periodic directions (mgs_mac.hh:45-56)
+    int zpr = 0;         // zero pressure past the right edge (mgs_mac.hh:50, 95)
 };
 struct fem_system {
     int m, n;            // nodes (a periodic direction: m nodes, m cells)
     double dx, dy;
     const double *irho;  // m*n (+m padding row) corner inverse densities, or null
     int px = 0, py = 0;  // periodic directions (mgs_fem.hh:47-52)
+    int zpr = 0;         // node column m-1 fixed at zero (pfix, mgs_fem.cc:21)
 };
 // y = A x (full operator, diagonal included)
 void mac_apply(const mac_system &s, const double *x, double *y);
 void fem_apply(const fem_system &s, const double *x, double *y);
 // Solve A z = b to ||b - A z||_2 <= tol ||b||_2 (b's mean is projected out
-// first). z is the warm start on entry. Returns V-cycles, or -1 on failure.
+// first, unless zpr makes the system definite; a FEM zpr solve zeroes b on
+// the fixed column as mgs_fem.cc:85 does). z is the warm start on entry.
+// Returns V-cycles, or -1 on failure.
 int mac_solve(const mac_system &s, const double *b, double *z, double tol,
               double *relres);
 int fem_solve(const fem_system &s, const double *b, double *z, double tol,
@@ -97,6 +101,10 @@ struct sim_spec {
     double mg_tol;  // relative residual for both projections
     int x_prd = 0, y_prd = 0;  // periodic directions (sim_type.hh:63-71)
     int vel = 0;    // sim_type::velocity: 0 rest, 1 four_rolls, 2 velocity_pulses
+    // sim_horiz_flow (sim_type.hh:116-145): zero pressure on the right edge,
+    // inflow (U, 0) on the left, initial velocity (U, 0)
+    int zpr = 0;
+    double U = 0;
 };
 
 class fluid;  // opaque

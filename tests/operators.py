@@ -18,13 +18,15 @@ import numpy as np
 import scipy.sparse as sp
 
 
-def mac_matrix(m, n, dx, dy, irho=None, px=False, py=False):
+def mac_matrix(m, n, dx, dy, irho=None, px=False, py=False, zpr=False):
     """Row c gets k_f (q_nb - q_c) / h^2 for each of its faces. Along a
     periodic direction the wall faces become the seam, whose neighbour is
     the opposite edge's cell; the right side of the seam uses the
     right-edge coefficient irho[2mn + j] and the top side irho[2mn + n + i]
     (mgs_mac.hh:81-96), so the matrix is symmetric only when those agree
-    with the left and bottom entries."""
+    with the left and bottom entries. zpr: q = 0 on the right edge face,
+    half a cell from the last cell's centre, so that face contributes the
+    flux k_f (0 - q_c) / (h^2 / 2) with k_f = irho[2mn + j]."""
     rows, cols, vals = [], [], []
     kx, ky = 1.0 / dx**2, 1.0 / dy**2
     mn = m * n
@@ -44,6 +46,10 @@ def mac_matrix(m, n, dx, dy, irho=None, px=False, py=False):
                 face(c, j * m + (i - 1) % m, kx * coef(2 * c))
             if i < m - 1 or px:
                 face(c, j * m + (i + 1) % m, kx * coef(2 * (c + 1) if i < m - 1 else 2 * mn + j))
+            elif zpr:
+                rows.append(c)
+                cols.append(c)
+                vals.append(-2 * kx * coef(2 * mn + j))
             if j > 0 or py:
                 face(c, ((j - 1) % n) * m + i, ky * coef(2 * c + 1))
             if j < n - 1 or py:
@@ -70,10 +76,12 @@ def bilinear_element(dx, dy):
     return K
 
 
-def fem_matrix(m, n, dx, dy, irho=None, px=False, py=False):
+def fem_matrix(m, n, dx, dy, irho=None, px=False, py=False, zpr=False):
     """m x n nodes; along a periodic direction there are m cells, the last
     one joining node m-1 to node 0 (mgs_fem.hh:99-106 wraps the corner
-    lookup the same way)."""
+    lookup the same way). zpr: node column m-1 is a Dirichlet column held
+    at zero: its rows keep only their diagonal, the other rows are those of
+    the Galerkin matrix."""
     Ke = bilinear_element(dx, dy)
     rows, cols, vals = [], [], []
     for cj in range(n if py else n - 1):
@@ -86,7 +94,16 @@ def fem_matrix(m, n, dx, dy, irho=None, px=False, py=False):
                     rows.append(nodes[p])
                     cols.append(nodes[q])
                     vals.append(k * Ke[p, q])
-    return sp.csr_matrix((vals, (rows, cols)), shape=(m * n, m * n))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(m * n, m * n))
+    if zpr:
+        A = A.tolil()
+        for j in range(n):
+            r = j * m + m - 1
+            d = A[r, r]
+            A[r, :] = 0
+            A[r, r] = d
+        A = A.tocsr()
+    return A
 
 
 def fem_matrix_sparse(m, n, dx, dy):

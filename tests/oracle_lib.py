@@ -113,6 +113,17 @@ def conv_spec(m, ctype, coarsest):
                 dt_ev_pad=0.8 * coarsest / m, x_prd=1, y_prd=1, vel=2, objects=objs)
 
 
+def horiz_flow_spec(m, n, U=1.0, y_prd=True):
+    """sim_horiz_flow(U, -1, 3, -1, 1) (sim_type.hh:116-145): inflow U from
+    the left, zero pressure on the right edge, periodic in y (or between
+    walls with y_prd False), with a soft circle off the mirror lines
+    (obj_circle(-0.0937, 0.0311, 0.3, 0), G=2, rhos=1.5)."""
+    return dict(m=m, n=n, ax=-1.0, bx=3.0, ay=-1.0, by=1.0, visc=1e-2, rhof=1.0, tw=2.5, pinw=2.5,
+                w_scale=3.0, w_force=20.0, ev_mult=0.8, dt_pad=0.4, dt_ev_pad=0.8, x_prd=0,
+                y_prd=int(y_prd), vel=0, zpr=1, U=float(U),
+                objects=[(CIRCLE, [-0.0937, 0.0311, 0.3, 0.0, 0.0, 0.0], 2.0, 1.5, 1.0)])
+
+
 class OracleFluid:
     def __init__(self, spec, mg_tol=1e-13):
         lib = oracle()
@@ -124,7 +135,7 @@ class OracleFluid:
         params = np.array([list(o[1]) + [o[2], o[3], o[4]] for o in objs], dtype=np.float64).ravel()
         D, I = ctypes.c_double, ctypes.c_int
         lib.orc_fluid_create.restype = ctypes.c_void_p
-        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp, I, I, I]
+        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp, I, I, I, I, D]
         lib.orc_fluid_destroy.argtypes = [ctypes.c_void_p]
         lib.orc_fluid_step.argtypes = [ctypes.c_void_p, D]
         lib.orc_fluid_dt_reg.argtypes = [ctypes.c_void_p]
@@ -139,7 +150,7 @@ class OracleFluid:
             spec["rhof"], spec["tw"], spec["pinw"], spec["w_scale"], spec["w_force"],
             spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"], mg_tol, len(objs),
             iptr(kinds), dptr(params), int(spec.get("x_prd", 0)), int(spec.get("y_prd", 0)),
-            int(spec.get("vel", 0)))
+            int(spec.get("vel", 0)), int(spec.get("zpr", 0)), float(spec.get("U", 0.0)))
         self.nobj = len(objs)
         dx, dy = (spec["bx"] - spec["ax"]) / self.m, (spec["by"] - spec["ay"]) / self.n
         self.eps = spec["tw"] * (dx if dx < dy else dy)

@@ -20,10 +20,10 @@ D, I = ctypes.c_double, ctypes.c_int
 
 def _lib():
     L = ol.oracle()
-    L.orc_mac_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I]
-    L.orc_fem_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I]
-    L.orc_mac_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I]
-    L.orc_fem_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I]
+    L.orc_mac_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I, I]
+    L.orc_fem_apply.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, I, I, I]
+    L.orc_mac_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I, I]
+    L.orc_fem_solve.argtypes = [I, I, D, D, ol._dp, ol._dp, ol._dp, D, ol._dp, I, I, I]
     return L
 
 
@@ -31,31 +31,31 @@ def _ptr(a):
     return None if a is None else ol.dptr(a)
 
 
-def mac_apply(m, n, dx, dy, irho, x, px=False, py=False):
+def mac_apply(m, n, dx, dy, irho, x, px=False, py=False, zpr=False):
     y = np.zeros(m * n)
-    _lib().orc_mac_apply(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py))
+    _lib().orc_mac_apply(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py), int(zpr))
     return y
 
 
-def fem_apply(m, n, dx, dy, irho, x, px=False, py=False):
+def fem_apply(m, n, dx, dy, irho, x, px=False, py=False, zpr=False):
     y = np.zeros(m * n)
-    _lib().orc_fem_apply(m, n, dx, dy, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py))
+    _lib().orc_fem_apply(m, n, dx, dy, _ptr(irho), ol.dptr(x), ol.dptr(y), int(px), int(py), int(zpr))
     return y
 
 
-def mac_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False):
+def mac_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False, zpr=False):
     z = np.zeros(m * n)
     r = D(0)
     c = _lib().orc_mac_solve(m, n, 1 / dx**2, 1 / dy**2, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r),
-                             int(px), int(py))
+                             int(px), int(py), int(zpr))
     return z, c, r.value
 
 
-def fem_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False):
+def fem_solve(m, n, dx, dy, irho, b, tol=1e-13, px=False, py=False, zpr=False):
     z = np.zeros(m * n)
     r = D(0)
     c = _lib().orc_fem_solve(m, n, dx, dy, _ptr(irho), ol.dptr(b), ol.dptr(z), tol, ctypes.byref(r),
-                             int(px), int(py))
+                             int(px), int(py), int(zpr))
     return z, c, r.value
 
 
