@@ -81,6 +81,39 @@ int incrmt_fluid_add_circle(incrmt_fluid *f, double cx, double cy, double cr, do
  * object.cc:41-44, 83-87): the square of half-width cr, at rest. */
 int incrmt_fluid_add_square(incrmt_fluid *f, double cx, double cy, double cr, double grav, double G, double rhos,
                             double ev_trans_mult);
+/* fluid_2d::add_object(new obj_full(), mat_const) (object.hh:83-86): a
+ * solid filling the whole box (phi = -1), as conv_test type 2 and ftest's
+ * solid-only case use with solid periodic boundaries. */
+int incrmt_fluid_add_full(incrmt_fluid *f, double G, double rhos, double ev_trans_mult);
+/* fluid_2d::solid_prd_bc (fluid_2d.hh:124; conv_test.cc:120, ftest.cc:233):
+ * every solid's reference map is periodic up to the box lengths in its
+ * ghost layers (set_boundaries_solid, fluid_2d.cc:818-835) and across the
+ * edge upwinding (edge_boundary_conditions_solid_pre/_post,
+ * fluid_2d.cc:784-815), and solids feel neither wall forces nor their own
+ * accelerations (fluid_2d.cc:1037). Needs x_prd = y_prd = 1. Before
+ * initialize. */
+int incrmt_fluid_set_solid_periodic(incrmt_fluid *f, int on);
+/* mat_const::set_velocity (obj_field.hh:25-28, obj_field.cc:62-75) of solid
+ * obj (in the order added): 1 (the default) blends the solid's own initial
+ * velocity into init_fields, 0 leaves the sim_type's velocity there
+ * (conv_test constructs its solids with 0, conv_test.cc:108). Before
+ * initialize. */
+int incrmt_fluid_set_solid_velocity(incrmt_fluid *f, int obj, int set_velocity);
+/* fluid_2d::add_object(new obj_flapper(cx,cy,cr,cl,ar,al,theta,m_stretch,T),
+ * mat_const) (object.hh:245-289, object.cc:284-308): a rounded rod of
+ * half-length cl and cap radius cr, rotated by theta, whose actuated region
+ * (half-length al, radius ar) stretches its deformation gradient by
+ * exp(con Y H(phi)) with con = -log(m_stretch)/ar sin^12(2 pi t / T)
+ * (pre_stress_setup at each stress evaluation). */
+int incrmt_fluid_add_flapper(incrmt_fluid *f, double cx, double cy, double cr, double cl, double ar, double al,
+                             double theta, double m_stretch, double T, double G, double rhos, double ev_trans_mult);
+/* fluid_2d::add_object(new obj_lamprey(cx,cy,theta,le,dis,w1,w2,T,mlam),
+ * mat_const) (object.hh:542-595, object.cc:554-580): the lamprey of Tytell
+ * et al. (head length dis, body length le-dis, half-widths w1, w2), rotated
+ * by theta, swimming by a contraction wave of period T and maximum stretch
+ * mlam ramped in over the first half period. */
+int incrmt_fluid_add_lamprey(incrmt_fluid *f, double cx, double cy, double theta, double le, double dis, double w1,
+                             double w2, double T, double mlam, double G, double rhos, double ev_trans_mult);
 /* The sim_type's initial fluid velocity, which init_fields blends with the
  * solids' momenta (fluid_2d.cc:210-238): at rest (sim_type.hh:67-70, the
  * default), sim_four_rolls (sim_type.cc:30-35) or sim_velocity_pulses

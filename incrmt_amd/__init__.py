@@ -61,6 +61,11 @@ _PROTOS = [
     ("incrmt_fluid_add_spin", _I, [_V, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     ("incrmt_fluid_add_circle", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
     ("incrmt_fluid_add_square", _I, [_V, _D, _D, _D, _D, _D, _D, _D]),
+    ("incrmt_fluid_add_flapper", _I, [_V] + [_D] * 12),
+    ("incrmt_fluid_set_solid_velocity", _I, [_V, _I, _I]),
+    ("incrmt_fluid_add_full", _I, [_V, _D, _D, _D]),
+    ("incrmt_fluid_set_solid_periodic", _I, [_V, _I]),
+    ("incrmt_fluid_add_lamprey", _I, [_V] + [_D] * 12),
     ("incrmt_fluid_initialize", _I, [_V, _D, _D, _D]),
     ("incrmt_fluid_set_solver", _I, [_V, _D, _I, _I]),
     ("incrmt_fluid_step", _I, [_V, _D]),
@@ -190,6 +195,33 @@ class Fluid2D:
 
     def add_square(self, cx, cy, cr, grav, G, rhos, ev_trans_mult=1.0):
         _check(lib().incrmt_fluid_add_square(self._h, cx, cy, cr, grav, G, rhos, ev_trans_mult))
+        self.nobj += 1
+
+    def add_full(self, G, rhos, ev_trans_mult=1.0):
+        """obj_full (object.hh:83-86): a solid filling the box."""
+        _check(lib().incrmt_fluid_add_full(self._h, G, rhos, ev_trans_mult))
+        self.nobj += 1
+
+    def set_solid_periodic(self, on=True):
+        """fluid_2d::solid_prd_bc: periodic reference maps shifted by the box
+        lengths, no wall forces on solids; doubly periodic boxes only."""
+        _check(lib().incrmt_fluid_set_solid_periodic(self._h, int(bool(on))))
+
+    def set_solid_velocity(self, k, set_velocity):
+        """mat_const::set_velocity of solid k (obj_field.hh:25-28): False
+        leaves the sim_type's initial velocity inside it (conv_test)."""
+        _check(lib().incrmt_fluid_set_solid_velocity(self._h, int(k), int(bool(set_velocity))))
+
+    def add_flapper(self, cx, cy, cr, cl, ar, al, theta, m_stretch, T, G, rhos, ev_trans_mult=1.0):
+        """obj_flapper(cx,cy,cr,cl,ar,al,theta,m_stretch,T) (object.hh:245-289)."""
+        _check(lib().incrmt_fluid_add_flapper(self._h, cx, cy, cr, cl, ar, al, theta, m_stretch, T, G, rhos,
+                                              ev_trans_mult))
+        self.nobj += 1
+
+    def add_lamprey(self, cx, cy, theta, le, dis, w1, w2, T, mlam, G, rhos, ev_trans_mult=1.0):
+        """obj_lamprey(cx,cy,theta,le,dis,w1,w2,T,mlam) (object.hh:542-595)."""
+        _check(lib().incrmt_fluid_add_lamprey(self._h, cx, cy, theta, le, dis, w1, w2, T, mlam, G, rhos,
+                                              ev_trans_mult))
         self.nobj += 1
 
     def initialize(self, ev_mult=0.8, dt_pad=0.4, dt_ev_pad=0.8):

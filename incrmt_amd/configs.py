@@ -5,7 +5,7 @@
     python -m incrmt_amd.configs multibody  2048^2 cells, 64 circles, mixed vs FP64
     python -m incrmt_amd.configs ensemble   256 rotor instances of uneven size
     python -m incrmt_amd.configs conv       resolution ladder of the rotor (conv_test)
-    python -m incrmt_amd.configs conv-test  conv_test itself: periodic pulses, type 0/1/4..7
+    python -m incrmt_amd.configs conv-test  conv_test itself: periodic pulses, types 0..7
 """
 import argparse
 import json

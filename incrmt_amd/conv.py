@@ -160,15 +160,21 @@ def conv_test_case(m, ctype, coarsest=280, rel_tol=1e-10, precision=0):
     periodic [-1,1]^2 box with sim_velocity_pulses, initialized with
     ev_mult 0.4 and the timestep paddings 0.4 and 0.8 scaled by
     coarsest / m (the reference scales by its coarsest grid, 280), so the
-    timestep falls as h^2."""
+    timestep falls as h^2. Its solids leave the pulses field inside them
+    (mat_const(G, rhos, ev_trans_mult, false), conv_test.cc:108)."""
     import incrmt_amd as ia
-    if ctype not in (0, 1, 3, 4, 5, 6, 7):
-        raise ValueError("conv_test types 0, 1 and 3..7 are built")
+    if ctype not in range(8):
+        raise ValueError("conv_test types are 0..7")
     f = ia.Fluid2D(m, m, visc=CONV_VISC[ctype], rhof=1.0, tw=2.5, pinw=2.5, x_prd=True, y_prd=True)
-    if ctype == 3:
+    if ctype == 2:  # obj_full with solid periodic boundaries (conv_test.cc:83-85, 120)
+        f.add_full(G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
+        f.set_solid_periodic(True)
+    elif ctype == 3:
         f.add_square(-0.1, 0.0, 0.6, 0.0, G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
     elif ctype >= 4:
         f.add_circle(-0.1, 0.0, 0.6, 0.0, G=1.0, rhos=float(CONV_RHOS[ctype]), ev_trans_mult=1.0)
+    if ctype >= 2:
+        f.set_solid_velocity(0, False)
     f.set_sim_velocity(ia.SIM_VELOCITY_PULSES)
     f.set_solver(rel_tol=rel_tol, precision=precision)
     f.initialize(0.4, 0.4 * coarsest / m, 0.8 * coarsest / m)

@@ -34,13 +34,17 @@ struct Grid {
     // columns (sim_type.cc:51-55)
     int zpr;
     double U;
+    // fluid_2d::solid_prd_bc: the reference maps are periodic up to the box
+    // lengths (set_boundaries_solid, edge_boundary_conditions_solid_pre /
+    // _post, fluid_2d.cc:784-835) and solids feel no wall force
+    int spbc;
     // cell (i,j), i in [-2,m+1], j in [-2,n+1]
     __host__ __device__ inline int at(int i, int j) const { return kOrigin + (j + 2) * P + i; }
     // level-set point (I,J) = cell (I-2,J-2)
     __host__ __device__ inline int ls(int I, int J) const { return kOrigin - 2 + J * P + I; }
 };
 
-enum ObjKind : int { kSpin = 0, kCircle = 1, kSquare = 2 };
+enum ObjKind : int { kSpin = 0, kCircle = 1, kSquare = 2, kFlapper = 3, kLamprey = 4, kFull = 5 };
 
 // sim_type::velocity of the reference's simulation types: at rest (the
 // default, sim_type.hh:67-70), four_rolls (sim_type.cc:30-35),
@@ -64,12 +68,17 @@ __host__ __device__ inline void sim_velocity(int kind, double x, double y, doubl
     }
 }
 
-// Constants of one solid (obj_field.hh:32-135, object.hh:128-138, 422-467).
+// Constants of one solid (obj_field.hh:32-135, object.hh:128-138, 245-289,
+// 422-467, 542-595).
 struct ObjConst {
     int kind;
+    int set_velocity;  // mat_const::set_velocity (obj_field.hh:25-28): the solid sets its initial velocity
     double eps, pinw, G, rhos, ev_trans_mult, ex_visc, tf1, tf2, tf3;
     double c0, c1, c2, piv_r, th_max, omega, K_stiff, piv_thresh;  // simple spin
-    double cx, cy, cr, grav;                                       // circle
+    double cx, cy, cr, grav;                                       // circle, square; centre of all
+    double cl, ar, al, amp;                                        // flapper (omega shared)
+    double dis, la, w1, w2, T, A;                                  // lamprey
+    double cth, sth;                                               // flapper / lamprey rotation
     double phi_down, phi_up;                                       // band cut-offs
 };
 
@@ -96,11 +105,46 @@ __device__ inline double tderiv_func_in(const ObjConst &o, double x) {
 __device__ inline double coll_func(const ObjConst &o, double x) {
     return x > o.eps ? 0 : 0.5 - x * o.tf1;  // obj_field.hh:97-99
 }
-// object::phi for the supported solids (object.cc:22-24, 83-87, 167-171)
+// object::actuate of the actuated solids: the deformation gradient
+// (Fa Fb; Fc Fd) scaled by an active stretch st, (a, c) by st and (b, d) by
+// 1/st. obj_flapper (object.cc:301-308): st = exp(con Y H(ph)) on the
+// rounded rod ph of the actuated region; obj_lamprey (object.cc:573-580): a
+// contraction wave of shift sh along the body. s0, s1: the solid's
+// pre_stress_setup state (con, sh).
+__device__ inline void actuate(const ObjConst &o, double s0, double s1, double X, double Y, double &Fa, double &Fb,
+                               double &Fc, double &Fd) {
+    if (o.kind == kFlapper) {
+        const double fX = fabs(X) - o.al, ph = (fX < 0 ? fabs(Y) : sqrt(fX * fX + Y * Y)) - o.ar;
+        if (ph < o.eps) {
+            const double fac = exp(s0 * Y * trans_func_in(o, ph)), inv = 1. / fac;
+            Fa *= fac; Fb *= inv;
+            Fc *= fac; Fd *= inv;
+        }
+    } else if (o.kind == kLamprey) {
+        const double dx = X + (Y > 0 ? 1.5 * o.la : o.la) + s1, wl = o.w1 - (o.w2 - o.w1) * X / o.la,
+                     fac = fabs(Y) / wl;
+        const double a = -X, b = o.la * 0.15 - fabs(dx - o.la * (int(dx / o.la) + 0.5));
+        const double st = exp(s0 * (fac < 1 ? fac * (2 - fac) : 1) * trans_func(o, -(b < a ? b : a))), inv = 1. / st;
+        Fa *= st; Fb *= inv;
+        Fc *= st; Fd *= inv;
+    }
+}
+// object::phi for the supported solids (object.hh:85, object.cc:22-24,
+// 83-87, 167-171, 284-287, 554-558)
 __host__ __device__ inline double obj_phi(const ObjConst &o, double X, double Y) {
+    if (o.kind == kFull) return -1;  // obj_full fills the box (object.hh:83-86)
     if (o.kind == kSpin) {
         double th = atan2(Y, X), c = cos(3 * th);
         return sqrt(X * X + Y * Y) - o.c0 - c * (o.c1 + o.c2 * c);
+    }
+    if (o.kind == kFlapper) {  // a rounded rod
+        const double fX = fabs(X) - o.cl;
+        return (fX < 0 ? fabs(Y) : sqrt(fX * fX + Y * Y)) - o.cr;
+    }
+    if (o.kind == kLamprey) {
+        if (X > 0) return o.dis * Y * Y / (o.w1 * o.w1) + X - o.dis;
+        const double wl = o.w1 - (o.w2 - o.w1) * X / o.la, a = -o.la - X, b = Y * Y / (wl * wl) - 1;
+        return a < b ? b : a;  // std::max
     }
     if (o.kind == kSquare) {
         X = fabs(X) - o.cr;

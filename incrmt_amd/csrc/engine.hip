@@ -135,14 +135,30 @@ struct FluidEngine {
         else dt_reg = dt1 < dt2 ? dt1 : dt2;
     }
 
-    // obj_simple_spin::pre_stress_setup (object.cc:175-181), host arithmetic
+    // object::pre_stress_setup at the current time, host arithmetic:
+    // obj_simple_spin (object.cc:175-181), obj_flapper (object.cc:291-296),
+    // obj_lamprey (object.cc:563-568)
     void pre_stress_setup() {
         for (size_t k = 0; k < obj.size(); k++) {
             const ObjConst &c = obj[k].c;
-            if (c.kind != kSpin) continue;
-            double ang = time * c.omega, theta = ang > 2 * M_PI ? 0 : c.th_max * (1 - std::cos(ang));
-            ph.rot[2 * k] = std::cos(theta);
-            ph.rot[2 * k + 1] = std::sin(theta);
+            if (c.kind == kSpin) {
+                double ang = time * c.omega, theta = ang > 2 * M_PI ? 0 : c.th_max * (1 - std::cos(ang));
+                ph.pss[2 * k] = std::cos(theta);
+                ph.pss[2 * k + 1] = std::sin(theta);
+            } else if (c.kind == kFlapper) {
+                double a = std::sin(c.omega * time);
+                a *= a * a;
+                a *= a;
+                a *= a;
+                ph.pss[2 * k] = -c.amp * a;
+                ph.pss[2 * k + 1] = 0;
+            } else if (c.kind == kLamprey) {
+                double sh = time / c.T;
+                const double o = 2 * sh;
+                ph.pss[2 * k] = o < 1 ? c.A * o * o * (3 - 2 * o) : c.A;
+                sh *= c.la;
+                ph.pss[2 * k + 1] = sh;
+            }
         }
     }
     int compute_stress() {
@@ -287,8 +303,16 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
             for (int i = -2; i < g.m + 2; i++) {
                 const int ij = g.at(i, j);
                 double xx = g.ax + g.dx * (0.5 + i), X, Y;
-                if (o.c.kind != kSpin) { X = xx - o.c.cx; Y = y - o.c.cy; }  // obj_standard::transform
-                else { X = xx; Y = y; }
+                if (o.c.kind == kFlapper || o.c.kind == kLamprey) {
+                    // translation and rotation (object.hh:270-274, 576-580)
+                    const double x0 = xx - o.c.cx, y0 = y - o.c.cy;
+                    X = x0 * o.c.cth + y0 * o.c.sth;
+                    Y = -x0 * o.c.sth + y0 * o.c.cth;
+                } else if (o.c.kind != kSpin) {
+                    X = xx - o.c.cx; Y = y - o.c.cy;  // obj_standard::transform
+                } else {
+                    X = xx; Y = y;
+                }
                 hX[ij] = X; hY[ij] = Y;
                 hphi[ij] = obj_phi(o.c, X, Y);
             }
@@ -329,7 +353,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
         if (int rc = ls_rebuild(s, g, o.c, o.a, o.w, &passes)) return rc;
     }
     if (int rc = launch_init_velocity(s, g, f, solids(), rhof)) return rc;
-    if (int rc = launch_set_boundaries(s, g, f)) return rc;
+    if (int rc = launch_set_boundaries(s, g, f, solids())) return rc;
     if (int rc = compute_stress()) return rc;
     if (int rc = set_rho()) return rc;
     HIP_CHECK(hipStreamSynchronize(s));
@@ -381,7 +405,10 @@ int FluidEngine::step(double dt) {
     if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
     st.reinit_passes = passes;
     // u,v are unchanged since the last set_boundaries, so its call here
-    // (fluid_2d.cc:340) would rewrite the same ghost values
+    // (fluid_2d.cc:340) would rewrite the same ghost values; the reference
+    // maps have moved by half a step, and with solid periodic boundaries
+    // their ghost values are its first part (set_boundaries_solid)
+    if ((rc = launch_set_boundaries_solid(s, g, o))) return rc;
     if ((rc = compute_stress())) return rc;
     if ((rc = launch_ustar(s, g, f, o, ph, dt))) return rc;
     time = ntime;
@@ -405,7 +432,7 @@ int FluidEngine::step(double dt) {
     }
     if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return rc;
     if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
-    if ((rc = launch_set_boundaries(s, g, f))) return rc;
+    if ((rc = launch_set_boundaries(s, g, f, o))) return rc;
     if ((rc = compute_stress())) return rc;
     if ((rc = tracers_stage2(s, g, f.u, f.v, d_oa, tr, dt, time))) return rc;
     st.steps++;
@@ -481,6 +508,7 @@ int add_object(incrmt_fluid *f, const ObjConst &c0) {
     if (f->e.initialized) return fail(INCRMT_ERR_ARG, "add_object after initialize");
     Solid so{};
     so.c = c0;
+    so.c.set_velocity = 1;  // mat_const's default
     FluidEngine &e = f->e;
     // obj_field constants (obj_field.cc:14-20)
     so.c.eps = e.eps;
@@ -569,7 +597,7 @@ int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by
     g.lax = ax - 1.5 * g.dx; g.lay = ay - 1.5 * g.dy;
     g.px = x_prd != 0; g.py = y_prd != 0;
     g.vel = kSimRest;
-    g.zpr = 0; g.U = 0;
+    g.zpr = 0; g.U = 0; g.spbc = 0;
     e.visc = visc; e.rhof = rhof; e.tw = tw; e.pinw = pinw; e.w_scale = w_scale; e.w_force = w_force;
     e.eps = tw * (g.dx < g.dy ? g.dx : g.dy);
     hipError_t er = hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking);
@@ -608,6 +636,54 @@ int incrmt_fluid_add_square(incrmt_fluid *f, double cx, double cy, double cr, do
     ObjConst c{};
     c.kind = kSquare;
     c.cx = cx; c.cy = cy; c.cr = cr; c.grav = grav;
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_set_solid_velocity(incrmt_fluid *f, int obj, int set_velocity) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "the initial velocity is set by initialize: set it before");
+    if (obj < 0 || obj >= (int)f->e.obj.size()) return fail(INCRMT_ERR_ARG, "no such solid");
+    f->e.obj[obj].c.set_velocity = set_velocity != 0;
+    return 0;
+}
+
+int incrmt_fluid_add_full(incrmt_fluid *f, double G, double rhos, double ev_trans_mult) {
+    ObjConst c{};
+    c.kind = kFull;
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_set_solid_periodic(incrmt_fluid *f, int on) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (f->e.initialized) return fail(INCRMT_ERR_ARG, "set solid periodic boundaries before initialize");
+    if (on && !(f->e.g.px && f->e.g.py)) return fail(INCRMT_ERR_ARG, "solid periodic boundaries need a doubly periodic box");
+    f->e.g.spbc = on != 0;
+    return 0;
+}
+
+int incrmt_fluid_add_flapper(incrmt_fluid *f, double cx, double cy, double cr, double cl, double ar, double al,
+                             double theta, double m_stretch, double T, double G, double rhos, double ev_trans_mult) {
+    if (!(m_stretch > 0) || !(ar > 0) || !(T > 0)) return fail(INCRMT_ERR_ARG, "bad flapper parameters");
+    ObjConst c{};
+    c.kind = kFlapper;
+    c.cx = cx; c.cy = cy; c.cr = cr; c.cl = cl; c.ar = ar; c.al = al;
+    c.amp = std::log(m_stretch) / ar;  // object.hh:264-267
+    c.omega = 2 * M_PI / T;
+    c.cth = std::cos(theta); c.sth = std::sin(theta);
+    c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
+    return add_object(f, c);
+}
+
+int incrmt_fluid_add_lamprey(incrmt_fluid *f, double cx, double cy, double theta, double le, double dis, double w1,
+                             double w2, double T, double mlam, double G, double rhos, double ev_trans_mult) {
+    if (!(mlam > 0) || !(T > 0) || !(le > dis) || !(w1 != 0)) return fail(INCRMT_ERR_ARG, "bad lamprey parameters");
+    ObjConst c{};
+    c.kind = kLamprey;
+    c.cx = cx; c.cy = cy; c.dis = dis; c.la = le - dis; c.w1 = w1; c.w2 = w2;  // object.hh:567-570
+    c.T = T; c.A = std::log(mlam);
+    c.cth = std::cos(theta); c.sth = std::sin(theta);
     c.G = G; c.rhos = rhos; c.ev_trans_mult = ev_trans_mult;
     return add_object(f, c);
 }

@@ -27,7 +27,10 @@ enum { UL = 0, VL, UR, VR, UD, VD, UU, VU };
 struct Phys {
     double visc, rhof, eps;
     double w, winv, k_rep;  // wall repulsion (fluid_2d.cc:1049)
-    double rot[2 * kMaxObj];  // per-solid anchor rotation (cos, sin)
+    // per-solid state of object::pre_stress_setup at the current time:
+    // simple spin (cos, sin) of the anchor rotation; flapper (con, 0);
+    // lamprey (con, sh)
+    double pss[2 * kMaxObj];
     const double *xrow;       // x of cell i formed as compute_ustar forms it
 };
 
@@ -51,7 +54,10 @@ int launch_ustar(hipStream_t s, const Grid &g, const Fields &f, Solids o, const 
 int launch_fem_src(hipStream_t s, const Grid &g, const Fields &f, double dt, double *b, int bp);
 int launch_pressure_velocity(hipStream_t s, const Grid &g, const Fields &f, Solids o,
                              const double *z, int zp, const double *pavg, double dt);
-int launch_set_boundaries(hipStream_t s, const Grid &g, const Fields &f);
+// set_boundaries (fluid_2d.cc:689-726) with set_boundaries_solid first when
+// the grid has solid periodic boundaries
+int launch_set_boundaries(hipStream_t s, const Grid &g, const Fields &f, Solids o);
+int launch_set_boundaries_solid(hipStream_t s, const Grid &g, Solids o);
 int launch_init_velocity(hipStream_t s, const Grid &g, const Fields &f, Solids o, double rhof);
 
 }  // namespace incrmt

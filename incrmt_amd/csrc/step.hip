@@ -143,6 +143,11 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         // m-1 the ghost's right state. Nothing else in the sweep reads or
         // writes these values, so the seam of row j is this thread's.
         const int r0 = g.at(0, j), gl = r0 - 1, gr = ij + 1;
+        if (!TANG && g.spbc)  // edge_boundary_conditions_solid_pre (fluid_2d.cc:784-797)
+            for (int k = 0; k < o.n; k++) {
+                const ObjArrays &a = o.oa[k];
+                a.Xr[gl] = a.Xr[ij] - (g.bx - g.ax); a.Yr[gl] = a.Yr[ij];
+            }
         double ur = e[UR][ij], vr = e[VR][ij], ul0 = e[UL][r0], vl0 = e[VL][r0];
         if (ur + ul0 > 0 && ur > 0) {
             ul0 = ur;
@@ -169,6 +174,11 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         e[UL][r0] = ul0; e[VL][r0] = vl0;
         e[UL][gr] = ul0; e[VL][gr] = vl0;
         e[UR][ij] = ur;
+        if (!TANG && g.spbc)  // edge_boundary_conditions_solid_post (fluid_2d.cc:802-815)
+            for (int k = 0; k < o.n; k++) {
+                const ObjArrays &a = o.oa[k];
+                a.Xl[gr] = a.Xl[r0] + (g.bx - g.ax); a.Yl[gr] = a.Yl[r0];
+            }
     }
     if (!g.py) {
         if (j == 0) { e[UD][ij] = 0; e[VD][ij] = 0; e[VU][ij - P] = 0; }
@@ -176,6 +186,11 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
     } else if (j == g.n - 1) {
         // periodic seam of column i (fluid_2d.cc:760-769), as above
         const int r0 = g.at(i, 0), gd = r0 - P, gu = ij + P;
+        if (!TANG && g.spbc)
+            for (int k = 0; k < o.n; k++) {
+                const ObjArrays &a = o.oa[k];
+                a.Xu[gd] = a.Xu[ij]; a.Yu[gd] = a.Yu[ij] - (g.by - g.ay);
+            }
         double vu = e[VU][ij], uu = e[UU][ij], vd0 = e[VD][r0], ud0 = e[UD][r0];
         if (vu + vd0 > 0 && vu > 0) {
             vd0 = vu;
@@ -202,6 +217,11 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         e[VD][r0] = vd0; e[UD][r0] = ud0;
         e[VD][gu] = vd0; e[UD][gu] = ud0;
         e[VU][ij] = vu;
+        if (!TANG && g.spbc)
+            for (int k = 0; k < o.n; k++) {
+                const ObjArrays &a = o.oa[k];
+                a.Xd[gu] = a.Xd[r0]; a.Yd[gu] = a.Yd[r0] + (g.by - g.ay);
+            }
     }
 }
 
@@ -236,13 +256,13 @@ __device__ inline void acceleration(const Grid &g, const Fields &f, const Solids
     for (int k = 0; k < o.n; k++) {
         const ObjConst &c = o.oc[k];
         const double phiv = o.oa[k].phi[ij];
-        if (phiv < c.eps) {
+        if (phiv < c.eps && !g.spbc) {  // fluid_2d.cc:1037
             wall_force(g, ph, x, y, phiv, accx, accy);
             if (c.kind == kSpin) {
                 double rsq = x * x + y * y;
                 if (rsq < c.piv_thresh) {
                     const double X = o.oa[k].X[ij], Y = o.oa[k].Y[ij];
-                    const double cth = ph.rot[2 * k], sth = ph.rot[2 * k + 1];
+                    const double cth = ph.pss[2 * k], sth = ph.pss[2 * k + 1];
                     double rx = X * cth - Y * sth, ry = X * sth + Y * cth,
                            K = c.K_stiff * trans_func_in(c, sqrt(rsq) - c.piv_r);
                     accx_ -= K * (x - rx);
@@ -415,8 +435,14 @@ __device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids 
             Xy = g.ysp * (X[ij] - X[ij - P]);
             Yy = g.ysp * (Y[ij] - Y[ij - P]);
         }
-        // F = [Yy,-Xy;-Yx,Xx], sigma = F F^T (mat.hh:159-161)
-        const double Fa = Yy, Fb = -Xy, Fc = -Yx, Fd = Xx;
+        // F = [Yy,-Xy;-Yx,Xx] with the solid's active part (object::actuate
+        // at the edge's reference map), sigma = F F^T (mat.hh:159-161)
+        double Fa = Yy, Fb = -Xy, Fc = -Yx, Fd = Xx;
+        if (c.kind == kFlapper || c.kind == kLamprey) {
+            const double Xe = LEFT ? 0.5 * (X[ij] + X[ij - 1]) : 0.5 * (X[ij] + X[ij - P]);
+            const double Ye = LEFT ? 0.5 * (Y[ij] + Y[ij - 1]) : 0.5 * (Y[ij] + Y[ij - P]);
+            actuate(c, ph.pss[2 * k], ph.pss[2 * k + 1], Xe, Ye, Fa, Fb, Fc, Fd);
+        }
         const double sa = Fa * Fa + Fb * Fb, sb = Fa * Fc + Fb * Fd, sd = Fc * Fc + Fd * Fd;
         double sfac = (1 / 3.0) * ((sa + sd) + 1), tf = trans_func_in(c, phiv), G = c.G * tf,
                ex_visc = c.ex_visc * tf * (1 + c.ev_trans_mult * tderiv_func_in(c, phiv));
@@ -631,6 +657,37 @@ __global__ void k_bc_rows(Grid g, Fields f) {
     u[t + 2 * P] = -u[t - P]; v[t + 2 * P] = -v[t - P];
 }
 
+// ---- fluid_2d::set_boundaries_solid (fluid_2d.cc:818-835): each solid's
+// reference map in the ghost layers is its periodic image shifted by the
+// box length (s_field::prd_bc). Columns first, then the rows over the full
+// ghosted width, as for the velocities.
+__global__ void k_bc_solid_cols(Grid g, Solids o) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.n) return;
+    const int r = g.at(0, j), m = g.m;
+    const double lx = g.bx - g.ax;
+    for (int k = 0; k < o.n; k++) {
+        double *X = o.oa[k].X, *Y = o.oa[k].Y;
+        X[r - 2] = X[r + m - 2] + -lx; Y[r - 2] = Y[r + m - 2] + 0;
+        X[r - 1] = X[r + m - 1] + -lx; Y[r - 1] = Y[r + m - 1] + 0;
+        X[r + m] = X[r] + lx; Y[r + m] = Y[r] + 0;
+        X[r + m + 1] = X[r + 1] + lx; Y[r + m + 1] = Y[r + 1] + 0;
+    }
+}
+__global__ void k_bc_solid_rows(Grid g, Solids o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - 2;
+    if (i >= g.m + 2) return;
+    const int P = g.P, b = g.at(i, 0), t = g.at(i, g.n - 1);
+    const double ly = g.by - g.ay;
+    for (int k = 0; k < o.n; k++) {
+        double *X = o.oa[k].X, *Y = o.oa[k].Y;
+        X[b - 2 * P] = X[t - P] + 0; Y[b - 2 * P] = Y[t - P] + -ly;
+        X[b - P] = X[t] + 0; Y[b - P] = Y[t] + -ly;
+        X[t + P] = X[b] + 0; Y[t + P] = Y[b] + ly;
+        X[t + 2 * P] = X[b + P] + 0; Y[t + 2 * P] = Y[b + P] + ly;
+    }
+}
+
 // ---- initial velocity from the solids' momenta (fluid_2d.cc:210-238)
 __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -641,6 +698,7 @@ __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
     double mu = 0, mv = 0, rho = 0, sfrac = 0;
     for (int k = 0; k < o.n; k++) {
         const ObjConst &c = o.oc[k];
+        if (!c.set_velocity) continue;  // obj_field::mom_contrib (obj_field.cc:62-75)
         double tf = trans_func(c, o.oa[k].phi[ij]), uu, vv;
         if (c.kind != kSpin) {  // obj_standard::velocity with zero speed
             uu = 0 - 0 * o.oa[k].Y[ij];
@@ -726,7 +784,16 @@ int launch_pressure_velocity(hipStream_t s, const Grid &g, const Fields &f, Soli
     LAUNCH_CHECK();
     return 0;
 }
-int launch_set_boundaries(hipStream_t s, const Grid &g, const Fields &f) {
+int launch_set_boundaries_solid(hipStream_t s, const Grid &g, Solids o) {
+    if (!g.spbc || !o.n) return 0;
+    k_bc_solid_cols<<<(g.n + 255) / 256, 256, 0, s>>>(g, o);
+    LAUNCH_CHECK();
+    k_bc_solid_rows<<<(g.m + 4 + 255) / 256, 256, 0, s>>>(g, o);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_set_boundaries(hipStream_t s, const Grid &g, const Fields &f, Solids o) {
+    if (int rc = launch_set_boundaries_solid(s, g, o)) return rc;
     k_bc_cols<<<(g.n + 255) / 256, 256, 0, s>>>(g, f);
     LAUNCH_CHECK();
     k_bc_rows<<<(g.m + 4 + 255) / 256, 256, 0, s>>>(g, f);

@@ -170,6 +170,12 @@ struct solid {
     double c0, c1, c2, piv_r, th_max, omega, K_stiff, piv_thresh, cth, sth;
     // obj_circle constants (object.hh:128-138)
     double cx, cy, cr, grav;
+    // obj_flapper (object.hh:245-289) and obj_lamprey (object.hh:542-595):
+    // shape, actuation, the transform's rotation (rc, rs) and the state of
+    // pre_stress_setup (con, sh)
+    double cl = 0, ar = 0, al = 0, amp = 0, fomega = 0;
+    double dis = 0, la = 0, w1 = 0, w2 = 0, T = 0, A = 0;
+    double rc = 1, rs = 0, con = 0, sh = 0;
     std::vector<scell> sb;
     std::vector<double> phi;  // level set on the ghosted grid
     std::vector<int> st;      // band status
@@ -184,6 +190,7 @@ struct solid {
     solid(const ls_grid &g_) : g(g_) {}
 
     double phi_of(double X, double Y) const {
+        if (sp.kind == OBJ_FULL) return -1;  // object.hh:85
         if (sp.kind == OBJ_SIMPLE_SPIN) {  // object.cc:167-171
             double th = std::atan2(Y, X), c = std::cos(3 * th);
             return std::sqrt(X * X + Y * Y) - c0 - c * (c1 + c2 * c);
@@ -193,15 +200,30 @@ struct solid {
             Y = std::fabs(Y) - cr;
             return X < 0 || Y < 0 ? std::max(X, Y) : std::sqrt(X * X + Y * Y);
         }
+        if (sp.kind == OBJ_FLAPPER) {  // object.cc:284-287
+            double fX = std::fabs(X) - cl;
+            return (fX < 0 ? std::fabs(Y) : std::sqrt(fX * fX + Y * Y)) - cr;
+        }
+        if (sp.kind == OBJ_LAMPREY) {  // object.cc:554-558
+            if (X > 0) return dis * Y * Y / (w1 * w1) + X - dis;
+            double wl = w1 - (w2 - w1) * X / la;
+            return std::max(-la - X, Y * Y / (wl * wl) - 1);
+        }
         return std::sqrt(X * X + Y * Y) - cr;  // object.cc:22-24
     }
+    bool actuated() const { return sp.kind == OBJ_FLAPPER || sp.kind == OBJ_LAMPREY; }
     void transform(double x, double y, double &X, double &Y) const {
-        if (sp.kind != OBJ_SIMPLE_SPIN) { X = x - cx; Y = y - cy; }  // obj_standard, object.hh:116-118
-        else { X = x; Y = y; }                                      // object.hh:26-28
+        if (actuated()) {  // object.hh:270-274, 576-580
+            x -= cx; y -= cy;
+            X = x * rc + y * rs;
+            Y = -x * rs + y * rc;
+        } else if (sp.kind != OBJ_SIMPLE_SPIN && sp.kind != OBJ_FULL) {
+            X = x - cx; Y = y - cy;  // obj_standard, object.hh:116-118
+        } else { X = x; Y = y; }     // object.hh:26-28
     }
     void velocity(double X, double Y, double &u, double &v) const {
-        if (sp.kind != OBJ_SIMPLE_SPIN) { u = 0 - 0 * Y; v = 0 + 0 * X; }  // obj_standard, object.cc:13-16
-        else { u = v = 0.; }
+        if (sp.kind == OBJ_CIRCLE || sp.kind == OBJ_SQUARE) { u = 0 - 0 * Y; v = 0 + 0 * X; }  // object.cc:13-16
+        else { u = v = 0.; }  // object.hh:31-33
     }
     // smoothed Heaviside and helpers (obj_field.hh:97-121)
     double trans_func_in(double x) const { return x < -eps ? 1 : 0.5 - x * tf1 - tf2 * std::sin(tf3 * x); }
@@ -209,9 +231,39 @@ struct solid {
     double tderiv_func_in(double x) const { return x < -eps ? 0 : 0.5 * (1 + std::cos(M_PI * x / eps)); }
     double coll_func(double x) const { return x > eps ? 0 : 0.5 - x * tf1; }
     void pre_stress_setup(double time) {
+        if (sp.kind == OBJ_FLAPPER) {  // object.cc:291-296
+            double a = std::sin(fomega * time);
+            a *= a * a; a *= a; a *= a;
+            con = -amp * a;
+            return;
+        }
+        if (sp.kind == OBJ_LAMPREY) {  // object.cc:563-568
+            sh = time / T;
+            double o = 2 * sh;
+            con = o < 1 ? A * o * o * (3 - 2 * o) : A;
+            sh *= la;
+            return;
+        }
         if (sp.kind != OBJ_SIMPLE_SPIN) return;  // object.cc:175-181
         double ang = time * omega, theta = ang > 2 * M_PI ? 0 : th_max * (1 - std::cos(ang));
         cth = std::cos(theta); sth = std::sin(theta);
+    }
+    // object::actuate (object.cc:301-308, 573-580) on F = (a b; c d)
+    void actuate(double X, double Y, double &a, double &b, double &c, double &d) const {
+        if (sp.kind == OBJ_FLAPPER) {
+            double fX = std::fabs(X) - al, ph = (fX < 0 ? std::fabs(Y) : std::sqrt(fX * fX + Y * Y)) - ar;
+            if (ph < eps) {
+                double fac = std::exp(con * Y * trans_func_in(ph));
+                a *= fac; b *= 1. / fac;
+                c *= fac; d *= 1. / fac;
+            }
+        } else if (sp.kind == OBJ_LAMPREY) {
+            double dx = X + (Y > 0 ? 1.5 * la : la) + sh, wl = w1 - (w2 - w1) * X / la, fac = std::fabs(Y) / wl,
+                   st = std::exp(con * (fac < 1 ? fac * (2 - fac) : 1)
+                                 * trans_func(-std::min(-X, la * 0.15 - std::fabs(dx - la * (int(dx / la) + 0.5)))));
+            a *= st; b *= 1. / st;
+            c *= st; d *= 1. / st;
+        }
     }
     void accel(double x, double y, double X, double Y, double phiv, double &ax, double &ay) const {
         if (sp.kind == OBJ_SIMPLE_SPIN) {  // object.cc:188-197
@@ -222,11 +274,46 @@ struct solid {
                 ax -= K * (x - rx);
                 ay -= K * (y - ry);
             }
-        } else {
+        } else if (sp.kind == OBJ_CIRCLE || sp.kind == OBJ_SQUARE) {
             ay -= grav * trans_func_in(phiv);  // object.cc:31-34
         }
     }
 };
+
+// The constants of one solid from its constructor's arguments and the
+// grid's transition half-width eps (obj_field.cc:14-20).
+void setup_solid(solid &so, const object_spec &o, double eps) {
+    so.sp = o;
+    so.eps = eps;
+    so.G = o.G; so.rhos = o.rhos; so.ev_trans_mult = o.ev_trans_mult;
+    so.tf1 = 0.5 / eps; so.tf2 = 0.5 / M_PI; so.tf3 = M_PI / eps;
+    if (o.kind == OBJ_SIMPLE_SPIN) {  // object.hh:449-455
+        const double rs = o.p[0], a1 = o.p[1], a2 = o.p[2];
+        const double scf = rs / (1 + a1 + a2);
+        so.c0 = scf * (1 - a2); so.c1 = scf * a1; so.c2 = 2 * scf * a2;
+        so.piv_r = o.p[3]; so.th_max = o.p[4]; so.omega = o.p[5];
+        so.cth = 1; so.sth = 0;
+        so.has_track = true;  // tracer at (0.9 r_scale, 0), X = x, Y = y (object.hh:456-461)
+        so.tk[0] = 0.9 * rs; so.tk[1] = 0;
+        so.tk[4] = so.tk[0]; so.tk[5] = so.tk[1];
+    } else if (o.kind == OBJ_FLAPPER) {  // object.hh:262-267
+        so.cx = o.p[0]; so.cy = o.p[1]; so.cr = o.p[2]; so.cl = o.p[3];
+        so.ar = o.p[4]; so.al = o.p[5];
+        so.amp = std::log(o.p[7]) / so.ar; so.fomega = 2 * M_PI / o.p[8];
+        so.rc = std::cos(o.p[6]); so.rs = std::sin(o.p[6]);
+        so.grav = 0;
+    } else if (o.kind == OBJ_LAMPREY) {  // object.hh:567-570
+        so.cx = o.p[0]; so.cy = o.p[1];
+        so.rc = std::cos(o.p[2]); so.rs = std::sin(o.p[2]);
+        so.dis = o.p[4]; so.la = o.p[3] - o.p[4]; so.w1 = o.p[5]; so.w2 = o.p[6];
+        so.T = o.p[7]; so.A = std::log(o.p[8]);
+        so.grav = 0;
+    } else if (o.kind == OBJ_FULL) {
+        so.grav = 0;
+    } else {
+        so.cx = o.p[0]; so.cy = o.p[1]; so.cr = o.p[2]; so.grav = o.p[3];
+    }
+}
 
 }  // namespace
 
@@ -304,22 +391,8 @@ fluid::fluid(const sim_spec &s, const std::vector<object_spec> &os)
     const ls_grid g(ml, nl, ax - 1.5 * dx, ay - 1.5 * dy, dx, dy, -eps - 4.5 * dx, eps + 4.5 * dx);
     for (const auto &o : os) {
         solid so(g);
-        so.sp = o;
-        so.eps = eps; so.pinw = s.pinw * dx;  // obj_field.cc:14-20
-        so.G = o.G; so.rhos = o.rhos; so.ev_trans_mult = o.ev_trans_mult;
-        so.tf1 = 0.5 / eps; so.tf2 = 0.5 / M_PI; so.tf3 = M_PI / eps;
-        if (o.kind == OBJ_SIMPLE_SPIN) {  // object.hh:449-455
-            const double rs = o.p[0], a1 = o.p[1], a2 = o.p[2];
-            const double scf = rs / (1 + a1 + a2);
-            so.c0 = scf * (1 - a2); so.c1 = scf * a1; so.c2 = 2 * scf * a2;
-            so.piv_r = o.p[3]; so.th_max = o.p[4]; so.omega = o.p[5];
-            so.cth = 1; so.sth = 0;
-            so.has_track = true;  // tracer at (0.9 r_scale, 0), X = x, Y = y (object.hh:456-461)
-            so.tk[0] = 0.9 * rs; so.tk[1] = 0;
-            so.tk[4] = so.tk[0]; so.tk[5] = so.tk[1];
-        } else {
-            so.cx = o.p[0]; so.cy = o.p[1]; so.cr = o.p[2]; so.grav = o.p[3];
-        }
+        setup_solid(so, o, eps);
+        so.pinw = s.pinw * dx;  // obj_field.cc:14-20
         so.sb.assign(ml * nl, scell());
         std::memset(so.sb.data(), 0, sizeof(scell) * so.sb.size());
         so.phi.assign(ml * nl, 0.0);
@@ -382,6 +455,7 @@ void fluid::init_fields() {  // fluid_2d.cc:195-254
             cellf &f = F(i, j);
             double mu = 0, mv = 0, rho = 0, sfrac = 0;
             for (auto &o : obj) {  // obj_field.cc:62-75
+                if (!o.sp.set_velocity) continue;
                 const scell &s = o.sb[at(i, j)];
                 double tf = o.trans_func(o.phi[at(i, j)]), uu, vv;
                 o.velocity(s.X, s.Y, uu, vv);
@@ -432,6 +506,21 @@ void fluid::sim_velocity(double x, double y, double &u, double &v) const {
 }
 
 void fluid::set_boundaries() {  // fluid_2d.cc:689-726: no-slip walls or periodic copies
+    if (sp.solid_prd_bc) {  // set_boundaries_solid (fluid_2d.cc:818-835), s_field::prd_bc
+        const double lx = bx - ax, ly = by - ay;
+        for (auto &o : obj) {
+            auto S = [&](int i, int j) -> scell & { return o.sb[at(i, j)]; };
+            auto pb = [&](scell &d, const scell &s, double ex, double ey) { d.X = s.X + ex; d.Y = s.Y + ey; };
+            for (int j = 0; j < n; j++) {
+                pb(S(-2, j), S(m - 2, j), -lx, 0); pb(S(-1, j), S(m - 1, j), -lx, 0);
+                pb(S(m, j), S(0, j), lx, 0); pb(S(m + 1, j), S(1, j), lx, 0);
+            }
+            for (int i = -2; i < m + 2; i++) {
+                pb(S(i, -2), S(i, n - 2), 0, -ly); pb(S(i, -1), S(i, n - 1), 0, -ly);
+                pb(S(i, n), S(i, 0), 0, ly); pb(S(i, n + 1), S(i, 1), 0, ly);
+            }
+        }
+    }
     for (int j = 0; j < n; j++) {
         if (sp.x_prd) {  // field::prd_bc (fields.hh:39-41)
             F(-2, j).u = F(m - 2, j).u; F(-2, j).v = F(m - 2, j).v;
@@ -505,9 +594,10 @@ void fluid::solid_stress(bool left, solid &o, int i, int j, double &s1s, double 
         Xy = ysp * (s->X - s[-ml].X);
         Yy = ysp * (s->Y - s[-ml].Y);
     }
-    (void)X; (void)Y;  // only read by actuated objects
-    // F = [Yy,-Xy;-Yx,Xx], sigma = F F^T (mat.hh:159-161)
-    const double Fa = Yy, Fb = -Xy, Fc = -Yx, Fd = Xx;
+    // F = [Yy,-Xy;-Yx,Xx] with the object's actuation, sigma = F F^T
+    // (fluid_2d.cc:533-541, mat.hh:159-161)
+    double Fa = Yy, Fb = -Xy, Fc = -Yx, Fd = Xx;
+    o.actuate(X, Y, Fa, Fb, Fc, Fd);
     const double sa = Fa * Fa + Fb * Fb, sb = Fa * Fc + Fb * Fd, sd = Fc * Fc + Fd * Fd;
     double sfac = (1 / 3.0) * ((sa + sd) + 1), tf = o.trans_func_in(phiv), G = o.G * tf,
            ex_visc = o.ex_visc * tf * (1 + o.ev_trans_mult * o.tderiv_func_in(phiv));
@@ -743,6 +833,18 @@ void fluid::set_du_tang(int ij) {
 }
 
 void fluid::edge_bc() {  // fluid_2d.cc:729-779: no-slip walls or periodic seams
+    const double lx = bx - ax, ly = by - ay;
+    if (sp.solid_prd_bc)  // edge_boundary_conditions_solid_pre (fluid_2d.cc:784-797)
+        for (auto &o : obj) {
+            for (int j = 0; j < n; j++) {
+                o.sb[at(-1, j)].Xr = o.sb[at(m - 1, j)].Xr - lx;
+                o.sb[at(-1, j)].Yr = o.sb[at(m - 1, j)].Yr;
+            }
+            for (int i = 0; i < m; i++) {
+                o.sb[at(i, -1)].Xu = o.sb[at(i, n - 1)].Xu;
+                o.sb[at(i, -1)].Yu = o.sb[at(i, n - 1)].Yu - ly;
+            }
+        }
     for (int j = 0; j < n; j++) {
         if (sp.x_prd) {
             F(-1, j).ur = F(m - 1, j).ur;
@@ -779,6 +881,17 @@ void fluid::edge_bc() {  // fluid_2d.cc:729-779: no-slip walls or periodic seams
         F(i, n).ud = F(i, n).vd = 0;
         F(i, n - 1).vu = 0;
     }
+    if (sp.solid_prd_bc)  // edge_boundary_conditions_solid_post (fluid_2d.cc:802-815)
+        for (auto &o : obj) {
+            for (int j = 0; j < n; j++) {
+                o.sb[at(m, j)].Xl = o.sb[at(0, j)].Xl + lx;
+                o.sb[at(m, j)].Yl = o.sb[at(0, j)].Yl;
+            }
+            for (int i = 0; i < m; i++) {
+                o.sb[at(i, n)].Xd = o.sb[at(i, 0)].Xd;
+                o.sb[at(i, n)].Yd = o.sb[at(i, 0)].Yd + ly;
+            }
+        }
 }
 
 void fluid::tangential() {  // fluid_2d.cc:953-975 (velocity part)
@@ -824,7 +937,7 @@ void fluid::acceleration(int i, int j, double &accx, double &accy, double x, dou
     }
     for (auto &o : obj) {
         const double phiv = o.phi[ij];
-        if (phiv < o.eps) {
+        if (phiv < o.eps && !sp.solid_prd_bc) {  // fluid_2d.cc:1037
             wall_force(x, y, phiv, accx, accy);
             o.accel(x, y, o.sb[ij].X, o.sb[ij].Y, phiv, accx_, accy_);
         }
@@ -1234,6 +1347,15 @@ int glibc_rand_probe(int n, int *out) {
 
 double mono_diff_probe(double f0, double f1, double f2, double f3, double f4) {
     return fluid::mono_diff(f0, f1, f2, f3, f4);
+}
+
+void object_eval(const object_spec &o, double eps, double time, double X, double Y, double *phi, double *F4) {
+    const ls_grid g(4, 4, 0, 0, 1, 1);
+    solid so(g);
+    setup_solid(so, o, eps);
+    so.pre_stress_setup(time);
+    *phi = so.phi_of(X, Y);
+    so.actuate(X, Y, F4[0], F4[1], F4[2], F4[3]);
 }
 
 fluid *fluid_create(const sim_spec &sp, const std::vector<object_spec> &objs) { return new fluid(sp, objs); }

@@ -91,26 +91,36 @@ extern "C" {
 // fluid_2d construction (fluid_2d.cc:26-43, 101-122) for a box with no-slip
 // walls or periodic directions, at rest or with a sim_type's velocity, or
 // sim_horiz_flow's inflow and zero-pressure outflow (zpr, U).
-// params: 9 doubles per object: 6 shape parameters, G, rhos, ev_trans_mult.
+// params: 13 doubles per object: 9 shape parameters, G, rhos, ev_trans_mult,
+// set_velocity (0 or 1).
 void *orc_fluid_create(int m, int n, double ax, double bx, double ay, double by,
                        double visc, double rhof, double tw, double pinw,
                        double w_scale, double w_force, double ev_mult,
                        double dt_pad, double dt_ev_pad, double mg_tol, int nobj,
                        const int *kinds, const double *params, int x_prd, int y_prd, int vel, int zpr,
-                       double U) {
+                       double U, int solid_prd_bc) {
     sim_spec sp{m, n, ax, bx, ay, by, visc, rhof, tw, pinw, w_scale, w_force,
-                ev_mult, dt_pad, dt_ev_pad, mg_tol, x_prd, y_prd, vel, zpr, U};
+                ev_mult, dt_pad, dt_ev_pad, mg_tol, x_prd, y_prd, vel, zpr, U, solid_prd_bc};
     std::vector<object_spec> os(nobj);
     for (int k = 0; k < nobj; k++) {
         os[k].kind = kinds[k];
-        for (int q = 0; q < 6; q++) os[k].p[q] = params[9 * k + q];
-        os[k].G = params[9 * k + 6];
-        os[k].rhos = params[9 * k + 7];
-        os[k].ev_trans_mult = params[9 * k + 8];
+        for (int q = 0; q < 9; q++) os[k].p[q] = params[13 * k + q];
+        os[k].G = params[13 * k + 9];
+        os[k].rhos = params[13 * k + 10];
+        os[k].ev_trans_mult = params[13 * k + 11];
+        os[k].set_velocity = params[13 * k + 12] != 0;
     }
     return fluid_create(sp, os);
 }
 void orc_fluid_destroy(void *h) { fluid_destroy((fluid *)h); }
+// object_eval for one solid given as kind + 9 constructor parameters
+void orc_object_eval(int kind, const double *p9, double eps, double time, double X, double Y, double *phi,
+                     double *F4) {
+    object_spec o{};
+    o.kind = kind;
+    for (int q = 0; q < 9; q++) o.p[q] = p9[q];
+    object_eval(o, eps, time, X, Y, phi, F4);
+}
 double orc_fluid_dt_reg(void *h) { return fluid_dt_reg((fluid *)h); }
 double orc_fluid_time(void *h) { return fluid_time((fluid *)h); }
 int orc_fluid_step(void *h, double dt) { return fluid_step((fluid *)h, dt); }

@@ -82,14 +82,17 @@ int fem_solve(const fem_system &s, const double *b, double *z, double tol,
               double *relres);
 
 // ------------------------------------------------------------ fluid step
-enum obj_kind { OBJ_SIMPLE_SPIN = 0, OBJ_CIRCLE = 1, OBJ_SQUARE = 2 };
+enum obj_kind { OBJ_SIMPLE_SPIN = 0, OBJ_CIRCLE = 1, OBJ_SQUARE = 2, OBJ_FLAPPER = 3, OBJ_LAMPREY = 4, OBJ_FULL = 5 };
 
 struct object_spec {
     int kind;
     // obj_simple_spin(r_scale,a1,a2,piv_r,th_max,omega) (object.hh:422-467)
     // or obj_circle(cx,cy,cr,grav) (object.hh:128-138)
-    double p[6];
+    // or obj_flapper(cx,cy,cr,cl,ar,al,theta,m_stretch,T) (object.hh:262-267)
+    // or obj_lamprey(cx,cy,theta,le,dis,w1,w2,T,mlam) (object.hh:567-570)
+    double p[9];
     double G, rhos, ev_trans_mult;
+    int set_velocity = 1;  // mat_const::set_velocity (obj_field.hh:25-28)
 };
 
 struct sim_spec {
@@ -105,11 +108,15 @@ struct sim_spec {
     // inflow (U, 0) on the left, initial velocity (U, 0)
     int zpr = 0;
     double U = 0;
+    int solid_prd_bc = 0;  // fluid_2d::solid_prd_bc (fluid_2d.hh:124)
 };
 
 class fluid;  // opaque
 
 fluid *fluid_create(const sim_spec &sp, const std::vector<object_spec> &objs);
+// One solid's object::phi at (X,Y) and object::actuate applied to F4 =
+// (a,b,c,d) after pre_stress_setup(time), for a transition half-width eps.
+void object_eval(const object_spec &o, double eps, double time, double X, double Y, double *phi, double *F4);
 void fluid_destroy(fluid *f);
 double fluid_dt_reg(fluid *f);
 double fluid_time(fluid *f);

@@ -85,6 +85,9 @@ def build_band_extrapolate(which, m, n, ax, ay, dx, dy, pd, pu, phi, fx, fy):
 SPIN = 0
 CIRCLE = 1
 SQUARE = 2
+FLAPPER = 3
+LAMPREY = 4
+FULL = 5
 
 
 def simple_spin_spec(m, omega=1.0):
@@ -102,15 +105,20 @@ def conv_spec(m, ctype, coarsest):
     0..7 without the option bits: sim_velocity_pulses in the periodic
     [-1,1]^2 box, 0/1 fluid only, 3 obj_square(-0.1,0,0.6), 4..7
     obj_circle(-0.1,0,0.6) with the listed densities and viscosities, G=1,
-    initialize(0.4, 0.4 coarsest/m, 0.8 coarsest/m). Type 2 (obj_full with
-    periodic solid boundaries) is not built."""
+    initialize(0.4, 0.4 coarsest/m, 0.8 coarsest/m); type 2 is obj_full
+    with solid periodic boundaries (conv_test.cc:120)."""
     rhos = [1, 1, 1, 1, 3, 1, 1, 1][ctype]
     visc = [1e-3, 4e-3, 1e-20, 1e-3, 1e-3, 1e-3, 4e-3, 16e-3][ctype]
     kind = SQUARE if ctype == 3 else CIRCLE
-    objs = [] if ctype in (0, 1) else [(kind, [-0.1, 0.0, 0.6, 0.0, 0.0, 0.0], 1.0, float(rhos), 1.0)]
+    # conv_test's solids leave the pulses inside them (set_velocity false,
+    # conv_test.cc:108)
+    objs = [] if ctype in (0, 1) else [(kind, [-0.1, 0.0, 0.6, 0.0, 0.0, 0.0], 1.0, float(rhos), 1.0, 0)]
+    if ctype == 2:
+        objs = [(FULL, [0.0] * 9, 1.0, float(rhos), 1.0, 0)]
     return dict(m=m, n=m, ax=-1.0, bx=1.0, ay=-1.0, by=1.0, visc=visc, rhof=1.0, tw=2.5, pinw=2.5,
                 w_scale=3.0, w_force=20.0, ev_mult=0.4, dt_pad=0.4 * coarsest / m,
-                dt_ev_pad=0.8 * coarsest / m, x_prd=1, y_prd=1, vel=2, objects=objs)
+                dt_ev_pad=0.8 * coarsest / m, x_prd=1, y_prd=1, vel=2, objects=objs,
+                solid_prd_bc=int(ctype == 2))
 
 
 def horiz_flow_spec(m, n, U=1.0, y_prd=True):
@@ -132,10 +140,12 @@ class OracleFluid:
         self.ml, self.nl = self.m + 4, self.n + 4
         objs = spec["objects"]
         kinds = np.array([o[0] for o in objs], dtype=np.int32)
-        params = np.array([list(o[1]) + [o[2], o[3], o[4]] for o in objs], dtype=np.float64).ravel()
+        # objects: (kind, shape parameters, G, rhos, ev_trans_mult[, set_velocity])
+        params = np.array([(list(o[1]) + [0.0] * 9)[:9] + [o[2], o[3], o[4], float(o[5] if len(o) > 5 else 1)]
+                           for o in objs], dtype=np.float64).ravel()
         D, I = ctypes.c_double, ctypes.c_int
         lib.orc_fluid_create.restype = ctypes.c_void_p
-        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp, I, I, I, I, D]
+        lib.orc_fluid_create.argtypes = [I, I] + [D] * 14 + [I, _ip, _dp, I, I, I, I, D, I]
         lib.orc_fluid_destroy.argtypes = [ctypes.c_void_p]
         lib.orc_fluid_step.argtypes = [ctypes.c_void_p, D]
         lib.orc_fluid_dt_reg.argtypes = [ctypes.c_void_p]
@@ -150,7 +160,8 @@ class OracleFluid:
             spec["rhof"], spec["tw"], spec["pinw"], spec["w_scale"], spec["w_force"],
             spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"], mg_tol, len(objs),
             iptr(kinds), dptr(params), int(spec.get("x_prd", 0)), int(spec.get("y_prd", 0)),
-            int(spec.get("vel", 0)), int(spec.get("zpr", 0)), float(spec.get("U", 0.0)))
+            int(spec.get("vel", 0)), int(spec.get("zpr", 0)), float(spec.get("U", 0.0)),
+            int(spec.get("solid_prd_bc", 0)))
         self.nobj = len(objs)
         dx, dy = (spec["bx"] - spec["ax"]) / self.m, (spec["by"] - spec["ay"]) / self.n
         self.eps = spec["tw"] * (dx if dx < dy else dy)

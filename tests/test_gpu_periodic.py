@@ -24,13 +24,20 @@ def gpu_from_spec(spec, tol=1e-12):
     f = ia.Fluid2D(spec["m"], spec["n"], visc=spec["visc"], rhof=spec["rhof"], tw=spec["tw"], pinw=spec["pinw"],
                    ax=spec["ax"], bx=spec["bx"], ay=spec["ay"], by=spec["by"], x_prd=bool(spec.get("x_prd")),
                    y_prd=bool(spec.get("y_prd")), w_scale=spec["w_scale"], w_force=spec["w_force"])
-    for kind, p, G, rhos, evm in spec["objects"]:
-        if kind == ol.SPIN:
+    for k, ob in enumerate(spec["objects"]):
+        kind, p, G, rhos, evm = ob[:5]
+        if kind == ol.FULL:
+            f.add_full(G=G, rhos=rhos, ev_trans_mult=evm)
+        elif kind == ol.SPIN:
             f.add_spin(*p, G=G, rhos=rhos, ev_trans_mult=evm)
         elif kind == ol.SQUARE:
             f.add_square(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
         else:
             f.add_circle(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
+        if len(ob) > 5:
+            f.set_solid_velocity(k, ob[5])
+    if spec.get("solid_prd_bc"):
+        f.set_solid_periodic(True)
     f.set_sim_velocity(spec.get("vel", 0))
     f.set_solver(rel_tol=tol, max_cycles=200)
     f.initialize(spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"])
@@ -60,9 +67,10 @@ def off_axis(spec):
     return spec
 
 
-@pytest.mark.parametrize("ctype", [0, 3, 4, 5])
+@pytest.mark.parametrize("ctype", [0, 2, 3, 4, 5])
 def test_conv_cases_per_step(ctype):
-    """conv_test types 0 (fluid only), 3 (square, obj_square), 4 (circle,
+    """conv_test types 0 (fluid only), 2 (obj_full with solid periodic
+    boundaries, fluid_2d.cc:784-835), 3 (square, obj_square), 4 (circle,
     density 3) and 5 (circle, density 1) in the doubly periodic box, 64^2,
     12 steps at a stable dt."""
     spec = off_axis(ol.conv_spec(64, ctype, 280))
@@ -91,6 +99,7 @@ def test_four_rolls_initial_field():
     circle's momentum (fluid_2d.cc:210-238), against the oracle."""
     spec = off_axis(ol.conv_spec(64, 4, 280))
     spec.update(vel=1)
+    spec["objects"] = [ob[:5] for ob in spec["objects"]]  # the circle sets its own velocity
     g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
     for nm in ("u", "v"):
         assert np.abs(g.get(nm) - o.get(nm)).max() <= 4e-16, nm
@@ -136,4 +145,22 @@ def test_square_in_walled_box_per_step():
         g.step_forward(g.dt_reg)
         o.step(o.dt_reg)
         compare(g, o, 64, dx, f"step {k}")
+    g.close()
+
+
+def test_full_solid_periodic_reference_map():
+    """conv_test type 2 at 64^2: the reference map's ghost layers are its
+    periodic image shifted by the box length (s_field::prd_bc), the level
+    set holds no interface (obj_full's phi = -1 is cleaned to the band's
+    lower cut-off, levelset.cc:408-413) and the solid fills every cell."""
+    spec = ol.conv_spec(64, 2, 280)
+    g = gpu_from_spec(spec)
+    for k in range(4):
+        g.step_forward(0.004)
+    X, Y, s = g.get("X"), g.get("Y"), g.get("s")
+    assert np.array_equal(X[2:-2, 0:2], X[2:-2, -4:-2] - 2.0) and np.array_equal(X[2:-2, -2:], X[2:-2, 2:4] + 2.0)
+    assert np.array_equal(Y[0:2, :], Y[-4:-2, :] - 2.0) and np.array_equal(Y[-2:, :], Y[2:4, :] + 2.0)
+    assert np.all(s == 0)
+    phi = g.get("phi")
+    assert np.all(phi == phi.flat[0]) and phi.flat[0] < -0.05
     g.close()

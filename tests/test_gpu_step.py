@@ -36,6 +36,10 @@ def compare(g, o, m, dx, tag):
             assert d <= 1e-9, f"{tag}: {nm}[{k}] max diff {d:.3e}"
         sg, so = g.get("s", k), o.get("s", k)
         band = (so >= 2) & (so <= 5)
+        if not band.any():  # no interface (obj_full): statuses and phi agree everywhere
+            assert np.array_equal(sg, so), f"{tag}: status mismatch"
+            assert np.abs(g.get("phi", k) - o.get("phi", k)).max() <= 0.05 * dx, f"{tag}: phi diff"
+            continue
         assert (sg[band] != so[band]).mean() <= 0.005, f"{tag}: status mismatch"
         dphi = np.abs(g.get("phi", k) - o.get("phi", k))[band & (sg == so)].max()
         assert dphi <= 0.05 * dx, f"{tag}: phi diff {dphi:.3e}"
