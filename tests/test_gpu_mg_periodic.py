@@ -112,3 +112,24 @@ def test_periodic_unsupported_sizes(kind, m, n):
     with pytest.raises(ia.IncrmtError) as e:
         ia.MgSolver(ia.MG_MAC if kind == "mac" else ia.MG_FEM, m, n, 0.1, 0.1, x_prd=True, y_prd=True)
     assert e.value.code == ia.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("px,py", PERIODS)
+def test_periodic_seam_faces_differ(px, py):
+    """A density that is not periodic across the seam (a solid crossing it)
+    gives the seam two different MAC faces: the left-edge entry left of
+    cell 0 and the right-edge entry right of cell m-1 (mgs_mac.hh:81-96).
+    Every fused launch shape still gives the per-colour sweep, which uses
+    the operator's own faces, bit for bit."""
+    m, n = 128, 64
+    dx, dy = 2.0 / m, 1.5 / n
+    irho = ops.mac_irho_from_rho(ops.blob_density(m, n, 3.0, seed=7), m, n)
+    s = ia.MgSolver(ia.MG_MAC, m, n, dx, dy, x_prd=px, y_prd=py)
+    s.set_irho(irho)
+    rng = np.random.default_rng(8)
+    b, z0 = rng.standard_normal(m * n), rng.standard_normal(m * n)
+    ref = s.smooth(b, z0, 4, -1)
+    for sh in [0, 1, 2, 3, 4, 5, 8, 9]:
+        got = s.smooth(b, z0, 4, sh)
+        assert np.array_equal(got, ref), sh
+    s.close()
