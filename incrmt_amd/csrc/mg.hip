@@ -164,6 +164,13 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
     if (!PER) { a.px = a.py = 0; pa.dr = 0; }
     constexpr int RW = kSX + 2 * H, RH = kSY + 2 * H, PW = RW + 1, PN = PW * (RH + 1);
     __shared__ T sz[PN], sb[PN], s1[PN], s2[KIND == kMAC ? PN : 1];
+    // A MAC seam's far faces: face nx (right of the last cell) per region
+    // row and face ny (above the last row) per region column, used by the
+    // region points whose wrapped index is the last one (the face slot of
+    // their wrapped neighbour holds face 0, the seam's other side).
+    constexpr bool SEAM = PER && KIND == kMAC;
+    __shared__ T sfr[SEAM ? RH + 1 : 1], sft[SEAM ? RW + 1 : 1];
+    __shared__ unsigned char lastc[SEAM ? RW + 1 : 1], lastr[SEAM ? RH + 1 : 1];
     const int i0 = blockIdx.x * kSX - H, j0 = blockIdx.y * kSY - H;
     const int tid = threadIdx.x, nt = smooth_threads<kSX, kSY>();
     // region points (li,lj) in [0,RW]x[0,RH]: the extra row/column holds the
@@ -172,11 +179,8 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
         const int li = e % PW, lj = e / PW;
         int i = i0 + li, j = j0 + lj;
         // periodic directions read the region's outside points from the
-        // opposite edge: they are real points, updated like the inside ones.
-        // A MAC seam face is then face 0 on both of its sides (the operator's
-        // face nx, used by the residual, is the same value whenever the
-        // density is periodic across the seam; where it is not, the sweep
-        // near the seam smooths a slightly different operator).
+        // opposite edge: they are real points, updated like the inside ones
+        // (a MAC seam's far face comes from sfr / sft below).
         if (a.px && (i < 0 || i >= a.nx)) i = wrap_idx(i, a.nx);
         if (a.py && (j < 0 || j >= a.ny)) j = wrap_idx(j, a.ny);
         const bool inpad = i >= -1 && i <= a.nx && j >= -1 && j <= a.ny;
@@ -187,6 +191,22 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
         sb[e] = inpad ? b[q] : T(0);
         s1[e] = inpad ? c1[c] : T(0);
         if (KIND == kMAC) s2[e] = inpad ? c2[c] : T(0);
+    }
+    if (SEAM) {
+        for (int e = tid; e < RH + 1; e += nt) {
+            int j = j0 + e;
+            if (a.py) j = wrap_idx(j, a.ny);
+            const bool in = j >= 0 && j < a.ny;
+            sfr[e] = (a.px && in) ? c1[j * a.cp + a.nx] : T(0);
+            lastr[e] = a.py && j == a.ny - 1;
+        }
+        for (int e = tid; e < RW + 1; e += nt) {
+            int i = i0 + e;
+            if (a.px) i = wrap_idx(i, a.nx);
+            const bool in = i >= 0 && i < a.nx;
+            sft[e] = (a.py && in) ? c2[a.ny * a.cp + i] : T(0);
+            lastc[e] = a.px && i == a.nx - 1;
+        }
     }
     __syncthreads();
     const int ncol = KIND == kMAC ? 2 : 4;
@@ -211,7 +231,15 @@ __global__ __launch_bounds__((smooth_threads<kSX, kSY>())) void k_smooth_fused(L
             if ((!a.px && (i < 0 || i >= a.nx)) || (!a.py && (j < 0 || j >= a.ny))) continue;
             const int q = lj * PW + li;
             T diag, off;
-            if (KIND == kMAC) off = mac_offdiag<T>(sz, s1, s2, q, q, PW, PW, T(a.kx), T(a.ky), diag);
+            if (SEAM) {
+                // mac_offdiag with the seam's far faces where they apply
+                const T wl = s1[q], sd = s2[q];
+                const T wr = lastc[li] ? sfr[lj] : s1[q + 1];
+                const T su = lastr[lj] ? sft[li] : s2[q + PW];
+                const T kx = T(a.kx), ky = T(a.ky);
+                diag = -(kx * (wl + wr) + ky * (sd + su));
+                off = kx * (wl * sz[q - 1] + wr * sz[q + 1]) + ky * (sd * sz[q - PW] + su * sz[q + PW]);
+            } else if (KIND == kMAC) off = mac_offdiag<T>(sz, s1, s2, q, q, PW, PW, T(a.kx), T(a.ky), diag);
             else off = fem_offdiag<T>(sz, s1, q, q, PW, PW, k, diag);
             if (diag != T(0)) sz[q] = (sb[q] - off) / diag;
         }
