@@ -253,9 +253,14 @@ int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, in
 int incrmt_mg_set_launch(incrmt_mg *h, int mode);
 /* No reference counterpart (for checking the smoother): `sweeps` forward
  * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused
- * launch shape `shape` (0..12; see incrmt_fluid_describe) or one launch per
+ * launch shape `shape` (0..16; see incrmt_fluid_describe; 14..16 and the
+ * MAC form of 13 fit in LDS only in single precision) or one launch per
  * colour when shape < 0. */
 int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape);
+/* The same in INCRMT_PREC_FP64 or, with INCRMT_PREC_MIXED, in single
+ * precision: b and z rounded to FP32, the FP32 sweeps of the mixed
+ * V-cycle, z widened back. */
+int incrmt_mg_smooth_prec(incrmt_mg *h, const double *b, double *z, int sweeps, int shape, int precision);
 /* y = A x with the solver's operator (mgs_*::mul_a + a_cc, full row). */
 int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y);
 /* Time `sweeps` fine-level smoother sweeps on the resident problem with HIP

@@ -100,6 +100,7 @@ _PROTOS = [
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
     ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
     ("incrmt_mg_smooth", _I, [_V, _dp, _dp, _I, _I]),
+    ("incrmt_mg_smooth_prec", _I, [_V, _dp, _dp, _I, _I, _I]),
     ("incrmt_mg_set_launch", _I, [_V, _I]),
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
@@ -432,12 +433,13 @@ class MgSolver:
         _check(lib().incrmt_mg_apply(self._h, _d(xx), _d(y)))
         return y
 
-    def smooth(self, b, z, sweeps, shape):
+    def smooth(self, b, z, sweeps, shape, precision=PREC_FP64):
         """`sweeps` Gauss-Seidel sweeps from z with fused launch shape
-        `shape`, or one launch per colour (shape -1). Returns the new z."""
+        `shape`, or one launch per colour (shape -1); PREC_MIXED sweeps in
+        single precision. Returns the new z."""
         bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
         zz = np.ascontiguousarray(z, dtype=np.float64).ravel().copy()
-        _check(lib().incrmt_mg_smooth(self._h, _d(bb), _d(zz), sweeps, shape))
+        _check(lib().incrmt_mg_smooth_prec(self._h, _d(bb), _d(zz), sweeps, shape, precision))
         return zz
 
     def time_smoother(self, sweeps, precision=PREC_FP64):

@@ -1216,11 +1216,17 @@ int incrmt_mg_set_launch(incrmt_mg *h, int mode) {
 }
 
 int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape) {
-    if (!h || !b || !z || sweeps < 1) return fail(INCRMT_ERR_ARG, "bad smooth arguments");
+    return incrmt_mg_smooth_prec(h, b, z, sweeps, shape, INCRMT_PREC_FP64);
+}
+
+int incrmt_mg_smooth_prec(incrmt_mg *h, const double *b, double *z, int sweeps, int shape, int precision) {
+    if (!h || !b || !z || sweeps < 1 || (precision != INCRMT_PREC_FP64 && precision != INCRMT_PREC_MIXED))
+        return fail(INCRMT_ERR_ARG, "bad smooth arguments");
     MgLevel &l = h->mg.fine();
     if (int rc = mg_copy(h, l.b, b, nullptr)) return rc;
     if (int rc = mg_copy(h, l.z, z, nullptr)) return rc;
-    if (int rc = h->mg.smooth_check(sweeps, shape)) return rc == 1 ? fail(INCRMT_ERR_ARG, "unknown shape") : rc;
+    if (int rc = h->mg.smooth_check(sweeps, shape, precision == INCRMT_PREC_MIXED))
+        return rc == 1 ? fail(INCRMT_ERR_ARG, "unknown shape") : rc;
     return mg_copy(h, l.z, nullptr, z);
 }
 

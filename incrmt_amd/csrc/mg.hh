@@ -104,10 +104,11 @@ class Multigrid {
     int solve_mixed(double tol, int max_cycles, MgStats *st);
     // Launch `sweeps` fine-level smoothing sweeps (for timing the smoother).
     int smooth_fine(int sweeps, bool fp32);
-    // `sweeps` forward FP64 sweeps on level 0 with fused launch shape
-    // `shape`, or with one launch per colour when shape < 0 (for checking
-    // that every form gives the same sweep)
-    int smooth_check(int sweeps, int shape);
+    // `sweeps` forward sweeps on level 0 (FP64, or FP32 on the rounded
+    // iterate with f32) with fused launch shape `shape`, or with one launch
+    // per colour when shape < 0 (for checking that every form gives the same
+    // sweep)
+    int smooth_check(int sweeps, int shape, bool f32 = false);
     // y = A x on level 0; x and y are padded level arrays (pointers at (0,0)).
     int apply(const double *x, double *y);
     // Time `sweeps` fine-level sweeps with HIP events: mean ms per launch.
@@ -138,7 +139,7 @@ class Multigrid {
   private:
     int vcycle(size_t l);
     int vcycle_f(size_t l);
-    int coarsen_coefs();
+    int coarsen_coefs(bool f32 = false);
     int reduce_norms(double *out3);  // ||b||^2, ||r||^2, sum b of level 0
     int reduce_dev();                // the same into d_red, no host copy
     template <class Cycle>
@@ -146,8 +147,9 @@ class Multigrid {
     int predict_ = 0;                // V-cycles run before the first residual check
     double *zprev_ = nullptr;        // the solution before the last one (extrapolation)
     bool f32_stale_ = true;          // coefficient copies for the FP32 cycle out of date
+    bool want_f32_ = false;          // a mixed solve has run: coefficient updates write FP32 copies too
     int make_f32_coefs();
-    int wrap_coefs(size_t l);
+    int wrap_coefs(size_t l, bool f32 = false);
     hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // FP64 cycle, mixed cycle
     std::vector<uintptr_t> graph_key_[2];                 // launch state each graph was captured in
     std::vector<uintptr_t> launch_state() const;

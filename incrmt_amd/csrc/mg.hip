@@ -838,19 +838,26 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
 // (fluid_2d.cc:1140-1160), zeroed on the walls. Across a periodic seam the
 // two sides are the reference's left-edge and right-edge values: face 0
 // from cells 0 and -1, face nx from cells nx-1 and nx (ghost densities).
-__global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *fw, double *fs) {
+// fwf, fsf (kcf below): single-precision copies written alongside, or null
+__global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *fw, double *fs, float *fwf,
+                                float *fsf) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i > a.nx || j > a.ny) return;
     const int c = j * a.cp + i;
     if (j < a.ny) {
         const int ij = g.at(i, j);
         // a zero-pressure right edge doubles its face (mgs_mac.hh:95)
-        fw[c] = (!a.px && (i == 0 || i == a.nx)) ? (a.dr && i == a.nx ? 2 * (0.5 / rho[ij] + 0.5 / rho[ij - 1]) : 0.0)
-                                                 : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
+        const double v = (!a.px && (i == 0 || i == a.nx))
+                             ? (a.dr && i == a.nx ? 2 * (0.5 / rho[ij] + 0.5 / rho[ij - 1]) : 0.0)
+                             : 0.5 / rho[ij] + 0.5 / rho[ij - 1];
+        fw[c] = v;
+        if (fwf) fwf[c] = float(v);
     }
     if (i < a.nx) {
         const int ij = g.at(i, j);
-        fs[c] = (!a.py && (j == 0 || j == a.ny)) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - g.P];
+        const double v = (!a.py && (j == 0 || j == a.ny)) ? 0.0 : 0.5 / rho[ij] + 0.5 / rho[ij - g.P];
+        fs[c] = v;
+        if (fsf) fsf[c] = float(v);
     }
 }
 // FEM fine cell weights 1/rho (fluid_2d.cc:1163-1170). A periodic
@@ -858,24 +865,40 @@ __global__ void k_mac_coef_fine(Grid g, const double *rho, LevelArgs a, double *
 // as mgs_fem_varying_rho wraps its corner lookup (mgs_fem.hh:99-106). A
 // zero-pressure edge also has nx cells, the last one between the last
 // unknown node and the fixed column.
-__global__ void k_fem_coef_fine(Grid g, const double *rho, LevelArgs a, double *kc) {
+__global__ void k_fem_coef_fine(Grid g, const double *rho, LevelArgs a, double *kc, float *kcf) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx - ((a.px || a.dr) ? 0 : 1) || j >= a.ny - (a.py ? 0 : 1)) return;
     const int ci = a.px && i == a.nx - 1 ? -1 : i, cj = a.py && j == a.ny - 1 ? -1 : j;
-    kc[j * a.cp + i] = 1. / rho[g.at(ci, cj)];
+    const double v = 1. / rho[g.at(ci, cj)];
+    kc[j * a.cp + i] = v;
+    if (kcf) kcf[j * a.cp + i] = float(v);
 }
 __global__ void k_mac_coef_coarsen(LevelArgs f, LevelArgs c, const double *fwf, const double *fsf, double *fwc,
-                                   double *fsc) {
+                                   double *fsc, float *fwc32, float *fsc32) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I > c.nx || J > c.ny) return;
-    if (J < c.ny) fwc[J * c.cp + I] = 0.5 * (fwf[2 * J * f.cp + 2 * I] + fwf[(2 * J + 1) * f.cp + 2 * I]);
-    if (I < c.nx) fsc[J * c.cp + I] = 0.5 * (fsf[2 * J * f.cp + 2 * I] + fsf[2 * J * f.cp + 2 * I + 1]);
+    if (J < c.ny) {
+        const double v = 0.5 * (fwf[2 * J * f.cp + 2 * I] + fwf[(2 * J + 1) * f.cp + 2 * I]);
+        fwc[J * c.cp + I] = v;
+        if (fwc32) fwc32[J * c.cp + I] = float(v);
+    }
+    if (I < c.nx) {
+        const double v = 0.5 * (fsf[2 * J * f.cp + 2 * I] + fsf[2 * J * f.cp + 2 * I + 1]);
+        fsc[J * c.cp + I] = v;
+        if (fsc32) fsc32[J * c.cp + I] = float(v);
+    }
 }
-__global__ void k_fem_coef_coarsen(LevelArgs f, LevelArgs c, const double *kf, double *kcc) {
+__global__ void k_fem_coef_coarsen(LevelArgs f, LevelArgs c, const double *kf, double *kcc, float *kcc32) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= c.nx - ((c.px || c.dr) ? 0 : 1) || J >= c.ny - (c.py ? 0 : 1)) return;
     const int q = 2 * J * f.cp + 2 * I;
-    kcc[J * c.cp + I] = 0.25 * (kf[q] + kf[q + 1] + kf[q + f.cp] + kf[q + f.cp + 1]);
+    const double v = 0.25 * (kf[q] + kf[q + 1] + kf[q + f.cp] + kf[q + f.cp + 1]);
+    kcc[J * c.cp + I] = v;
+    if (kcc32) kcc32[J * c.cp + I] = float(v);
+}
+__global__ void k_to_f64(const float *a, double *b, long long n) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = double(a[k]);
 }
 __global__ void k_to_f32(const double *a, float *b, long long n) {
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1197,24 +1220,29 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
 }
 
 
-int Multigrid::coarsen_coefs() {
+int Multigrid::coarsen_coefs(bool f32) {
+    // f32: the caller wrote level 0's single-precision copies with its
+    // coefficients; every coarse level writes its copies too
+    if (resampled_) f32 = false;
     for (size_t l = 0; l + 1 < L.size(); l++) {
         const MgLevel &f = L[l];
         MgLevel &c = L[l + 1];
         if (kind_ == kMAC) {
-            k_mac_coef_coarsen<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.fw, f.fs,
-                                                                                    c.fw, c.fs);
+            k_mac_coef_coarsen<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(
+                args_of(f), args_of(c), f.fw, f.fs, c.fw, c.fs, f32 ? c.fwf : nullptr, f32 ? c.fsf : nullptr);
         } else if (l == 0 && resampled_) {
             k_fem_coef_rs<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc, rx_.clo,
                                                                       rx_.chi, ry_.clo, ry_.chi, rx_.hf, rx_.hc,
                                                                       ry_.hf, ry_.hc);
         } else {
-            k_fem_coef_coarsen<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc);
+            k_fem_coef_coarsen<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc,
+                                                                           f32 ? c.kcf : nullptr);
         }
         KCHECK();
-        if (int rc = wrap_coefs(l + 1)) return rc;
+        if (int rc = wrap_coefs(l + 1, f32)) return rc;
     }
-    f32_stale_ = true;  // the single-precision copies are made when a mixed solve needs them
+    // otherwise the single-precision copies are made when a mixed solve needs them
+    f32_stale_ = !f32;
     return 0;
 }
 
@@ -1236,22 +1264,30 @@ int Multigrid::make_f32_coefs() {
 
 int Multigrid::set_density(const Grid &g, const double *rho) {
     MgLevel &f = L[0];
+    // once a mixed solve has run, the single-precision copies are written
+    // with the coefficients instead of converted afterwards
+    const bool f32 = want_f32_ && !resampled_;
     if (kind_ == kMAC)
-        k_mac_coef_fine<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.fw, f.fs);
+        k_mac_coef_fine<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.fw, f.fs,
+                                                                          f32 ? f.fwf : nullptr, f32 ? f.fsf : nullptr);
     else
-        k_fem_coef_fine<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.kc);
+        k_fem_coef_fine<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(g, rho, args_of(f), f.kc, f32 ? f.kcf : nullptr);
     KCHECK();
-    if (int rc = wrap_coefs(0)) return rc;
-    return coarsen_coefs();
+    if (int rc = wrap_coefs(0, f32)) return rc;
+    return coarsen_coefs(f32);
 }
 
 // FEM cell weights of a periodic level: ring cells mirror the last cells
-int Multigrid::wrap_coefs(size_t l) {
+int Multigrid::wrap_coefs(size_t l, bool f32) {
     const MgLevel &v = L[l];
     if (kind_ != kFEM || !(v.px || v.py)) return 0;
     const int n = 2 * (v.nx + 2) + 2 * (v.ny + 2);
     k_wrap_ring<double><<<(n + 255) / 256, 256, 0, s_>>>(args_of(v), v.kc);
     KCHECK();
+    if (f32) {
+        k_wrap_ring<float><<<(n + 255) / 256, 256, 0, s_>>>(args_of(v), v.kcf);
+        KCHECK();
+    }
     return 0;
 }
 
@@ -1337,14 +1373,26 @@ int Multigrid::collect_events() {
 struct SmoothShape {
     int sx, sy, h;
 };
-// gfx950 gives a workgroup up to 160 KiB of LDS; the larger tiles (up to
-// 110 KiB of FP64 tile data) halve the halo re-reads of the small ones.
-constexpr SmoothShape kShapes[] = {{64, 16, 4}, {64, 8, 4},  {64, 4, 4},  {64, 16, 2}, {64, 8, 2},
-                                   {64, 4, 2},  {64, 8, 8},  {64, 4, 8},  {64, 32, 4}, {128, 16, 4},
-                                   {64, 32, 8}, {128, 16, 8}, {128, 8, 8}};
+// gfx950 gives a workgroup up to 160 KiB of LDS; the larger tiles halve
+// the halo re-reads of the small ones. The last three fit only in single
+// precision (the tile data of a shape is (SX+2H+1)(SY+2H+1) points of 4 MAC
+// or 3 FEM arrays).
+constexpr SmoothShape kShapes[] = {{64, 16, 4}, {64, 8, 4},   {64, 4, 4},   {64, 16, 2}, {64, 8, 2},
+                                   {64, 4, 2},  {64, 8, 8},   {64, 4, 8},   {64, 32, 4}, {128, 16, 4},
+                                   {64, 32, 8}, {128, 16, 8}, {128, 8, 8},  {64, 64, 8}, {128, 32, 4},
+                                   {128, 32, 8}, {128, 64, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
-inline bool shape_valid(MgKind kind, const SmoothShape &v) { return kind == kMAC ? v.h <= 4 : v.h >= 4; }
+// LDS bytes of a fused launch (the periodic seam's extra rows and columns
+// included), against gfx950's 160 KiB per workgroup
+constexpr long smooth_lds(int kind, int h, int sx, int sy, int tsize) {
+    return (long)(sx + 2 * h + 1) * (sy + 2 * h + 1) * (kind == kMAC ? 4 : 3) * tsize
+           + (kind == kMAC ? (long)(sx + sy + 4 * h + 2) * (tsize + 1) : 0);
+}
+constexpr long kLdsBudget = 160 * 1024 - 1024;
+inline bool shape_valid(MgKind kind, const SmoothShape &v, bool f32 = false) {
+    return (kind == kMAC ? v.h <= 4 : v.h >= 4) && smooth_lds(kind, v.h, v.sx, v.sy, f32 ? 4 : 8) <= kLdsBudget;
+}
 
 template <class T, int KIND, int H, int SX, int SY>
 static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2, int sweeps,
@@ -1372,14 +1420,19 @@ static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, c
 template <class T, int KIND>
 static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2,
                        int sweeps, bool reverse, hipStream_t s, int *nl, ProArgs pa) {
-#define SHAPE(id, H, SX, SY) \
-    case id: return launch_shape<T, KIND, H, SX, SY>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa);
+#define SHAPE(id, H, SX, SY)                                                                          \
+    case id:                                                                                          \
+        if constexpr (smooth_lds(KIND, H, SX, SY, sizeof(T)) <= kLdsBudget)                          \
+            return launch_shape<T, KIND, H, SX, SY>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); \
+        else                                                                                          \
+            return INCRMT_ERR_ARG_SHAPE;
     switch (shape) {
         SHAPE(0, 4, 64, 16)
         SHAPE(1, 4, 64, 8)
         SHAPE(2, 4, 64, 4)
         SHAPE(8, 4, 64, 32)
         SHAPE(9, 4, 128, 16)
+        SHAPE(14, 4, 128, 32)
     }
     if constexpr (KIND == kMAC) {
         switch (shape) {
@@ -1394,6 +1447,9 @@ static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, con
             SHAPE(10, 8, 64, 32)
             SHAPE(11, 8, 128, 16)
             SHAPE(12, 8, 128, 8)
+            SHAPE(13, 8, 64, 64)
+            SHAPE(15, 8, 128, 32)
+            SHAPE(16, 8, 128, 64)
         }
     }
 #undef SHAPE
@@ -1426,7 +1482,7 @@ int Multigrid::autotune() {
             float best = 1e30f;
             int pick = 0;
             for (int sh = 0; sh < kNumShapes && !rc; sh++) {
-                if (!shape_valid(kind_, kShapes[sh])) continue;
+                if (!shape_valid(kind_, kShapes[sh], prec == 1)) continue;
                 // scratch iterate: the level's residual array (always written
                 // before it is read); b and the coefficients are the level's
                 auto run = [&]() -> int {
@@ -1469,11 +1525,45 @@ std::string Multigrid::shapes() const {
     return s;
 }
 
-int Multigrid::smooth_check(int sweeps, int shape) {
+int Multigrid::smooth_check(int sweeps, int shape, bool f32) {
     MgLevel &v = L[0];
     const LevelArgs a = args_of(v);
     const FemC k = fem_consts(v.hx, v.hy);
     if (int rc = wrap_fine(v.z)) return rc;
+    if (f32) {
+        // the same sweeps in single precision: z and b rounded to FP32 (rings
+        // included), the result widened back into z
+        if (shape >= kNumShapes || (shape >= 0 && !shape_valid(kind_, kShapes[shape], true))) return INCRMT_ERR_ARG_SHAPE;
+        if (f32_stale_)
+            if (int rc = make_f32_coefs()) return rc;
+        const long long len = (long long)(v.ny + 2) * v.pitch;
+        const int nb = (int)((len + 255) / 256);
+        k_to_f32<<<nb, 256, 0, s_>>>(v.z - v.pitch - 1, v.zf - v.pitch - 1, len);
+        k_to_f32<<<nb, 256, 0, s_>>>(v.b - v.pitch - 1, v.bf - v.pitch - 1, len);
+        KCHECK();
+        if (shape < 0) {
+            const int ncol = kind_ == kMAC ? 2 : 4;
+            for (int sw = 0; sw < sweeps; sw++)
+                for (int c = 0; c < ncol; c++) {
+                    if (kind_ == kMAC) {
+                        const dim3 grid((v.nx / 2 + 1 + BX - 1) / BX, (v.ny + BY - 1) / BY);
+                        k_mac_smooth<float><<<grid, dim3(BX, BY), 0, s_>>>(a, v.zf, v.bf, v.fwf, v.fsf, c);
+                    } else {
+                        const dim3 grid(((v.nx + 1) / 2 + BX - 1) / BX, ((v.ny + 1) / 2 + BY - 1) / BY);
+                        k_fem_smooth<float><<<grid, dim3(BX, BY), 0, s_>>>(a, k, v.zf, v.bf, v.kcf, c);
+                    }
+                    KCHECK();
+                }
+        } else {
+            int nl;
+            if (int rc = launch_smooth<float>(kind_, shape, a, k, &v.zf, &v.zf2, v.bf, kind_ == kMAC ? v.fwf : v.kcf,
+                                              v.fsf, sweeps, false, s_, &nl))
+                return rc;
+        }
+        k_to_f64<<<nb, 256, 0, s_>>>(v.zf - v.pitch - 1, v.z - v.pitch - 1, len);
+        KCHECK();
+        return 0;
+    }
     if (shape < 0) {  // one launch per colour: the reference form of the sweep
         const int ncol = kind_ == kMAC ? 2 : 4;
         for (int sw = 0; sw < sweeps; sw++)
@@ -1842,6 +1932,7 @@ int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
 
 int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
     const MgLevel &f = L[0];
+    want_f32_ = true;
     if (f32_stale_)
         if (int rc = make_f32_coefs()) return rc;
     const FemC k = fem_consts(dx_, dy_);

@@ -154,8 +154,17 @@ def test_fused_smoother_shapes_bit_identical(kind, sweeps):
     b = rng.standard_normal(m * n)
     z = rng.standard_normal(m * n)
     ref = s.smooth(b, z, sweeps, -1)
-    shapes = (0, 1, 2, 3, 4, 5, 8, 9) if kind == "mac" else (0, 1, 2, 6, 7, 8, 9, 10, 11, 12)
+    shapes = (0, 1, 2, 3, 4, 5, 8, 9) if kind == "mac" else (0, 1, 2, 6, 7, 8, 9, 10, 11, 12, 13, 14)
     for sh in shapes:
         assert np.array_equal(s.smooth(b, z, sweeps, sh), ref), sh
     with pytest.raises(ia.IncrmtError):
         s.smooth(b, z, sweeps, 6 if kind == "mac" else 3)
+    with pytest.raises(ia.IncrmtError):  # fits in LDS only in single precision
+        s.smooth(b, z, sweeps, 14 if kind == "mac" else 15)
+    # single precision: every shape, including those only FP32 tiles fit,
+    # gives the per-colour FP32 sweep bit for bit
+    ref32 = s.smooth(b, z, sweeps, -1, precision=ia.PREC_MIXED)
+    assert np.abs(ref32 - ref).max() <= 1e-4 * np.abs(ref).max()
+    shapes32 = shapes + ((14,) if kind == "mac" else (15, 16))
+    for sh in shapes32:
+        assert np.array_equal(s.smooth(b, z, sweeps, sh, precision=ia.PREC_MIXED), ref32), ("fp32", sh)
