@@ -27,6 +27,32 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
+// Wavefront-aggregated list append: the lanes of a wavefront (among those
+// executing the call) with pred set take consecutive slots of `list` with
+// one atomic on `counter`. The order inside a list carries no meaning: each
+// entry is processed on its own.
+__device__ inline void wave_append(bool pred, int *counter, int *list, int value) {
+    const unsigned long long m = __ballot(pred);
+    if (!m) return;
+    const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(m & ((1ull << lane) - 1))] = value;
+}
+// Wavefront-aggregated flag set: one atomicOr per distinct tile index.
+__device__ inline void wave_flag(bool pred, int *flags, int tile, int bit) {
+    unsigned long long m = __ballot(pred);
+    const int lane = __lane_id();
+    while (m) {
+        const int l = __ffsll((long long)m) - 1;
+        const int t = __shfl(tile, l, 64);
+        const unsigned long long same = __ballot(pred && tile == t);
+        if (lane == l) atomicOr(&flags[t], bit);
+        m &= ~same;
+    }
+}
+
 // Rectangle of level-set points [I0,I1) x [J0,J1) a rebuild processes.
 struct Win {
     int I0, I1, J0, J1;
@@ -228,10 +254,8 @@ __global__ void k_scan(Grid g, ObjArrays a, LsWork w, Win wn) {
     if (J > 0 && (phi[ij - P] > 0) != pos) iface = true;
     if (J < g.nl - 1 && (phi[ij + P] > 0) != pos) iface = true;
     a.st[ij] = pos ? (iface ? 4 : 7) : (iface ? 3 : 0);
-    if (iface) {
-        const int k = atomicAdd(&w.cnt[kCntIface], 1);
-        w.iface[k] = ij;
-    } else {
+    wave_append(iface, &w.cnt[kCntIface], w.iface, ij);
+    if (!iface) {
         const double inf = pos ? INFINITY : -INFINITY;
         w.va[ij] = inf;
         w.vb[ij] = inf;
@@ -290,11 +314,8 @@ __global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
             hit = w.mark[ij + d * g.P] != 0;
         }
     w.cflag[ij] = hit;
-    if (hit) {
-        const int k = atomicAdd(&w.cnt[kCntCand], 1);
-        w.cand[k] = ij;
-        atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 1);
-    }
+    wave_append(hit, &w.cnt[kCntCand], w.cand, ij);
+    wave_flag(hit, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 1);
 }
 
 __global__ void k_bbox_init(LsWork w) {
@@ -481,11 +502,9 @@ __device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const 
         bb[0][lane] = I; bb[1][lane] = I; bb[2][lane] = J; bb[3][lane] = J;
     }
     a.phi[ij] = phi;
-    if (ns == 4 || ns == 5) {
-        const int k = atomicAdd(&w.cnt[kCntPband], 1);
-        w.pband[k] = ij;
-        atomicOr(&w.tflag[(J / kTileY) * w.ntx + I / kTileX], 2);
-    }
+    const bool pb = ns == 4 || ns == 5;
+    wave_append(pb, &w.cnt[kCntPband], w.pband, ij);
+    wave_flag(pb, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 2);
 }
 __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
