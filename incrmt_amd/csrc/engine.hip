@@ -202,7 +202,13 @@ int FluidEngine::init_tracers() {
             }
             return false;
         };
+        // rejection sampling as fluid_2d::init_tracers (fluid_2d.cc:1376-1386),
+        // which loops for ever when the solids leave no fluid (obj_full): give
+        // up with an error after a bounded number of draws instead
+        long long draws = 0;
+        const long long max_draws = 1000LL * nt + 10000000LL;
         for (size_t q = 0; q < tm.size();) {
+            if (++draws > max_draws) return fail(INCRMT_ERR_ARG, "no fluid left outside the solids for tracers");
             double tx = rng.rnd(g.ax, g.bx);
             double ty = rng.rnd(g.ay, g.by);
             if (!inside(tx, ty)) { tm[q] = tx; tm[q + 1] = ty; q += 4; }

@@ -164,3 +164,17 @@ def test_full_solid_periodic_reference_map():
     phi = g.get("phi")
     assert np.all(phi == phi.flat[0]) and phi.flat[0] < -0.05
     g.close()
+
+
+def test_full_solid_leaves_no_room_for_tracers():
+    """fluid_2d::init_tracers draws tracer positions until they fall outside
+    every solid (fluid_2d.cc:1376-1386), for ever when obj_full fills the
+    box; the library stops after a bounded number of draws with an error."""
+    f = ia.Fluid2D(32, 32, visc=1e-3, rhof=1.0, x_prd=True, y_prd=True)
+    f.add_full(G=1.0, rhos=1.0)
+    f.set_solid_periodic(True)
+    f.set_ntrace(8)
+    with pytest.raises(ia.IncrmtError) as e:
+        f.initialize(0.4, 0.4, 0.8)
+    assert e.value.code == ia.ERR_ARG
+    f.close()
