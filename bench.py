@@ -163,8 +163,8 @@ def main():
                          "passes: two profiler runs of it differ by exactly the steps")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU sharding is a design "
-              "in DESIGN.md, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): the path would shard as row slabs "
+              "with halo exchange, which is not built", file=sys.stderr)
         sys.exit(2)
     import incrmt_amd as ia
     m = a.size
