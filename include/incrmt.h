@@ -149,6 +149,10 @@ int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int
  * solve stops at the same relative residual. */
 #define INCRMT_GUESS_PREVIOUS 0
 #define INCRMT_GUESS_EXTRAPOLATE 1
+/* No reference counterpart: the quadratic (Lagrange) extrapolation in time
+ * of the last three solutions (linear while only two exist), O(dt^3) from
+ * the new one instead of O(dt^2). */
+#define INCRMT_GUESS_QUADRATIC 2
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
 /* How the projections' V-cycles are issued: captured once into a HIP graph
  * and replayed (INCRMT_LAUNCH_GRAPH, the default) or launched kernel by

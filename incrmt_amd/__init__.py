@@ -27,7 +27,7 @@ OK = 0
 ERR_ARG, ERR_NOT_CONVERGED, ERR_NEWTON, ERR_BAND, ERR_SINGULAR_FIT, ERR_HIP, ERR_UNSUPPORTED = range(1, 8)
 MG_MAC, MG_FEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_PREVIOUS, GUESS_EXTRAPOLATE = 0, 1
+GUESS_PREVIOUS, GUESS_EXTRAPOLATE, GUESS_QUADRATIC = 0, 1, 2
 LAUNCH_DIRECT, LAUNCH_GRAPH = 0, 1
 SIM_REST, SIM_FOUR_ROLLS, SIM_VELOCITY_PULSES = 0, 1, 2
 

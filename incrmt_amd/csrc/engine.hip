@@ -1003,8 +1003,9 @@ int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int p
 
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
-    if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE) return fail(INCRMT_ERR_ARG, "bad guess mode");
-    f->e.mac.extrapolate = f->e.fem.extrapolate = mode == INCRMT_GUESS_EXTRAPOLATE;
+    if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE && mode != INCRMT_GUESS_QUADRATIC)
+        return fail(INCRMT_ERR_ARG, "bad guess mode");
+    f->e.mac.extrapolate = f->e.fem.extrapolate = mode;
     return 0;
 }
 

@@ -127,7 +127,9 @@ class Multigrid {
     // start each solve from the linear extrapolation in time of the last two
     // solutions (mgs_base::initial_guess with mg_lin_t_interp), the solve's
     // time being t_solve (the step's projections set both)
-    bool extrapolate = false;
+    // 0: previous solution; 1: linear (mg_lin_t_interp); 2: quadratic in
+    // time through the last three solutions (linear while two exist)
+    int extrapolate = 0;
     double t_solve = 0;
     // replay captured V-cycles (HIP graphs) instead of launching them
     bool use_graphs = true;
@@ -156,6 +158,9 @@ class Multigrid {
     void restore_buffers(const std::vector<uintptr_t> &key);
     int run_cycle(int which, const std::function<int()> &cycle);
     int scount_ = 0;                 // solves recorded for the extrapolation (0, 1, 2)
+    int nq_ = 0;                     // solutions known to the quadratic start (0..3)
+    double tq_[3] = {0, 0, 0};       // their times, newest first
+    double *zq1_ = nullptr, *zq2_ = nullptr;  // the second and third newest (the newest is z)
     double t0_ = 0, tprev_ = 0;      // times of the last two solves
     bool resampled_ = false;         // level 1 is a uniform resampling of level 0
     Resample rx_{}, ry_{};

@@ -988,6 +988,17 @@ __global__ void k_extrapolate(LevelArgs a, double *z, double *zp, double cb, dou
     put_z(a, z, i, j, cb * zz - cc * zp[q]);
     zp[q] = zz;
 }
+// Quadratic start: z <- w0 z + w1 r1 + w2 r2, then the history shifts
+// (r2 <- r1, r1 <- the old z).
+__global__ void k_extrapolate2(LevelArgs a, double *z, double *r1, double *r2, double w0, double w1, double w2) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i;
+    const double z0 = z[q], z1 = r1[q], z2 = r2[q];
+    put_z(a, z, i, j, w0 * z0 + w1 * z1 + w2 * z2);
+    r2[q] = z1;
+    r1[q] = z0;
+}
 __global__ void k_copy_level(LevelArgs a, const double *z, double *zp) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
@@ -1793,7 +1804,48 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     const MgLevel &f = L[0];
     int rc;
     if ((rc = wrap_fine(f.z))) return rc;  // the iterate may come from outside
-    if (extrapolate) {
+    if (extrapolate == 2) {
+        // Lagrange extrapolation in time through the newest solutions: z
+        // holds the newest (time tq_[0]), zq1_ and zq2_ the two before.
+        const double t = t_solve;
+        if (!zq1_) {
+            const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+            for (double **p : {&zq1_, &zq2_}) {
+                void *q;
+                HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+                HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+                allocs.push_back(q);
+                *p = (double *)q + f.pitch + 1;
+            }
+        }
+        const double *T = tq_;
+        const double eps = 100. * std::numeric_limits<double>::epsilon();
+        double w0 = 1, w1 = 0, w2 = 0;
+        bool ok = true;
+        if (nq_ >= 3) {
+            ok = std::fabs(T[0] - T[1]) > eps && std::fabs(T[0] - T[2]) > eps && std::fabs(T[1] - T[2]) > eps;
+            if (ok) {
+                w0 = (t - T[1]) * (t - T[2]) / ((T[0] - T[1]) * (T[0] - T[2]));
+                w1 = (t - T[0]) * (t - T[2]) / ((T[1] - T[0]) * (T[1] - T[2]));
+                w2 = (t - T[0]) * (t - T[1]) / ((T[2] - T[0]) * (T[2] - T[1]));
+            }
+        } else if (nq_ == 2) {
+            ok = std::fabs(T[0] - T[1]) > eps;
+            if (ok) {
+                w0 = (t - T[1]) / (T[0] - T[1]);
+                w1 = (t - T[0]) / (T[1] - T[0]);
+            }
+        }
+        if (!ok) { w0 = 1; w1 = w2 = 0; }  // repeated times: keep the newest as it is
+        if (nq_ >= 1) {
+            k_extrapolate2<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zq1_, zq2_, w0, w1, w2);
+            KCHECK();
+            tq_[2] = tq_[1];
+            tq_[1] = tq_[0];
+        }
+        tq_[0] = t;  // the time of the solution this solve leaves in z
+        if (nq_ < 3) nq_++;
+    } else if (extrapolate) {
         // mgs_base::initial_guess (mgs_common.cc:33-66): start from the
         // linear interpolation in time of the last two solutions, at the
         // solve time t_solve: the first call records its time, the second

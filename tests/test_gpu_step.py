@@ -153,13 +153,15 @@ def test_solver_failure_is_reported():
     assert e.value.code == ia.ERR_NOT_CONVERGED
 
 
-def test_extrapolated_guess_same_solution():
-    # starting the projections from 2 z_n - z_(n-1) changes the iterates,
-    # not the answer: per-step parity with the oracle as above, fewer cycles
+@pytest.mark.parametrize("guess", [ia.GUESS_EXTRAPOLATE, ia.GUESS_QUADRATIC])
+def test_extrapolated_guess_same_solution(guess):
+    # starting the projections from the linear (mg_lin_t_interp) or
+    # quadratic extrapolation in time changes the iterates, not the answer:
+    # per-step parity with the oracle as above, fewer cycles
     m = 64
     spec = ol.simple_spin_spec(m)
     g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
-    g.set_guess(ia.GUESS_EXTRAPOLATE)
+    g.set_guess(guess)
     base = gpu_from_spec(spec)
     l, adt = g.timestep_select(4 * math.pi / 240)
     dx = 2.0 / m
