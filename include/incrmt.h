@@ -155,9 +155,12 @@ int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int
 #define INCRMT_GUESS_QUADRATIC 2
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
 /* How the projections' V-cycles are issued: captured once into a HIP graph
- * and replayed (INCRMT_LAUNCH_GRAPH, the default) or launched kernel by
- * kernel (INCRMT_LAUNCH_DIRECT). Results are bit-identical. No reference
- * counterpart (the reference's V-cycles are host loops in TGMG). */
+ * and replayed (INCRMT_LAUNCH_GRAPH) or launched kernel by kernel
+ * (INCRMT_LAUNCH_DIRECT). Results are bit-identical. The default is graphs
+ * for solvers whose finest level has fewer than 512^2 unknowns (launch
+ * latency dominates) and direct launches above (a 1024^2 step is 1% faster
+ * so). No reference counterpart (the reference's V-cycles are host loops in
+ * TGMG). */
 #define INCRMT_LAUNCH_DIRECT 0
 #define INCRMT_LAUNCH_GRAPH 1
 int incrmt_fluid_set_launch(incrmt_fluid *f, int mode);
@@ -251,9 +254,9 @@ int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
  * except with a zero-pressure edge). */
 int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
                     int *cycles, double *rel_res);
-/* V-cycle issue mode of incrmt_mg_solve (INCRMT_LAUNCH_GRAPH, the default,
- * or INCRMT_LAUNCH_DIRECT; see incrmt_fluid_set_launch). No reference
- * counterpart. */
+/* V-cycle issue mode of incrmt_mg_solve (INCRMT_LAUNCH_GRAPH or
+ * INCRMT_LAUNCH_DIRECT, default by size as for incrmt_fluid_set_launch). No
+ * reference counterpart. */
 int incrmt_mg_set_launch(incrmt_mg *h, int mode);
 /* No reference counterpart (for checking the smoother): `sweeps` forward
  * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused

@@ -131,8 +131,10 @@ class Multigrid {
     // time through the last three solutions (linear while two exist)
     int extrapolate = 0;
     double t_solve = 0;
-    // replay captured V-cycles (HIP graphs) instead of launching them
+    // replay captured V-cycles (HIP graphs) instead of launching them; init
+    // sets it for hierarchies whose finest level is below kGraphMaxPoints
     bool use_graphs = true;
+    static constexpr long long kGraphMaxPoints = 512LL * 512;
     double smoother_ms = 0;
     long smoother_launches = 0;
     double smoother_bytes = 0;  // algorithmic bytes of the timed sweeps

@@ -1145,6 +1145,9 @@ int Multigrid::make_resample(int nf, double hf, int nc, Resample &r) {
 
 int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px, bool py, bool dr) {
     kind_ = kind; s_ = s; dx_ = dx; dy_ = dy; dr_ = dr;
+    // graph replay pays where launch latency dominates; a large finest level
+    // keeps the queue full with direct launches and skips the replay cost
+    use_graphs = (long long)m * n < kGraphMaxPoints;
     int nx = m, ny = n;
     double kx = 1. / (dx * dx), ky = 1. / (dy * dy), hx = dx, hy = dy;
     // a periodic direction is coloured across its seam: it needs an even
