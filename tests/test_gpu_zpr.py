@@ -189,3 +189,18 @@ def test_horiz_flow_properties():
         assert np.all(p[2:-1, 2 + 256] == 0.0)
         assert np.all(g.get("u")[:, 0:2] == 1.0)
     g.close()
+
+
+def test_uniform_flow_stays_uniform():
+    """No solid, y periodic: the uniform inflow is a steady state of the
+    horizontal-flow step, kept to the last bit (the oracle's own check in
+    tests/test_oracle_zpr.py)."""
+    f = ia.Fluid2D(64, 32, visc=1e-2, rhof=1.0, ax=-1.0, bx=3.0, ay=-1.0, by=1.0, x_prd=False, y_prd=True)
+    f.set_horizontal_flow(1.0)
+    f.set_solver(rel_tol=1e-12, max_cycles=200)
+    f.initialize(0.8, 0.4, 0.8)
+    for k in range(4):
+        f.step_forward(0.01)
+    u, v, p = f.get("u"), f.get("v"), f.get("p")
+    assert np.all(u[2:-2, 2:-2] == 1.0) and np.all(v[2:-2, 2:-2] == 0.0) and np.all(p[2:-2, 2:-2] == 0.0)
+    f.close()

@@ -114,3 +114,20 @@ def test_horiz_flow_step_properties():
         # right ghost columns: field::extrap
         assert np.array_equal(u[2:-2, -2], 2 * u[2:-2, -3] - u[2:-2, -4])
     o.close()
+
+
+def test_uniform_flow_stays_uniform():
+    """Closed form: with no solid, y periodic, the uniform inflow u = U,
+    v = 0 is a steady state of the horizontal-flow step, and it is kept
+    exactly only if every boundary piece agrees with it: the inflow ghosts
+    (U, 0), the inflow face left as cell 0's own state (not a zeroed wall
+    face) in the MAC source, the ghost velocity in the FEM source left of
+    node 0 (not zero), the fixed node column and no mean removed."""
+    spec = ol.horiz_flow_spec(32, 16, U=1.0)
+    spec["objects"] = []
+    o = ol.OracleFluid(spec, mg_tol=1e-12)
+    for k in range(4):
+        o.step(0.02)
+    u, v, p = o.get("u"), o.get("v"), o.get("p")
+    assert np.all(u[2:-2, 2:-2] == 1.0) and np.all(v[2:-2, 2:-2] == 0.0) and np.all(p[2:-2, 2:-2] == 0.0)
+    o.close()
