@@ -178,3 +178,21 @@ def test_full_solid_leaves_no_room_for_tracers():
         f.initialize(0.4, 0.4, 0.8)
     assert e.value.code == ia.ERR_ARG
     f.close()
+
+
+def test_translating_full_solid():
+    """obj_full at a uniform velocity with solid periodic boundaries is a
+    rigid translation (X = x - U t, zero stress): the GPU keeps it to
+    rounding, as the oracle does (tests/test_oracle_zpr.py)."""
+    spec = ol.conv_spec(64, 2, 280)
+    spec["vel"] = 0
+    g = gpu_from_spec(spec)
+    u = g.get("u"); u[:] = 0.5; g.set("u", u)
+    v = g.get("v"); v[:] = 0.0; g.set("v", v)
+    X0 = g.get("X").copy()
+    for k in range(5):
+        g.step_forward(0.005)
+    u, v, X = g.get("u"), g.get("v"), g.get("X")
+    assert np.abs(u[2:-2, 2:-2] - 0.5).max() <= 1e-13 and np.abs(v[2:-2, 2:-2]).max() <= 1e-13
+    assert np.abs(X - (X0 - 0.5 * 0.025))[2:-2, 2:-2].max() <= 1e-13
+    g.close()
