@@ -65,8 +65,8 @@ int incrmt_copy_probe(long long bytes, int reps, double *ms_per_copy);
  * (sim_type.hh:63-71): no-slip walls, or periodic along x_prd / y_prd
  * (fluid_2d.cc:689-872, 1193-1235, 1317-1348; a periodic direction needs an
  * even cell count whose halvings end at 2 or 3, checked at initialize).
- * The outflow (zpr) and periodic-solid (solid_prd_bc) variants are not
- * implemented. */
+ * The outflow (zpr) and periodic-solid (solid_prd_bc) variants are set by
+ * incrmt_fluid_set_horizontal_flow and incrmt_fluid_set_solid_periodic. */
 int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
                         double w_scale, double w_force, double visc, double rhof, double tw, double pinw,
                         incrmt_fluid **out);
@@ -159,8 +159,8 @@ int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
  * (INCRMT_LAUNCH_DIRECT). Results are bit-identical. The default is graphs
  * for solvers whose finest level has fewer than 512^2 unknowns (launch
  * latency dominates) and direct launches above (a 1024^2 step is 1% faster
- * so). No reference counterpart (the reference's V-cycles are host loops in
- * TGMG). */
+ * so). A call before initialize is kept and applied there. No reference
+ * counterpart (the reference's V-cycles are host loops in TGMG). */
 #define INCRMT_LAUNCH_DIRECT 0
 #define INCRMT_LAUNCH_GRAPH 1
 int incrmt_fluid_set_launch(incrmt_fluid *f, int mode);
@@ -193,7 +193,8 @@ int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
 int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned int fflags);
 /* No reference counterpart: the fused-smoother launch shape the autotuner
  * picked for each multigrid level ("nx x ny: SXxSY hH/SXxSY hH", FP64/FP32
- * sweeps), for reports. */
+ * sweeps) and each solver's V-cycle issue mode ("launch MAC graph, FEM
+ * direct"), for reports. */
 int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len);
 /* No reference counterpart: time `sweeps` fine-level smoother sweeps of one
  * projection system (INCRMT_MG_MAC / INCRMT_MG_FEM) back to back between two

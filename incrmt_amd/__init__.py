@@ -232,10 +232,13 @@ class Fluid2D:
         _check(lib().incrmt_fluid_set_solver(self._h, rel_tol, max_cycles, precision))
 
     def set_guess(self, mode):
-        """GUESS_PREVIOUS (the reference's warm start) or GUESS_EXTRAPOLATE
-        (the linear extrapolation in time of the last two solutions,
+        """The projections' starting iterate: GUESS_PREVIOUS (the
+        reference's warm start, the default), GUESS_EXTRAPOLATE (the linear
+        extrapolation in time of the last two solutions,
         mgs_base::initial_guess with mg_lin_t_interp, mgs_common.cc:33-66)
-        as the projections' starting iterate."""
+        or GUESS_QUADRATIC (the quadratic extrapolation through the last
+        three). Extrapolated starts are made mean-free for the singular
+        (pure Neumann) systems; changing the mode clears the history."""
         _check(lib().incrmt_fluid_set_guess(self._h, mode))
 
     def set_sim_velocity(self, kind):
@@ -251,8 +254,10 @@ class Fluid2D:
         _check(lib().incrmt_fluid_set_horizontal_flow(self._h, float(U)))
 
     def set_launch(self, mode):
-        """LAUNCH_GRAPH (V-cycles captured once as HIP graphs and replayed;
-        the default) or LAUNCH_DIRECT (kernel by kernel); bit-identical."""
+        """LAUNCH_GRAPH (V-cycles captured once as HIP graphs and replayed)
+        or LAUNCH_DIRECT (kernel by kernel); bit-identical. Without a call
+        the solvers pick by size: graphs below 512^2 unknowns, direct
+        launches above. A call before initialize is kept and applied there."""
         _check(lib().incrmt_fluid_set_launch(self._h, mode))
 
     def step_forward(self, dt):
@@ -424,7 +429,8 @@ class MgSolver:
         return self.z.copy(), c.value, r.value
 
     def set_launch(self, mode):
-        """LAUNCH_GRAPH (default) or LAUNCH_DIRECT V-cycles; bit-identical."""
+        """LAUNCH_GRAPH or LAUNCH_DIRECT V-cycles (default by size: graphs
+        below 512^2 unknowns); bit-identical."""
         _check(lib().incrmt_mg_set_launch(self._h, mode))
 
     def apply(self, x):

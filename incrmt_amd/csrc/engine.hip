@@ -100,6 +100,7 @@ struct FluidEngine {
     int max_cycles = 100, precision = INCRMT_PREC_FP64;
     incrmt_stats st{};
     int ntrace_req = 0;   // fluid tracers asked for before initialize
+    int launch_req = -1;  // V-cycle issue mode asked for (-1: the solvers' default by size)
     TracerSet tr;
 
     ~FluidEngine() {
@@ -353,6 +354,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
         return fail(rc0, "a periodic direction or a zero-pressure edge needs an even cell count whose halvings end "
                          "at most 16 points");
     if (rc0) return rc0;
+    if (launch_req >= 0) mac.use_graphs = fem.use_graphs = launch_req == INCRMT_LAUNCH_GRAPH;
     // fluid_2d::init_fields (fluid_2d.cc:195-254)
     for (auto &o : obj) {
         int passes;
@@ -1005,7 +1007,8 @@ int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE && mode != INCRMT_GUESS_QUADRATIC)
         return fail(INCRMT_ERR_ARG, "bad guess mode");
-    f->e.mac.extrapolate = f->e.fem.extrapolate = mode;
+    f->e.mac.set_extrapolate(mode);
+    f->e.fem.set_extrapolate(mode);
     return 0;
 }
 
@@ -1031,7 +1034,10 @@ int incrmt_fluid_set_horizontal_flow(incrmt_fluid *f, double U) {
 int incrmt_fluid_set_launch(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_LAUNCH_DIRECT && mode != INCRMT_LAUNCH_GRAPH) return fail(INCRMT_ERR_ARG, "bad launch mode");
-    f->e.mac.use_graphs = f->e.fem.use_graphs = mode == INCRMT_LAUNCH_GRAPH;
+    f->e.launch_req = mode;
+    // before initialize the request is kept and applied after the solvers'
+    // set-up (which otherwise picks the mode by size)
+    if (f->e.initialized) f->e.mac.use_graphs = f->e.fem.use_graphs = mode == INCRMT_LAUNCH_GRAPH;
     return 0;
 }
 
@@ -1126,7 +1132,9 @@ int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec
 int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len) {
     if (int rc = check_ready(f)) return rc;
     if (!buf || len <= 0) return fail(INCRMT_ERR_ARG, "null buffer");
-    const std::string d = "MAC " + f->e.mac.shapes() + "; FEM " + f->e.fem.shapes();
+    const std::string d = "MAC " + f->e.mac.shapes() + "; FEM " + f->e.fem.shapes() + "; launch MAC " +
+                          (f->e.mac.use_graphs ? "graph" : "direct") + ", FEM " +
+                          (f->e.fem.use_graphs ? "graph" : "direct");
     snprintf(buf, len, "%s", d.c_str());
     return 0;
 }

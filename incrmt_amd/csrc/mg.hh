@@ -131,6 +131,8 @@ class Multigrid {
     // time through the last three solutions (linear while two exist)
     int extrapolate = 0;
     double t_solve = 0;
+    // change the starting-iterate mode; a change clears the solution history
+    void set_extrapolate(int mode);
     // replay captured V-cycles (HIP graphs) instead of launching them; init
     // sets it for hierarchies whose finest level is below kGraphMaxPoints
     bool use_graphs = true;

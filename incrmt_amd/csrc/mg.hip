@@ -1884,6 +1884,20 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
             t0_ = t;
         }
     }
+    if (extrapolate && singular()) {
+        // an extrapolated start also carries the constant null-space part
+        // of the iterates forward, and the quadratic weights (3,-3,1 at
+        // equal steps) amplify any drift of it from solve to solve: the
+        // start is made mean-free (the caller uses only gradients and
+        // mean-free values, so the solution is unchanged)
+        k_norms<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), fem_consts(dx_, dy_), f.z, f.z, f.fw, f.fs,
+                                                    f.kc, d_red + 8);
+        KCHECK();
+        k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_red);
+        KCHECK();
+        k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, d_red);
+        KCHECK();
+    }
     if (singular()) {
         // pure Neumann: only the mean-free part of b is solvable
         if ((rc = reduce_dev())) return rc;
@@ -1979,6 +1993,15 @@ int Multigrid::run_cycle(int which, const std::function<int()> &cycle) {
     }
     HIP_CHECK(hipGraphLaunch(graph_exec_[which], s_));
     return 0;
+}
+
+void Multigrid::set_extrapolate(int mode) {
+    if (mode != extrapolate) {
+        // the history of one mode means nothing to another: start afresh
+        scount_ = 0;
+        nq_ = 0;
+    }
+    extrapolate = mode;
 }
 
 int Multigrid::solve(double tol, int max_cycles, MgStats *st) {

@@ -363,6 +363,7 @@ class fluid {
     int pin();
     void compute_ustar(double dt);
     int fem_projection(double dt);
+    void fem_source(double dt);
     void copy_pressure();
     void update_velocity(double dt);
     int step(double dt);
@@ -1089,7 +1090,20 @@ void fluid::compute_ustar(double dt) {  // fluid_2d.cc:1278-1291
     }
 }
 
-int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_fem.cc:104-121
+int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313; mgs_fem.cc:104-121
+    fem_source(dt);
+    const double *ir = nullptr;
+    if (!const_rho) {
+        for (int j = 0; j < n_fem; j++)
+            for (int i = 0; i < m_fem; i++) irho_fem[j * m_fem + i] = 1. / F(i - 1, j - 1).rho;
+        ir = irho_fem.data();
+    }
+    fem_system s{m_fem, n_fem, dx, dy, ir, sp.x_prd, sp.y_prd, sp.zpr};
+    fem_cycles = fem_solve(s, src.data(), zfem.data(), sp.mg_tol, &fem_res);
+    return fem_cycles < 0 ? 3 : 0;
+}
+
+void fluid::fem_source(double dt) {  // fluid_2d.cc:1297-1309, 839-872
     for (int j = 0; j < n; j++) {
         if (sp.x_prd) {
             F(-1, j).c0 = F(m - 1, j).c0;
@@ -1119,15 +1133,6 @@ int fluid::fem_projection(double dt) {  // fluid_2d.cc:1297-1313, 839-872; mgs_f
             src[j * m_fem + i] = hx * (fp[-ml - 1].c0 + fp[-1].c0 - fp[-ml].c0 - fp->c0)
                                  + hy * (fp[-ml - 1].c1 - fp[-1].c1 + fp[-ml].c1 - fp->c1);
         }
-    const double *ir = nullptr;
-    if (!const_rho) {
-        for (int j = 0; j < n_fem; j++)
-            for (int i = 0; i < m_fem; i++) irho_fem[j * m_fem + i] = 1. / F(i - 1, j - 1).rho;
-        ir = irho_fem.data();
-    }
-    fem_system s{m_fem, n_fem, dx, dy, ir, sp.x_prd, sp.y_prd, sp.zpr};
-    fem_cycles = fem_solve(s, src.data(), zfem.data(), sp.mg_tol, &fem_res);
-    return fem_cycles < 0 ? 3 : 0;
 }
 
 void fluid::copy_pressure() {  // fluid_2d.cc:1317-1348
@@ -1381,6 +1386,18 @@ static bool field_ptr(fluid *f, const std::string &nm, int k, size_t &off, bool 
     else if (nm == "c1") off = offsetof(cellf, c1);
     else if (nm == "ul") off = offsetof(cellf, ul);
     else if (nm == "vd") off = offsetof(cellf, vd);
+    else if (nm == "vl") off = offsetof(cellf, vl);
+    else if (nm == "ur") off = offsetof(cellf, ur);
+    else if (nm == "vr") off = offsetof(cellf, vr);
+    else if (nm == "ud") off = offsetof(cellf, ud);
+    else if (nm == "uu") off = offsetof(cellf, uu);
+    else if (nm == "vu") off = offsetof(cellf, vu);
+    else if (nm == "ux") off = offsetof(cellf, ux);
+    else if (nm == "uy") off = offsetof(cellf, uy);
+    else if (nm == "vx") off = offsetof(cellf, vx);
+    else if (nm == "vy") off = offsetof(cellf, vy);
+    else if (nm == "c2") off = offsetof(cellf, c2);
+    else if (nm == "c3") off = offsetof(cellf, c3);
     else {
         if (k < 0 || k >= (int)f->obj.size()) return false;
         solidf = true;
@@ -1388,12 +1405,64 @@ static bool field_ptr(fluid *f, const std::string &nm, int k, size_t &off, bool 
         else if (nm == "Y") off = offsetof(scell, Y);
         else if (nm == "nX") off = offsetof(scell, nX);
         else if (nm == "nY") off = offsetof(scell, nY);
+        else if (nm == "Xx") off = offsetof(scell, Xx);
+        else if (nm == "Xy") off = offsetof(scell, Xy);
+        else if (nm == "Yx") off = offsetof(scell, Yx);
+        else if (nm == "Yy") off = offsetof(scell, Yy);
+        else if (nm == "Xl") off = offsetof(scell, Xl);
+        else if (nm == "Xr") off = offsetof(scell, Xr);
+        else if (nm == "Xd") off = offsetof(scell, Xd);
+        else if (nm == "Xu") off = offsetof(scell, Xu);
+        else if (nm == "Yl") off = offsetof(scell, Yl);
+        else if (nm == "Yr") off = offsetof(scell, Yr);
+        else if (nm == "Yd") off = offsetof(scell, Yd);
+        else if (nm == "Yu") off = offsetof(scell, Yu);
         else if (nm == "phi") off = (size_t)-1;
         else if (nm == "s") { off = (size_t)-1; isint = true; }
         else return false;
     }
     return true;
 }
+
+// One phase of fluid_2d::step_forward on the current state, so that tests
+// can pin a single sweep by a closed form. The names and the reference
+// code each runs:
+//   normal_derivatives  fluid_2d.cc:877-950      godunov_tang  :642-686 (tang)
+//   edge_bc             :729-779                 tangential    :953-975
+//   half_time_edges     :979-1108                godunov       :1112-1121
+//   mac_projection      :1175-1236               update_reference_map :1242-1273
+//   stress              :381-600                 ustar         :1278-1291
+//   fem_source          :1297-1309, 839-872 (out: the m_fem x n_fem source)
+//   copy_pressure       :1317-1348 (in: the m_fem x n_fem solution)
+//   update_velocity     :1354-1374               set_boundaries :689-726
+//   set_rho             :1124-1137
+// Returns 0, or 1 for an unknown name.
+int fluid_phase(fluid *f, const char *name, double dt, const double *in, double *out) {
+    const std::string nm(name);
+    if (nm == "normal_derivatives") f->normal_derivatives(dt);
+    else if (nm == "godunov_tang") f->godunov(true);
+    else if (nm == "edge_bc") f->edge_bc();
+    else if (nm == "tangential") f->tangential();
+    else if (nm == "half_time_edges") f->half_time_edges(dt);
+    else if (nm == "godunov") f->godunov(false);
+    else if (nm == "mac_projection") return f->mac_projection() ? 2 : 0;
+    else if (nm == "update_reference_map") f->update_reference_map(dt);
+    else if (nm == "stress") f->compute_stress();
+    else if (nm == "ustar") f->compute_ustar(dt);
+    else if (nm == "fem_source") {
+        f->fem_source(dt);
+        std::memcpy(out, f->src.data(), sizeof(double) * f->m_fem * f->n_fem);
+    } else if (nm == "copy_pressure") {
+        std::memcpy(f->zfem.data(), in, sizeof(double) * f->m_fem * f->n_fem);
+        f->copy_pressure();
+    } else if (nm == "update_velocity") f->update_velocity(dt);
+    else if (nm == "set_boundaries") f->set_boundaries();
+    else if (nm == "set_rho") f->set_rho();
+    else return 1;
+    return 0;
+}
+
+void fluid_set_time(fluid *f, double t) { f->time = t; }
 
 int fluid_get(fluid *f, const char *name, int k, double *out) {
     size_t off; bool so, isint;

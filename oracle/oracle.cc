@@ -127,6 +127,10 @@ int orc_fluid_step(void *h, double dt) { return fluid_step((fluid *)h, dt); }
 int orc_fluid_get(void *h, const char *name, int k, double *out) { return fluid_get((fluid *)h, name, k, out); }
 int orc_fluid_set(void *h, const char *name, int k, const double *in) { return fluid_set((fluid *)h, name, k, in); }
 void orc_fluid_stats(void *h, double *out4) { fluid_solver_stats((fluid *)h, out4); }
+int orc_fluid_phase(void *h, const char *name, double dt, const double *in, double *out) {
+    return fluid_phase((fluid *)h, name, dt, in, out);
+}
+void orc_fluid_set_time(void *h, double t) { fluid_set_time((fluid *)h, t); }
 void orc_fluid_init_tracers(void *h, int ntrace) { fluid_init_tracers((fluid *)h, ntrace); }
 void orc_fluid_frame_output(void *h) { fluid_frame_output((fluid *)h); }
 int orc_fluid_ntrace(void *h) { return fluid_ntrace((fluid *)h); }

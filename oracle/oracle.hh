@@ -128,6 +128,11 @@ int fluid_step(fluid *f, double dt);
 int fluid_get(fluid *f, const char *name, int k, double *out);
 int fluid_set(fluid *f, const char *name, int k, const double *in);
 void fluid_solver_stats(fluid *f, double *out4);
+// One named phase of step_forward on the current state (names and the
+// reference lines they follow are listed at the definition); in/out carry
+// the FEM solution or source where a phase takes or makes one.
+int fluid_phase(fluid *f, const char *name, double dt, const double *in, double *out);
+void fluid_set_time(fluid *f, double t);
 // Tracers: fluid_2d::init_tracers with glibc's unseeded rand() sequence
 // (fluid_2d.cc:1376-1386); the per-step improved-Euler update is part of
 // fluid_step. frame_output is the object part of write_files (track.dat).

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# INCRMT_ORACLE_SO points the tests at another build of the oracle (the
+# planted-mistake builds of tests/test_oracle_pin_mutants.py)
+ORACLE_SO = os.environ.get("INCRMT_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libreflevel.so")
 REF_INTERP_SO = os.path.join(ROOT, "oracle", "_ref", "librefinterp.so")
 
@@ -163,6 +165,7 @@ class OracleFluid:
             int(spec.get("vel", 0)), int(spec.get("zpr", 0)), float(spec.get("U", 0.0)),
             int(spec.get("solid_prd_bc", 0)))
         self.nobj = len(objs)
+        self.x_prd, self.y_prd = int(spec.get("x_prd", 0)), int(spec.get("y_prd", 0))
         dx, dy = (spec["bx"] - spec["ax"]) / self.m, (spec["by"] - spec["ay"]) / self.n
         self.eps = spec["tw"] * (dx if dx < dy else dy)
 
@@ -194,6 +197,24 @@ class OracleFluid:
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         if self.lib.orc_fluid_set(self.h, name.encode(), k, dptr(a)):
             raise KeyError(name)
+
+    def phase(self, name, dt=0.0, zfem=None):
+        """One phase of the step on the current state (oracle fluid_phase):
+        returns the FEM source for "fem_source"; "copy_pressure" takes the
+        FEM solution zfem (m_fem x n_fem)."""
+        mf = self.m if getattr(self, "x_prd", 0) else self.m + 1
+        nf = self.n if getattr(self, "y_prd", 0) else self.n + 1
+        out = np.zeros(mf * nf)
+        zin = np.zeros(mf * nf) if zfem is None else np.ascontiguousarray(zfem, dtype=np.float64).ravel()
+        f = self.lib.orc_fluid_phase
+        f.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, _dp, _dp]
+        if f(self.h, name.encode(), dt, dptr(zin), dptr(out)):
+            raise KeyError(name)
+        return out.reshape(nf, mf)
+
+    def set_time(self, t):
+        self.lib.orc_fluid_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        self.lib.orc_fluid_set_time(self.h, t)
 
     def stats(self):
         out = np.zeros(4)

@@ -183,7 +183,7 @@ def test_full_solid_leaves_no_room_for_tracers():
 def test_translating_full_solid():
     """obj_full at a uniform velocity with solid periodic boundaries is a
     rigid translation (X = x - U t, zero stress): the GPU keeps it to
-    rounding, as the oracle does (tests/test_oracle_zpr.py)."""
+    rounding, as the oracle does (tests/test_oracle_solid_periodic.py)."""
     spec = ol.conv_spec(64, 2, 280)
     spec["vel"] = 0
     g = gpu_from_spec(spec)

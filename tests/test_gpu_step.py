@@ -45,7 +45,7 @@ def compare(g, o, m, dx, tag):
         assert dphi <= 0.05 * dx, f"{tag}: phi diff {dphi:.3e}"
 
 
-def gpu_from_spec(spec, tol=1e-12):
+def gpu_from_spec(spec, tol=1e-12, precision=ia.PREC_FP64, guess=ia.GUESS_PREVIOUS):
     f = ia.Fluid2D(spec["m"], spec["n"], visc=spec["visc"], rhof=spec["rhof"], tw=spec["tw"], pinw=spec["pinw"],
                    ax=spec["ax"], bx=spec["bx"], ay=spec["ay"], by=spec["by"])
     for kind, p, G, rhos, evm in spec["objects"]:
@@ -53,15 +53,23 @@ def gpu_from_spec(spec, tol=1e-12):
             f.add_spin(*p, G=G, rhos=rhos, ev_trans_mult=evm)
         else:
             f.add_circle(p[0], p[1], p[2], p[3], G=G, rhos=rhos, ev_trans_mult=evm)
-    f.set_solver(rel_tol=tol, max_cycles=200)
+    f.set_solver(rel_tol=tol, max_cycles=200, precision=precision)
+    f.set_guess(guess)
     f.initialize(spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"])
     return f
 
 
+# (precision, starting iterate): the reference's FP64 warm start, and the
+# benchmark's solver (FP32 V-cycles in FP64 refinement from the quadratic
+# extrapolation in time)
+SOLVERS = [(ia.PREC_FP64, ia.GUESS_PREVIOUS), (ia.PREC_MIXED, ia.GUESS_QUADRATIC)]
+
+
+@pytest.mark.parametrize("solver", SOLVERS, ids=["fp64", "mixed+quadratic"])
 @pytest.mark.parametrize("m", [64, 128])
-def test_simple_spin_per_step(m):
+def test_simple_spin_per_step(m, solver):
     spec = ol.simple_spin_spec(m)
-    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    g, o = gpu_from_spec(spec, precision=solver[0], guess=solver[1]), ol.OracleFluid(spec, mg_tol=1e-13)
     assert g.dt_reg == o.dt_reg
     l, adt = g.timestep_select(4 * math.pi / 240)
     dx = 2.0 / m
@@ -153,16 +161,17 @@ def test_solver_failure_is_reported():
     assert e.value.code == ia.ERR_NOT_CONVERGED
 
 
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("guess", [ia.GUESS_EXTRAPOLATE, ia.GUESS_QUADRATIC])
-def test_extrapolated_guess_same_solution(guess):
+def test_extrapolated_guess_same_solution(guess, precision):
     # starting the projections from the linear (mg_lin_t_interp) or
     # quadratic extrapolation in time changes the iterates, not the answer:
     # per-step parity with the oracle as above, fewer cycles
     m = 64
     spec = ol.simple_spin_spec(m)
-    g, o = gpu_from_spec(spec), ol.OracleFluid(spec, mg_tol=1e-13)
+    g, o = gpu_from_spec(spec, precision=precision), ol.OracleFluid(spec, mg_tol=1e-13)
     g.set_guess(guess)
-    base = gpu_from_spec(spec)
+    base = gpu_from_spec(spec, precision=precision)
     l, adt = g.timestep_select(4 * math.pi / 240)
     dx = 2.0 / m
     for k in range(1, 16):
@@ -172,3 +181,73 @@ def test_extrapolated_guess_same_solution(guess):
         compare(g, o, m, dx, f"step {k}")
     sg, sb = g.stats(), base.stats()
     assert sg["total_mac_cycles"] + sg["total_fem_cycles"] < sb["total_mac_cycles"] + sb["total_fem_cycles"]
+
+
+@pytest.mark.parametrize("m", [64, 128])
+def test_benchmark_solver_at_its_tolerance(m):
+    # the benchmark's exact configuration (mixed precision, quadratic start,
+    # projections stopped at relative residual 1e-8) against the oracle
+    # solved to 1e-13, 20 steps. Stated tolerances for a 1e-8 solve: u, v
+    # relative L2 <= 1e-6, p relative L2 <= 1e-5, X, Y max |difference|
+    # <= 1e-9; the band as above.
+    spec = ol.simple_spin_spec(m)
+    g = gpu_from_spec(spec, tol=1e-8, precision=ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC)
+    o = ol.OracleFluid(spec, mg_tol=1e-13)
+    l, adt = g.timestep_select(4 * math.pi / 240)
+    dx = 2.0 / m
+    inner = (slice(2, -2), slice(2, -2))
+    worst = {}
+    for k in range(1, 21):
+        g.step_forward(adt)
+        o.step(adt)
+        st = g.stats()
+        assert st["mac_rel_res"] <= 1e-8 and st["fem_rel_res"] <= 1e-8
+        for nm, tol in (("u", 1e-6), ("v", 1e-6), ("p", 1e-5)):
+            r = rel(g.get(nm)[inner], o.get(nm)[inner])
+            worst[nm] = max(worst.get(nm, 0), r)
+            assert r <= tol, f"step {k}: {nm} relative L2 {r:.3e}"
+        for nm in ("X", "Y"):
+            d = np.abs(g.get(nm) - o.get(nm))[inner].max()
+            assert d <= 1e-9, f"step {k}: {nm} max diff {d:.3e}"
+    print(f"benchmark solver at {m}^2, worst relative L2 over 20 steps:", worst)
+
+
+def test_quadratic_start_long_run():
+    # 1200 steps at 64^2 from the quadratic start: the iterates' constant
+    # part stays bounded (the extrapolated start is kept mean-free) and the
+    # fields stay with those of the previous-solution start
+    spec = ol.simple_spin_spec(64)
+    q = gpu_from_spec(spec, tol=1e-10, precision=ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC)
+    b = gpu_from_spec(spec, tol=1e-10, precision=ia.PREC_FP64, guess=ia.GUESS_PREVIOUS)
+    l, adt = q.timestep_select(4 * math.pi / 240)
+    inner = (slice(2, -2), slice(2, -2))
+    for k in range(12):
+        q.steps(adt, 100)
+        b.steps(adt, 100)
+        for nm in ("u", "v", "p"):
+            r = rel(q.get(nm)[inner], b.get(nm)[inner])
+            assert r <= 1e-6, f"after {100 * (k + 1)} steps: {nm} relative L2 {r:.3e}"
+    # switching the mode clears the history: a linear start after the
+    # quadratic run is as good as a fresh one
+    q.set_guess(ia.GUESS_EXTRAPOLATE)
+    q.steps(adt, 5)
+    b.steps(adt, 5)
+    assert rel(q.get("p")[inner], b.get("p")[inner]) <= 1e-6
+    assert q.stats()["mac_cycles"] <= b.stats()["mac_cycles"] + 2
+
+
+def test_launch_mode_before_initialize_is_kept():
+    # LAUNCH_DIRECT asked for before initialize survives the solvers'
+    # set-up (which would pick graphs at 64^2); results are bit-identical
+    spec = ol.simple_spin_spec(64)
+    f = ia.Fluid2D(64, 64, visc=spec["visc"], rhof=spec["rhof"], tw=spec["tw"], pinw=spec["pinw"])
+    f.add_spin(*spec["objects"][0][1], G=48.0, rhos=3.0, ev_trans_mult=1.0)
+    f.set_launch(ia.LAUNCH_DIRECT)
+    f.set_solver(rel_tol=1e-12, max_cycles=200)
+    f.initialize(spec["ev_mult"], spec["dt_pad"], spec["dt_ev_pad"])
+    g = gpu_from_spec(spec)
+    for h in (f, g):
+        h.steps(h.dt_reg, 3)
+    assert np.array_equal(f.get("p"), g.get("p"))
+    assert f.describe().endswith("launch MAC direct, FEM direct")
+    assert g.describe().endswith("launch MAC graph, FEM graph")

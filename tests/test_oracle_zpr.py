@@ -131,4 +131,3 @@ def test_uniform_flow_stays_uniform():
     u, v, p = o.get("u"), o.get("v"), o.get("p")
     assert np.all(u[2:-2, 2:-2] == 1.0) and np.all(v[2:-2, 2:-2] == 0.0) and np.all(p[2:-2, 2:-2] == 0.0)
     o.close()
-
