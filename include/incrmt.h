@@ -283,6 +283,18 @@ void incrmt_mg_destroy(incrmt_mg *h);
  * fixed-point passes the rebuild took. */
 int incrmt_levelset_build_band(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
                                double phi_up, double *phi, int *s, int *passes);
+/* No reference counterpart: incrmt_levelset_build_band for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). phi (in/out) and s
+ * (out) are DEVICE arrays of m*n points, row-major; everything else the
+ * rebuild needs lives in one static per-device scratch array of 12 MiB,
+ * so m and n may be at most 520 (INCRMT_ERR_ARG beyond). Runs on the
+ * default stream of the current device and returns when the result is in
+ * phi and s. Results are identical to incrmt_levelset_build_band's. Every
+ * call on a device shares that device's scratch: calls that overlap on one
+ * GPU (from several threads) race. */
+int incrmt_levelset_build_band_noalloc(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                                       double phi_up, double *phi, int *s, int *passes);
 /* build_band followed by levelset::extrapolate_fields<E>() of two fields
  * (traverse.hh:235-260), as obj_field::extrapolate does for (nX,nY). */
 int incrmt_levelset_extrapolate(int m, int n, double ax, double ay, double dx, double dy, double phi_down,

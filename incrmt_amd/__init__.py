@@ -105,6 +105,7 @@ _PROTOS = [
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
+    ("incrmt_levelset_build_band_noalloc", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _V, _V, _ip]),
     ("incrmt_levelset_extrapolate", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _dp, _dp, _ip]),
 ]
 SYMBOLS = [p[0] for p in _PROTOS]

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 #include <limits>
+#include <atomic>
 #include "step.hh"
 #include "mg.hh"
 #include "levelset.hh"
@@ -1325,6 +1326,61 @@ int ls_args_ok(int m, int n, double dx, double dy, double pd, double pu) {
         return fail(INCRMT_ERR_ARG, "bad level-set arguments");
     return 0;
 }
+
+// ---- the band rebuild without allocation ---------------------------------
+// Everything the rebuild needs beyond the caller's phi and status arrays
+// (the padded working copies, the fixed-point buffers, the point and tile
+// lists, the counters) lives in `scratch`, one file-scope device array sized
+// for level-set grids of up to kNoAllocSide x kNoAllocSide points (12 MiB).
+// Each GPU holds its own copy until the process ends. Host code never uses
+// the array's name as an address: the address hipGetSymbolAddress returns on
+// each device is looked up once per device and kept in g_scratch_dev.
+constexpr int kNoAllocSide = 520;
+constexpr long long kNoAllocPitch = (kNoAllocSide + 31) / 32 * 32;
+constexpr long long kNoAllocN = kOrigin + kNoAllocSide * kNoAllocPitch + 64;
+constexpr long long kNoAllocL = (long long)kNoAllocSide * kNoAllocSide + 64;
+constexpr long long kNoAllocT =
+    (long long)((kNoAllocSide + kTileX - 1) / kTileX) * ((kNoAllocSide + kTileY - 1) / kTileY) + 64;
+constexpr size_t a256(long long b) { return (size_t)((b + 255) / 256 * 256); }
+constexpr size_t kNoAllocBytes = 3 * a256(kNoAllocN * 8) + a256(kNoAllocN * 4) + 2 * a256(kNoAllocN) +
+                                 3 * a256(kNoAllocL * 4) + 3 * a256(kNoAllocT * 4) +
+                                 a256((kCntIter + kIterSlots) * 4) + a256(16);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::atomic<unsigned char *> g_scratch_dev[kMaxDevices];
+// host mirror of the rebuild's counters (shared by every call in the process)
+int g_noalloc_hcnt[kCntIter + kIterSlots];
+
+int scratch_address(unsigned char **out) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(INCRMT_ERR_ARG, "device index beyond the scratch table");
+    unsigned char *p = g_scratch_dev[dev].load();
+    if (!p) {
+        void *q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        p = (unsigned char *)q;
+        g_scratch_dev[dev].store(p);
+    }
+    *out = p;
+    return 0;
+}
+
+__global__ void k_fill_words(unsigned *p, long long n, unsigned v) {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+int device_array(const void *p) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();  // a host pointer leaves an error behind: clear it
+        return fail(INCRMT_ERR_ARG, "phi and s must be device arrays");
+    }
+    return 0;
+}
 }  // namespace
 
 int incrmt_levelset_build_band(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
@@ -1342,6 +1398,79 @@ int incrmt_levelset_build_band(int m, int n, double ax, double ay, double dx, do
     if (passes) *passes = p;
     if (int r2 = L.get(L.a.phi, phi)) return r2;
     return L.get_status(s);
+}
+
+int incrmt_levelset_build_band_noalloc(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
+                                       double phi_up, double *phi, int *s, int *passes) {
+    if (int rc = ls_args_ok(m, n, dx, dy, phi_down, phi_up)) return rc;
+    if (!phi || !s) return fail(INCRMT_ERR_ARG, "null array");
+    if (m > kNoAllocSide || n > kNoAllocSide)
+        return fail(INCRMT_ERR_ARG, "level-set grid larger than the no-allocation limit of 520 x 520 points");
+    if (int rc = device_array(phi)) return rc;
+    if (int rc = device_array(s)) return rc;
+    unsigned char *base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    // the grid as incrmt_levelset_build_band lays it out
+    Grid g{};
+    g.m = m - 4; g.n = n - 4; g.ml = m; g.nl = n;
+    g.P = round_up(m, 32);
+    g.N = (long long)kOrigin + (long long)n * g.P + 64;
+    g.dx = dx; g.dy = dy; g.ax = ax + 1.5 * dx; g.ay = ay + 1.5 * dy;
+    g.lax = ax; g.lay = ay;
+    g.bx = g.ax + g.m * dx; g.by = g.ay + g.n * dy;
+    g.xsp = 1 / dx; g.ysp = 1 / dy; g.xxsp = g.xsp * g.xsp; g.yysp = g.ysp * g.ysp;
+    ObjConst c{};
+    c.phi_down = phi_down; c.phi_up = phi_up;
+    ObjArrays a{};
+    LsWork w{};
+    w.ntx = (g.ml + kTileX - 1) / kTileX;
+    w.nty = (g.nl + kTileY - 1) / kTileY;
+    const long long N = g.N, L = (long long)m * n + 64, T = (long long)w.ntx * w.nty + 64;
+    // carve the scratch (every piece 256-B aligned); sizes are within the
+    // maxima the array was sized for, since m, n <= kNoAllocSide
+    size_t off = 0;
+    auto take = [&](long long bytes) { unsigned char *p = base + off; off += a256(bytes); return p; };
+    a.phi = (double *)take(N * 8);
+    w.va = (double *)take(N * 8);
+    w.vb = (double *)take(N * 8);
+    a.st = (int *)take(N * 4);
+    w.mark = take(N);
+    w.cflag = take(N);
+    w.iface = (int *)take(L * 4);
+    w.cand = (int *)take(L * 4);
+    w.pband = (int *)take(L * 4);
+    w.tflag = (int *)take(T * 4);
+    w.tiles_fim = (int *)take(T * 4);
+    w.tiles_ext = (int *)take(T * 4);
+    w.cnt = (int *)take((kCntIter + kIterSlots) * 4);
+    w.bbox = (int *)take(16);
+    if (off > kNoAllocBytes) return fail(INCRMT_ERR_ARG, "scratch layout exceeds the array");
+    w.hcnt = g_noalloc_hcnt;  // no window is kept: no host box, no event
+    hipStream_t st = nullptr;
+    // the scratch keeps the previous call's contents: clear what the rebuild
+    // expects to start at zero (with a kernel, not a memset node)
+    auto clear = [&](void *p, long long bytes) -> int {
+        k_fill_words<<<256, 256, 0, st>>>((unsigned *)p, (long long)(a256(bytes) / 4), 0u);
+        HIP_CHECK(hipGetLastError());
+        return 0;
+    };
+    int rc = 0;
+    if ((rc = clear(a.phi, N * 8)) || (rc = clear(w.va, N * 8)) || (rc = clear(w.vb, N * 8)) ||
+        (rc = clear(a.st, N * 4)) || (rc = clear(w.mark, N)) || (rc = clear(w.cflag, N)) || (rc = clear(w.tflag, T * 4)))
+        return rc;
+    HIP_CHECK(hipMemcpy2DAsync(a.phi + g.ls(0, 0), g.P * sizeof(double), phi, m * sizeof(double), m * sizeof(double), n,
+                               hipMemcpyDeviceToDevice, st));
+    int p = 0;
+    rc = ls_rebuild(st, g, c, a, w, &p);
+    if (rc == 3) return fail(INCRMT_ERR_NEWTON, "Newton iteration failed.");
+    if (rc) return rc;
+    HIP_CHECK(hipMemcpy2DAsync(phi, m * sizeof(double), a.phi + g.ls(0, 0), g.P * sizeof(double), m * sizeof(double), n,
+                               hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipMemcpy2DAsync(s, m * sizeof(int), a.st + g.ls(0, 0), g.P * sizeof(int), m * sizeof(int), n,
+                               hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (passes) *passes = p;
+    return 0;
 }
 
 int incrmt_levelset_extrapolate(int m, int n, double ax, double ay, double dx, double dy, double phi_down,

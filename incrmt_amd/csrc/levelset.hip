@@ -831,9 +831,11 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     KCHECK();
     k_finalize<<<grid_win(wn), bb, 0, s>>>(g, c, a, w, bufs[passes & 1], wn);
     KCHECK();
-    HIP_CHECK(hipMemcpyAsync(w.hbbox, w.bbox, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipEventRecord(w.ev_bbox, s));
-    w.bbox_pending = true;
+    if (w.ev_bbox) {  // a work set without the event keeps no window (one-off rebuilds)
+        HIP_CHECK(hipMemcpyAsync(w.hbbox, w.bbox, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(w.ev_bbox, s));
+        w.bbox_pending = true;
+    }
     w.have_win = true;
     if (whole) {  // windows grow from the band's box, not from the whole grid
         w.win[0] = w.win[1] = w.win[2] = w.win[3] = 0;
