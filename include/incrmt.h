@@ -153,7 +153,38 @@ int incrmt_fluid_set_solver(incrmt_fluid *f, double rel_tol, int max_cycles, int
  * of the last three solutions (linear while only two exist), O(dt^3) from
  * the new one instead of O(dt^2). */
 #define INCRMT_GUESS_QUADRATIC 2
+/* No reference counterpart: the quadratic start corrected by the learned
+ * guess network (see "learned projection guess" below) along its
+ * prediction, by the step that minimises the error's energy norm; when the
+ * relative residual after one V-cycle exceeds the fallback tolerance (1e-4
+ * by default) the solve restarts from the quadratic start with FP64
+ * V-cycles; after 8 such misses in a row the learned start is switched off
+ * for 32 solves, then probed again (new parameters or a training call switch
+ * it back on at once). The learned start is used once the quadratic start
+ * has its three solutions (from the third solve on). */
+#define INCRMT_GUESS_LEARNED 3
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode);
+/* No reference counterpart: train the guess networks online, one Adam step
+ * (learning rate lr) per converged projection on its own start: the loss is
+ * the relative energy norm of the error left after the learned start's
+ * step along the prediction d, 1 - (e.Ad)^2 / ((e.Ae)(d.Ad)), e being the
+ * solution minus the quadratic start; fallback_tol as above. Works in any
+ * guess mode (the networks see every solve's start either way). */
+int incrmt_fluid_set_guess_training(incrmt_fluid *f, int train, float lr, double fallback_tol);
+/* The guess network of one projection (INCRMT_MG_MAC on the m x n cells,
+ * INCRMT_MG_FEM on the nodes): packed parameter count, get, set (after
+ * initialize). */
+int incrmt_fluid_guess_nparams(incrmt_fluid *f, int kind, long long *n);
+int incrmt_fluid_get_guess_params(incrmt_fluid *f, int kind, float *params);
+int incrmt_fluid_set_guess_params(incrmt_fluid *f, int kind, const float *params);
+/* learned starts used, fallbacks, training steps, last training loss,
+ * learned start switched off (0/1), misses in a row, the relative residual
+ * of the last learned start and after its first V-cycle */
+int incrmt_fluid_guess_stats(incrmt_fluid *f, int kind, double *out8);
+/* No reference counterpart (host logic, no GPU): the learned start's
+ * accept/fallback rule run over a sequence of relative residuals after the
+ * first V-cycle: decision 1 accepted, 0 fallback, -1 learned start off. */
+int incrmt_guess_policy_run(const double *rel, int n, double tol, int *decision);
 /* How the projections' V-cycles are issued: captured once into a HIP graph
  * and replayed (INCRMT_LAUNCH_GRAPH) or launched kernel by kernel
  * (INCRMT_LAUNCH_DIRECT). Results are bit-identical. The default is graphs
@@ -299,6 +330,32 @@ int incrmt_levelset_build_band_noalloc(int m, int n, double ax, double ay, doubl
  * (traverse.hh:235-260), as obj_field::extrapolate does for (nX,nY). */
 int incrmt_levelset_extrapolate(int m, int n, double ax, double ay, double dx, double dy, double phi_down,
                                 double phi_up, double *phi, int *s, double *fx, double *fy, int *passes);
+
+/* ---- learned projection guess ----------------------------------------------
+ * No reference counterpart. A 4-level U-Net on an nx x ny node grid (levels of
+ * (n+1)/2 nodes; 16/32/48/64 channels; 3x3 convolutions; BF16 weights and
+ * activations on the MFMA units with FP32 accumulation; ReLU; stride-2
+ * down-sampling, vertex-centred bilinear up-sampling, skip connections by
+ * concatenation; a linear one-channel output) that maps a projection's
+ * starting residual, level set and source to a correction of its starting
+ * iterate: the hook of mgs_base::initial_guess (mgs_common.cc:33-66).
+ * Parameters travel as one packed float vector: per layer, in forward order
+ * (E0a 4->16, E0b 16->16, D1 16->32 /2, E1 32->32, D2 32->48 /2, E2 48->48,
+ * D3 48->64 /2, B3 64->64, U2 (64+48)->48, U1 (48+32)->32, U0 (32+16)->16,
+ * Out 16->1), the weights [cout][tap = 3 (dy+1) + (dx+1)][cin] then the bias
+ * [cout]. Host arrays: in3 = nx*ny nodes x 3 channels (node-major), out and
+ * target = nx*ny; the loss is 0.5 mean((out - target)^2). */
+typedef struct incrmt_unet incrmt_unet;
+int incrmt_unet_create(int nx, int ny, unsigned seed, incrmt_unet **out);
+int incrmt_unet_nparams(const incrmt_unet *h, long long *n);
+int incrmt_unet_get_params(incrmt_unet *h, float *params);
+int incrmt_unet_set_params(incrmt_unet *h, const float *params);
+int incrmt_unet_forward(incrmt_unet *h, const float *in3, float *out);
+/* loss and d loss / d params (packed) of one sample, parameters unchanged */
+int incrmt_unet_gradient(incrmt_unet *h, const float *in3, const float *target, double *loss, float *grads);
+/* one Adam step (0.9, 0.999, 1e-8) on one sample */
+int incrmt_unet_train_step(incrmt_unet *h, const float *in3, const float *target, float lr, double *loss);
+void incrmt_unet_destroy(incrmt_unet *h);
 
 #ifdef __cplusplus
 }

@@ -27,13 +27,14 @@ OK = 0
 ERR_ARG, ERR_NOT_CONVERGED, ERR_NEWTON, ERR_BAND, ERR_SINGULAR_FIT, ERR_HIP, ERR_UNSUPPORTED = range(1, 8)
 MG_MAC, MG_FEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_PREVIOUS, GUESS_EXTRAPOLATE, GUESS_QUADRATIC = 0, 1, 2
+GUESS_PREVIOUS, GUESS_EXTRAPOLATE, GUESS_QUADRATIC, GUESS_LEARNED = 0, 1, 2, 3
 LAUNCH_DIRECT, LAUNCH_GRAPH = 0, 1
 SIM_REST, SIM_FOUR_ROLLS, SIM_VELOCITY_PULSES = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _D, _I, _V = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class IncrmtError(RuntimeError):
@@ -106,6 +107,20 @@ _PROTOS = [
     ("incrmt_mg_destroy", None, [_V]),
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
     ("incrmt_levelset_build_band_noalloc", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _V, _V, _ip]),
+    ("incrmt_fluid_set_guess_training", _I, [_V, _I, ctypes.c_float, _D]),
+    ("incrmt_fluid_guess_nparams", _I, [_V, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    ("incrmt_fluid_get_guess_params", _I, [_V, _I, _fp]),
+    ("incrmt_fluid_set_guess_params", _I, [_V, _I, _fp]),
+    ("incrmt_fluid_guess_stats", _I, [_V, _I, _dp]),
+    ("incrmt_guess_policy_run", _I, [_dp, _I, _D, _ip]),
+    ("incrmt_unet_create", _I, [_I, _I, ctypes.c_uint, ctypes.POINTER(_V)]),
+    ("incrmt_unet_nparams", _I, [_V, ctypes.POINTER(ctypes.c_longlong)]),
+    ("incrmt_unet_get_params", _I, [_V, _fp]),
+    ("incrmt_unet_set_params", _I, [_V, _fp]),
+    ("incrmt_unet_forward", _I, [_V, _fp, _fp]),
+    ("incrmt_unet_gradient", _I, [_V, _fp, _fp, _dp, _fp]),
+    ("incrmt_unet_train_step", _I, [_V, _fp, _fp, ctypes.c_float, _dp]),
+    ("incrmt_unet_destroy", None, [_V]),
     ("incrmt_levelset_extrapolate", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _dp, _dp, _ip]),
 ]
 SYMBOLS = [p[0] for p in _PROTOS]
@@ -241,6 +256,32 @@ class Fluid2D:
         three). Extrapolated starts are made mean-free for the singular
         (pure Neumann) systems; changing the mode clears the history."""
         _check(lib().incrmt_fluid_set_guess(self._h, mode))
+
+    def set_guess_training(self, train=True, lr=1e-3, fallback_tol=1e-4):
+        """Online training of the two learned-guess networks: one Adam step
+        per converged projection (the loss is the relative energy norm of
+        the error left after the learned start's step along the
+        prediction);
+        fallback_tol is the relative residual after one V-cycle above which
+        a learned start is dropped for the quadratic start and FP64 cycles."""
+        _check(lib().incrmt_fluid_set_guess_training(self._h, int(train), lr, fallback_tol))
+
+    def guess_params(self, kind):
+        n = ctypes.c_longlong(0)
+        _check(lib().incrmt_fluid_guess_nparams(self._h, kind, ctypes.byref(n)))
+        p = np.zeros(n.value, dtype=np.float32)
+        _check(lib().incrmt_fluid_get_guess_params(self._h, kind, _f(p)))
+        return p
+
+    def set_guess_params(self, kind, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().incrmt_fluid_set_guess_params(self._h, kind, _f(p)))
+
+    def guess_stats(self, kind):
+        o = np.zeros(8)
+        _check(lib().incrmt_fluid_guess_stats(self._h, kind, _d(o)))
+        return dict(learned_solves=int(o[0]), fallbacks=int(o[1]), train_steps=int(o[2]), loss=o[3],
+                    off=bool(o[4]), misses=int(o[5]), rel0=o[6], rel1=o[7])
 
     def set_sim_velocity(self, kind):
         """SIM_REST, SIM_FOUR_ROLLS or SIM_VELOCITY_PULSES: the sim_type's
@@ -492,3 +533,80 @@ def multibody(m=2048, count=64, seed=20260102, rel_tol=1e-8, precision=PREC_MIXE
     f.set_solver(rel_tol=rel_tol, max_cycles=max_cycles, precision=precision)
     f.initialize(0.8, 0.4, 0.8)
     return f
+
+
+# ------------------------------------------------------------ learned guess
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+class UNet:
+    """The learned projection guess network (include/incrmt.h): a 4-level
+    U-Net in BF16 on the MFMA units for an nx x ny node grid. Parameters are
+    one packed float32 vector (per layer weights [cout][tap][cin], bias)."""
+
+    def __init__(self, nx, ny, seed=1):
+        h = _V()
+        _check(lib().incrmt_unet_create(nx, ny, seed, ctypes.byref(h)))
+        self._h, self.nx, self.ny = h, nx, ny
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().incrmt_unet_destroy(self._h)
+            self._h = None
+
+    @property
+    def nparams(self):
+        n = ctypes.c_longlong(0)
+        _check(lib().incrmt_unet_nparams(self._h, ctypes.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().incrmt_unet_get_params(self._h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check(lib().incrmt_unet_set_params(self._h, _f(p)))
+
+    def _in(self, in3):
+        a = np.ascontiguousarray(in3, dtype=np.float32)
+        assert a.size == self.nx * self.ny * 3
+        return a
+
+    def forward(self, in3):
+        """in3: (ny, nx, 3) float32; returns the (ny, nx) prediction."""
+        a = self._in(in3)
+        out = np.zeros(self.nx * self.ny, dtype=np.float32)
+        _check(lib().incrmt_unet_forward(self._h, _f(a), _f(out)))
+        return out.reshape(self.ny, self.nx)
+
+    def gradient(self, in3, target):
+        a = self._in(in3)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = _D(0)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().incrmt_unet_gradient(self._h, _f(a), _f(t), ctypes.byref(loss), _f(g)))
+        return loss.value, g
+
+    def train_step(self, in3, target, lr):
+        a = self._in(in3)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = _D(0)
+        _check(lib().incrmt_unet_train_step(self._h, _f(a), _f(t), lr, ctypes.byref(loss)))
+        return loss.value
+
+
+def guess_policy_run(rel, tol=1e-4):
+    """The learned start's accept/fallback rule (host logic, no GPU) over a
+    sequence of relative residuals after the first V-cycle: 1 accepted, 0
+    fallback, -1 learned start switched off."""
+    r = np.ascontiguousarray(rel, dtype=np.float64)
+    d = np.zeros(r.size, dtype=np.int32)
+    _check(lib().incrmt_guess_policy_run(_d(r), r.size, tol, d.ctypes.data_as(_ip)))
+    return d

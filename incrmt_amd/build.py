@@ -13,7 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libincrmt_hip.so")
-SOURCES = ["step.hip", "mg.hip", "levelset.hip", "tracers.hip", "engine.hip"]
+SOURCES = ["step.hip", "mg.hip", "levelset.hip", "tracers.hip", "unet.hip", "engine.hip"]
 ARCH = os.environ.get("INCRMT_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

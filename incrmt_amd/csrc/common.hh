@@ -164,6 +164,8 @@ __host__ __device__ inline double obj_phi(const ObjConst &o, double X, double Y)
 
 namespace incrmt {
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
+// Record msg as the calling thread's last error and return code.
+int set_error(int code, const char *msg);
 // Clear device memory and wait for it. A plain hipMemset runs on the null
 // stream and may return before it is done; the library's streams are
 // non-blocking, so their next kernels are not ordered after it.

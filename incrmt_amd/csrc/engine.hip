@@ -18,6 +18,7 @@
 #include "mg.hh"
 #include "levelset.hh"
 #include "tracers.hh"
+#include "unet.hh"
 #include "../../include/incrmt.h"
 
 namespace incrmt {
@@ -34,6 +35,7 @@ static int fail(int code, const char *msg) {
     g_last_error = msg;
     return code;
 }
+int set_error(int code, const char *msg) { return fail(code, msg); }
 
 namespace {
 
@@ -74,6 +76,25 @@ __global__ void k_pavg_final(const double *part, int nb, double scale, double *o
     if (threadIdx.x == 0) *out = s[0] * scale;
 }
 
+// The level-set channel of the learned projection guess on a solver's
+// level-0 grid: the smallest solid phi at each MAC cell centre or, for the
+// FEM, at each node (the mean of its four cells), over six cell widths,
+// clamped to [-1, 1]; 1 without solids.
+__global__ void k_guess_feature(Grid g, const ObjArrays *oa, int nobj, int fem, int nx, int ny, int zp,
+                                double *feat) {
+    const int i = blockIdx.x * BXe + threadIdx.x, j = blockIdx.y * BYe + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    double v = 1e300;
+    for (int k = 0; k < nobj; k++) {
+        const double *ph = oa[k].phi;
+        const double p = fem ? 0.25 * (ph[g.at(i - 1, j - 1)] + ph[g.at(i, j - 1)] + ph[g.at(i - 1, j)] + ph[g.at(i, j)])
+                             : ph[g.at(i, j)];
+        v = fmin(v, p);
+    }
+    const double h = fmin(g.dx, g.dy);
+    feat[(size_t)j * zp + i] = nobj ? fmax(-1.0, fmin(1.0, v / (6 * h))) : 1.0;
+}
+
 }  // namespace
 
 struct Solid {
@@ -101,10 +122,21 @@ struct FluidEngine {
     int max_cycles = 100, precision = INCRMT_PREC_FP64;
     incrmt_stats st{};
     int ntrace_req = 0;   // fluid tracers asked for before initialize
+    // learned projection guess: one network per projection (MAC cells, FEM
+    // nodes) and the level-set channel on each solver's grid
+    int guess_mode = INCRMT_GUESS_PREVIOUS;
+    bool guess_train = false;
+    float guess_lr = 1e-3f;
+    double guess_tol = 1e-4;
+    UNet *gnet[2] = {nullptr, nullptr};
+    double *gfeat[2] = {nullptr, nullptr};
+    int ensure_guess();
+    int guess_features(Multigrid &mg, int which);
     int launch_req = -1;  // V-cycle issue mode asked for (-1: the solvers' default by size)
     TracerSet tr;
 
     ~FluidEngine() {
+        for (UNet *n : gnet) delete n;
         for (auto &o : obj) ls_free(o.w);
         for (void *p : allocs) (void)hipFree(p);
         if (s) (void)hipStreamDestroy(s);
@@ -255,6 +287,41 @@ int FluidEngine::init_tracers() {
     return 0;
 }
 
+// The two guess networks (seeded, so runs repeat) and their feature
+// arrays, once the solvers exist; the solvers' hooks point at them.
+int FluidEngine::ensure_guess() {
+    Multigrid *mg[2] = {&mac, &fem};
+    for (int w = 0; w < 2; w++) {
+        MgLevel &f = mg[w]->fine();
+        if (!gnet[w]) {
+            gnet[w] = new UNet;
+            if (int rc = gnet[w]->init(f.nx, f.ny, 20260105u + (unsigned)w, s)) return rc;
+            const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+            void *q;
+            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+            allocs.push_back(q);
+            gfeat[w] = (double *)q + f.pitch + 1;
+        }
+        mg[w]->net = gnet[w];
+        mg[w]->feat = gfeat[w];
+        mg[w]->train = guess_train;
+        mg[w]->lr = guess_lr;
+        mg[w]->guess_tol = guess_tol;
+    }
+    return 0;
+}
+
+int FluidEngine::guess_features(Multigrid &mg, int which) {
+    if (!mg.net || !(mg.extrapolate == INCRMT_GUESS_LEARNED || mg.train)) return 0;
+    const MgLevel &f = mg.fine();
+    const dim3 grid((f.nx + BXe - 1) / BXe, (f.ny + BYe - 1) / BYe);
+    k_guess_feature<<<grid, dim3(BXe, BYe), 0, s>>>(g, d_oa, (int)obj.size(), which, f.nx, f.ny, f.pitch,
+                                                     gfeat[which]);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if ((int)obj.size() > kMaxObj) return fail(INCRMT_ERR_ARG, "too many solids");
     choose_ev_and_dt(ev_mult, dt_pad, dt_ev_pad);
@@ -356,6 +423,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
                          "at most 16 points");
     if (rc0) return rc0;
     if (launch_req >= 0) mac.use_graphs = fem.use_graphs = launch_req == INCRMT_LAUNCH_GRAPH;
+    if ((guess_mode == INCRMT_GUESS_LEARNED || guess_train) && (rc0 = ensure_guess())) return rc0;
     // fluid_2d::init_fields (fluid_2d.cc:195-254)
     for (auto &o : obj) {
         int passes;
@@ -398,6 +466,7 @@ int FluidEngine::step(double dt) {
     if (!const_rho && (rc = mac.set_density(g, f.rho))) return rc;
     MgStats ms{};
     mac.t_solve = time;  // the MAC solve's time, t + dt/2 (initial_guess(f.time))
+    if ((rc = guess_features(mac, 0))) return rc;
     rc = precision == INCRMT_PREC_MIXED ? mac.solve_mixed(mg_tol, max_cycles, &ms) : mac.solve(mg_tol, max_cycles, &ms);
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in MAC problem");
     if (rc) return rc;
@@ -426,6 +495,7 @@ int FluidEngine::step(double dt) {
     if ((rc = launch_fem_src(s, g, f, dt, fl.b, fl.pitch))) return rc;
     if (!const_rho && (rc = fem.set_density(g, f.rho))) return rc;
     fem.t_solve = time;  // t + dt
+    if ((rc = guess_features(fem, 1))) return rc;
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
     if (rc) return rc;
@@ -1006,10 +1076,85 @@ int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int p
 
 int incrmt_fluid_set_guess(incrmt_fluid *f, int mode) {
     if (!f) return fail(INCRMT_ERR_ARG, "null handle");
-    if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE && mode != INCRMT_GUESS_QUADRATIC)
+    if (mode != INCRMT_GUESS_PREVIOUS && mode != INCRMT_GUESS_EXTRAPOLATE && mode != INCRMT_GUESS_QUADRATIC &&
+        mode != INCRMT_GUESS_LEARNED)
         return fail(INCRMT_ERR_ARG, "bad guess mode");
+    f->e.guess_mode = mode;
     f->e.mac.set_extrapolate(mode);
     f->e.fem.set_extrapolate(mode);
+    if (mode == INCRMT_GUESS_LEARNED && f->e.initialized) return f->e.ensure_guess();
+    return 0;
+}
+
+int incrmt_fluid_set_guess_training(incrmt_fluid *f, int train, float lr, double fallback_tol) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (!(lr > 0) || !(fallback_tol >= 0)) return fail(INCRMT_ERR_ARG, "bad learning rate or tolerance");
+    FluidEngine &e = f->e;
+    e.guess_train = train != 0;
+    e.guess_lr = lr;
+    e.guess_tol = fallback_tol;
+    e.mac.policy.reset();
+    e.fem.policy.reset();
+    if (e.initialized && (e.guess_train || e.guess_mode == INCRMT_GUESS_LEARNED)) return e.ensure_guess();
+    e.mac.train = e.fem.train = e.guess_train;
+    return 0;
+}
+
+static int guess_net(incrmt_fluid *f, int kind, UNet **out) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
+    if (!f->e.initialized) return fail(INCRMT_ERR_ARG, "the guess networks are made by initialize");
+    if (int rc = f->e.ensure_guess()) return rc;
+    *out = f->e.gnet[kind];
+    return 0;
+}
+
+int incrmt_fluid_guess_nparams(incrmt_fluid *f, int kind, long long *n) {
+    UNet *net;
+    if (int rc = guess_net(f, kind, &net)) return rc;
+    if (!n) return fail(INCRMT_ERR_ARG, "null argument");
+    *n = (long long)net->nparams();
+    return 0;
+}
+
+int incrmt_fluid_get_guess_params(incrmt_fluid *f, int kind, float *params) {
+    UNet *net;
+    if (int rc = guess_net(f, kind, &net)) return rc;
+    if (!params) return fail(INCRMT_ERR_ARG, "null argument");
+    return net->get_params(params);
+}
+
+int incrmt_fluid_set_guess_params(incrmt_fluid *f, int kind, const float *params) {
+    UNet *net;
+    if (int rc = guess_net(f, kind, &net)) return rc;
+    if (!params) return fail(INCRMT_ERR_ARG, "null argument");
+    (kind == INCRMT_MG_MAC ? f->e.mac : f->e.fem).policy.reset();
+    return net->set_params(params);
+}
+
+int incrmt_fluid_guess_stats(incrmt_fluid *f, int kind, double *out8) {
+    if (!f || !out8) return fail(INCRMT_ERR_ARG, "null argument");
+    double *out6 = out8;
+    if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
+    Multigrid &mg = kind == INCRMT_MG_MAC ? f->e.mac : f->e.fem;
+    double loss = 0;
+    if (mg.net && mg.train_steps > 0)
+        if (int rc = mg.read_train_loss(&loss)) return rc;
+    out6[0] = (double)mg.learned_solves;
+    out6[1] = (double)mg.guess_fallbacks;
+    out6[2] = (double)mg.train_steps;
+    out6[3] = loss;
+    out6[4] = mg.policy.off ? 1 : 0;
+    out6[5] = (double)mg.policy.misses;
+    out8[6] = mg.last_rel0;
+    out8[7] = mg.last_rel1;
+    return 0;
+}
+
+int incrmt_guess_policy_run(const double *rel, int n, double tol, int *decision) {
+    if (!rel || !decision || n < 0) return fail(INCRMT_ERR_ARG, "null argument");
+    GuessPolicy p;
+    for (int k = 0; k < n; k++) decision[k] = !p.use() ? -1 : (p.accept(rel[k], tol) ? 1 : 0);
     return 0;
 }
 

@@ -11,6 +11,7 @@
 #ifndef INCRMT_MG_HH
 #define INCRMT_MG_HH
 
+#include <cmath>
 #include <cstdint>
 #include <functional>
 #include <string>
@@ -26,7 +27,35 @@
 
 namespace incrmt {
 
+class UNet;
+
 enum MgKind { kMAC = 0, kFEM = 1 };
+
+// When a learned start is used and when it is given up (host logic): a start
+// is accepted when the relative residual after its first V-cycle is finite
+// and at most the tolerance. After kMaxMisses rejections in a row the
+// learned start is switched off for kRetry solves, then probed once: one
+// accepted start switches it back on, one more rejection off again.
+struct GuessPolicy {
+    static constexpr int kMaxMisses = 8, kRetry = 32;
+    int misses = 0, cooldown = 0;
+    bool off = false;
+    // whether this solve may use the learned start (counts the solves off)
+    bool use() {
+        if (!off) return true;
+        if (--cooldown > 0) return false;
+        off = false;
+        misses = kMaxMisses - 1;  // a probe: one more miss switches it off again
+        return true;
+    }
+    bool accept(double rel, double tol) {
+        const bool ok = std::isfinite(rel) && rel <= tol;
+        misses = ok ? 0 : misses + 1;
+        if (misses >= kMaxMisses) { off = true; cooldown = kRetry; }
+        return ok;
+    }
+    void reset() { misses = 0; cooldown = 0; off = false; }
+};
 
 // One level: nx x ny unknowns stored with a one-point zero ghost ring, so
 // stencils read neighbours without branches. Element (i,j) lives at
@@ -133,6 +162,27 @@ class Multigrid {
     double t_solve = 0;
     // change the starting-iterate mode; a change clears the solution history
     void set_extrapolate(int mode);
+    // training loss of the last training step (waits for the stream)
+    int read_train_loss(double *loss);
+    // Learned start (extrapolate == 3): the quadratic start plus alpha d,
+    // where d is the network's prediction from the starting residual, the
+    // feature channel and the source, and alpha = r.d / d.Ad the step along
+    // d that minimises the error's energy norm (computable from the
+    // residual r alone). If the relative residual after the first
+    // V-cycle exceeds guess_tol (or is not finite) the solve restarts from
+    // the quadratic start with FP64 V-cycles (the fallback). With train set,
+    // every solve that converges also takes one Adam step on its (input,
+    // starting residual) pair: the loss is the relative energy norm of the
+    // error left after that step, the error being the solution minus the
+    // quadratic start.
+    UNet *net = nullptr;
+    const double *feat = nullptr;  // level-0 array: the level-set channel (null: zero)
+    double guess_tol = 1e-4;
+    bool train = false;
+    float lr = 1e-3f;
+    long learned_solves = 0, guess_fallbacks = 0, train_steps = 0;
+    double last_rel0 = 0, last_rel1 = 0;  // relative residual of the last learned start, after its first cycle
+    GuessPolicy policy;
     // replay captured V-cycles (HIP graphs) instead of launching them; init
     // sets it for hierarchies whose finest level is below kGraphMaxPoints
     bool use_graphs = true;
@@ -161,6 +211,11 @@ class Multigrid {
     std::vector<uintptr_t> launch_state() const;
     void restore_buffers(const std::vector<uintptr_t> &key);
     int run_cycle(int which, const std::function<int()> &cycle);
+    double *zs_ = nullptr, *dz_ = nullptr;  // learned start: the quadratic start, the network's direction
+    float *tgt_ = nullptr;                  // training: d loss / d prediction (dense)
+    double *d_gs_ = nullptr;                // r.d, d.Ad, |r|^2 or e.r, training loss
+    int learned_start(bool apply);
+    int train_step();
     int scount_ = 0;                 // solves recorded for the extrapolation (0, 1, 2)
     int nq_ = 0;                     // solutions known to the quadratic start (0..3)
     double tq_[3] = {0, 0, 0};       // their times, newest first

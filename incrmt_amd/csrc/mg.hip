@@ -15,6 +15,7 @@
 #include <limits>
 #include <type_traits>
 #include "mg.hh"
+#include "unet.hh"
 
 namespace incrmt {
 
@@ -1030,6 +1031,144 @@ __global__ void k_wrap_ring(LevelArgs a, T *z) {
     z[j * a.zp + i] = z[sj * a.zp + si];
 }
 
+// ------------------------------------------------------- learned start
+// Network input at level 0 (channels-last BF16, 4 channels): the starting
+// residual r = b - A z and the source b, each scaled to unit root mean
+// square (sums: |b|^2, |r|^2 over the level), the feature channel, zero.
+__global__ void k_guess_input(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *fw,
+                              const double *fs, const double *kc, const double *feat, const double *sums,
+                              bf16_t *in) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    const double N = double(a.nx) * a.ny;
+    const double sr = sums[1] > 0 ? sqrt(N / sums[1]) : 0., sb = sums[0] > 0 ? sqrt(N / sums[0]) : 0.;
+    double diag, off;
+    if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+    else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
+    const double r = b[q] - off - diag * z[q];
+    bf16_t *o = in + 4 * ((size_t)j * a.nx + i);
+    o[0] = f2bf((float)(r * sr));
+    o[1] = f2bf(feat ? (float)feat[q] : 0.f);
+    o[2] = f2bf((float)(b[q] * sb));
+    o[3] = 0;
+}
+// the network's dense FP32 prediction into a level array
+__global__ void k_guess_dir(LevelArgs a, const float *d, double *dz) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    put_z(a, dz, i, j, (double)d[(size_t)j * a.nx + i]);
+}
+// partial sums per block: (r . dz, dz . A dz, |r|^2) with r = b - A z
+__global__ void k_guess_dots(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *dz,
+                             const double *fw, const double *fs, const double *kc, double *part) {
+    __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
+    double ra = 0, aa = 0, rr = 0;
+    const int N = a.nx * a.ny;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int i = p % a.nx, j = p / a.nx, q = j * a.zp + i, c = j * a.cp + i;
+        double diag, off, dd, od;
+        if (kind == kMAC) {
+            off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+            od = mac_offdiag<double>(dz, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, dd);
+        } else {
+            off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
+            od = fem_offdiag<double>(dz, kc, q, c, a.zp, a.cp, k, dd);
+        }
+        const double r = b[q] - off - diag * z[q], ad = od + dd * dz[q];
+        ra += r * dz[q];
+        aa += dz[q] * ad;
+        rr += r * r;
+    }
+    s0[threadIdx.x] = ra; s1[threadIdx.x] = aa; s2[threadIdx.x] = rr;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+// z += alpha dz with alpha = (r . dz) / (dz . A dz), the step that minimises
+// the energy norm of the error along dz (0 for a null direction)
+__global__ void k_guess_apply(LevelArgs a, double *z, const double *dz, const double *s) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const double alpha = s[1] != 0 && isfinite(s[0] / s[1]) ? s[0] / s[1] : 0.;
+    const int q = j * a.zp + i;
+    put_z(a, z, i, j, z[q] + alpha * dz[q]);
+}
+// Training loss of the learned start: the relative energy norm of the
+// error left after the optimal step along the prediction d,
+//   L = |e - alpha d|_A^2 / |e|_A^2 = 1 - (e.Ad)^2 / ((e.Ae)(d.Ad)),
+// where e = z - zs is the correction the solve made to the quadratic start
+// zs, r = b - A zs = A e its starting residual (to the solve's tolerance),
+// so e.Ad = r.d, e.Ae = e.r and alpha = r.d / d.Ad is the step the learned
+// start takes. dL/dd = -2 (r.d) / ((e.r)(d.Ad)) (r - alpha A d).
+// Partial sums per block: (r.d, d.Ad, e.r).
+__global__ void k_guess_edots(int kind, LevelArgs a, FemC k, const double *z, const double *zs, const double *b,
+                              const double *dz, const double *fw, const double *fs, const double *kc, double *part) {
+    __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
+    double rd = 0, dad = 0, er = 0;
+    const int N = a.nx * a.ny;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int i = p % a.nx, j = p / a.nx, q = j * a.zp + i, c = j * a.cp + i;
+        double diag, off, dd, od;
+        if (kind == kMAC) {
+            off = mac_offdiag<double>(zs, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+            od = mac_offdiag<double>(dz, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, dd);
+        } else {
+            off = fem_offdiag<double>(zs, kc, q, c, a.zp, a.cp, k, diag);
+            od = fem_offdiag<double>(dz, kc, q, c, a.zp, a.cp, k, dd);
+        }
+        const double r = b[q] - off - diag * zs[q], ad = od + dd * dz[q];
+        rd += r * dz[q];
+        dad += dz[q] * ad;
+        er += (z[q] - zs[q]) * r;
+    }
+    s0[threadIdx.x] = rd; s1[threadIdx.x] = dad; s2[threadIdx.x] = er;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+// the loss gradient with respect to the (dense FP32) prediction, and the
+// loss itself into sums[3]
+__global__ void k_guess_egrad(int kind, LevelArgs a, FemC k, const double *zs, const double *b, const double *dz,
+                              const double *fw, const double *fs, const double *kc, double *sums, float *g) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    // the MAC operator is negative definite, the FEM one positive: the
+    // energies e.Ae and d.Ad share the operator's sign
+    const double rd = sums[0], dad = sums[1], er = sums[2];
+    const bool ok = dad * er > 0;
+    const double alpha = ok ? rd / dad : 0., f = ok ? -2. * rd / (er * dad) : 0.;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    double diag, off, dd, od;
+    if (kind == kMAC) {
+        off = mac_offdiag<double>(zs, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+        od = mac_offdiag<double>(dz, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, dd);
+    } else {
+        off = fem_offdiag<double>(zs, kc, q, c, a.zp, a.cp, k, diag);
+        od = fem_offdiag<double>(dz, kc, q, c, a.zp, a.cp, k, dd);
+    }
+    g[(size_t)j * a.nx + i] = (float)(f * ((b[q] - off - diag * zs[q]) - alpha * (od + dd * dz[q])));
+    if (i == 0 && j == 0) sums[3] = ok ? 1. - rd * rd / (er * dad) : 1.;
+}
+
 FemC fem_consts(double dx, double dy) {
     // mgs_fem.cc:12-21
     const double dydx = dy / dx, dxdy = dx / dy;
@@ -1807,7 +1946,8 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     const MgLevel &f = L[0];
     int rc;
     if ((rc = wrap_fine(f.z))) return rc;  // the iterate may come from outside
-    if (extrapolate == 2) {
+    const bool quad_start = extrapolate >= 2 && nq_ >= 3;  // this solve starts from a quadratic extrapolation
+    if (extrapolate >= 2) {
         // Lagrange extrapolation in time through the newest solutions: z
         // holds the newest (time tq_[0]), zq1_ and zq2_ the two before.
         const double t = t_solve;
@@ -1904,26 +2044,60 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
         k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.b, d_red);
         KCHECK();
     }
+    // the learned start (mgs_base::initial_guess's hook, mgs_common.cc:33-66);
+    // with training on, a solve without it still runs the network's forward
+    // pass on its input (the pair it will train on)
+    // (only once the quadratic start has its three solutions: the first
+    // two solves start from the previous solution and a linear
+    // extrapolation, which the network was not trained on)
+    const bool learned = extrapolate == 3 && net && quad_start && policy.use();
+    const bool sample = train && net;
+    if ((learned || sample) && (rc = learned_start(learned))) return rc;
     double nr[3];
     if ((rc = reduce_norms(nr))) return rc;
     const double bn = std::sqrt(nr[0]);
     double rel = bn > 0 ? std::sqrt(nr[1]) / bn : 0;
     int cycles = 0;
-    if (rel > tol && bn > 0 && predict_ > 0) {
+    bool fallback = false;
+    auto fp64_cycle = [&]() -> int { return run_cycle(0, [&]() { return vcycle(0); }); };
+    auto next = [&]() -> int { return fallback ? fp64_cycle() : cycle(); };
+    if (learned && rel > tol && bn > 0 && max_cycles > 0) {
+        // one V-cycle, then the judgement in FP64: a learned start that has
+        // not reached guess_tol is dropped for the quadratic start and plain
+        // FP64 V-cycles
+        last_rel0 = rel;
+        if ((rc = cycle())) return rc;
+        cycles = 1;
+        if ((rc = reduce_norms(nr))) return rc;
+        rel = std::sqrt(nr[1]) / bn;
+        last_rel1 = rel;
+        learned_solves++;
+        if (!policy.accept(rel, guess_tol)) {
+            guess_fallbacks++;
+            fallback = true;
+            k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), zs_, f.z);
+            KCHECK();
+            if ((rc = wrap_fine(f.z))) return rc;
+            if ((rc = reduce_norms(nr))) return rc;
+            rel = std::sqrt(nr[1]) / bn;
+        }
+    }
+    if (rel > tol && bn > 0 && predict_ > cycles) {
         const int ahead = predict_ < max_cycles ? predict_ : max_cycles;
         for (; cycles < ahead; cycles++)
-            if ((rc = cycle())) return rc;
+            if ((rc = next())) return rc;
         if ((rc = reduce_norms(nr))) return rc;
         rel = std::sqrt(nr[1]) / bn;
     }
     while (rel > tol && bn > 0) {
         if (cycles >= max_cycles) break;
-        if ((rc = cycle())) return rc;
+        if ((rc = next())) return rc;
         cycles++;
         if ((rc = reduce_norms(nr))) return rc;
         rel = std::sqrt(nr[1]) / bn;
     }
     if (cycles > 0) predict_ = cycles - 1;
+    if (sample && rel <= tol && bn > 0 && (rc = train_step())) return rc;
     if (time_smoother && (rc = collect_events())) return rc;
     if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
     return rel <= tol || bn == 0 ? 0 : 2;
@@ -1992,6 +2166,82 @@ int Multigrid::run_cycle(int which, const std::function<int()> &cycle) {
         graph_key_[which] = key;
     }
     HIP_CHECK(hipGraphLaunch(graph_exec_[which], s_));
+    return 0;
+}
+
+int Multigrid::learned_start(bool apply) {
+    const MgLevel &f = L[0];
+    if (!zs_) {
+        const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+        for (double **p : {&zs_, &dz_}) {
+            void *q;
+            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+            allocs.push_back(q);
+            *p = (double *)q + f.pitch + 1;
+        }
+        void *q;
+        HIP_CHECK(hipMalloc(&q, (size_t)f.nx * f.ny * sizeof(float)));
+        allocs.push_back(q);
+        tgt_ = (float *)q;
+        HIP_CHECK(hipMalloc(&q, 4 * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, 4 * sizeof(double)));
+        allocs.push_back(q);
+        d_gs_ = (double *)q;
+    }
+    const FemC k = fem_consts(dx_, dy_);
+    k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zs_);
+    KCHECK();
+    if (int rc = reduce_dev()) return rc;
+    k_guess_input<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, feat,
+                                                              d_red, net->input_buffer());
+    KCHECK();
+    if (int rc = net->forward(net->input_buffer(), nullptr)) return rc;
+    if (!apply) return 0;
+    k_guess_dir<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), net->output_buffer(), dz_);
+    KCHECK();
+    k_guess_dots<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), k, f.z, f.b, dz_, f.fw, f.fs, f.kc,
+                                                    d_red + 8);
+    KCHECK();
+    k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_gs_);
+    KCHECK();
+    k_guess_apply<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, dz_, d_gs_);
+    KCHECK();
+    if (singular()) {  // keep the start mean-free
+        k_norms<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), k, f.z, f.z, f.fw, f.fs, f.kc, d_red + 8);
+        KCHECK();
+        k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_red);
+        KCHECK();
+        k_sub_mean<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, d_red);
+        KCHECK();
+    }
+    return 0;
+}
+
+int Multigrid::train_step() {
+    const MgLevel &f = L[0];
+    const FemC k = fem_consts(dx_, dy_);
+    k_guess_dir<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), net->output_buffer(), dz_);
+    KCHECK();
+    k_guess_edots<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, args_of(f), k, f.z, zs_, f.b, dz_, f.fw, f.fs, f.kc,
+                                                     d_red + 8);
+    KCHECK();
+    k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_gs_);
+    KCHECK();
+    k_guess_egrad<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, zs_, f.b, dz_, f.fw, f.fs, f.kc,
+                                                              d_gs_, tgt_);
+    KCHECK();
+    if (int rc = net->backward_grad(tgt_)) return rc;
+    if (int rc = net->adam(lr)) return rc;
+    train_steps++;
+    return 0;
+}
+
+int Multigrid::read_train_loss(double *loss) {
+    if (!d_gs_) { *loss = 0; return 0; }
+    HIP_CHECK(hipMemcpyAsync(h_red + 4, d_gs_ + 3, sizeof(double), hipMemcpyDeviceToHost, s_));
+    HIP_CHECK(hipStreamSynchronize(s_));
+    *loss = h_red[4];
     return 0;
 }
 
