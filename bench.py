@@ -8,10 +8,13 @@ reference initialises it. One step = one step_forward of adt = (4pi/240)/l.
 
 Prints ONE JSON line. value = cell-steps/s = m*n*K / seconds over K timed
 steps (state resident in HBM, no host transfers inside the timed region).
-The roofline object is for the dominant kernel, the fine-level smoother of
-the projections, timed with HIP events on the library's stream in a second
-pass of the same K steps; cpu_baseline is the project's OpenMP CPU oracle on
-a bounded sample of the same workload.
+The headline solver is mixed precision (FP32 V-cycles in FP64 refinement)
+started from the quadratic extrapolation corrected by the learned guess,
+whose networks are first trained (untimed) on the run's own projections;
+the other solvers are reported beside it. The roofline object is for the
+fine-level smoother of the projections, timed with HIP events on the
+library's stream in a further pass of the same K steps; cpu_baseline is the
+project's OpenMP CPU oracle on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -26,11 +29,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None):
+def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, params=None):
     import incrmt_amd as ia
     f = ia.simple_spin(m, rel_tol=1e-8, precision=precision, guess=guess)
     if launch is not None:  # otherwise the library's default for the size
         f.set_launch(launch)
+    if params is not None:  # trained learned-guess networks (MAC, FEM)
+        for k in (ia.MG_MAC, ia.MG_FEM):
+            f.set_guess_params(k, params[k])
     l, adt = f.timestep_select(4 * math.pi / 240)
     f.steps(adt, warmup)
     f.sync()
@@ -47,8 +53,16 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None):
                mac_rel_res=s1["mac_rel_res"], fem_rel_res=s1["fem_rel_res"],
                reinit_passes=s1["reinit_passes"], extend_passes=s1["extend_passes"],
                smoother_shapes=shapes)
+    # projection-only time: a second pass of the same steps with HIP events
+    # around the two projections (their starts included)
+    f.time_projections(True)
+    f.steps(adt, steps)
+    out["projection_ms_per_step"] = f.stats()["proj_ms"] / steps
+    f.time_projections(False)
+    if guess == ia.GUESS_LEARNED:
+        out["learned_start"] = {("mac" if k == ia.MG_MAC else "fem"): f.guess_stats(k) for k in (ia.MG_MAC, ia.MG_FEM)}
     if timing:
-        # second pass with HIP events around every fine-level smoother launch
+        # a further pass with HIP events around every fine-level smoother launch
         f.time_smoothers(True)
         f.steps(adt, steps)
         f.sync()
@@ -68,21 +82,82 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None):
     return out
 
 
-def cpu_baseline(m, steps):
-    """The oracle (OpenMP C++ restatement of the reference step) on the
-    host, a bounded sample: `steps` steps after one warm-up step."""
+def train_guess(m, steps, lr=1e-3):
+    """Train the two learned-guess networks online on the solver's own
+    projections of the same workload (one Adam step per projection, the
+    quadratic start's corrections as targets), untimed for the benchmark."""
+    import incrmt_amd as ia
+    f = ia.simple_spin(m, rel_tol=1e-8, precision=ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC)
+    f.set_guess_training(True, lr, 1e-4)
+    l, adt = f.timestep_select(4 * math.pi / 240)
+    t0 = time.perf_counter()
+    f.steps(adt, steps)
+    f.sync()
+    dt = time.perf_counter() - t0
+    params = [f.guess_params(k) for k in (ia.MG_MAC, ia.MG_FEM)]
+    st = {("mac" if k == ia.MG_MAC else "fem"): f.guess_stats(k) for k in (ia.MG_MAC, ia.MG_FEM)}
+    f.close()
+    return params, dict(steps=steps, projections=2 * steps, seconds=round(dt, 3),
+                        ms_per_step_training=round(dt / steps * 1e3, 3),
+                        final_loss={k: round(v["loss"], 5) for k, v in st.items()},
+                        learning_rate=lr, parameters_per_network=int(params[0].size))
+
+
+def cpu_info():
+    """lscpu's model name and physical core count of this host."""
+    import subprocess
+    model, cores = "unknown", os.cpu_count() or 1
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        p = subprocess.run(["lscpu", "-p=Core,Socket"], capture_output=True, text=True, timeout=10).stdout
+        pairs = {l for l in p.splitlines() if l and not l.startswith("#")}
+        if pairs:
+            cores = len(pairs)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, cores
+
+
+def cpu_baseline(m, steps, warmup=10):
+    """The oracle (OpenMP C++ restatement of the reference step, g++ -O3
+    -fopenmp) on the host: OMP_NUM_THREADS = the physical cores with
+    OMP_PROC_BIND=close and OMP_PLACES=cores, `warmup` untimed steps, then
+    `steps` timed; then one thread, one warm-up step and one timed step."""
+    model, cores = cpu_info()
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    import ctypes
     from tests import oracle_lib as ol
     f = ol.OracleFluid(ol.simple_spin_spec(m), mg_tol=1e-8)
     l, adt = ol.timestep_select(4 * math.pi / 240, f.dt_reg)
-    f.step(adt)
+    for _ in range(warmup):
+        f.step(adt)
     t0 = time.perf_counter()
     for _ in range(steps):
         f.step(adt)
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=m * m * steps / dt, unit="cell-steps/s", cores=threads, kind="port",
-                sample=f"{steps} oracle steps at {m}x{m} after 1 warm-up step, OpenMP {threads} threads, "
-                       f"projections to relative residual 1e-8")
+    out = dict(value=round(m * m * steps / dt, 1), unit="cell-steps/s", cores=threads, kind="port",
+               cpu=model, physical_cores=cores,
+               sample=f"{steps} oracle steps at {m}x{m} after {warmup} warm-up steps, OpenMP {threads} threads "
+                      f"(OMP_PROC_BIND={os.environ['OMP_PROC_BIND']}, OMP_PLACES={os.environ['OMP_PLACES']}), "
+                      f"projections to relative residual 1e-8")
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        f.step(adt)
+        t0 = time.perf_counter()
+        f.step(adt)
+        dt1 = time.perf_counter() - t0
+        out["one_thread"] = dict(value=round(m * m / dt1, 1), sample="1 step after 1 warm-up step, 1 thread")
+        gomp.omp_set_num_threads(threads)
+    except OSError:
+        out["one_thread"] = None
+    return out
 
 
 def smoother_probe(m, sweeps):
@@ -156,12 +231,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=500,
+                    help="steps of online training of the learned guess before its timed run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS fine-level smoother sweeps of each system (for counter runs)")
     ap.add_argument("--headline-only", action="store_true",
-                    help="only the headline variant, --warmup then --steps steps (may be 0), no timing "
-                         "passes: two profiler runs of it differ by exactly the steps")
+                    help="only the mixed+quadratic variant, --warmup then --steps steps (may be 0), no "
+                         "timing passes: two profiler runs of it differ by exactly the steps")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the path would shard as row slabs "
@@ -189,8 +266,9 @@ def main():
     # the reference does (mgs_common.hh:48-50), against FP32 V-cycles in FP64
     # refinement started from a prediction in time: the linear extrapolation
     # of the last two solutions (mgs_base::initial_guess with mg_lin_t_interp,
-    # mgs_common.cc:33-66) and the quadratic one of the last three (the
-    # headline; no trained network)
+    # mgs_common.cc:33-66), the quadratic one of the last three, and the
+    # quadratic start corrected by the learned guess (the headline: the
+    # networks are first trained on this workload's own projections, untimed)
     fp64 = gpu_run(m, a.steps, a.warmup, ia.PREC_FP64, timing=True, guess=ia.GUESS_PREVIOUS)
     mixed = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_PREVIOUS)
     pred = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_EXTRAPOLATE)
@@ -199,8 +277,10 @@ def main():
     # library's default below 512^2 unknowns; above it launches them kernel
     # by kernel, as the runs above do)
     graph = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC, launch=ia.LAUNCH_GRAPH)
-    best = quad
-    best_name = "mixed+quadratic_guess"
+    params, training = train_guess(m, a.train_steps)
+    learned = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_LEARNED, params=params)
+    best = learned
+    best_name = "mixed+learned_guess"
     # roofline of the fine-level smoother (FP64 pass): algorithmic bytes per
     # sweep are, per point, z, b and the coefficients read and z written:
     # MAC red-black 8+8+16+8 = 40 B/cell (two face inverse densities), FEM
@@ -217,31 +297,37 @@ def main():
                     back_to_back_us_per_launch={k: round(v, 2) for k, v in fp64["back_to_back_us_per_launch"].items()},
                     algorithmic_bytes_per_launch=round(fp64["smoother_bytes"] / launches))
     name, hbm = ia.device_info()
+    skip = ("back_to_back_us_per_launch", "learned_start")
     line = {
         "metric": "cell-steps/s at 1024^2 (plain FP64 MG vs mixed+learned), rel. div <=1e-8",
         "value": round(best["cell_steps_per_s"], 1),
         "unit": "cell-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(best["ms_per_step"], 3),
+        "projection_ms_per_step": round(best["projection_ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+f32",
-        "data": "synthetic (simple-spin rotor initial state)",
+        "dtype": "f64+f32+bf16",
+        "data": "synthetic (simple-spin rotor initial state); learned-guess networks trained on the run's own projections",
         "config": {"workload": f"simple-spin rotor, full step_forward, {m}x{m} cells, projections to rel. residual 1e-8",
                    "solver": best_name, "baseline_solver": "fp64 multigrid, previous-solution warm start",
                    "grid": [m, m]},
-        "variants": {"fp64": {k: sig(v) for k, v in fp64.items() if k != "back_to_back_us_per_launch"},
+        "variants": {"fp64": {k: sig(v) for k, v in fp64.items() if k not in skip},
                      "mixed": {k: sig(v) for k, v in mixed.items()},
                      "mixed+predicted_guess": {k: sig(v) for k, v in pred.items()},
                      "mixed+quadratic_guess": {k: sig(v) for k, v in quad.items()},
                      "mixed+quadratic_guess, graph replay": {k: sig(v) for k, v in graph.items()
-                                                             if k != "smoother_shapes"}},
+                                                             if k != "smoother_shapes"},
+                     "mixed+learned_guess": {k: sig(v) for k, v in learned.items() if k not in skip}},
+        "learned_guess": {"training": training,
+                          "learned_start": {k: {kk: sig(vv) for kk, vv in v.items()}
+                                            for k, v in learned["learned_start"].items()}},
         "speedup_vs_plain_fp64": round(best["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
+        "speedup_quadratic_vs_plain_fp64": round(quad["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "roofline": roofline,
         "device": name,
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(m, a.cpu_steps)
-        line["cpu_baseline"]["value"] = round(line["cpu_baseline"]["value"], 1)
     print(json.dumps(line))
 
 

@@ -48,6 +48,7 @@ typedef struct incrmt_stats {
     double smoother_ms;               /* summed fine-level smoother time (timing on) */
     long smoother_launches;           /* fine-level smoother kernel launches timed */
     double smoother_bytes;            /* their algorithmic bytes (z, b, coefficients read; z written) */
+    double proj_ms;                   /* summed time of the two projections per step (timing on) */
 } incrmt_stats;
 
 /* Last error message of the calling thread. */
@@ -233,6 +234,9 @@ int incrmt_fluid_describe(incrmt_fluid *f, char *buf, int len);
  * iterate); returns the mean milliseconds per launch. */
 int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int precision, double *ms_per_launch,
                                     int *launches);
+/* No reference counterpart: record HIP events around each step's two
+ * projections, their starts included (stats.proj_ms; enabling resets it). */
+int incrmt_fluid_time_projections(incrmt_fluid *f, int enable);
 /* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
 void incrmt_fluid_destroy(incrmt_fluid *f);

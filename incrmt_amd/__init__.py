@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("mac_cycles", _I), ("fem_cycles", _I), ("mac_rel_res", _D), ("fem_rel_res", _D),
                 ("total_mac_cycles", ctypes.c_long), ("total_fem_cycles", ctypes.c_long),
                 ("steps", ctypes.c_long), ("reinit_passes", _I), ("extend_passes", _I),
-                ("smoother_ms", _D), ("smoother_launches", ctypes.c_long), ("smoother_bytes", _D)]
+                ("smoother_ms", _D), ("smoother_launches", ctypes.c_long), ("smoother_bytes", _D),
+                ("proj_ms", _D)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +109,7 @@ _PROTOS = [
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
     ("incrmt_levelset_build_band_noalloc", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _V, _V, _ip]),
     ("incrmt_fluid_set_guess_training", _I, [_V, _I, ctypes.c_float, _D]),
+    ("incrmt_fluid_time_projections", _I, [_V, _I]),
     ("incrmt_fluid_guess_nparams", _I, [_V, _I, ctypes.POINTER(ctypes.c_longlong)]),
     ("incrmt_fluid_get_guess_params", _I, [_V, _I, _fp]),
     ("incrmt_fluid_set_guess_params", _I, [_V, _I, _fp]),
@@ -265,6 +267,10 @@ class Fluid2D:
         fallback_tol is the relative residual after one V-cycle above which
         a learned start is dropped for the quadratic start and FP64 cycles."""
         _check(lib().incrmt_fluid_set_guess_training(self._h, int(train), lr, fallback_tol))
+
+    def time_projections(self, enable=True):
+        """HIP events around each step's two projections (stats proj_ms)."""
+        _check(lib().incrmt_fluid_time_projections(self._h, int(enable)))
 
     def guess_params(self, kind):
         n = ctypes.c_longlong(0)

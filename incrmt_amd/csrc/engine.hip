@@ -132,10 +132,43 @@ struct FluidEngine {
     double *gfeat[2] = {nullptr, nullptr};
     int ensure_guess();
     int guess_features(Multigrid &mg, int which);
+    // HIP events around the two projections of every step (their learned
+    // starts and solves), summed into proj_ms when the stats are read
+    bool time_proj = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pev;
+    size_t pev_used = 0;
+    double proj_ms = 0;
+    int proj_begin() {
+        if (!time_proj) return 0;
+        if (pev_used == pev.size()) {
+            hipEvent_t a, b;
+            HIP_CHECK(hipEventCreate(&a));
+            HIP_CHECK(hipEventCreate(&b));
+            pev.push_back({a, b});
+        }
+        HIP_CHECK(hipEventRecord(pev[pev_used].first, s));
+        return 0;
+    }
+    int proj_end() {
+        if (!time_proj) return 0;
+        HIP_CHECK(hipEventRecord(pev[pev_used++].second, s));
+        return 0;
+    }
+    int proj_collect() {
+        for (size_t k = 0; k < pev_used; k++) {
+            HIP_CHECK(hipEventSynchronize(pev[k].second));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, pev[k].first, pev[k].second));
+            proj_ms += ms;
+        }
+        pev_used = 0;
+        return 0;
+    }
     int launch_req = -1;  // V-cycle issue mode asked for (-1: the solvers' default by size)
     TracerSet tr;
 
     ~FluidEngine() {
+        for (auto &e : pev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (UNet *n : gnet) delete n;
         for (auto &o : obj) ls_free(o.w);
         for (void *p : allocs) (void)hipFree(p);
@@ -466,8 +499,10 @@ int FluidEngine::step(double dt) {
     if (!const_rho && (rc = mac.set_density(g, f.rho))) return rc;
     MgStats ms{};
     mac.t_solve = time;  // the MAC solve's time, t + dt/2 (initial_guess(f.time))
+    if ((rc = proj_begin())) return rc;
     if ((rc = guess_features(mac, 0))) return rc;
     rc = precision == INCRMT_PREC_MIXED ? mac.solve_mixed(mg_tol, max_cycles, &ms) : mac.solve(mg_tol, max_cycles, &ms);
+    if (!rc) rc = proj_end();
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in MAC problem");
     if (rc) return rc;
     st.mac_cycles = ms.cycles; st.mac_rel_res = ms.rel_res; st.total_mac_cycles += ms.cycles;
@@ -495,8 +530,10 @@ int FluidEngine::step(double dt) {
     if ((rc = launch_fem_src(s, g, f, dt, fl.b, fl.pitch))) return rc;
     if (!const_rho && (rc = fem.set_density(g, f.rho))) return rc;
     fem.t_solve = time;  // t + dt
+    if ((rc = proj_begin())) return rc;
     if ((rc = guess_features(fem, 1))) return rc;
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
+    if (!rc) rc = proj_end();
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
     if (rc) return rc;
     st.fem_cycles = ms.cycles; st.fem_rel_res = ms.rel_res; st.total_fem_cycles += ms.cycles;
@@ -852,10 +889,20 @@ int incrmt_fluid_sync(incrmt_fluid *f) {
 
 int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out) {
     if (!f || !out) return fail(INCRMT_ERR_ARG, "null argument");
+    if (int rc = f->e.proj_collect()) return rc;
     *out = f->e.st;
+    out->proj_ms = f->e.proj_ms;
     out->smoother_ms = f->e.mac.smoother_ms + f->e.fem.smoother_ms;
     out->smoother_launches = f->e.mac.smoother_launches + f->e.fem.smoother_launches;
     out->smoother_bytes = f->e.mac.smoother_bytes + f->e.fem.smoother_bytes;
+    return 0;
+}
+
+int incrmt_fluid_time_projections(incrmt_fluid *f, int enable) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    if (int rc = f->e.proj_collect()) return rc;
+    f->e.time_proj = enable != 0;
+    f->e.proj_ms = 0;
     return 0;
 }
 

@@ -212,8 +212,10 @@ class Multigrid {
     void restore_buffers(const std::vector<uintptr_t> &key);
     int run_cycle(int which, const std::function<int()> &cycle);
     double *zs_ = nullptr, *dz_ = nullptr;  // learned start: the quadratic start, the network's direction
+    double *eprev_ = nullptr;               // the last solve's correction to its quadratic start
+    int guess_buffers();
     float *tgt_ = nullptr;                  // training: d loss / d prediction (dense)
-    double *d_gs_ = nullptr;                // r.d, d.Ad, |r|^2 or e.r, training loss
+    double *d_gs_ = nullptr;                // r.d, d.Ad, |r|^2 or e.r, training loss; |eprev|^2
     int learned_start(bool apply);
     int train_step();
     int scount_ = 0;                 // solves recorded for the extrapolation (0, 1, 2)

@@ -1034,10 +1034,12 @@ __global__ void k_wrap_ring(LevelArgs a, T *z) {
 // ------------------------------------------------------- learned start
 // Network input at level 0 (channels-last BF16, 4 channels): the starting
 // residual r = b - A z and the source b, each scaled to unit root mean
-// square (sums: |b|^2, |r|^2 over the level), the feature channel, zero.
+// square (sums: |b|^2, |r|^2 over the level), the feature channel, and the
+// previous solve's correction to its quadratic start scaled likewise
+// (esum: its sum of squares).
 __global__ void k_guess_input(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *fw,
                               const double *fs, const double *kc, const double *feat, const double *sums,
-                              bf16_t *in) {
+                              const double *eprev, const double *esum, bf16_t *in) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
     const int q = j * a.zp + i, c = j * a.cp + i;
@@ -1051,7 +1053,28 @@ __global__ void k_guess_input(int kind, LevelArgs a, FemC k, const double *z, co
     o[0] = f2bf((float)(r * sr));
     o[1] = f2bf(feat ? (float)feat[q] : 0.f);
     o[2] = f2bf((float)(b[q] * sb));
-    o[3] = 0;
+    const double se = esum[0] > 0 ? sqrt(N / esum[0]) : 0.;
+    o[3] = f2bf((float)(eprev[q] * se));
+}
+// the correction this solve made to its quadratic start, kept as the next
+// solve's input, and per-block partial sums of its square
+__global__ void k_guess_eprev(LevelArgs a, const double *z, const double *zs, double *e, double *part) {
+    __shared__ double s0[kRedThreads];
+    double t2 = 0;
+    const int N = a.nx * a.ny;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int i = p % a.nx, j = p / a.nx, q = j * a.zp + i;
+        const double t = z[q] - zs[q];
+        put_z(a, e, i, j, t);
+        t2 += t * t;
+    }
+    s0[threadIdx.x] = t2;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s0[threadIdx.x] += s0[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = 0; part[3 * blockIdx.x + 2] = 0; }
 }
 // the network's dense FP32 prediction into a level array
 __global__ void k_guess_dir(LevelArgs a, const float *d, double *dz) {
@@ -2052,6 +2075,14 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     // extrapolation, which the network was not trained on)
     const bool learned = extrapolate == 3 && net && quad_start && policy.use();
     const bool sample = train && net;
+    // the network's inputs (the start before the learned correction, the
+    // last correction) are kept whenever a network is attached and used
+    const bool keep = net && (extrapolate >= 2 || train);
+    if (keep) {
+        if ((rc = guess_buffers())) return rc;
+        k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zs_);
+        KCHECK();
+    }
     if ((learned || sample) && (rc = learned_start(learned))) return rc;
     double nr[3];
     if ((rc = reduce_norms(nr))) return rc;
@@ -2098,6 +2129,12 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     }
     if (cycles > 0) predict_ = cycles - 1;
     if (sample && rel <= tol && bn > 0 && (rc = train_step())) return rc;
+    if (keep && rel <= tol && bn > 0) {
+        k_guess_eprev<<<kRedBlocks, kRedThreads, 0, s_>>>(args_of(f), f.z, zs_, eprev_, d_red + 8);
+        KCHECK();
+        k_sum_partials<<<1, kRedThreads, 0, s_>>>(d_red + 8, kRedBlocks, d_gs_ + 4);
+        KCHECK();
+    }
     if (time_smoother && (rc = collect_events())) return rc;
     if (st) { st->cycles = cycles; st->rel_res = rel; st->bnorm = bn; st->rnorm = rel * bn; }
     return rel <= tol || bn == 0 ? 0 : 2;
@@ -2169,32 +2206,34 @@ int Multigrid::run_cycle(int which, const std::function<int()> &cycle) {
     return 0;
 }
 
+int Multigrid::guess_buffers() {
+    if (zs_) return 0;
+    const MgLevel &f = L[0];
+    const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+    for (double **p : {&zs_, &dz_, &eprev_}) {
+        void *q;
+        HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+        allocs.push_back(q);
+        *p = (double *)q + f.pitch + 1;
+    }
+    void *q;
+    HIP_CHECK(hipMalloc(&q, (size_t)f.nx * f.ny * sizeof(float)));
+    allocs.push_back(q);
+    tgt_ = (float *)q;
+    HIP_CHECK(hipMalloc(&q, 8 * sizeof(double)));
+    HIP_CHECK(memset_sync(q, 0, 8 * sizeof(double)));
+    allocs.push_back(q);
+    d_gs_ = (double *)q;
+    return 0;
+}
+
 int Multigrid::learned_start(bool apply) {
     const MgLevel &f = L[0];
-    if (!zs_) {
-        const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
-        for (double **p : {&zs_, &dz_}) {
-            void *q;
-            HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
-            allocs.push_back(q);
-            *p = (double *)q + f.pitch + 1;
-        }
-        void *q;
-        HIP_CHECK(hipMalloc(&q, (size_t)f.nx * f.ny * sizeof(float)));
-        allocs.push_back(q);
-        tgt_ = (float *)q;
-        HIP_CHECK(hipMalloc(&q, 4 * sizeof(double)));
-        HIP_CHECK(memset_sync(q, 0, 4 * sizeof(double)));
-        allocs.push_back(q);
-        d_gs_ = (double *)q;
-    }
     const FemC k = fem_consts(dx_, dy_);
-    k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zs_);
-    KCHECK();
     if (int rc = reduce_dev()) return rc;
     k_guess_input<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc, feat,
-                                                              d_red, net->input_buffer());
+                                                              d_red, eprev_, d_gs_ + 4, net->input_buffer());
     KCHECK();
     if (int rc = net->forward(net->input_buffer(), nullptr)) return rc;
     if (!apply) return 0;

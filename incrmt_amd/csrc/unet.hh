@@ -36,7 +36,9 @@ __host__ __device__ inline float bf2f(bf16_t h) {
 // ((n + 1) / 2 points). Channels 16/32/48/64 at levels 0..3.
 constexpr int kUnetLevels = 4;
 constexpr int kUnetCh[kUnetLevels] = {16, 32, 48, 64};
-constexpr int kUnetIn = 4;  // input channels: residual, level set, source, zero pad
+// input channels: starting residual, level set, source, the previous
+// solve's correction (zero where a caller has none)
+constexpr int kUnetIn = 4;
 
 // Convolution layers in forward order.
 enum UnetLayer { kE0a = 0, kE0b, kD1, kE1, kD2, kE2, kD3, kB3, kU2, kU1, kU0, kOut, kUnetLayers };

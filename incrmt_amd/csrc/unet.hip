@@ -547,10 +547,7 @@ int UNet::init(int nx, int ny, unsigned seed, hipStream_t s) {
         for (int co = 0; co < c.cout; co++)
             for (int t = 0; t < 9; t++)
                 for (int ci = 0; ci < cin; ci++) {
-                    // the fourth input channel is padding: its weights stay zero
-                    const double v = nd(rng);
-                    if (l == kE0a && ci >= 3) continue;
-                    h[c.w_off + (size_t)co * c.kp + kidx(t, ci)] = (float)v;
+                    h[c.w_off + (size_t)co * c.kp + kidx(t, ci)] = (float)nd(rng);
                 }
     }
     HIP_CHECK(hipMemcpy(w32_, h.data(), off * 4, hipMemcpyHostToDevice));
