@@ -75,8 +75,8 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, par
         f.time_smoothers(False)
         # the same kernels back to back (no event between launches), for
         # comparison with a profiler's per-dispatch durations
-        mac_ms, mac_n = f.time_fine_smoother(ia.MG_MAC, 40)
-        fem_ms, fem_n = f.time_fine_smoother(ia.MG_FEM, 40)
+        mac_ms, mac_n = f.time_fine_smoother(ia.MG_MAC, 40, precision)
+        fem_ms, fem_n = f.time_fine_smoother(ia.MG_FEM, 40, precision)
         out["back_to_back_us_per_launch"] = {"mac": mac_ms * 1e3, "fem": fem_ms * 1e3}
     f.close()
     return out
@@ -193,7 +193,7 @@ def smoother_probe(m, sweeps):
     return out
 
 
-def measured_traffic(shapes, mac_cycles, fem_cycles, nu=2):
+def measured_traffic(shapes, mac_cycles, fem_cycles, nu=2, fp32=False):
     """HBM bytes per fine-level smoother launch in the timed pass, from the
     calibrated FETCH_SIZE/WRITE_SIZE counts per launch shape measured on the
     same fine levels (profiles/smoother_fine_1024_traffic.json), weighted by
@@ -207,9 +207,10 @@ def measured_traffic(shapes, mac_cycles, fem_cycles, nu=2):
     total = 0.0
     count = 0
     for kind, part, cycles in ((0, shapes.split(";")[0], mac_cycles), (1, shapes.split(";")[1], fem_cycles)):
-        m = re.search(r"(\d+)x(\d+):(\d+)x(\d+)h(\d+)/", part)
-        sx, sy, h = int(m.group(3)), int(m.group(4)), int(m.group(5))
-        name = f"k_smooth_fused<double, {kind}, {h}, {sx}, {sy}>"
+        m = re.search(r"(\d+)x(\d+):(\d+)x(\d+)h(\d+)/(\d+)x(\d+)h(\d+)", part)
+        g = 6 if fp32 else 3
+        sx, sy, h = int(m.group(g)), int(m.group(g + 1)), int(m.group(g + 2))
+        name = f"k_smooth_fused<{'float' if fp32 else 'double'}, {kind}, {h}, {sx}, {sy}>"
         if name not in tab:
             return None
         per_launch = h // (2 if kind == 0 else 4)
@@ -278,24 +279,30 @@ def main():
     # by kernel, as the runs above do)
     graph = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC, launch=ia.LAUNCH_GRAPH)
     params, training = train_guess(m, a.train_steps)
-    learned = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_LEARNED, params=params)
+    learned = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, timing=True, guess=ia.GUESS_LEARNED, params=params)
     best = learned
     best_name = "mixed+learned_guess"
-    # roofline of the fine-level smoother (FP64 pass): algorithmic bytes per
-    # sweep are, per point, z, b and the coefficients read and z written:
-    # MAC red-black 8+8+16+8 = 40 B/cell (two face inverse densities), FEM
-    # 4-colour 8+8+8+8 = 32 B/node (one cell inverse density); the library
-    # sums them over the timed sweeps and times each launch with HIP events.
-    launches = fp64["smoother_launches"]
-    avg_ms = fp64["smoother_ms"] / launches
-    achieved = fp64["smoother_bytes"] / (fp64["smoother_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(fp64["smoother_shapes"], fp64["mac_cycles_timed"], fp64["fem_cycles_timed"])
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="fine-level projection smoother k_smooth_fused<double> (MAC and FEM)",
-                    avg_launch_us=round(avg_ms * 1e3, 2),
-                    back_to_back_us_per_launch={k: round(v, 2) for k, v in fp64["back_to_back_us_per_launch"].items()},
-                    algorithmic_bytes_per_launch=round(fp64["smoother_bytes"] / launches))
+    # roofline of the fine-level smoother: algorithmic bytes per sweep are,
+    # per point, z, b and the coefficients read and z written: MAC
+    # red-black 8+8+16+8 = 40 B/cell in FP64 (two face inverse densities),
+    # 20 in FP32; FEM 4-colour 8+8+8+8 = 32 B/node (one cell inverse
+    # density), 16 in FP32; the library sums them over the timed sweeps and
+    # times each launch with HIP events. The headline's V-cycles smooth in
+    # FP32 (roofline), the FP64 baseline's in FP64 (roofline_fp64).
+    def smoother_roofline(r, fp32):
+        launches = r["smoother_launches"]
+        achieved = r["smoother_bytes"] / (r["smoother_ms"] * 1e-3) / 1e9
+        t = "float" if fp32 else "double"
+        return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(r["smoother_shapes"], r["mac_cycles_timed"], r["fem_cycles_timed"],
+                                             fp32=fp32),
+                    kernel=f"fine-level projection smoother k_smooth_fused<{t}> (MAC and FEM)",
+                    avg_launch_us=round(r["smoother_ms"] / launches * 1e3, 2),
+                    back_to_back_us_per_launch={k: round(v, 2) for k, v in r["back_to_back_us_per_launch"].items()},
+                    algorithmic_bytes_per_launch=round(r["smoother_bytes"] / launches))
+    roofline = smoother_roofline(learned, True)
+    roofline_fp64 = smoother_roofline(fp64, False)
     name, hbm = ia.device_info()
     skip = ("back_to_back_us_per_launch", "learned_start")
     line = {
@@ -324,6 +331,7 @@ def main():
         "speedup_vs_plain_fp64": round(best["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "speedup_quadratic_vs_plain_fp64": round(quad["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
         "device": name,
     }
     if not a.no_cpu:

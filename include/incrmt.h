@@ -296,9 +296,9 @@ int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, in
 int incrmt_mg_set_launch(incrmt_mg *h, int mode);
 /* No reference counterpart (for checking the smoother): `sweeps` forward
  * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused
- * launch shape `shape` (0..16; see incrmt_fluid_describe; 14..16 and the
- * MAC form of 13 fit in LDS only in single precision) or one launch per
- * colour when shape < 0. */
+ * launch shape `shape` (0..16; see incrmt_fluid_describe; the FEM form of
+ * 16 fits in LDS only in single precision) or one launch per colour when
+ * shape < 0. */
 int incrmt_mg_smooth(incrmt_mg *h, const double *b, double *z, int sweeps, int shape);
 /* The same in INCRMT_PREC_FP64 or, with INCRMT_PREC_MIXED, in single
  * precision: b and z rounded to FP32, the FP32 sweeps of the mixed
@@ -309,6 +309,9 @@ int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y);
 /* Time `sweeps` fine-level smoother sweeps on the resident problem with HIP
  * events; returns the mean milliseconds per smoother launch. */
 int incrmt_mg_time_smoother(incrmt_mg *h, int sweeps, int precision, double *ms_per_launch, int *launches);
+/* The same with fused launch shape `shape` (as for incrmt_mg_smooth) in
+ * place of the one the autotuner picked, for measuring every shape. */
+int incrmt_mg_time_shape(incrmt_mg *h, int sweeps, int precision, int shape, double *ms_per_launch, int *launches);
 void incrmt_mg_destroy(incrmt_mg *h);
 
 /* ---- levelset -------------------------------------------------------------

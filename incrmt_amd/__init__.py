@@ -105,6 +105,7 @@ _PROTOS = [
     ("incrmt_mg_smooth_prec", _I, [_V, _dp, _dp, _I, _I, _I]),
     ("incrmt_mg_set_launch", _I, [_V, _I]),
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
+    ("incrmt_mg_time_shape", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
     ("incrmt_levelset_build_band", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _dp, _ip, _ip]),
     ("incrmt_levelset_build_band_noalloc", _I, [_I, _I, _D, _D, _D, _D, _D, _D, _V, _V, _ip]),
@@ -496,9 +497,11 @@ class MgSolver:
         _check(lib().incrmt_mg_smooth_prec(self._h, _d(bb), _d(zz), sweeps, shape, precision))
         return zz
 
-    def time_smoother(self, sweeps, precision=PREC_FP64):
+    def time_smoother(self, sweeps, precision=PREC_FP64, shape=-1):
+        """Mean ms per fine-level smoother launch over `sweeps` sweeps and
+        the launch count, on the autotuned shape or on `shape`."""
         ms, n = _D(0), _I(0)
-        _check(lib().incrmt_mg_time_smoother(self._h, sweeps, precision, ctypes.byref(ms), ctypes.byref(n)))
+        _check(lib().incrmt_mg_time_shape(self._h, sweeps, precision, shape, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
 

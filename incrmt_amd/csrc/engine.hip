@@ -1448,9 +1448,14 @@ int incrmt_mg_apply(incrmt_mg *h, const double *x, double *y) {
 }
 
 int incrmt_mg_time_smoother(incrmt_mg *h, int sweeps, int precision, double *ms_per_launch, int *launches) {
+    return incrmt_mg_time_shape(h, sweeps, precision, -1, ms_per_launch, launches);
+}
+
+int incrmt_mg_time_shape(incrmt_mg *h, int sweeps, int precision, int shape, double *ms_per_launch, int *launches) {
     if (!h || sweeps < 1) return fail(INCRMT_ERR_ARG, "bad arguments");
     long n = 0;
-    int rc = h->mg.time_fine_smoother(sweeps, precision == INCRMT_PREC_MIXED, ms_per_launch, &n);
+    int rc = h->mg.time_fine_smoother(sweeps, precision == INCRMT_PREC_MIXED, ms_per_launch, &n, shape);
+    if (rc == 1) return fail(INCRMT_ERR_ARG, "unknown shape");
     if (launches) *launches = (int)n;
     return rc;
 }

@@ -141,7 +141,8 @@ class Multigrid {
     // y = A x on level 0; x and y are padded level arrays (pointers at (0,0)).
     int apply(const double *x, double *y);
     // Time `sweeps` fine-level sweeps with HIP events: mean ms per launch.
-    int time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches);
+    // shape >= 0: time that launch shape instead of the level's own
+    int time_fine_smoother(int sweeps, bool fp32, double *ms_per_launch, long *launches, int shape = -1);
     MgLevel &fine() { return L[0]; }
     const std::vector<MgLevel> &levels() const { return L; }
     MgKind kind() const { return kind_; }
