@@ -238,8 +238,10 @@ def main():
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS fine-level smoother sweeps of each system (for counter runs)")
     ap.add_argument("--headline-only", action="store_true",
-                    help="only the mixed+quadratic variant, --warmup then --steps steps (may be 0), no "
-                         "timing passes: two profiler runs of it differ by exactly the steps")
+                    help="only the mixed+quadratic variant (with --learned: mixed+learned, after the "
+                         "untimed training), --warmup then --steps steps (may be 0), no timing passes: "
+                         "two profiler runs of it differ by exactly the steps")
+    ap.add_argument("--learned", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the path would shard as row slabs "
@@ -251,7 +253,12 @@ def main():
         print(json.dumps(smoother_probe(m, a.smoother_probe)))
         return
     if a.headline_only:
-        f = ia.simple_spin(m, rel_tol=1e-8, precision=ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC)
+        params = train_guess(m, a.train_steps)[0] if a.learned else None
+        f = ia.simple_spin(m, rel_tol=1e-8, precision=ia.PREC_MIXED,
+                           guess=ia.GUESS_LEARNED if a.learned else ia.GUESS_QUADRATIC)
+        if params is not None:
+            for k in (ia.MG_MAC, ia.MG_FEM):
+                f.set_guess_params(k, params[k])
         l, adt = f.timestep_select(4 * math.pi / 240)
         f.steps(adt, a.warmup)
         f.sync()
