@@ -64,8 +64,11 @@ int incrmt_copy_probe(long long bytes, int reps, double *ms_per_copy);
  * fluid_2d::fluid_2d(m,n,sim_type&,visc,rhof,tw,pinw,ntrace,fflags,dir)
  * (fluid_2d.cc:26-43) with sim_type(ax,bx,ay,by,x_prd,y_prd,w_scale,w_force)
  * (sim_type.hh:63-71): no-slip walls, or periodic along x_prd / y_prd
- * (fluid_2d.cc:689-872, 1193-1235, 1317-1348; a periodic direction needs an
- * even cell count whose halvings end at 2 or 3, checked at initialize).
+ * (fluid_2d.cc:689-872, 1193-1235, 1317-1348). A box periodic in one
+ * direction only needs an even cell count there whose halvings end at 2 or
+ * 3; a doubly periodic box takes any size of at least 8 (conv_test's ladder
+ * 280..2520 included): when halving does not end on a tiny level the
+ * solvers resample to a 2^k or 3 2^k grid (checked at initialize).
  * The outflow (zpr) and periodic-solid (solid_prd_bc) variants are set by
  * incrmt_fluid_set_horizontal_flow and incrmt_fluid_set_solid_periodic. */
 int incrmt_fluid_create(int m, int n, double ax, double bx, double ay, double by, int x_prd, int y_prd,
@@ -270,7 +273,10 @@ int incrmt_fluid_track(incrmt_fluid *f, int obj, double *rec, int cap, int *nrec
  * A periodic direction (mgs_mac.hh:45-56, mgs_fem.hh:47-52; a periodic FEM
  * direction has m nodes, node m being node 0) needs an even size whose
  * repeated halving ends on a coarsest level of at most 16 points (e.g.
- * 2^k or 3 * 2^k); others return INCRMT_ERR_UNSUPPORTED. */
+ * 2^k or 3 * 2^k) unless both directions are periodic: then any size of at
+ * least 8 is taken (the first coarse level resamples to 2^k or 3 * 2^k
+ * points per direction, an odd finest size is smoothed in nine colours);
+ * others return INCRMT_ERR_UNSUPPORTED. */
 int incrmt_mg_create(int kind, int m, int n, int x_prd, int y_prd, double dx, double dy, incrmt_mg **out);
 /* The same with the zero-pressure right edge of mgs_base(m,n,x_prd,y_prd,zpr)
  * (mgs_common.cc:15-17; the reference sets it through fluid_2d::zpr): MAC

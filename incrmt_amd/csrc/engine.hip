@@ -452,8 +452,8 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (!rc0)
         rc0 = fem.init(kFEM, (g.px || g.zpr) ? g.m : g.m + 1, g.py ? g.n : g.n + 1, g.dx, g.dy, s, g.px, g.py, g.zpr);
     if (rc0 == INCRMT_ERR_UNSUPPORTED)
-        return fail(rc0, "a periodic direction or a zero-pressure edge needs an even cell count whose halvings end "
-                         "at most 16 points");
+        return fail(rc0, "a singly periodic direction or a zero-pressure edge needs an even cell count whose "
+                         "halvings end at most 16 points (a doubly periodic box: at least 8 cells)");
     if (rc0) return rc0;
     if (launch_req >= 0) mac.use_graphs = fem.use_graphs = launch_req == INCRMT_LAUNCH_GRAPH;
     if ((guess_mode == INCRMT_GUESS_LEARNED || guess_train) && (rc0 = ensure_guess())) return rc0;
@@ -1357,8 +1357,8 @@ int incrmt_mg_create_bc(int kind, int m, int n, int x_prd, int y_prd, int zpr, d
     int rc = h->mg.init(kind == INCRMT_MG_MAC ? kMAC : kFEM, fixcol ? m - 1 : m, n, dx, dy, h->s, x_prd != 0,
                         y_prd != 0, zpr != 0);
     if (rc == INCRMT_ERR_UNSUPPORTED)
-        rc = fail(rc, "a periodic direction or a zero-pressure edge needs an even size whose halvings end at most "
-                      "16 points");
+        rc = fail(rc, "a singly periodic direction or a zero-pressure edge needs an even size whose halvings end "
+                      "at most 16 points (a doubly periodic box: at least 8 points)");
     if (!rc) {
         const MgLevel &l = h->mg.fine();
         const size_t len = (size_t)(l.ny + 2) * l.pitch + 64;

@@ -75,6 +75,7 @@ struct MgLevel {
     double kx, ky;  // MAC 1/dx^2, 1/dy^2 of this level
     double hx, hy;  // FEM node spacing of this level
     int shape64 = 0, shape32 = 0;  // fused-smoother launch shape (FP64, FP32 sweeps)
+    bool odd = false;              // odd size along a periodic direction: nine-colour sweeps
     // periodic directions: the ghost ring of every iterate holds the
     // opposite edge (each write of an edge point also writes its mirrors),
     // MAC faces 0 and nx are the two sides of the seam face, FEM has nx
@@ -225,8 +226,19 @@ class Multigrid {
     double *zq1_ = nullptr, *zq2_ = nullptr;  // the second and third newest (the newest is z)
     double t0_ = 0, tprev_ = 0;      // times of the last two solves
     bool resampled_ = false;         // level 1 is a uniform resampling of level 0
+    bool rs_per_ = false;            // ... of a doubly periodic level 0
     Resample rx_{}, ry_{};
     int make_resample(int nf, double hf, int nc, Resample &r);
+    // periodic: nf and nc points (cell centres, off = 0.5, or nodes, off = 0)
+    // spread evenly over the same period
+    int make_resample_per(int nf, double hf, int nc, double off, Resample &r);
+    template <class T>
+    int smooth_colours(size_t l, int sweeps, bool reverse);
+    // restriction weight of a periodic resampling: FEM the transpose of the
+    // prolongation, MAC its average (nc/nf per direction)
+    double rs_scale() const {
+        return kind_ == kMAC ? (double)rx_.nc / rx_.nf * ((double)ry_.nc / ry_.nf) : 1.0;
+    }
   public:
     // Time every fused-smoother launch shape on each level large enough to
     // matter and keep the fastest (results do not depend on the shape).

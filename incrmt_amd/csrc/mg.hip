@@ -109,6 +109,39 @@ __global__ void k_fem_smooth(LevelArgs a, FemC k, T *z, const T *b, const T *kc,
     if (diag != T(0)) put_z(a, z, i, j, (b[q] - off) / diag);
 }
 
+// Nine-colour Gauss-Seidel for a level of odd size along a periodic
+// direction, where the two colours of i % 2 meet across the seam: the last
+// point of such a direction takes a third colour, so no two neighbours of
+// the 5- or 9-point stencil share a colour (i, j) -> cx + 3 cy. Colours
+// with no points return at once.
+template <class T>
+__global__ void k_smooth_colour9(int kind, LevelArgs a, FemC k, T *z, const T *b, const T *c1, const T *c2,
+                                 int colour) {
+    const int cx = colour % 3, cy = colour / 3;
+    const int t = blockIdx.x * BX + threadIdx.x, u = blockIdx.y * BY + threadIdx.y;
+    const bool ox = a.px && (a.nx & 1), oy = a.py && (a.ny & 1);
+    int i, j;
+    if (cx == 2) {
+        if (!ox || t) return;
+        i = a.nx - 1;
+    } else {
+        i = 2 * t + cx;
+        if (i >= a.nx - (ox ? 1 : 0)) return;
+    }
+    if (cy == 2) {
+        if (!oy || u) return;
+        j = a.ny - 1;
+    } else {
+        j = 2 * u + cy;
+        if (j >= a.ny - (oy ? 1 : 0)) return;
+    }
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    T diag, off;
+    if (kind == kMAC) off = mac_offdiag<T>(z, c1, c2, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
+    else off = fem_offdiag<T>(z, c1, q, c, a.zp, a.cp, k, diag);
+    if (diag != T(0)) put_z(a, z, i, j, (b[q] - off) / diag);
+}
+
 // Fused colored Gauss-Seidel: a whole multi-colour sweep (MAC: two sweeps)
 // in one launch. A tile plus a halo of one point per colour phase is loaded
 // into LDS with coalesced row reads; phase p updates its colour on the
@@ -511,6 +544,113 @@ __global__ void k_fem_prolong_rs(LevelArgs f, LevelArgs cl, T *z, const T *e, Rs
     const T up = (T(1) - tx) * e[(J + 1) * p + I] + tx * e[(J + 1) * p + I + 1];
     z[j * f.zp + i] += (T(1) - ty) * lo + ty * up;
 }
+// Transfers between a doubly periodic level and its resampling: nf fine and
+// nc coarse points (cell centres or nodes) spread evenly over one period.
+// Fine point i sits at coarse coordinate i0[i] + t[i] (i0 wrapped into
+// [0, nc)): prolongation is linear interpolation between coarse points i0
+// and i0 + 1 (wrapped), restriction its transpose, gathered per coarse
+// point over the fine window lo..hi (indices wrapped), times `scale`
+// (FEM: 1, the transpose itself; MAC: nc/nf per direction, an average).
+template <class T>
+__global__ void k_restrict_rs_per(LevelArgs f, LevelArgs cl, const T *r, T *bc, T *zc, RsArgs x, RsArgs y, T scale) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= cl.nx || J >= cl.ny) return;
+    auto w = [](const RsArgs &d, int i, int I, int nc) -> T {
+        const int a = d.i0[i];
+        return a == I ? T(1 - d.t[i]) : ((a + 1 == nc ? 0 : a + 1) == I ? T(d.t[i]) : T(0));
+    };
+    T s = 0;
+    for (int jj = y.lo[J]; jj <= y.hi[J]; jj++) {
+        const int j = wrap_idx(jj, f.ny);
+        const T wy = w(y, j, J, cl.ny);
+        if (wy == T(0)) continue;
+        T sr = 0;
+        for (int ii = x.lo[I]; ii <= x.hi[I]; ii++) {
+            const int i = wrap_idx(ii, f.nx);
+            sr += w(x, i, I, cl.nx) * r[j * f.zp + i];
+        }
+        s += wy * sr;
+    }
+    bc[J * cl.zp + I] = scale * s;
+    put_z(cl, zc, I, J, T(0));
+}
+template <class T>
+__global__ void k_prolong_rs_per(LevelArgs f, LevelArgs cl, T *z, const T *e, RsArgs x, RsArgs y) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= f.nx || j >= f.ny) return;
+    const int I = x.i0[i], J = y.i0[j], p = cl.zp;
+    const int I1 = I + 1 == cl.nx ? 0 : I + 1, J1 = J + 1 == cl.ny ? 0 : J + 1;
+    const T tx = T(x.t[i]), ty = T(y.t[j]);
+    const T lo = (T(1) - tx) * e[J * p + I] + tx * e[J * p + I1];
+    const T up = (T(1) - tx) * e[J1 * p + I] + tx * e[J1 * p + I1];
+    put_z(f, z, i, j, z[j * f.zp + i] + ((T(1) - ty) * lo + ty * up));
+}
+// MAC residual r = b - A z of one level
+template <class T>
+__global__ void k_mac_residual(LevelArgs a, const T *z, const T *b, const T *fw, const T *fs, T *r) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    T diag;
+    const T off = mac_offdiag<T>(z, fw, fs, q, c, a.zp, a.cp, T(a.kx), T(a.ky), diag);
+    r[q] = b[q] - off - diag * z[q];
+}
+// Overlap of [a0, a1) and [b0, b1)
+__device__ inline double overlap(double a0, double a1, double b0, double b1) {
+    const double v = fmin(a1, b1) - fmax(a0, b0);
+    return v > 0 ? v : 0;
+}
+// Resampled coefficients of a doubly periodic level, in units of one
+// period per direction (fine spacing 1/f.nx, coarse 1/c.nx). FEM: a coarse
+// cell's weight is its fine cells' average weighted by overlap. MAC: a
+// coarse face's inverse density is the fine faces' value interpolated
+// linearly along the face normal and averaged with overlap along the face.
+__global__ void k_fem_coef_rs_per(LevelArgs f, LevelArgs c, const double *kf, double *kcc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= c.nx || J >= c.ny) return;
+    const double hfx = 1.0 / f.nx, hfy = 1.0 / f.ny, hcx = 1.0 / c.nx, hcy = 1.0 / c.ny;
+    const int i0 = (int)(I * hcx / hfx), i1 = min(f.nx - 1, (int)((I + 1) * hcx / hfx));
+    const int j0 = (int)(J * hcy / hfy), j1 = min(f.ny - 1, (int)((J + 1) * hcy / hfy));
+    double s = 0;
+    for (int j = j0; j <= j1; j++) {
+        const double oy = overlap(j * hfy, (j + 1) * hfy, J * hcy, (J + 1) * hcy);
+        for (int i = i0; i <= i1; i++) s += oy * overlap(i * hfx, (i + 1) * hfx, I * hcx, (I + 1) * hcx) * kf[j * f.cp + i];
+    }
+    kcc[J * c.cp + I] = s / (hcx * hcy);
+}
+__global__ void k_mac_coef_rs_per(LevelArgs f, LevelArgs c, const double *fwf, const double *fsf, double *fwc,
+                                  double *fsc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I > c.nx || J > c.ny) return;
+    const double hfx = 1.0 / f.nx, hfy = 1.0 / f.ny, hcx = 1.0 / c.nx, hcy = 1.0 / c.ny;
+    if (J < c.ny) {  // x-face I of row J (faces 0..nx, face nx the seam's copy of face 0)
+        const double u = I * hcx / hfx;
+        int a = (int)u;
+        if (a > f.nx) a = f.nx;
+        const double t = u - a;
+        const int b = a < f.nx ? a + 1 : a;
+        const int j0 = (int)(J * hcy / hfy), j1 = min(f.ny - 1, (int)((J + 1) * hcy / hfy));
+        double s = 0;
+        for (int j = j0; j <= j1; j++)
+            s += overlap(j * hfy, (j + 1) * hfy, J * hcy, (J + 1) * hcy) *
+                 ((1 - t) * fwf[j * f.cp + a] + t * fwf[j * f.cp + b]);
+        fwc[J * c.cp + I] = s / hcy;
+    }
+    if (I < c.nx) {  // y-face J of column I
+        const double u = J * hcy / hfy;
+        int a = (int)u;
+        if (a > f.ny) a = f.ny;
+        const double t = u - a;
+        const int b = a < f.ny ? a + 1 : a;
+        const int i0 = (int)(I * hcx / hfx), i1 = min(f.nx - 1, (int)((I + 1) * hcx / hfx));
+        double s = 0;
+        for (int i = i0; i <= i1; i++)
+            s += overlap(i * hfx, (i + 1) * hfx, I * hcx, (I + 1) * hcx) *
+                 ((1 - t) * fsf[a * f.cp + i] + t * fsf[b * f.cp + i]);
+        fsc[J * c.cp + I] = s / hcx;
+    }
+}
+
 // coarse cell coefficient: fine cell values averaged with their overlap
 __global__ void k_fem_coef_rs(LevelArgs f, LevelArgs c, const double *kf, double *kcc, int *cxlo, int *cxhi,
                               int *cylo, int *cyhi, double hfx, double hcx, double hfy, double hcy) {
@@ -1440,6 +1580,39 @@ int Multigrid::make_resample(int nf, double hf, int nc, Resample &r) {
     return 0;
 }
 
+int Multigrid::make_resample_per(int nf, double hf, int nc, double off, Resample &r) {
+    const double ratio = (double)nc / nf;  // coarse points per fine point
+    std::vector<int> i0(nf), lo(nc), hi(nc);
+    std::vector<double> t(nf);
+    for (int i = 0; i < nf; i++) {
+        const double u = (i + off) * ratio - off, fl = std::floor(u);
+        t[i] = u - fl;
+        int I = (int)fl % nc;
+        if (I < 0) I += nc;
+        i0[i] = I;
+    }
+    // the fine points within one coarse spacing of coarse point I
+    for (int I = 0; I < nc; I++) {
+        lo[I] = (int)std::floor((I + off - 1) / ratio - off);
+        hi[I] = (int)std::ceil((I + off + 1) / ratio - off);
+        if (hi[I] - lo[I] + 1 > nf) return INCRMT_ERR_GRID;  // a window wrapping onto itself
+    }
+    auto up = [&](auto &v, auto *&d) -> int {
+        using E = typename std::remove_reference<decltype(v[0])>::type;
+        void *q;
+        HIP_CHECK(hipMalloc(&q, v.size() * sizeof(E)));
+        allocs.push_back(q);
+        HIP_CHECK(hipMemcpy(q, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+        d = (decltype(d))q;
+        return 0;
+    };
+    int rc;
+    if ((rc = up(i0, r.i0)) || (rc = up(lo, r.lo)) || (rc = up(hi, r.hi)) || (rc = up(t, r.t))) return rc;
+    r.clo = r.chi = nullptr;
+    r.nf = nf; r.nc = nc; r.hf = hf; r.hc = hf * nf / nc;
+    return 0;
+}
+
 int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t s, bool px, bool py, bool dr) {
     kind_ = kind; s_ = s; dx_ = dx; dy_ = dy; dr_ = dr;
     // graph replay pays where launch latency dominates; a large finest level
@@ -1450,7 +1623,10 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
     // a periodic direction is coloured across its seam: it needs an even
     // size on every level that is smoothed; a zero-pressure edge closes a
     // non-periodic x direction
-    if ((px && nx % 2) || (py && ny % 2) || (dr && px)) return INCRMT_ERR_GRID;
+    // (an odd size is taken on a doubly periodic finest level that is
+    // resampled below, and smoothed there in nine colours)
+    if (dr && px) return INCRMT_ERR_GRID;
+    if (((px && nx % 2) || (py && ny % 2)) && !(px && py)) return INCRMT_ERR_GRID;
     while (true) {
         MgLevel l{};
         l.nx = nx; l.ny = ny;
@@ -1498,6 +1674,33 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
             nx = cx; ny = cy;
             continue;
         }
+        if (px && py && !dr && L.size() == 1) {
+            // a doubly periodic level whose halving does not end in a tiny
+            // (directly solved) level: the next level is the largest 2^k or
+            // 3 2^k grid of at most half the points on the same period
+            // (conv_test's ladder 280..2520 halves to 35, 45, 63, 105 or 315)
+            auto reaches = [](int a, int b) {
+                while (a % 2 == 0 && b % 2 == 0 && a >= 4 && b >= 4) { a /= 2; b /= 2; }
+                return (long long)a * b <= 16;
+            };
+            auto target = [](int k) {
+                int best = 0;
+                for (int c = 4; c <= k / 2; c *= 2) best = c;
+                for (int c = 6; c <= k / 2; c *= 2) best = std::max(best, c);
+                return best;
+            };
+            const int cx = target(nx), cy = target(ny);
+            if (!reaches(nx, ny) && cx >= 4 && cy >= 4 && reaches(cx, cy)) {
+                const double off = kind == kMAC ? 0.5 : 0.0;
+                if ((rc = make_resample_per(nx, hx, cx, off, rx_)) || (rc = make_resample_per(ny, hy, cy, off, ry_)))
+                    return rc;
+                resampled_ = rs_per_ = true;
+                hx = rx_.hc; hy = ry_.hc;
+                kx = 1. / (hx * hx); ky = 1. / (hy * hy);
+                nx = cx; ny = cy;
+                continue;
+            }
+        }
         // MAC halves even cell counts; FEM halves 2^k+1 node counts, or even
         // node counts along a periodic direction or up to a zero-pressure
         // edge (nodes 2I are kept, the fixed column stays the ring)
@@ -1508,6 +1711,8 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         if (kind == kMAC) { nx /= 2; ny /= 2; kx *= 0.25; ky *= 0.25; }
         else { nx = (px || dr) ? nx / 2 : (nx + 1) / 2; ny = py ? ny / 2 : (ny + 1) / 2; hx *= 2; hy *= 2; }
     }
+    L[0].odd = (px && m % 2) || (py && n % 2);
+    if (L[0].odd && !rs_per_) return INCRMT_ERR_GRID;
     // a periodic or zero-pressure coarsest level is solved exactly (dense
     // elimination in the tail cycle), so it must be tiny
     if ((px || py || dr) && (long long)L.back().nx * L.back().ny > 16) return INCRMT_ERR_GRID;
@@ -1538,9 +1743,14 @@ int Multigrid::coarsen_coefs(bool f32) {
     for (size_t l = 0; l + 1 < L.size(); l++) {
         const MgLevel &f = L[l];
         MgLevel &c = L[l + 1];
-        if (kind_ == kMAC) {
+        if (kind_ == kMAC && l == 0 && rs_per_) {
+            k_mac_coef_rs_per<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.fw, f.fs,
+                                                                                c.fw, c.fs);
+        } else if (kind_ == kMAC) {
             k_mac_coef_coarsen<<<grid2(c.nx + 1, c.ny + 1), dim3(BX, BY), 0, s_>>>(
                 args_of(f), args_of(c), f.fw, f.fs, c.fw, c.fs, f32 ? c.fwf : nullptr, f32 ? c.fsf : nullptr);
+        } else if (l == 0 && rs_per_) {
+            k_fem_coef_rs_per<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc);
         } else if (l == 0 && resampled_) {
             k_fem_coef_rs<<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.kc, c.kc, rx_.clo,
                                                                       rx_.chi, ry_.clo, ry_.chi, rx_.hf, rx_.hc,
@@ -1809,7 +2019,7 @@ int Multigrid::autotune() {
     int rc = 0;
     for (size_t l = 0; l < L.size() && l < tail_ && !rc; l++) {
         MgLevel &v = L[l];
-        if ((long long)v.nx * v.ny < 64 * 64) continue;
+        if ((long long)v.nx * v.ny < 64 * 64 || v.odd) continue;
         const LevelArgs a = args_of(v);
         const FemC k = fem_consts(v.hx, v.hy);
         for (int prec = 0; prec < 2 && !rc; prec++) {
@@ -1933,8 +2143,28 @@ int Multigrid::smooth_check(int sweeps, int shape, bool f32) {
                                  false, s_, &nl);
 }
 
+template <class T>
+int Multigrid::smooth_colours(size_t l, int sweeps, bool reverse) {
+    MgLevel &v = L[l];
+    const bool f32 = std::is_same<T, float>::value;
+    T *z = f32 ? (T *)v.zf : (T *)v.z;
+    const T *b = f32 ? (const T *)v.bf : (const T *)v.b;
+    const T *c1 = f32 ? (const T *)(kind_ == kMAC ? v.fwf : v.kcf) : (const T *)(kind_ == kMAC ? v.fw : v.kc);
+    const T *c2 = f32 ? (const T *)v.fsf : (const T *)v.fs;
+    const dim3 grid((v.nx + 1) / 2 / BX + 1, (v.ny + 1) / 2 / BY + 1);
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int cc = 0; cc < 9; cc++) {
+            const int col = reverse ? 8 - cc : cc;
+            k_smooth_colour9<T><<<grid, dim3(BX, BY), 0, s_>>>(kind_, args_of(v), fem_consts(v.hx, v.hy), z, b, c1, c2,
+                                                               col);
+            KCHECK();
+        }
+    return 0;
+}
+
 int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
     const MgLevel &v = L[l];
+    if (v.odd) return smooth_colours<double>(l, sweeps, reverse);
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
@@ -1957,6 +2187,7 @@ int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
 
 int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
     const MgLevel &v = L[l];
+    if (v.odd) return smooth_colours<float>(l, sweeps, reverse);
     const bool timed = l == 0;
     if (timed && record_begin()) return 6;
     int nl = 0;
@@ -2030,7 +2261,15 @@ int Multigrid::vcycle(size_t l) {
     int rc;
     if ((rc = smooth(l, nu1, false))) return rc;
     const MgLevel &c = L[l + 1];
-    if (kind_ == kMAC) {
+    if (l == 0 && rs_per_) {
+        if (kind_ == kMAC)
+            k_mac_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, f.b, f.fw, f.fs, f.r);
+        else
+            k_fem_residual<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.z, f.b, f.kc, f.r);
+        KCHECK();
+        k_restrict_rs_per<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.r, c.b, c.z,
+                                                                             rs_args(rx_), rs_args(ry_), rs_scale());
+    } else if (kind_ == kMAC) {
         k_mac_restrict<double><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, f.b, f.fw,
                                                                             f.fs, c.b, c.z);
     } else {
@@ -2049,8 +2288,12 @@ int Multigrid::vcycle(size_t l) {
     // the coarse-grid correction enters in the post-smoother's load, except
     // across a resampled (non-nested) level pair
     if (l == 0 && resampled_) {
-        k_fem_prolong_rs<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z,
-                                                                          rs_args(rx_), rs_args(ry_));
+        if (rs_per_)
+            k_prolong_rs_per<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z,
+                                                                              rs_args(rx_), rs_args(ry_));
+        else
+            k_fem_prolong_rs<double><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.z, c.z,
+                                                                              rs_args(rx_), rs_args(ry_));
         KCHECK();
         return smooth(l, nu2, true, false);
     }
@@ -2083,7 +2326,15 @@ int Multigrid::vcycle_f(size_t l) {
     int rc;
     if ((rc = smooth_f(l, nu1, false))) return rc;
     const MgLevel &c = L[l + 1];
-    if (kind_ == kMAC) {
+    if (l == 0 && rs_per_) {
+        if (kind_ == kMAC)
+            k_mac_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.zf, f.bf, f.fwf, f.fsf, f.rf);
+        else
+            k_fem_residual<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), k, f.zf, f.bf, f.kcf, f.rf);
+        KCHECK();
+        k_restrict_rs_per<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(
+            args_of(f), args_of(c), f.rf, c.bf, c.zf, rs_args(rx_), rs_args(ry_), float(rs_scale()));
+    } else if (kind_ == kMAC) {
         k_mac_restrict<float><<<grid2(c.nx, c.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, f.bf, f.fwf,
                                                                            f.fsf, c.bf, c.zf);
     } else {
@@ -2102,8 +2353,12 @@ int Multigrid::vcycle_f(size_t l) {
     // the coarse-grid correction enters in the post-smoother's load, except
     // across a resampled (non-nested) level pair
     if (l == 0 && resampled_) {
-        k_fem_prolong_rs<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf,
-                                                                          rs_args(rx_), rs_args(ry_));
+        if (rs_per_)
+            k_prolong_rs_per<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf,
+                                                                             rs_args(rx_), rs_args(ry_));
+        else
+            k_fem_prolong_rs<float><<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), args_of(c), f.zf, c.zf,
+                                                                             rs_args(rx_), rs_args(ry_));
         KCHECK();
         return smooth_f(l, nu2, true, false);
     }

@@ -34,6 +34,11 @@ struct level {
     std::vector<double> kc;      // FEM: cell inverse densities cx()*cy()
     double kx, ky;               // MAC scalings 1/dx^2, 1/dy^2
     double dxdy, dydx;           // FEM ratios
+    // a resampling of the doubly periodic level above it: fine point i of
+    // that level sits at coarse coordinate ix[i] + tx[i] (ix wrapped)
+    bool rs = false;
+    std::vector<int> ix, iy;
+    std::vector<double> tx, ty;
     int cx() const { return px ? nx : nx - 1; }  // FEM cells per row
     int cy() const { return py ? ny : ny - 1; }
     void alloc() {
@@ -119,8 +124,16 @@ void residual(level &L) {
         }
 }
 
-// Colored Gauss-Seidel: colors (i%2, j%2), a valid GS ordering for 9 points.
+void smooth_serial(level &L);
+
+// Colored Gauss-Seidel: colors (i%2, j%2), a valid GS ordering for 9 points
+// (a level of odd size along a periodic direction, where two points of one
+// colour meet across the seam, is swept in lexicographic order instead).
 void smooth(level &L, int sweeps, bool reverse) {
+    if ((L.px && L.nx % 2) || (L.py && L.ny % 2)) {
+        for (int s = 0; s < sweeps; s++) smooth_serial(L);
+        return;
+    }
     for (int s = 0; s < sweeps; s++)
         for (int cc = 0; cc < 4; cc++) {
             const int col = reverse ? 3 - cc : cc, ci = col & 1, cj = col >> 1;
@@ -151,7 +164,109 @@ double mean(const std::vector<double> &v) {
     return v.empty() ? 0 : s / v.size();
 }
 
+// Periodic resampling weights: nf fine points (cell centres, off = 0.5, or
+// nodes, off = 0) onto nc coarse points over the same period.
+void resample_weights(int nf, int nc, double off, std::vector<int> &i0, std::vector<double> &t) {
+    i0.resize(nf);
+    t.resize(nf);
+    for (int i = 0; i < nf; i++) {
+        const double u = (i + off) * nc / (double)nf - off, fl = std::floor(u);
+        t[i] = u - fl;
+        i0[i] = (((int)fl % nc) + nc) % nc;
+    }
+}
+
+// Restriction onto a resampled level: the transpose of linear
+// interpolation (FEM), times nc/nf per direction (MAC, an average).
+void restrict_rs(const level &F, level &C) {
+    std::fill(C.b.begin(), C.b.end(), 0.0);
+    const double scale = F.cell ? (double)C.nx / F.nx * ((double)C.ny / F.ny) : 1.0;
+    for (int j = 0; j < F.ny; j++) {
+        const int J0 = C.iy[j], J1 = (J0 + 1) % C.ny;
+        const double wy[2] = {1 - C.ty[j], C.ty[j]};
+        for (int i = 0; i < F.nx; i++) {
+            const int I0 = C.ix[i], I1 = (I0 + 1) % C.nx;
+            const double wx[2] = {1 - C.tx[i], C.tx[i]}, r = scale * F.r[j * F.nx + i];
+            C.b[J0 * C.nx + I0] += wy[0] * wx[0] * r;
+            C.b[J0 * C.nx + I1] += wy[0] * wx[1] * r;
+            C.b[J1 * C.nx + I0] += wy[1] * wx[0] * r;
+            C.b[J1 * C.nx + I1] += wy[1] * wx[1] * r;
+        }
+    }
+}
+
+void prolong_rs(level &F, const level &C) {
+    for (int j = 0; j < F.ny; j++) {
+        const int J0 = C.iy[j], J1 = (J0 + 1) % C.ny;
+        const double ty = C.ty[j];
+        for (int i = 0; i < F.nx; i++) {
+            const int I0 = C.ix[i], I1 = (I0 + 1) % C.nx;
+            const double tx = C.tx[i];
+            F.z[j * F.nx + i] += (1 - ty) * ((1 - tx) * C.z[J0 * C.nx + I0] + tx * C.z[J0 * C.nx + I1])
+                                 + ty * ((1 - tx) * C.z[J1 * C.nx + I0] + tx * C.z[J1 * C.nx + I1]);
+        }
+    }
+}
+
+double overlap(double a0, double a1, double b0, double b1) {
+    const double v = std::min(a1, b1) - std::max(a0, b0);
+    return v > 0 ? v : 0;
+}
+
+// The coefficients of a resampled doubly periodic level, in units of one
+// period: FEM cell weights averaged over the fine cells with their overlap;
+// MAC face inverse densities interpolated linearly along the face normal
+// and averaged with overlap along the face.
+void resample_coefs(const level &F, level &C) {
+    const double hfx = 1.0 / F.nx, hfy = 1.0 / F.ny, hcx = 1.0 / C.nx, hcy = 1.0 / C.ny;
+    if (!F.cell) {
+        C.kc.assign((size_t)C.nx * C.ny, 0.0);
+        for (int J = 0; J < C.ny; J++)
+            for (int I = 0; I < C.nx; I++) {
+                double s = 0;
+                for (int j = 0; j < F.ny; j++) {
+                    const double oy = overlap(j * hfy, (j + 1) * hfy, J * hcy, (J + 1) * hcy);
+                    if (oy == 0) continue;
+                    for (int i = 0; i < F.nx; i++)
+                        s += oy * overlap(i * hfx, (i + 1) * hfx, I * hcx, (I + 1) * hcx) * F.kc[j * F.nx + i];
+                }
+                C.kc[J * C.nx + I] = s / (hcx * hcy);
+            }
+        return;
+    }
+    C.fw.assign((size_t)(C.nx + 1) * C.ny, 0.0);
+    C.fs.assign((size_t)C.nx * (C.ny + 1), 0.0);
+    for (int J = 0; J < C.ny; J++)
+        for (int I = 0; I <= C.nx; I++) {
+            const double u = I * (double)F.nx / C.nx;
+            const int a = std::min((int)std::floor(u), F.nx), b = std::min(a + 1, F.nx);
+            const double t = u - a;
+            double s = 0;
+            for (int j = 0; j < F.ny; j++) {
+                const double oy = overlap(j * hfy, (j + 1) * hfy, J * hcy, (J + 1) * hcy);
+                if (oy > 0) s += oy * ((1 - t) * F.fw[j * (F.nx + 1) + a] + t * F.fw[j * (F.nx + 1) + b]);
+            }
+            C.fw[J * (C.nx + 1) + I] = s / hcy;
+        }
+    for (int J = 0; J <= C.ny; J++)
+        for (int I = 0; I < C.nx; I++) {
+            const double u = J * (double)F.ny / C.ny;
+            const int a = std::min((int)std::floor(u), F.ny), b = std::min(a + 1, F.ny);
+            const double t = u - a;
+            double s = 0;
+            for (int i = 0; i < F.nx; i++) {
+                const double ox = overlap(i * hfx, (i + 1) * hfx, I * hcx, (I + 1) * hcx);
+                if (ox > 0) s += ox * ((1 - t) * F.fs[a * F.nx + i] + t * F.fs[b * F.nx + i]);
+            }
+            C.fs[J * C.nx + I] = s / hcx;
+        }
+}
+
 void restrict_to(const level &F, level &C) {
+    if (C.rs) {
+        restrict_rs(F, C);
+        return;
+    }
     if (F.cell) {
         for (int J = 0; J < C.ny; J++)
             for (int I = 0; I < C.nx; I++) {
@@ -181,6 +296,10 @@ void restrict_to(const level &F, level &C) {
 }
 
 void prolong_add(level &F, const level &C) {
+    if (C.rs) {
+        prolong_rs(F, C);
+        return;
+    }
     if (F.cell) {
         for (int j = 0; j < F.ny; j++)
             for (int i = 0; i < F.nx; i++) {
@@ -218,6 +337,43 @@ struct hierarchy {
             level C;
             C.cell = F.cell;
             C.px = F.px; C.py = F.py; C.zpr = F.zpr;
+            if (F.px && F.py && !F.zpr && L.size() == 1) {
+                // a doubly periodic level whose halving does not end on a
+                // tiny level: the next level resamples it onto the largest
+                // 2^k or 3 2^k grid of at most half the points
+                auto reaches = [](int a, int b) {
+                    while (a % 2 == 0 && b % 2 == 0 && a >= 4 && b >= 4) { a /= 2; b /= 2; }
+                    return (long long)a * b <= 16;
+                };
+                auto target = [](int k) {
+                    int best = 0;
+                    for (int c = 4; c <= k / 2; c *= 2) best = c;
+                    for (int c = 6; c <= k / 2; c *= 2) best = std::max(best, c);
+                    return best;
+                };
+                const int cx = target(F.nx), cy = target(F.ny);
+                if (!reaches(F.nx, F.ny) && cx >= 4 && cy >= 4 && reaches(cx, cy)) {
+                    C.nx = cx; C.ny = cy;
+                    C.rs = true;
+                    const double off = F.cell ? 0.5 : 0.0;
+                    resample_weights(F.nx, cx, off, C.ix, C.tx);
+                    resample_weights(F.ny, cy, off, C.iy, C.ty);
+                    resample_coefs(F, C);
+                    if (F.cell) {
+                        C.kx = F.kx * ((double)cx / F.nx) * ((double)cx / F.nx);
+                        C.ky = F.ky * ((double)cy / F.ny) * ((double)cy / F.ny);
+                        C.alloc();
+                        mac_stencil(C);
+                    } else {
+                        const double rx = (double)F.nx / cx, ry = (double)F.ny / cy;  // spacing growth
+                        C.dxdy = F.dxdy * rx / ry; C.dydx = F.dydx * ry / rx;
+                        C.alloc();
+                        fem_stencil(C);
+                    }
+                    L.push_back(std::move(C));
+                    continue;
+                }
+            }
             if (F.cell) {
                 if ((F.nx & 1) || (F.ny & 1) || F.nx < 4 || F.ny < 4) break;
                 C.nx = F.nx / 2; C.ny = F.ny / 2;

@@ -108,10 +108,31 @@ def test_periodic_graph_equals_direct():
 
 @pytest.mark.parametrize("kind,m,n", [("mac", 30, 16), ("mac", 80, 80), ("fem", 31, 16), ("fem", 40, 40)])
 def test_periodic_unsupported_sizes(kind, m, n):
-    """Odd periodic sizes, and sizes whose halvings end above 16 points."""
+    """One periodic direction: odd sizes along it, and sizes whose
+    halvings end above 16 points."""
     with pytest.raises(ia.IncrmtError) as e:
-        ia.MgSolver(ia.MG_MAC if kind == "mac" else ia.MG_FEM, m, n, 0.1, 0.1, x_prd=True, y_prd=True)
+        ia.MgSolver(ia.MG_MAC if kind == "mac" else ia.MG_FEM, m, n, 0.1, 0.1, x_prd=True, y_prd=False)
     assert e.value.code == ia.ERR_UNSUPPORTED
+    with pytest.raises(ia.IncrmtError):  # too small to resample
+        ia.MgSolver(ia.MG_MAC if kind == "mac" else ia.MG_FEM, 7, 7, 0.1, 0.1, x_prd=True, y_prd=True)
+
+
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+@pytest.mark.parametrize("m", [10, 15, 35, 45, 63])
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
+def test_doubly_periodic_resampled_matches_direct(kind, m, precision):
+    """Doubly periodic sizes whose halving does not end on a tiny level
+    (10 -> 5; 35, 45, 63: where conv_test's 280, 360 and 504 halve to; 15
+    odd, smoothed in nine colours) coarsen through a resampled level: the
+    solve still equals the mean-free direct solution."""
+    s, A = (_mac if kind == "mac" else _fem)(m, m, True, True, True, seed=4)
+    b = np.random.default_rng(5).standard_normal(m * m)
+    z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=200, precision=precision)
+    print(kind, m, precision, cyc, rr, flush=True)
+    assert rr <= 1e-10 and cyc < 40
+    zd = ops.neumann_solve(A, b)
+    assert np.abs((z - z.mean()) - zd).max() <= 1e-8 * np.abs(zd).max()
+    s.close()
 
 
 @pytest.mark.parametrize("px,py", PERIODS)

@@ -216,3 +216,29 @@ def test_periodic_poisson_second_order():
         errs.append(np.abs((z - z.mean()) - (ue - ue.mean())).max())
     r = [errs[k] / errs[k + 1] for k in range(2)]
     assert all(3.6 < q < 4.4 for q in r), r
+
+
+@pytest.mark.parametrize("kind", ["mac", "fem"])
+@pytest.mark.parametrize("m", [10, 15, 35])
+def test_periodic_resampled_solves_match_direct(kind, m):
+    """Doubly periodic sizes whose halving does not end on a tiny level
+    (10 -> 5, 35 odd, 15 odd: conv_test's ladder halves to 35, 45, 63, 105
+    or 315) coarsen through a resampled 2^k / 3 2^k level; odd sizes are
+    swept lexicographically. The solve still equals the direct one."""
+    n = m
+    R = ops.periodic_density(m, n, True, True, seed=6)
+    rng = np.random.default_rng(9)
+    dx, dy = 2.0 / m, 2.0 / n
+    if kind == "mac":
+        irho = ops.mac_irho_from_rho(R, m, n)
+        A = ops.mac_matrix(m, n, dx, dy, irho, True, True)
+        b = rng.standard_normal(m * n)
+        z, cyc, rr = mac_solve(m, n, dx, dy, irho, b - b.mean(), 1e-12, True, True)
+    else:
+        irho = ops.fem_irho_periodic(R, m, n, True, True)
+        A = ops.fem_matrix(m, n, dx, dy, irho, True, True)
+        b = rng.standard_normal(m * n)
+        z, cyc, rr = fem_solve(m, n, dx, dy, irho, b - b.mean(), 1e-12, True, True)
+    assert 0 < cyc < 60 and rr <= 1e-12, (cyc, rr)
+    zd = ops.neumann_solve(A, b)
+    assert np.abs((z - z.mean()) - zd).max() <= 1e-9 * np.abs(zd).max()
