@@ -242,6 +242,12 @@ int incrmt_fluid_time_fine_smoother(incrmt_fluid *f, int kind, int sweeps, int p
 int incrmt_fluid_time_projections(incrmt_fluid *f, int enable);
 /* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
+/* No reference counterpart (the reference sweeps every solid over the whole
+ * grid, fluid_2d.cc:915-950, 1082-1108, 1242-1273, 510-560): with `on` (the
+ * default) each solid's per-step sweeps cover only its band's bounding box
+ * widened by a few cells, which gives the same fields bit for bit; off,
+ * they cover the whole grid. Takes effect at the next band rebuild. */
+int incrmt_fluid_set_solid_windows(incrmt_fluid *f, int on);
 void incrmt_fluid_destroy(incrmt_fluid *f);
 
 /* ---- tracers -------------------------------------------------------------

@@ -81,6 +81,7 @@ _PROTOS = [
     ("incrmt_fluid_sync", _I, [_V]),
     ("incrmt_fluid_get_stats", _I, [_V, ctypes.POINTER(Stats)]),
     ("incrmt_fluid_time_smoothers", _I, [_V, _I]),
+    ("incrmt_fluid_set_solid_windows", _I, [_V, _I]),
     ("incrmt_fluid_destroy", None, [_V]),
     ("incrmt_fluid_set_ntrace", _I, [_V, _I]),
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
@@ -363,6 +364,12 @@ class Fluid2D:
 
     def time_smoothers(self, enable=True):
         _check(lib().incrmt_fluid_time_smoothers(self._h, int(enable)))
+
+    def set_solid_windows(self, on=True):
+        """Per-solid sweeps over each solid's band box (default) or the
+        whole grid; the fields are the same bit for bit. Set before
+        initialize to have no windowed step at all."""
+        _check(lib().incrmt_fluid_set_solid_windows(self._h, int(on)))
 
     def write_frame(self, directory, sn, fflags=8 | 16 | 64 | 128 | 256 | 512):
         """fluid_2d::write_files(sn): the fflags-selected fields as float32

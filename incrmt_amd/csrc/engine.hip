@@ -102,6 +102,10 @@ struct Solid {
     ObjArrays a;
     LsWork w;
     double r_scale = 0;  // obj_simple_spin's r_scale (its tracer starts at 0.9 r_scale)
+    // steps left whose sweeps cover the whole grid: after an upload of the
+    // solid's fields, one full step restores X = nX outside the band,
+    // which windowed sweeps rely on
+    int full_steps = 0;
 };
 
 struct FluidEngine {
@@ -112,6 +116,8 @@ struct FluidEngine {
     std::vector<Solid> obj;
     ObjConst *d_oc = nullptr;
     ObjArrays *d_oa = nullptr;
+    SolidWin *d_win = nullptr;  // per-solid sweep windows (device)
+    bool windows = true;        // windowed per-solid sweeps (else the whole grid)
     Fields f{};
     Phys ph{};
     std::vector<void *> allocs;
@@ -182,7 +188,11 @@ struct FluidEngine {
         p = (double *)q;
         return 0;
     }
-    Solids solids() const { return Solids{d_oc, d_oa, (int)obj.size()}; }
+    Solids solids() const { return Solids{d_oc, d_oa, (int)obj.size(), d_win}; }
+    // solid k's sweep window after a rebuild of its band (full: the whole grid)
+    int update_window(size_t k, bool full = false) {
+        return launch_solid_window(s, g, obj[k].w.bbox, d_win + k, full || !windows || obj[k].full_steps > 0);
+    }
 
     // fluid_2d::choose_ev_and_dt + obj_field::set_ex_visc (fluid_2d.cc:152-192, obj_field.cc:35-49)
     void choose_ev_and_dt(double ev_mult, double dt_pad, double dt_ev_pad) {
@@ -435,8 +445,10 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (!obj.empty()) {
         HIP_CHECK(hipMalloc((void **)&d_oc, obj.size() * sizeof(ObjConst)));
         HIP_CHECK(hipMalloc((void **)&d_oa, obj.size() * sizeof(ObjArrays)));
+        HIP_CHECK(hipMalloc((void **)&d_win, obj.size() * sizeof(SolidWin)));
         allocs.push_back(d_oc);
         allocs.push_back(d_oa);
+        allocs.push_back(d_win);
         std::vector<ObjConst> hc;
         std::vector<ObjArrays> ha;
         for (auto &o : obj) { hc.push_back(o.c); ha.push_back(o.a); }
@@ -458,9 +470,11 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (launch_req >= 0) mac.use_graphs = fem.use_graphs = launch_req == INCRMT_LAUNCH_GRAPH;
     if ((guess_mode == INCRMT_GUESS_LEARNED || guess_train) && (rc0 = ensure_guess())) return rc0;
     // fluid_2d::init_fields (fluid_2d.cc:195-254)
-    for (auto &o : obj) {
+    for (size_t k = 0; k < obj.size(); k++) {
+        Solid &o = obj[k];
         int passes;
         if (int rc = ls_rebuild(s, g, o.c, o.a, o.w, &passes)) return rc;
+        if (int rc = update_window(k)) return rc;
     }
     if (int rc = launch_init_velocity(s, g, f, solids(), rhof)) return rc;
     if (int rc = launch_set_boundaries(s, g, f, solids())) return rc;
@@ -473,9 +487,11 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
 }
 
 int FluidEngine::pin_all(int *passes) {
-    for (auto &o : obj) {
+    for (size_t k = 0; k < obj.size(); k++) {
+        Solid &o = obj[k];
         int p = 0;
         if (int rc = ls_pin_rebuild(s, g, o.c, o.a, o.w, &p)) return rc;
+        if (int rc = update_window(k)) return rc;
         if (passes) *passes = p;
     }
     if (!const_rho) return set_rho();
@@ -548,6 +564,8 @@ int FluidEngine::step(double dt) {
     }
     if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return rc;
     if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
+    for (auto &ob : obj)
+        if (ob.full_steps > 0) ob.full_steps--;
     if ((rc = launch_set_boundaries(s, g, f, o))) return rc;
     if ((rc = compute_stress())) return rc;
     if ((rc = tracers_stage2(s, g, f.u, f.v, d_oa, tr, dt, time))) return rc;
@@ -864,8 +882,11 @@ int incrmt_fluid_upload(incrmt_fluid *f, const char *name, int obj, const double
     FluidEngine &e = f->e;
     HIP_CHECK(hipStreamSynchronize(e.s));
     const std::string nm = name;
-    if (nm == "X" || nm == "Y" || nm == "nX" || nm == "nY" || nm == "phi" || nm == "s")
+    if (nm == "X" || nm == "Y" || nm == "nX" || nm == "nY" || nm == "phi" || nm == "s") {
         ls_reset_window(e.obj[obj].w);  // the next band rebuild of this solid covers the whole grid
+        e.obj[obj].full_steps = 1;          // and its sweeps the whole grid for one step
+        if (int rc = e.update_window(obj, true)) return rc;
+    }
     if (!is_int) return copy_field(e, p, const_cast<double *>(host), false);
     std::vector<int> tmp(e.g.N, 0);
     HIP_CHECK(hipMemcpy(tmp.data(), p, e.g.N * sizeof(int), hipMemcpyDeviceToHost));
@@ -903,6 +924,12 @@ int incrmt_fluid_time_projections(incrmt_fluid *f, int enable) {
     if (int rc = f->e.proj_collect()) return rc;
     f->e.time_proj = enable != 0;
     f->e.proj_ms = 0;
+    return 0;
+}
+
+int incrmt_fluid_set_solid_windows(incrmt_fluid *f, int on) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    f->e.windows = on != 0;
     return 0;
 }
 

@@ -34,10 +34,30 @@ struct Phys {
     const double *xrow;       // x of cell i formed as compute_ustar forms it
 };
 
+// A solid's sweep window: the cells [i0, i1) x [j0, j1) (ghost cells are
+// -2, -1 and m, m+1) outside which the solid leaves nothing to compute.
+// Written on the device after every band rebuild (launch_solid_window):
+// the band's bounding box (statuses 1..6) widened by kWinMargin cells when
+// that stays clear of the grid's edges, else the whole ghosted grid (a band
+// touching an edge, a periodic grid, no band). Inside a closed interface the
+// solid's interior lies in that box, so outside the window every point of
+// the solid is far outside it (status 7, phi at the positive cut-off, above
+// the transition half-width): no transition function, wall force,
+// stress, collision, reference-map update or density term of the solid is
+// non-zero there.
+struct SolidWin {
+    int i0, i1, j0, j1;
+};
+constexpr int kWinMargin = 6;
+__device__ inline bool in_win(const SolidWin &w, int i, int j) {
+    return i >= w.i0 && i < w.i1 && j >= w.j0 && j < w.j1;
+}
+
 struct Solids {
     const ObjConst *oc;  // device
     const ObjArrays *oa; // device
     int n;
+    const SolidWin *win; // device, n windows
 };
 
 int launch_normal_derivs(hipStream_t s, const Grid &g, const Fields &f, Solids o, double dt);
@@ -59,6 +79,9 @@ int launch_pressure_velocity(hipStream_t s, const Grid &g, const Fields &f, Soli
 int launch_set_boundaries(hipStream_t s, const Grid &g, const Fields &f, Solids o);
 int launch_set_boundaries_solid(hipStream_t s, const Grid &g, Solids o);
 int launch_init_velocity(hipStream_t s, const Grid &g, const Fields &f, Solids o, double rhof);
+// solid k's window from the band box `bbox` (level-set coordinates, as the
+// rebuild leaves it on the device), or the whole grid (full)
+int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full);
 
 }  // namespace incrmt
 

@@ -54,14 +54,34 @@ __global__ void k_normal_derivs(Grid g, Fields f, Solids o, double dt) {
     f.eA[VD][ij] = vc - 0.5 * (g.dy + dt * vc) * vy;
     f.eA[UU][ij] = uc + 0.5 * (g.dy - dt * vc) * uy;
     f.eA[VU][ij] = vc + 0.5 * (g.dy - dt * vc) * vy;
-    for (int k = 0; k < o.n; k++) {
-        const ObjArrays &a = o.oa[k];
+}
+
+// Per-solid sweeps over the solid's window only: workgroup (x, z) walks the
+// BX x BY tiles x, x + gridDim.x, ... of solid z's window (clipped to the
+// cells [0, m) x [0, n)).
+constexpr int kWinBlocks = 96;
+template <class F>
+__device__ inline void window_cells(const Grid &g, const SolidWin &w0, F f) {
+    const int i0 = max(w0.i0, 0), i1 = min(w0.i1, g.m), j0 = max(w0.j0, 0), j1 = min(w0.j1, g.n);
+    const int tw = (i1 - i0 + BX - 1) / BX, th = (j1 - j0 + BY - 1) / BY;
+    for (int t = blockIdx.x; t < tw * th; t += gridDim.x) {
+        const int i = i0 + (t % tw) * BX + threadIdx.x, j = j0 + (t / tw) * BY + threadIdx.y;
+        if (i < i1 && j < j1) f(i, j);
+    }
+}
+
+// fluid_2d::normal_derivatives_refmap's slopes (fluid_2d.cc:915-950)
+__global__ void k_refmap_slopes(Grid g, Solids o) {
+    const int k = blockIdx.z, P = g.P;
+    const ObjArrays a = o.oa[k];
+    window_cells(g, o.win[k], [&](int i, int j) {
+        const int ij = g.at(i, j);
         const double *X = a.X, *Y = a.Y;
         a.Xx[ij] = g.xsp * mono_diff(X[ij - 2], X[ij - 1], X[ij], X[ij + 1], X[ij + 2]);
         a.Yx[ij] = g.xsp * mono_diff(Y[ij - 2], Y[ij - 1], Y[ij], Y[ij + 1], Y[ij + 2]);
         a.Xy[ij] = g.ysp * mono_diff(X[ij - 2 * P], X[ij - P], X[ij], X[ij + P], X[ij + 2 * P]);
         a.Yy[ij] = g.ysp * mono_diff(Y[ij - 2 * P], Y[ij - P], Y[ij], Y[ij + P], Y[ij + 2 * P]);
-    }
+    });
 }
 
 // ---- fluid_2d::godunov_upwinding_set<tang> + edge_boundary_conditions
@@ -82,23 +102,12 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         if (ur + ulr > 0 && ur > 0) {
             e[UL][ij + 1] = ur;
             e[VL][ij + 1] = e[VR][ij];
-            if (!TANG)
-                for (int k = 0; k < o.n; k++) {
-                    const ObjArrays &a = o.oa[k];
-                    a.Xl[ij + 1] = a.Xr[ij]; a.Yl[ij + 1] = a.Yr[ij];
-                }
         } else if (ur + ulr < 0 && ulr < 0) {
             if (TANG) e[UR][ij] = ulr;
         } else {
             if (TANG) e[UR][ij] = 0.5 * (ur + ulr);
             e[UL][ij + 1] = 0;
             e[VL][ij + 1] = 0.5 * (e[VR][ij] + e[VL][ij + 1]);
-            if (!TANG)
-                for (int k = 0; k < o.n; k++) {
-                    const ObjArrays &a = o.oa[k];
-                    a.Xl[ij + 1] = 0.5 * (a.Xr[ij] + a.Xl[ij + 1]);
-                    a.Yl[ij + 1] = 0.5 * (a.Yr[ij] + a.Yl[ij + 1]);
-                }
         }
     }
     if (j < g.n - 1) {
@@ -106,23 +115,12 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
         if (vu + vdu > 0 && vu > 0) {
             e[VD][ij + P] = vu;
             e[UD][ij + P] = e[UU][ij];
-            if (!TANG)
-                for (int k = 0; k < o.n; k++) {
-                    const ObjArrays &a = o.oa[k];
-                    a.Xd[ij + P] = a.Xu[ij]; a.Yd[ij + P] = a.Yu[ij];
-                }
         } else if (vu + vdu < 0 && vdu < 0) {
             if (TANG) e[VU][ij] = vdu;
         } else {
             if (TANG) e[VU][ij] = 0.5 * (vu + vdu);
             e[VD][ij + P] = 0;
             e[UD][ij + P] = 0.5 * (e[UU][ij] + e[UD][ij + P]);
-            if (!TANG)
-                for (int k = 0; k < o.n; k++) {
-                    const ObjArrays &a = o.oa[k];
-                    a.Xd[ij + P] = 0.5 * (a.Xu[ij] + a.Xd[ij + P]);
-                    a.Yd[ij + P] = 0.5 * (a.Yu[ij] + a.Yd[ij + P]);
-                }
         }
     }
     // no-slip wall faces (fluid_2d.cc:754-776)
@@ -225,6 +223,37 @@ __global__ void k_godunov(Grid g, Fields f, Solids o) {
     }
 }
 
+// The reference-map half of godunov_upwinding_set<false> on the interior
+// faces (fluid_2d.cc:642-686, 1112-1121): per solid over its window, run
+// before the velocity pass so that it reads the same unmodified face
+// velocities that pass decides on.
+__global__ void k_refmap_upwind(Grid g, Fields f, Solids o) {
+    const int k = blockIdx.z, P = g.P;
+    const ObjArrays a = o.oa[k];
+    double *const *e = f.eB;
+    window_cells(g, o.win[k], [&](int i, int j) {
+        const int ij = g.at(i, j);
+        if (i < g.m - 1) {
+            const double ur = e[UR][ij], ulr = e[UL][ij + 1];
+            if (ur + ulr > 0 && ur > 0) {
+                a.Xl[ij + 1] = a.Xr[ij]; a.Yl[ij + 1] = a.Yr[ij];
+            } else if (!(ur + ulr < 0 && ulr < 0)) {
+                a.Xl[ij + 1] = 0.5 * (a.Xr[ij] + a.Xl[ij + 1]);
+                a.Yl[ij + 1] = 0.5 * (a.Yr[ij] + a.Yl[ij + 1]);
+            }
+        }
+        if (j < g.n - 1) {
+            const double vu = e[VU][ij], vdu = e[VD][ij + P];
+            if (vu + vdu > 0 && vu > 0) {
+                a.Xd[ij + P] = a.Xu[ij]; a.Yd[ij + P] = a.Yu[ij];
+            } else if (!(vu + vdu < 0 && vdu < 0)) {
+                a.Xd[ij + P] = 0.5 * (a.Xu[ij] + a.Xd[ij + P]);
+                a.Yd[ij + P] = 0.5 * (a.Yu[ij] + a.Yd[ij + P]);
+            }
+        }
+    });
+}
+
 // ---- fluid_2d::acceleration + wall_force + object::accel
 // (fluid_2d.cc:1019-1078, object.cc:31-34, 188-197)
 __device__ inline void wall_force(const Grid &g, const Phys &ph, double x, double y, double phiv,
@@ -242,7 +271,7 @@ __device__ inline void wall_force(const Grid &g, const Phys &ph, double x, doubl
     }
 }
 
-__device__ inline void acceleration(const Grid &g, const Fields &f, const Solids &o, const Phys &ph,
+__device__ inline void acceleration(const Grid &g, const Fields &f, const Solids &o, const Phys &ph, int i, int j,
                                     int ij, double &accx, double &accy, double x, double y, bool pressure) {
     const int P = g.P;
     double accx_ = 0, accy_ = 0;
@@ -254,6 +283,7 @@ __device__ inline void acceleration(const Grid &g, const Fields &f, const Solids
         accy -= 0.5 * g.ysp * (p[ij + P + 1] - p[ij + 1] + p[ij + P] - p[ij]);
     }
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], i, j)) continue;
         const ObjConst &c = o.oc[k];
         const double phiv = o.oa[k].phi[ij];
         if (phiv < c.eps && !g.spbc) {  // fluid_2d.cc:1037
@@ -297,7 +327,7 @@ __global__ void k_tang_half(Grid g, Fields f, Solids o, Phys ph, double dt) {
     const double c3 = v_adv * (A[VU][ij] - A[VU][ij - P]);
     const double y = g.ay + g.dy * (0.5 + j), x = g.ax + g.dx * (0.5 + i);
     double accx, accy;
-    acceleration(g, f, o, ph, ij, accx, accy, x, y, true);
+    acceleration(g, f, o, ph, i, j, ij, accx, accy, x, y, true);
     const double u = f.u[ij], v = f.v[ij], ux = f.ux[ij], vx = f.vx[ij], uy = f.uy[ij], vy = f.vy[ij];
     double *const *B = f.eB;
     B[UL][ij] = u + 0.5 * (-g.dx * ux + dt * (-u * ux - c2 + accx));
@@ -308,8 +338,16 @@ __global__ void k_tang_half(Grid g, Fields f, Solids o, Phys ph, double dt) {
     B[VD][ij] = v + 0.5 * (-g.dy * vy + dt * (-v * vy - c1 + accy));
     B[UU][ij] = u + 0.5 * (g.dy * uy + dt * (-v * uy - c0 + accx));
     B[VU][ij] = v + 0.5 * (g.dy * vy + dt * (-v * vy - c1 + accy));
-    for (int k = 0; k < o.n; k++) {
-        const ObjArrays &a = o.oa[k];
+}
+
+// compute_half_time_edge_reference_map (fluid_2d.cc:1082-1108), per solid
+// over its window
+__global__ void k_refmap_half(Grid g, Fields f, Solids o, double dt) {
+    const int k = blockIdx.z;
+    const ObjArrays a = o.oa[k];
+    window_cells(g, o.win[k], [&](int i, int j) {
+        const int ij = g.at(i, j);
+        const double u = f.u[ij], v = f.v[ij];
         const double X = a.X[ij], Y = a.Y[ij], Xx = a.Xx[ij], Xy = a.Xy[ij], Yx = a.Yx[ij], Yy = a.Yy[ij];
         double Xdt = dt * (-u * Xx - v * Xy), Ydt = dt * (-u * Yx - v * Yy);
         a.Xl[ij] = X + 0.5 * (-g.dx * Xx + Xdt);
@@ -320,7 +358,7 @@ __global__ void k_tang_half(Grid g, Fields f, Solids o, Phys ph, double dt) {
         a.Yd[ij] = Y + 0.5 * (-g.dy * Yy + Ydt);
         a.Xu[ij] = X + 0.5 * (g.dy * Xy + Xdt);
         a.Yu[ij] = Y + 0.5 * (g.dy * Yy + Ydt);
-    }
+    });
 }
 
 // ---- MAC source: the divergence of the half-time face velocities
@@ -367,6 +405,7 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
     (void)q;
     const double hx = 0.5 * dt * g.xsp, hy = 0.5 * dt * g.ysp;
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         if (a.st[ij] >= 4) continue;
         double cX = -hx * (ulr + ulc) * (a.Xl[ij + 1] - a.Xl[ij]) - hy * (vdu + vdc) * (a.Xd[ij + P] - a.Xd[ij]);
@@ -386,6 +425,7 @@ __global__ void k_half_timestep(Grid g, Solids o) {
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j);
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         if (a.st[ij] < 6) {
             a.X[ij] = 0.5 * (a.X[ij] + a.nX[ij]);
@@ -401,6 +441,7 @@ __global__ void k_set_rho(Grid g, Fields f, Solids o, double rhof) {
     const int ij = g.ls(I, J);
     double rho = 0, sfrac = 0;
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], I - 2, J - 2)) continue;
         double tf = trans_func(o.oc[k], o.oa[k].phi[ij]);
         sfrac += tf;
         rho += tf * o.oc[k].rhos;
@@ -410,13 +451,17 @@ __global__ void k_set_rho(Grid g, Fields f, Solids o, double rhof) {
 
 // ---- fluid_2d::compute_stress (fluid_2d.cc:381-600) on the (m+1)x(n+1)
 // left and down edges, with solid_stress, fluid_stress and collision_stress.
+// (a solid takes part at an edge only inside its window: the edge's own
+// cell (i, j) is in it, and with it, for any edge where the solid is
+// present, the neighbour across the edge)
 template <bool LEFT>
-__device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids &o, const Phys &ph,
+__device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids &o, const Phys &ph, int i, int j,
                                    int ij, double &a, double &b) {
     const int P = g.P, d = LEFT ? 1 : P;
     double s1s = 0, s2s = 0, sfrac = 0;
     int ncol = 0;
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], i, j)) continue;
         const ObjConst &c = o.oc[k];
         const ObjArrays &A = o.oa[k];
         double phiv = 0.5 * (A.phi[ij] + A.phi[ij - d]);
@@ -498,10 +543,12 @@ __device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids 
         };
         const double small_number = 100. * 2.220446049250313e-16;
         for (int k1 = 0; k1 < o.n; k1++) {
+            if (!in_win(o.win[k1], i, j)) continue;
             if (0.5 * (o.oa[k1].phi[ij] + o.oa[k1].phi[ij - d]) > o.oc[k1].eps) continue;
             double p1, p1x, p1y;
             edge(k1, p1, p1x, p1y);
             for (int k2 = k1 + 1; k2 < o.n; k2++) {
+                if (!in_win(o.win[k2], i, j)) continue;
                 if (0.5 * (o.oa[k2].phi[ij] + o.oa[k2].phi[ij - d]) > o.oc[k2].eps) continue;
                 double p2, p2x, p2y;
                 edge(k2, p2, p2x, p2y);
@@ -525,9 +572,9 @@ __global__ void k_stress(Grid g, Fields f, Solids o, Phys ph) {
     if (i > g.m || j > g.n) return;
     const int ij = g.at(i, j);
     double a, b;
-    edge_stress<true>(g, f, o, ph, ij, a, b);
+    edge_stress<true>(g, f, o, ph, i, j, ij, a, b);
     f.s11[ij] = a; f.s21[ij] = b;
-    edge_stress<false>(g, f, o, ph, ij, a, b);
+    edge_stress<false>(g, f, o, ph, i, j, ij, a, b);
     f.s12[ij] = a; f.s22[ij] = b;
 }
 
@@ -540,7 +587,7 @@ __global__ void k_ustar(Grid g, Fields f, Solids o, Phys ph, double dt) {
     const int ij = g.at(i, j), P = g.P;
     const double y = g.ay + g.dy * (.5 + j), x = ph.xrow[i];
     double accx, accy;
-    acceleration(g, f, o, ph, ij, accx, accy, x, y, false);
+    acceleration(g, f, o, ph, i, j, ij, accx, accy, x, y, false);
     const double hx = 0.5 * dt * g.xsp, hy = 0.5 * dt * g.ysp;
     const double *ul = f.ulc, *vl = f.eB[VL], *ud = f.eB[UD], *vd = f.vdc;
     f.c0[ij] = f.u[ij] - hx * (ul[ij + 1] + ul[ij]) * (ul[ij + 1] - ul[ij]) - hy * (vd[ij + P] + vd[ij]) * (ud[ij + P] - ud[ij]) + dt * accx;
@@ -595,7 +642,9 @@ __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z,
     const double rhoinv = 1. / f.rho[ij], u_star = f.c0[ij], v_star = f.c1[ij];
     f.u[ij] = u_star - 0.5 * dt * rhoinv * g.xsp * (p11 + p10 - p01 - p00);
     f.v[ij] = v_star - 0.5 * dt * rhoinv * g.ysp * (p11 - p10 + p01 - p00);
+    // (outside a solid's window X = nX already: neither changes there)
     for (int k = 0; k < o.n; k++) {
+        if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         a.X[ij] = a.nX[ij];
         a.Y[ij] = a.nY[ij];
@@ -729,9 +778,17 @@ __global__ void k_init_velocity(Grid g, Fields f, Solids o, double rhof) {
 int launch_normal_derivs(hipStream_t s, const Grid &g, const Fields &f, Solids o, double dt) {
     k_normal_derivs<<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o, dt);
     LAUNCH_CHECK();
+    if (o.n) {
+        k_refmap_slopes<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, o);
+        LAUNCH_CHECK();
+    }
     return 0;
 }
 int launch_godunov(hipStream_t s, const Grid &g, const Fields &f, Solids o, bool tang) {
+    if (!tang && o.n) {
+        k_refmap_upwind<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o);
+        LAUNCH_CHECK();
+    }
     if (tang) k_godunov<true><<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o);
     else k_godunov<false><<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o);
     LAUNCH_CHECK();
@@ -739,6 +796,29 @@ int launch_godunov(hipStream_t s, const Grid &g, const Fields &f, Solids o, bool
 }
 int launch_tang_half(hipStream_t s, const Grid &g, const Fields &f, Solids o, const Phys &ph, double dt) {
     k_tang_half<<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o, ph, dt);
+    LAUNCH_CHECK();
+    if (o.n) {
+        k_refmap_half<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o, dt);
+        LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+// ---- a solid's sweep window from its band's box (see SolidWin)
+__global__ void k_solid_window(Grid g, const int *bbox, SolidWin *win, int full) {
+    if (threadIdx.x) return;
+    const SolidWin all{-2, g.m + 2, -2, g.n + 2};
+    if (full || g.px || g.py || bbox[1] < 0) {
+        *win = all;
+        return;
+    }
+    // level-set coordinates are cell coordinates + 2
+    const int i0 = bbox[0] - 2 - kWinMargin, i1 = bbox[1] - 2 + 1 + kWinMargin;
+    const int j0 = bbox[2] - 2 - kWinMargin, j1 = bbox[3] - 2 + 1 + kWinMargin;
+    *win = (i0 < 0 || j0 < 0 || i1 > g.m || j1 > g.n) ? all : SolidWin{i0, i1, j0, j1};
+}
+int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full) {
+    k_solid_window<<<1, 64, 0, s>>>(g, bbox, win, full);
     LAUNCH_CHECK();
     return 0;
 }
