@@ -243,10 +243,13 @@ int incrmt_fluid_time_projections(incrmt_fluid *f, int enable);
 /* Record HIP events around every fine-level smoother launch (stats.smoother_ms). */
 int incrmt_fluid_time_smoothers(incrmt_fluid *f, int enable);
 /* No reference counterpart (the reference sweeps every solid over the whole
- * grid, fluid_2d.cc:915-950, 1082-1108, 1242-1273, 510-560): with `on` (the
- * default) each solid's per-step sweeps cover only its band's bounding box
- * widened by a few cells, which gives the same fields bit for bit; off,
- * they cover the whole grid. Takes effect at the next band rebuild. */
+ * grid, fluid_2d.cc:915-950, 1082-1108, 1242-1273, 510-560, and rebuilds
+ * one solid's band after another): with `on` (the default) each solid's
+ * per-step sweeps cover only its band's bounding box widened by a few
+ * cells, and the band passes of all solids run in shared launches, which
+ * gives the same fields bit for bit; off, the sweeps cover the whole grid
+ * and each solid's band is rebuilt alone. Takes effect at the next band
+ * rebuild. */
 int incrmt_fluid_set_solid_windows(incrmt_fluid *f, int on);
 void incrmt_fluid_destroy(incrmt_fluid *f);
 

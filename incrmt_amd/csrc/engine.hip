@@ -117,7 +117,9 @@ struct FluidEngine {
     ObjConst *d_oc = nullptr;
     ObjArrays *d_oa = nullptr;
     SolidWin *d_win = nullptr;  // per-solid sweep windows (device)
-    bool windows = true;        // windowed per-solid sweeps (else the whole grid)
+    bool windows = true;        // windowed per-solid sweeps and batched band passes (else
+                                // whole-grid sweeps and one solid's passes at a time)
+    LsBatch lsb{};              // batched band passes of the solids
     Fields f{};
     Phys ph{};
     std::vector<void *> allocs;
@@ -177,6 +179,7 @@ struct FluidEngine {
         for (auto &e : pev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (UNet *n : gnet) delete n;
         for (auto &o : obj) ls_free(o.w);
+        ls_batch_free(lsb);
         for (void *p : allocs) (void)hipFree(p);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -190,6 +193,11 @@ struct FluidEngine {
     }
     Solids solids() const { return Solids{d_oc, d_oa, (int)obj.size(), d_win}; }
     // solid k's sweep window after a rebuild of its band (full: the whole grid)
+    std::vector<LsJob> jobs() {
+        std::vector<LsJob> j;
+        for (auto &o : obj) j.push_back(LsJob{&o.c, &o.a, &o.w});
+        return j;
+    }
     int update_window(size_t k, bool full = false) {
         return launch_solid_window(s, g, obj[k].w.bbox, d_win + k, full || !windows || obj[k].full_steps > 0);
     }
@@ -487,6 +495,17 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
 }
 
 int FluidEngine::pin_all(int *passes) {
+    if (obj.size() > 1 && windows) {
+        if (!lsb.cap)
+            if (int rc = ls_batch_alloc((int)obj.size(), lsb)) return rc;
+        int p = 0;
+        if (int rc = ls_pin_rebuild_batch(s, g, jobs(), lsb, &p)) return rc;
+        for (size_t k = 0; k < obj.size(); k++)
+            if (int rc = update_window(k)) return rc;
+        if (passes) *passes = p;
+        if (!const_rho) return set_rho();
+        return 0;
+    }
     for (size_t k = 0; k < obj.size(); k++) {
         Solid &o = obj[k];
         int p = 0;
@@ -524,10 +543,19 @@ int FluidEngine::step(double dt) {
     st.mac_cycles = ms.cycles; st.mac_rel_res = ms.rel_res; st.total_mac_cycles += ms.cycles;
     if ((rc = launch_mac_correct_predict(s, g, f, o, ml.z, ml.pitch, ml.fw, ml.fs, ml.cp, const_rho, dt))) return rc;
     // update_reference_map (fluid_2d.cc:1242-1273)
-    for (auto &ob : obj) {
+    if (obj.size() > 1 && windows) {
         int p = 0;
-        if ((rc = ls_extend(s, g, ob.a, ob.w, &p))) return rc == 5 ? fail(INCRMT_ERR_SINGULAR_FIT, "singular extension fit") : rc;
+        if (!lsb.cap && (rc = ls_batch_alloc((int)obj.size(), lsb))) return rc;
+        if ((rc = ls_extend_batch(s, g, jobs(), lsb, &p)))
+            return rc == 5 ? fail(INCRMT_ERR_SINGULAR_FIT, "singular extension fit") : rc;
         st.extend_passes = p;
+    } else {
+        for (auto &ob : obj) {
+            int p = 0;
+            if ((rc = ls_extend(s, g, ob.a, ob.w, &p)))
+                return rc == 5 ? fail(INCRMT_ERR_SINGULAR_FIT, "singular extension fit") : rc;
+            st.extend_passes = p;
+        }
     }
     if ((rc = launch_half_timestep(s, g, o))) return rc;
     int passes = 0;

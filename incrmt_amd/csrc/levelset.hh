@@ -3,6 +3,7 @@
 #ifndef INCRMT_LEVELSET_HH
 #define INCRMT_LEVELSET_HH
 
+#include <vector>
 #include "common.hh"
 
 namespace incrmt {
@@ -47,6 +48,28 @@ struct LsWork {
 
 int ls_alloc(const Grid &g, LsWork &w);
 void ls_free(LsWork &w);
+
+// Rebuilds and extensions of several solids of one grid run their passes
+// together: one launch per pass for every solid, one counter read-back per
+// batch of passes. Every solid gets exactly what the single-solid call
+// gives it. The batch work area holds up to `cap` solids.
+struct LsJob {
+    const ObjConst *c;
+    const ObjArrays *a;
+    LsWork *w;
+};
+struct LsBatch {
+    void *d_items = nullptr, *h_items = nullptr;
+    int *d_cnt = nullptr, *h_cnt = nullptr;
+    int cap = 0;
+};
+int ls_batch_alloc(int cap, LsBatch &bt);
+void ls_batch_free(LsBatch &bt);
+// ls_pin_rebuild of every job; *fim_iters: the most passes any took
+int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                         int *fim_iters);
+// ls_extend of every job; *ext_iters: the most passes any took
+int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt, int *ext_iters);
 
 // obj_field::pin + levelset::build_band (obj_field.cc:92-97,
 // levelset.cc:199-335). Returns 0, or an INCRMT_ERR_* code.

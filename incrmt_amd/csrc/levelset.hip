@@ -17,6 +17,7 @@
 // alone, so a Jacobi iteration over the outside band reaches the sequential
 // result exactly once every dependency chain has been walked; it stops when
 // an iteration changes no value.
+#include <algorithm>
 #include <cstdio>
 #include "levelset.hh"
 #include "eikonal.hh"
@@ -380,8 +381,8 @@ struct LdsAcc {
     __device__ double val(int q) const { return sg * v[q]; }
 };
 
-__global__ __launch_bounds__(kTileThreads) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
-                                                   const double *src, double *dst, int slot, int max_local) {
+__device__ __forceinline__ void fim_tiles(const Grid &g, const EikGeom &eg, const ObjConst &c, const ObjArrays &a,
+                                          const LsWork &w, const double *src, double *dst, int slot, int max_local) {
     __shared__ double sv[2][kFN];
     __shared__ int sst[kFN];
     if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
@@ -437,6 +438,25 @@ __global__ __launch_bounds__(kTileThreads) void k_fim_tiles(Grid g, EikGeom eg, 
         __syncthreads();
     }
     if (__any(changed_any) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
+}
+__global__ __launch_bounds__(kTileThreads) void k_fim_tiles(Grid g, EikGeom eg, ObjConst c, ObjArrays a, LsWork w,
+                                                   const double *src, double *dst, int slot, int max_local) {
+    fim_tiles(g, eg, c, a, w, src, dst, slot, max_local);
+}
+// The march passes of several solids in one launch: workgroup (x, y) runs
+// the pass of solid y with x as its block index, on its own buffers and
+// counters (pass p reads buf[p & 1], writes the other), so each solid's
+// passes are exactly those it runs alone.
+struct FimItem {
+    ObjConst c;
+    ObjArrays a;
+    LsWork w;
+    double *buf[2];
+};
+__global__ __launch_bounds__(kTileThreads) void k_fim_tiles_batch(Grid g, EikGeom eg, const FimItem *it, int p,
+                                                         int slot, int max_local) {
+    const FimItem &x = it[blockIdx.y];
+    fim_tiles(g, eg, x.c, x.a, x.w, x.buf[p & 1], x.buf[(p + 1) & 1], slot, max_local);
 }
 
 // Final band (levelset.cc:300-335, 408-413): accepted points get their
@@ -521,9 +541,8 @@ __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
 // read the previous values (sx,sy), inside points their predictor values.
 constexpr int kEH = 7, kEW = kTileX + 2 * kEH, kEHt = kTileY + 2 * kEH, kEN = kEW * kEHt;
 
-__global__ __launch_bounds__(kTileThreads) void k_extend_tiles(Grid g, ObjArrays a, LsWork w, const double *srcx,
-                                                      const double *srcy, double *dstx, double *dsty, int slot,
-                                                      int max_local) {
+__device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, const LsWork &w, const double *srcx,
+                                             const double *srcy, double *dstx, double *dsty, int slot, int max_local) {
     __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
     __shared__ int sst[kEN];
     if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
@@ -660,6 +679,36 @@ __global__ __launch_bounds__(kTileThreads) void k_extend_tiles(Grid g, ObjArrays
     }
     if (__any(changed_any) && (threadIdx.x & 63) == 0) atomicAdd(&w.cnt[kCntIter + slot], 1);
 }
+__global__ __launch_bounds__(kTileThreads) void k_extend_tiles(Grid g, ObjArrays a, LsWork w, const double *srcx,
+                                                      const double *srcy, double *dstx, double *dsty, int slot,
+                                                      int max_local) {
+    extend_tiles(g, a, w, srcx, srcy, dstx, dsty, slot, max_local);
+}
+// Extension passes of several solids in one launch (as k_fim_tiles_batch)
+struct ExtItem {
+    ObjArrays a;
+    LsWork w;
+    double *bx[2], *by[2];
+};
+__global__ __launch_bounds__(kTileThreads) void k_extend_tiles_batch(Grid g, const ExtItem *it, int p, int slot,
+                                                            int max_local) {
+    const ExtItem &x = it[blockIdx.y];
+    extend_tiles(g, x.a, x.w, x.bx[p & 1], x.by[p & 1], x.bx[(p + 1) & 1], x.by[(p + 1) & 1], slot, max_local);
+}
+// Counters of a batch of solids: clear the pass slots of every solid, or
+// gather its error flags and the first `batch` slots into out (batch + 1
+// ints per solid)
+template <class Item>
+__global__ void k_batch_zero(const Item *it, int n) {
+    for (int k = blockIdx.x; k < n; k += gridDim.x)
+        for (int e = threadIdx.x; e < kIterSlots; e += blockDim.x) it[k].w.cnt[kCntIter + e] = 0;
+}
+template <class Item>
+__global__ void k_batch_gather(const Item *it, int n, int batch, int *out) {
+    for (int k = blockIdx.x; k < n; k += gridDim.x)
+        for (int e = threadIdx.x; e <= batch; e += blockDim.x)
+            out[k * (batch + 1) + e] = e == 0 ? it[k].w.cnt[kCntErr] : it[k].w.cnt[kCntIter + e - 1];
+}
 
 // Seed both extension buffers with the current nX,nY of the outside band,
 // or copy the converged buffer back into nX,nY.
@@ -713,7 +762,84 @@ int iterate(hipStream_t s, LsWork &w, int &predict, int max_passes, F pass, int 
     return 7;
 }
 
+// The same iteration for several solids at once: every pass is one launch
+// for all solids not yet at their fixed point (launch(p, slot, items, n)),
+// one gather of their counters per batch. Each solid's passes, results,
+// pass count and prediction are those of iterate on it alone; its error
+// flags go to its host mirror (hcnt[kCntErr]).
+template <class Item, class F>
+int iterate_batch(hipStream_t s, LsBatch &bt, std::vector<Item> items, std::vector<int *> predict, int max_passes,
+                  F launch, std::vector<int> &passes_out) {
+    const size_t n0 = items.size();
+    std::vector<size_t> idx(n0);
+    for (size_t k = 0; k < n0; k++) idx[k] = k;
+    passes_out.assign(n0, 0);
+    Item *d_items = (Item *)bt.d_items, *h_items = (Item *)bt.h_items;
+    int passes = 0;
+    while (!idx.empty()) {
+        if (passes >= max_passes) {
+            for (size_t k : idx) passes_out[k] = passes;
+            return 7;
+        }
+        const int n = (int)idx.size();
+        int batch = 2;
+        for (size_t k : idx) batch = std::max(batch, *predict[k] < 2 ? 2 : *predict[k]);
+        if (batch > kIterSlots) batch = kIterSlots;
+        for (int k = 0; k < n; k++) h_items[k] = items[idx[k]];
+        HIP_CHECK(hipMemcpyAsync(d_items, h_items, n * sizeof(Item), hipMemcpyHostToDevice, s));
+        k_batch_zero<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n);
+        KCHECK();
+        for (int b = 0; b < batch; b++)
+            if (int rc = launch(passes + b, b, d_items, n)) return rc;
+        k_batch_gather<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n, batch, bt.d_cnt);
+        KCHECK();
+        HIP_CHECK(hipMemcpyAsync(bt.h_cnt, bt.d_cnt, (size_t)n * (batch + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<size_t> rest;
+        for (int k = 0; k < n; k++) {
+            const int *c = bt.h_cnt + (size_t)k * (batch + 1);
+            const size_t id = idx[k];
+            items[id].w.hcnt[kCntErr] = c[0];
+            if (c[0]) {
+                passes_out[id] = passes + batch;
+                continue;
+            }
+            int done = -1;
+            for (int b = 0; b < batch && done < 0; b++)
+                if (c[1 + b] == 0) done = b;
+            if (done >= 0) {
+                passes_out[id] = passes + done + 1;
+                *predict[id] = passes + done + 1;
+            } else {
+                rest.push_back(id);
+            }
+        }
+        passes += batch;
+        for (size_t id : rest) *predict[id] = passes + 2;
+        idx.swap(rest);
+    }
+    return 0;
+}
+
 }  // namespace
+
+int ls_batch_alloc(int cap, LsBatch &bt) {
+    const size_t bytes = (size_t)cap * std::max(sizeof(FimItem), sizeof(ExtItem));
+    HIP_CHECK(hipMalloc(&bt.d_items, bytes));
+    HIP_CHECK(hipHostMalloc(&bt.h_items, bytes, hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void **)&bt.d_cnt, (size_t)cap * (kIterSlots + 1) * sizeof(int)));
+    HIP_CHECK(hipHostMalloc((void **)&bt.h_cnt, (size_t)cap * (kIterSlots + 1) * sizeof(int), hipHostMallocDefault));
+    bt.cap = cap;
+    return 0;
+}
+
+void ls_batch_free(LsBatch &bt) {
+    if (bt.d_items) (void)hipFree(bt.d_items);
+    if (bt.h_items) (void)hipHostFree(bt.h_items);
+    if (bt.d_cnt) (void)hipFree(bt.d_cnt);
+    if (bt.h_cnt) (void)hipHostFree(bt.h_cnt);
+    bt = LsBatch{};
+}
 
 int ls_alloc(const Grid &g, LsWork &w) {
     const size_t N = (size_t)g.N;
@@ -785,16 +911,21 @@ static Win next_window(const Grid &g, LsWork &w, int R) {
     return v;
 }
 
-static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
-                          int *fim_iters) {
+// A rebuild in three parts: the set-up (pinning, interface list, Newton
+// closest points, dilation to the march region, active tiles), the march
+// passes, and the finish (final band and statuses, the extension's tiles).
+// *live = false: nothing of the solid is left on the grid (nothing to do).
+static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, Win &wn,
+                         bool &whole, bool *live) {
     const dim3 bb(BX, BY);
     // the march cannot accept a point further than limit/h + 2 cells (max
     // norm) from the interface layer: beyond it every path is longer
     const double reach = fmax(-c.phi_down, c.phi_up) / fmin(g.dx, g.dy) + 3;
     const int R = reach > g.ml + g.nl ? g.ml + g.nl : (int)reach;
-    const bool whole = !w.have_win;
-    const Win wn = next_window(g, w, R);
-    if (wn.I1 <= wn.I0 || wn.J1 <= wn.J0) return 0;  // nothing of this solid left on the grid
+    whole = !w.have_win;
+    wn = next_window(g, w, R);
+    *live = !(wn.I1 <= wn.I0 || wn.J1 <= wn.J0);
+    if (!*live) return 0;
     // the row dilation also covers R rows either side, which the column pass reads
     const Win wr{wn.I0, wn.I1, wn.J0 - R < 0 ? 0 : wn.J0 - R, wn.J1 + R > g.nl ? g.nl : wn.J1 + R};
     if (w.pin_first) {
@@ -813,18 +944,13 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     KCHECK();
     k_compact_tiles<<<64, 256, 0, s>>>(w, 1, w.tiles_fim, kCntTileFim);
     KCHECK();
-    EikGeom eg;
-    eg.init(g.dx, g.dy);
+    return 0;
+}
+
+static int rebuild_finish(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                          const Win &wn, bool whole, int passes) {
+    const dim3 bb(BX, BY);
     double *bufs[2] = {w.va, w.vb};
-    int passes = 0;
-    int rc = iterate(s, w, w.pred_fim, 4096, [&](int p, int slot) -> int {
-        k_fim_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
-        KCHECK();
-        return 0;
-    }, &passes);
-    if (rc) return rc;
-    if (w.hcnt[kCntErr] & kErrNewton) return 3;
-    if (fim_iters) *fim_iters = passes;
     k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
     KCHECK();
     k_bbox_init<<<1, 64, 0, s>>>(w);
@@ -846,6 +972,75 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     KCHECK();
     k_commit_status<<<grid_win(wn), bb, 0, s>>>(g, a, w, wn);
     KCHECK();
+    return 0;
+}
+
+static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                          int *fim_iters) {
+    Win wn;
+    bool whole, live;
+    if (int rc = rebuild_setup(s, g, c, a, w, wn, whole, &live)) return rc;
+    if (!live) return 0;
+    EikGeom eg;
+    eg.init(g.dx, g.dy);
+    double *bufs[2] = {w.va, w.vb};
+    int passes = 0;
+    int rc = iterate(s, w, w.pred_fim, 4096, [&](int p, int slot) -> int {
+        k_fim_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
+        KCHECK();
+        return 0;
+    }, &passes);
+    if (rc) return rc;
+    if (w.hcnt[kCntErr] & kErrNewton) return 3;
+    if (fim_iters) *fim_iters = passes;
+    return rebuild_finish(s, g, c, a, w, wn, whole, passes);
+}
+
+// Workgroups per solid of a batched pass: the single-solid count shared out,
+// at least 64
+static int batch_blocks(int n) { return std::max(64, kTileBlocks / std::max(1, (n + 3) / 4)); }
+
+int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                         int *fim_iters) {
+    const size_t n = jobs.size();
+    if (n > (size_t)bt.cap) return 7;
+    std::vector<Win> wn(n);
+    std::vector<char> whole(n), live(n);
+    std::vector<FimItem> items;
+    std::vector<int *> pred;
+    std::vector<size_t> who;
+    for (size_t k = 0; k < n; k++) {
+        LsWork &w = *jobs[k].w;
+        w.pin_first = true;
+        bool wh, lv;
+        if (int rc = rebuild_setup(s, g, *jobs[k].c, *jobs[k].a, w, wn[k], wh, &lv)) return rc;
+        whole[k] = wh;
+        live[k] = lv;
+        if (!lv) continue;
+        items.push_back(FimItem{*jobs[k].c, *jobs[k].a, w, {w.va, w.vb}});
+        pred.push_back(&w.pred_fim);
+        who.push_back(k);
+    }
+    if (items.empty()) return 0;
+    EikGeom eg;
+    eg.init(g.dx, g.dy);
+    const int nb = batch_blocks((int)items.size());
+    std::vector<int> passes;
+    int rc = iterate_batch(s, bt, items, pred, 4096, [&](int p, int slot, const FimItem *d, int m) -> int {
+        k_fim_tiles_batch<<<dim3(nb, m), kTileThreads, 0, s>>>(g, eg, d, p, slot, 64);
+        KCHECK();
+        return 0;
+    }, passes);
+    if (rc) return rc;
+    int most = 0;
+    for (size_t t = 0; t < who.size(); t++) {
+        const size_t k = who[t];
+        LsWork &w = *jobs[k].w;
+        if (w.hcnt[kCntErr] & kErrNewton) return 3;
+        most = std::max(most, passes[t]);
+        if ((rc = rebuild_finish(s, g, *jobs[k].c, *jobs[k].a, w, wn[k], whole[k], passes[t]))) return rc;
+    }
+    if (fim_iters) *fim_iters = most;
     return 0;
 }
 
@@ -883,6 +1078,40 @@ int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *
     if (ext_iters) *ext_iters = passes;
     k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(a, w, bx[passes & 1], by[passes & 1], 1);
     KCHECK();
+    return 0;
+}
+
+int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt, int *ext_iters) {
+    const size_t n = jobs.size();
+    if (n > (size_t)bt.cap) return 7;
+    std::vector<ExtItem> items;
+    std::vector<int *> pred;
+    for (size_t k = 0; k < n; k++) {
+        LsWork &w = *jobs[k].w;
+        k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(*jobs[k].a, w, w.ex, w.ey, 0);
+        KCHECK();
+        items.push_back(ExtItem{*jobs[k].a, w, {w.ex, w.fx}, {w.ey, w.fy}});
+        pred.push_back(&w.pred_ext);
+    }
+    if (items.empty()) return 0;
+    const int nb = batch_blocks((int)items.size());
+    std::vector<int> passes;
+    int rc = iterate_batch(s, bt, items, pred, 4096, [&](int p, int slot, const ExtItem *d, int m) -> int {
+        k_extend_tiles_batch<<<dim3(nb, m), kTileThreads, 0, s>>>(g, d, p, slot, 128);
+        KCHECK();
+        return 0;
+    }, passes);
+    if (rc) return rc;
+    int most = 0;
+    for (size_t k = 0; k < n; k++) {
+        LsWork &w = *jobs[k].w;
+        if (w.hcnt[kCntErr] & kErrSingular) return 5;
+        most = std::max(most, passes[k]);
+        const int q = passes[k];
+        k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(*jobs[k].a, w, q & 1 ? w.fx : w.ex, q & 1 ? w.fy : w.ey, 1);
+        KCHECK();
+    }
+    if (ext_iters) *ext_iters = most;
     return 0;
 }
 
