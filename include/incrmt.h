@@ -305,6 +305,13 @@ int incrmt_mg_set_irho(incrmt_mg *h, const double *irho);
  * except with a zero-pressure edge). */
 int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, int max_cycles, int precision,
                     int *cycles, double *rel_res);
+/* No reference counterpart (for measuring the solver on the device): `reps`
+ * solves of the last incrmt_mg_solve's right-hand side, kept on the
+ * device, each from a zero start; *ms_per_solve is the mean time between
+ * HIP events recorded on the solver's stream around the solve alone (no
+ * host copy of b or z inside). */
+int incrmt_mg_time_solve(incrmt_mg *h, double rel_tol, int max_cycles, int precision, int reps,
+                         double *ms_per_solve, int *cycles, double *rel_res);
 /* V-cycle issue mode of incrmt_mg_solve (INCRMT_LAUNCH_GRAPH or
  * INCRMT_LAUNCH_DIRECT, default by size as for incrmt_fluid_set_launch). No
  * reference counterpart. */

@@ -101,6 +101,7 @@ _PROTOS = [
     ("incrmt_mg_create_bc", _I, [_I, _I, _I, _I, _I, _I, _D, _D, ctypes.POINTER(_V)]),
     ("incrmt_mg_set_irho", _I, [_V, _dp]),
     ("incrmt_mg_solve", _I, [_V, _dp, _dp, _D, _I, _I, _ip, _dp]),
+    ("incrmt_mg_time_solve", _I, [_V, _D, _I, _I, _I, _dp, _ip, _dp]),
     ("incrmt_mg_apply", _I, [_V, _dp, _dp]),
     ("incrmt_mg_smooth", _I, [_V, _dp, _dp, _I, _I]),
     ("incrmt_mg_smooth_prec", _I, [_V, _dp, _dp, _I, _I, _I]),
@@ -483,6 +484,15 @@ class MgSolver:
         _check(lib().incrmt_mg_solve(self._h, _d(bb), _d(self.z), rel_tol, max_cycles, precision,
                                      ctypes.byref(c), ctypes.byref(r)))
         return self.z.copy(), c.value, r.value
+
+    def time_solve(self, rel_tol=1e-10, max_cycles=200, precision=PREC_FP64, reps=3):
+        """Device time of solves of the last solve's right-hand side (kept
+        on the device) from a zero start: (ms per solve, cycles, relative
+        residual)."""
+        ms, c, r = _D(0), _I(0), _D(0)
+        _check(lib().incrmt_mg_time_solve(self._h, rel_tol, max_cycles, precision, reps, ctypes.byref(ms),
+                                          ctypes.byref(c), ctypes.byref(r)))
+        return ms.value, c.value, r.value
 
     def set_launch(self, mode):
         """LAUNCH_GRAPH or LAUNCH_DIRECT V-cycles (default by size: graphs

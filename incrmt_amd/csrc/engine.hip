@@ -612,6 +612,7 @@ struct incrmt_mg {
     Multigrid mg;
     hipStream_t s = nullptr;
     double *x = nullptr, *y = nullptr;
+    double *bkeep = nullptr;  // the last solve's right-hand side (timed solves)
     // host row length: the solver's nx, plus the fixed node column of a FEM
     // system with zero pressure on the right edge (read as zero, written
     // as zero)
@@ -619,6 +620,7 @@ struct incrmt_mg {
     ~incrmt_mg() {
         if (x) (void)hipFree(x);
         if (y) (void)hipFree(y);
+        if (bkeep) (void)hipFree(bkeep);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -1461,11 +1463,49 @@ int incrmt_mg_solve(incrmt_mg *h, const double *b, double *z, double rel_tol, in
     MgLevel &l = h->mg.fine();
     if (int rc = mg_copy(h, l.b, b, nullptr)) return rc;
     if (int rc = mg_copy(h, l.z, z, nullptr)) return rc;
+    const size_t len = (size_t)(l.ny + 2) * l.pitch;
+    if (!h->bkeep) HIP_CHECK(hipMalloc((void **)&h->bkeep, len * sizeof(double)));
+    HIP_CHECK(hipMemcpyAsync(h->bkeep, l.b - l.pitch - 1, len * sizeof(double), hipMemcpyDeviceToDevice, h->s));
     MgStats st{};
     int rc = precision == INCRMT_PREC_MIXED ? h->mg.solve_mixed(rel_tol, max_cycles, &st)
                                             : h->mg.solve(rel_tol, max_cycles, &st);
     if (rc && rc != 2) return rc;
     if (int r2 = mg_copy(h, l.z, nullptr, z)) return r2;
+    if (cycles) *cycles = st.cycles;
+    if (rel_res) *rel_res = st.rel_res;
+    return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;
+}
+
+int incrmt_mg_time_solve(incrmt_mg *h, double rel_tol, int max_cycles, int precision, int reps,
+                         double *ms_per_solve, int *cycles, double *rel_res) {
+    if (!h || !(rel_tol > 0) || max_cycles < 1 || reps < 1) return fail(INCRMT_ERR_ARG, "bad arguments");
+    if (!h->bkeep) return fail(INCRMT_ERR_ARG, "no right-hand side: solve once first");
+    MgLevel &l = h->mg.fine();
+    const size_t len = (size_t)(l.ny + 2) * l.pitch;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    double total = 0;
+    MgStats st{};
+    int rc = 0;
+    for (int r = 0; r < reps && !rc; r++) {
+        // the right-hand side as uploaded and a zero start, then the solve
+        // alone between the two events
+        HIP_CHECK(hipMemcpyAsync(l.b - l.pitch - 1, h->bkeep, len * sizeof(double), hipMemcpyDeviceToDevice, h->s));
+        HIP_CHECK(hipMemsetAsync(l.z - l.pitch - 1, 0, len * sizeof(double), h->s));
+        HIP_CHECK(hipEventRecord(e0, h->s));
+        rc = precision == INCRMT_PREC_MIXED ? h->mg.solve_mixed(rel_tol, max_cycles, &st)
+                                            : h->mg.solve(rel_tol, max_cycles, &st);
+        HIP_CHECK(hipEventRecord(e1, h->s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc && rc != 2) return rc;
+    if (ms_per_solve) *ms_per_solve = total / reps;
     if (cycles) *cycles = st.cycles;
     if (rel_res) *rel_res = st.rel_res;
     return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;

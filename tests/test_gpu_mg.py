@@ -175,3 +175,17 @@ def test_fused_smoother_shapes_bit_identical(kind, sweeps, merged):
     shapes32 = shapes + (() if kind == "mac" else (16,))
     for sh in shapes32:
         assert np.array_equal(s.smooth(b, z, sweeps, sh, precision=ia.PREC_MIXED), ref32), ("fp32", sh)
+
+
+@pytest.mark.parametrize("precision", [ia.PREC_FP64, ia.PREC_MIXED])
+def test_time_solve_on_device(precision):
+    """The device-timed solve (b kept on the device, zero start) runs the
+    same solve as incrmt_mg_solve from a zero start: same V-cycles, same
+    residual, a positive event time."""
+    m = 129
+    s, _ = _fem(m, m, True, seed=3)
+    b = np.random.default_rng(4).standard_normal(m * m)
+    z, cyc, rr = s.solve(b, rel_tol=1e-10, max_cycles=100, precision=precision, z=np.zeros(m * m))
+    ms, c2, r2 = s.time_solve(1e-10, 100, precision, reps=2)
+    assert c2 == cyc and r2 == rr and ms > 0
+    s.close()
