@@ -56,7 +56,24 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER = os.path.join(HERE, "adapter_driver")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_driver(out=DRIVER):
+    """The C++ driver written against include/incrmt_adapter.hh (the
+    reference-side adapters), linked to the library beside it."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++11", "-Wall", "-I", INCLUDE, os.path.join(CSRC, "adapter_driver.cc"),
+           "-L", HERE, "-lincrmt_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"adapter driver build failed\n{r.stdout}\n{r.stderr}")
+    return out
 
 
 if __name__ == "__main__":
