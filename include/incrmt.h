@@ -226,6 +226,15 @@ int incrmt_fluid_get_stats(incrmt_fluid *f, incrmt_stats *out);
  * reference's float32 Gnuplot matrix format, the tracers as "trace.<sn>",
  * and the objects' output (a track.dat record; the file is rewritten whole). */
 int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned int fflags);
+/* fluid_2d::solve(duration, frames) (fluid_2d.cc:259-296): the header file
+ * (save_header, fluid_2d_io.cc:13-19), frame 0 on the first call, then per
+ * frame timestep_select's l steps, write_frame (when dir is not null) and
+ * one line on stdout: with fflags & 8192 the machine-readable "%d %d %.8g
+ * %.8g  %d %.8g %d %.8g" (frame, steps, step and output wall seconds, then
+ * per projection its V-cycles in the frame, standing in for TGMG's
+ * vcount_extra, and its solver wall seconds), else "# Output frame k [l,
+ * step s, output s] {MAC v, FEM v}" with mean V-cycles per solve. */
+int incrmt_fluid_solve(incrmt_fluid *f, const char *dir, double duration, int frames, unsigned int fflags);
 /* No reference counterpart: the fused-smoother launch shape the autotuner
  * picked for each multigrid level ("nx x ny: SXxSY hH/SXxSY hH", FP64/FP32
  * sweeps) and each solver's V-cycle issue mode ("launch MAC graph, FEM

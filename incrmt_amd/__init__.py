@@ -91,6 +91,7 @@ _PROTOS = [
     ("incrmt_fluid_set_horizontal_flow", _I, [_V, _D]),
     ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_fluid_write_frame", _I, [_V, ctypes.c_char_p, _I, ctypes.c_uint]),
+    ("incrmt_fluid_solve", _I, [_V, ctypes.c_char_p, _D, _I, ctypes.c_uint]),
     ("incrmt_fluid_get_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_set_tracers", _I, [_V, _dp]),
     ("incrmt_fluid_get_track_state", _I, [_V, _I, _dp]),
@@ -376,6 +377,14 @@ class Fluid2D:
         """fluid_2d::write_files(sn): the fflags-selected fields as float32
         Gnuplot matrices in `directory` (ftest's default flags)."""
         _check(lib().incrmt_fluid_write_frame(self._h, str(directory).encode(), sn, fflags))
+
+    def solve(self, duration, frames, directory=None, fflags=8 | 16 | 64 | 128 | 256 | 512):
+        """fluid_2d::solve(duration, frames): `frames` frames of
+        timestep_select's steps, each written to `directory` (when given)
+        and reported by one line on the process's stdout (machine-readable
+        with fflags & 8192)."""
+        d = None if directory is None else str(directory).encode()
+        _check(lib().incrmt_fluid_solve(self._h, d, duration, frames, fflags))
 
     def time_fine_smoother(self, kind, sweeps, precision=PREC_FP64):
         """Back-to-back fine-level sweeps of one projection system between two

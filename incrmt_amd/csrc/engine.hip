@@ -14,6 +14,7 @@
 #include <vector>
 #include <limits>
 #include <atomic>
+#include <chrono>
 #include "step.hh"
 #include "mg.hh"
 #include "levelset.hh"
@@ -120,6 +121,12 @@ struct FluidEngine {
     bool windows = true;        // windowed per-solid sweeps and batched band passes (else
                                 // whole-grid sweeps and one solid's passes at a time)
     LsBatch lsb{};              // batched band passes of the solids
+    // the frame loop's diagnostics (fluid_2d::solve, fluid_2d.cc:259-296):
+    // frames written so far, and per projection the wall seconds, V-cycles
+    // and solves since the last frame line
+    int f_num = 0;
+    double wc[2] = {0, 0};
+    long vcyc[2] = {0, 0}, nsolve[2] = {0, 0};
     Fields f{};
     Phys ph{};
     std::vector<void *> allocs;
@@ -536,7 +543,11 @@ int FluidEngine::step(double dt) {
     mac.t_solve = time;  // the MAC solve's time, t + dt/2 (initial_guess(f.time))
     if ((rc = proj_begin())) return rc;
     if ((rc = guess_features(mac, 0))) return rc;
+    auto w0 = std::chrono::steady_clock::now();
     rc = precision == INCRMT_PREC_MIXED ? mac.solve_mixed(mg_tol, max_cycles, &ms) : mac.solve(mg_tol, max_cycles, &ms);
+    wc[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    vcyc[0] += ms.cycles;
+    nsolve[0]++;
     if (!rc) rc = proj_end();
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in MAC problem");
     if (rc) return rc;
@@ -576,7 +587,11 @@ int FluidEngine::step(double dt) {
     fem.t_solve = time;  // t + dt
     if ((rc = proj_begin())) return rc;
     if ((rc = guess_features(fem, 1))) return rc;
+    w0 = std::chrono::steady_clock::now();
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
+    wc[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    vcyc[1] += ms.cycles;
+    nsolve[1]++;
     if (!rc) rc = proj_end();
     if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
     if (rc) return rc;
@@ -1059,6 +1074,58 @@ int write_matrix(const std::string &path, const HostFrame &h, int mode, int li, 
 }
 
 }  // namespace
+
+int incrmt_fluid_solve(incrmt_fluid *f, const char *dir, double duration, int frames, unsigned int fflags) {
+    if (int rc = check_ready(f)) return rc;
+    if (!(duration > 0) || frames < 1) return fail(INCRMT_ERR_ARG, "bad duration or frame count");
+    FluidEngine &e = f->e;
+    auto now = []() { return std::chrono::steady_clock::now(); };
+    auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double>(b - a).count();
+    };
+    double adt;
+    const int l = incrmt_fluid_timestep_select(f, duration / frames, &adt);
+    if (l < 0) return fail(INCRMT_ERR_ARG, "timestep_select failed");
+    if (dir) {  // save_header (fluid_2d_io.cc:13-19)
+        const std::string hp = std::string(dir) + "/header";
+        FILE *fp = std::fopen(hp.c_str(), e.f_num == 0 ? "w" : "a");
+        if (!fp) return fail(INCRMT_ERR_ARG, "cannot open the header file");
+        std::fprintf(fp, "%g %g %d\n", e.time, e.time + duration, frames);
+        std::fclose(fp);
+    }
+    if (e.f_num == 0) {
+        if (dir)
+            if (int rc = incrmt_fluid_write_frame(f, dir, 0, fflags)) return rc;
+        std::puts("# Output frame 0");
+    }
+    auto t0 = now();
+    for (int k = 1; k <= frames; k++) {
+        if (int rc = incrmt_fluid_steps(f, adt, l)) return rc;
+        HIP_CHECK(hipStreamSynchronize(e.s));
+        const auto t1 = now();
+        if (dir)
+            if (int rc = incrmt_fluid_write_frame(f, dir, k + e.f_num, fflags)) return rc;
+        const auto t2 = now();
+        // the reference prints TGMG's V-cycle count and the solver wall
+        // time with 8192 (its tgmg_predict counters are not available: the
+        // V-cycles run in the frame stand in), else the mean V-cycles per
+        // solve
+        const double avg0 = e.nsolve[0] ? double(e.vcyc[0]) / e.nsolve[0] : 0;
+        const double avg1 = e.nsolve[1] ? double(e.vcyc[1]) / e.nsolve[1] : 0;
+        if (fflags & 8192)
+            std::printf("%d %d %.8g %.8g  %ld %.8g %ld %.8g\n", k + e.f_num, l, secs(t0, t1), secs(t1, t2), e.vcyc[0],
+                        e.wc[0], e.vcyc[1], e.wc[1]);
+        else
+            std::printf("# Output frame %d [%d, %.8g s, %.8g s] {MAC %.2f, FEM %.2f}\n", k + e.f_num, l, secs(t0, t1),
+                        secs(t1, t2), avg0, avg1);
+        std::fflush(stdout);
+        e.wc[0] = e.wc[1] = 0;
+        e.vcyc[0] = e.vcyc[1] = e.nsolve[0] = e.nsolve[1] = 0;
+        t0 = t2;
+    }
+    e.f_num += frames;
+    return 0;
+}
 
 int incrmt_fluid_write_frame(incrmt_fluid *f, const char *dir, int sn, unsigned int fflags) {
     if (int rc = check_ready(f)) return rc;

@@ -75,3 +75,32 @@ def test_rigid_rotation_vorticity(tmp_path):
     assert np.allclose(W, 2 * om, rtol=1e-6)
     _, _, D = fr.read_matrix(os.path.join(tmp_path, "div.0"))
     assert D.shape == (m, m) and np.abs(D).max() <= 1e-6
+
+
+def test_solve_frame_lines(tmp_path, capfd):
+    """fluid_2d::solve's frame loop (fluid_2d.cc:259-296): the header file,
+    frame 0, then per frame l steps, the frame's files and one line: with
+    fflags & 8192 "frame steps t_step t_out  mac_vcycles mac_s fem_vcycles
+    fem_s" (the V-cycles of the frame, which the step statistics count too),
+    else "# Output frame k [l, ...] {MAC avg, FEM avg}"."""
+    f = ia.simple_spin(64, rel_tol=1e-8)
+    l, adt = f.timestep_select(0.02)
+    c0 = f.stats()
+    f.solve(0.04, 2, tmp_path, fflags=8 | 8192)
+    c1 = f.stats()
+    f.solve(0.02, 1, tmp_path, fflags=8)
+    out = capfd.readouterr().out.strip().splitlines()
+    assert out[0] == "# Output frame 0"
+    rows = [line.split() for line in out[1:3]]
+    assert [int(r[0]) for r in rows] == [1, 2] and all(int(r[1]) == l for r in rows)
+    assert all(len(r) == 8 and float(r[2]) > 0 and float(r[5]) > 0 and float(r[7]) > 0 for r in rows)
+    assert sum(int(r[4]) for r in rows) == c1["total_mac_cycles"] - c0["total_mac_cycles"]
+    assert sum(int(r[6]) for r in rows) == c1["total_fem_cycles"] - c0["total_fem_cycles"]
+    assert out[3].startswith(f"# Output frame 3 [{l}, ") and "{MAC " in out[3] and ", FEM " in out[3]
+    hdr = (tmp_path / "header").read_text().split("\n")
+    t0, t1, fr = hdr[0].split()
+    assert float(t0) == 0 and abs(float(t1) - 0.04) < 1e-12 and fr == "2"
+    assert hdr[1].split()[2] == "1"
+    for k in range(4):
+        assert (tmp_path / f"p.{k}").exists()
+    f.close()
