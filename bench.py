@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, params=None):
+def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, params=None, cost_aware=False):
     import incrmt_amd as ia
     f = ia.simple_spin(m, rel_tol=1e-8, precision=precision, guess=guess)
     if launch is not None:  # otherwise the library's default for the size
@@ -37,6 +37,8 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, par
     if params is not None:  # trained learned-guess networks (MAC, FEM)
         for k in (ia.MG_MAC, ia.MG_FEM):
             f.set_guess_params(k, params[k])
+    if cost_aware:  # the learned start only where it saves more than it costs
+        f.set_guess_cost(1)
     l, adt = f.timestep_select(4 * math.pi / 240)
     f.steps(adt, warmup)
     f.sync()
@@ -60,7 +62,8 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, par
     out["projection_ms_per_step"] = f.stats()["proj_ms"] / steps
     f.time_projections(False)
     if guess == ia.GUESS_LEARNED:
-        out["learned_start"] = {("mac" if k == ia.MG_MAC else "fem"): f.guess_stats(k) for k in (ia.MG_MAC, ia.MG_FEM)}
+        out["learned_start"] = {("mac" if k == ia.MG_MAC else "fem"): dict(f.guess_stats(k), **f.guess_cost_stats(k))
+                                for k in (ia.MG_MAC, ia.MG_FEM)}
     if timing:
         # a further pass with HIP events around every fine-level smoother launch
         f.time_smoothers(True)
@@ -259,6 +262,7 @@ def main():
         if params is not None:
             for k in (ia.MG_MAC, ia.MG_FEM):
                 f.set_guess_params(k, params[k])
+            f.set_guess_cost(1)
         l, adt = f.timestep_select(4 * math.pi / 240)
         f.steps(adt, a.warmup)
         f.sync()
@@ -286,9 +290,13 @@ def main():
     # by kernel, as the runs above do)
     graph = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC, launch=ia.LAUNCH_GRAPH)
     params, training = train_guess(m, a.train_steps)
-    learned = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, timing=True, guess=ia.GUESS_LEARNED, params=params)
+    # the learned start used on every solve, and (the headline) only where
+    # it saves more V-cycles than its network costs (measured in V-cycles)
+    always = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, guess=ia.GUESS_LEARNED, params=params)
+    learned = gpu_run(m, a.steps, a.warmup, ia.PREC_MIXED, timing=True, guess=ia.GUESS_LEARNED, params=params,
+                      cost_aware=True)
     best = learned
-    best_name = "mixed+learned_guess"
+    best_name = "mixed+learned_guess (cost-aware)"
     # roofline of the fine-level smoother: algorithmic bytes per sweep are,
     # per point, z, b and the coefficients read and z written: MAC
     # red-black 8+8+16+8 = 40 B/cell in FP64 (two face inverse densities),
@@ -331,10 +339,13 @@ def main():
                      "mixed+quadratic_guess": {k: sig(v) for k, v in quad.items()},
                      "mixed+quadratic_guess, graph replay": {k: sig(v) for k, v in graph.items()
                                                              if k != "smoother_shapes"},
-                     "mixed+learned_guess": {k: sig(v) for k, v in learned.items() if k not in skip}},
+                     "mixed+learned_guess, every solve": {k: sig(v) for k, v in always.items() if k not in skip},
+                     "mixed+learned_guess (cost-aware)": {k: sig(v) for k, v in learned.items() if k not in skip}},
         "learned_guess": {"training": training,
                           "learned_start": {k: {kk: sig(vv) for kk, vv in v.items()}
-                                            for k, v in learned["learned_start"].items()}},
+                                            for k, v in learned["learned_start"].items()},
+                          "learned_start_every_solve": {k: {kk: sig(vv) for kk, vv in v.items()}
+                                                        for k, v in always["learned_start"].items()}},
         "speedup_vs_plain_fp64": round(best["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "speedup_quadratic_vs_plain_fp64": round(quad["cell_steps_per_s"] / fp64["cell_steps_per_s"], 3),
         "roofline": roofline,

@@ -185,6 +185,18 @@ int incrmt_fluid_set_guess_params(incrmt_fluid *f, int kind, const float *params
  * learned start switched off (0/1), misses in a row, the relative residual
  * of the last learned start and after its first V-cycle */
 int incrmt_fluid_guess_stats(incrmt_fluid *f, int kind, double *out8);
+/* No reference counterpart: the learned start used only where it pays.
+ * mode 0: always (the default); 1: its cost in V-cycles is measured once
+ * (HIP events around the network's start and the first V-cycle after it);
+ * 2: the cost is `cycles`. One solve in four then runs the quadratic start
+ * as a control, and once three learned and one control solves are averaged
+ * (exponentially), a learned start that saves no more V-cycles than its
+ * cost is switched off for 256 solves, then tried again. */
+int incrmt_fluid_set_guess_cost(incrmt_fluid *f, int mode, double cycles);
+/* cost in V-cycles (< 0 unmeasured), mean V-cycles of learned and of
+ * control solves, switched off for cost (0/1), learned and control solves
+ * counted since the last decision */
+int incrmt_fluid_guess_cost_stats(incrmt_fluid *f, int kind, double *out6);
 /* No reference counterpart (host logic, no GPU): the learned start's
  * accept/fallback rule run over a sequence of relative residuals after the
  * first V-cycle: decision 1 accepted, 0 fallback, -1 learned start off. */

@@ -118,6 +118,8 @@ _PROTOS = [
     ("incrmt_fluid_get_guess_params", _I, [_V, _I, _fp]),
     ("incrmt_fluid_set_guess_params", _I, [_V, _I, _fp]),
     ("incrmt_fluid_guess_stats", _I, [_V, _I, _dp]),
+    ("incrmt_fluid_set_guess_cost", _I, [_V, _I, _D]),
+    ("incrmt_fluid_guess_cost_stats", _I, [_V, _I, _dp]),
     ("incrmt_guess_policy_run", _I, [_dp, _I, _D, _ip]),
     ("incrmt_unet_create", _I, [_I, _I, ctypes.c_uint, ctypes.POINTER(_V)]),
     ("incrmt_unet_nparams", _I, [_V, ctypes.POINTER(ctypes.c_longlong)]),
@@ -271,6 +273,17 @@ class Fluid2D:
         fallback_tol is the relative residual after one V-cycle above which
         a learned start is dropped for the quadratic start and FP64 cycles."""
         _check(lib().incrmt_fluid_set_guess_training(self._h, int(train), lr, fallback_tol))
+
+    def set_guess_cost(self, mode=1, cycles=0.0):
+        """Use the learned start only where it pays: mode 1 measures its
+        cost in V-cycles, mode 2 takes `cycles`, mode 0 always uses it."""
+        _check(lib().incrmt_fluid_set_guess_cost(self._h, mode, cycles))
+
+    def guess_cost_stats(self, kind):
+        o = (ctypes.c_double * 6)()
+        _check(lib().incrmt_fluid_guess_cost_stats(self._h, kind, o))
+        return dict(cost=o[0], mean_learned=o[1], mean_control=o[2], off_for_cost=bool(o[3]),
+                    learned_counted=int(o[4]), control_counted=int(o[5]))
 
     def time_projections(self, enable=True):
         """HIP events around each step's two projections (stats proj_ms)."""

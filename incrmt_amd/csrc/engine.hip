@@ -1322,6 +1322,29 @@ int incrmt_fluid_guess_stats(incrmt_fluid *f, int kind, double *out8) {
     return 0;
 }
 
+int incrmt_fluid_set_guess_cost(incrmt_fluid *f, int mode, double cycles) {
+    if (!f || mode < 0 || mode > 2 || (mode == 2 && !(cycles >= 0))) return fail(INCRMT_ERR_ARG, "bad arguments");
+    for (Multigrid *mg : {&f->e.mac, &f->e.fem}) {
+        mg->policy.reset();
+        mg->policy.cost_aware = mode != 0;
+        mg->policy.cost = mode == 2 ? cycles : -1;
+    }
+    return 0;
+}
+
+int incrmt_fluid_guess_cost_stats(incrmt_fluid *f, int kind, double *out6) {
+    if (!f || !out6) return fail(INCRMT_ERR_ARG, "null argument");
+    if (kind != INCRMT_MG_MAC && kind != INCRMT_MG_FEM) return fail(INCRMT_ERR_ARG, "bad solver kind");
+    const GuessPolicy &p = (kind == INCRMT_MG_MAC ? f->e.mac : f->e.fem).policy;
+    out6[0] = p.cost;
+    out6[1] = p.ema_learn;
+    out6[2] = p.ema_quad;
+    out6[3] = p.benefit_off ? 1 : 0;
+    out6[4] = p.nlearn;
+    out6[5] = p.nquad;
+    return 0;
+}
+
 int incrmt_guess_policy_run(const double *rel, int n, double tol, int *decision) {
     if (!rel || !decision || n < 0) return fail(INCRMT_ERR_ARG, "null argument");
     GuessPolicy p;

@@ -54,7 +54,53 @@ struct GuessPolicy {
         if (misses >= kMaxMisses) { off = true; cooldown = kRetry; }
         return ok;
     }
-    void reset() { misses = 0; cooldown = 0; off = false; }
+    void reset() {
+        misses = 0; cooldown = 0; off = false;
+        ema_learn = ema_quad = -1; nlearn = nquad = since_control = 0; benefit_off = false; benefit_cool = 0;
+    }
+
+    // Cost-aware use (when enabled): the learned start pays when the
+    // V-cycles it saves against the quadratic start exceed its own cost in
+    // V-cycles (the network's forward pass timed against one V-cycle). One
+    // solve in kControlEvery runs the quadratic start as a control; after
+    // kMinLearn learned and kMinQuad control solves, a learned start whose
+    // mean V-cycles (exponential average) are not below the controls' by
+    // more than its cost is switched off for kBenefitRetry solves, then
+    // probed again from fresh averages.
+    static constexpr int kControlEvery = 4, kMinLearn = 3, kMinQuad = 1, kBenefitRetry = 256;
+    static constexpr double kEma = 0.25;
+    bool cost_aware = false;
+    double cost = -1;  // the learned start's cost in V-cycles (< 0: not measured yet)
+    double ema_learn = -1, ema_quad = -1;
+    int nlearn = 0, nquad = 0, since_control = 0, benefit_cool = 0;
+    bool benefit_off = false;
+    // whether this solve is a control (quadratic start) or off for cost
+    bool skip_for_cost() {
+        if (!cost_aware) return false;
+        if (benefit_off) {
+            if (--benefit_cool > 0) return true;
+            benefit_off = false;
+            ema_learn = ema_quad = -1;
+            nlearn = nquad = since_control = 0;
+        }
+        if (++since_control >= kControlEvery) {
+            since_control = 0;
+            return true;
+        }
+        return false;
+    }
+    // V-cycles of a solve that could have used the learned start (learned:
+    // whether it did)
+    void record(bool learned, int cycles) {
+        if (!cost_aware || benefit_off) return;
+        double &e = learned ? ema_learn : ema_quad;
+        e = e < 0 ? cycles : (1 - kEma) * e + kEma * cycles;
+        (learned ? nlearn : nquad)++;
+        if (nlearn >= kMinLearn && nquad >= kMinQuad && cost >= 0 && ema_quad - ema_learn <= cost) {
+            benefit_off = true;
+            benefit_cool = kBenefitRetry;
+        }
+    }
 };
 
 // One level: nx x ny unknowns stored with a one-point zero ghost ring, so
@@ -185,6 +231,7 @@ class Multigrid {
     long learned_solves = 0, guess_fallbacks = 0, train_steps = 0;
     double last_rel0 = 0, last_rel1 = 0;  // relative residual of the last learned start, after its first cycle
     GuessPolicy policy;
+    hipEvent_t cev_[3] = {nullptr, nullptr, nullptr};  // the learned start's cost measurement
     // replay captured V-cycles (HIP graphs) instead of launching them; init
     // sets it for hierarchies whose finest level is below kGraphMaxPoints
     bool use_graphs = true;

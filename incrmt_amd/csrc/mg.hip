@@ -1534,6 +1534,8 @@ Multigrid::~Multigrid() {
     for (auto &g : graph_exec_)
         if (g) (void)hipGraphExecDestroy(g);
     for (auto &e : ev_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (hipEvent_t e : cev_)
+        if (e) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
     if (h_red) (void)hipHostFree(h_red);
 }
@@ -2505,7 +2507,14 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
     // (only once the quadratic start has its three solutions: the first
     // two solves start from the previous solution and a linear
     // extrapolation, which the network was not trained on)
-    const bool learned = extrapolate == 3 && net && quad_start && policy.use();
+    const bool eligible = extrapolate == 3 && net && quad_start;
+    bool learned = eligible && policy.use();
+    if (learned && policy.skip_for_cost()) learned = false;  // a control solve, or off for its cost
+    // the learned start's cost in V-cycles, measured once: events around
+    // the network's start and around the first V-cycle after it
+    const bool measure = learned && policy.cost_aware && policy.cost < 0;
+    if (measure && !cev_[0])
+        for (hipEvent_t &e : cev_) HIP_CHECK(hipEventCreate(&e));
     const bool sample = train && net;
     // the network's inputs (the start before the learned correction, the
     // last correction) are kept whenever a network is attached and used
@@ -2515,7 +2524,9 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
         k_copy_level<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(args_of(f), f.z, zs_);
         KCHECK();
     }
+    if (measure) HIP_CHECK(hipEventRecord(cev_[0], s_));
     if ((learned || sample) && (rc = learned_start(learned))) return rc;
+    if (measure) HIP_CHECK(hipEventRecord(cev_[1], s_));
     double nr[3];
     if ((rc = reduce_norms(nr))) return rc;
     const double bn = std::sqrt(nr[0]);
@@ -2530,6 +2541,14 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
         // FP64 V-cycles
         last_rel0 = rel;
         if ((rc = cycle())) return rc;
+        if (measure) {
+            HIP_CHECK(hipEventRecord(cev_[2], s_));
+            HIP_CHECK(hipEventSynchronize(cev_[2]));
+            float t_net = 0, t_cycle = 0;
+            HIP_CHECK(hipEventElapsedTime(&t_net, cev_[0], cev_[1]));
+            HIP_CHECK(hipEventElapsedTime(&t_cycle, cev_[1], cev_[2]));
+            policy.cost = t_cycle > 0 ? t_net / t_cycle : 0;
+        }
         cycles = 1;
         if ((rc = reduce_norms(nr))) return rc;
         rel = std::sqrt(nr[1]) / bn;
@@ -2560,6 +2579,7 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
         rel = std::sqrt(nr[1]) / bn;
     }
     if (cycles > 0) predict_ = cycles - 1;
+    if (eligible) policy.record(learned, cycles);
     if (sample && rel <= tol && bn > 0 && (rc = train_step())) return rc;
     if (keep && rel <= tol && bn > 0) {
         k_guess_eprev<<<kRedBlocks, kRedThreads, 0, s_>>>(args_of(f), f.z, zs_, eprev_, d_red + 8);

@@ -72,3 +72,30 @@ def test_trained_guess_saves_v_cycles():
           f"learned-start stats {learned['stats']}")
     assert learned["cycles"][1] <= 0.95 * quad["cycles"][1]       # FEM
     assert learned["cycles"].sum() < quad["cycles"].sum()
+
+
+def test_cost_aware_use():
+    """With the learned start's cost given as 100 V-cycles (mode 2), which no
+    start can save, the policy runs one control solve (quadratic start) in
+    four and switches the learned start off once three learned and one
+    control solve are counted: 3 learned solves per projection, then none.
+    With the cost measured (mode 1) it is a positive number of V-cycles.
+    Every projection still reaches its residual."""
+    f = ia.simple_spin(64, rel_tol=1e-8, precision=ia.PREC_MIXED, guess=ia.GUESS_LEARNED)
+    f.set_guess_training(False, 1e-3, 1.0)   # every learned start accepted: no misses
+    f.set_guess_cost(2, 100.0)
+    l, adt = f.timestep_select(4 * math.pi / 240)
+    for _ in range(30):
+        f.step_forward(adt)
+        st = f.stats()
+        assert st["mac_rel_res"] <= 1e-8 and st["fem_rel_res"] <= 1e-8
+    for kind in (ia.MG_MAC, ia.MG_FEM):
+        cs, gs = f.guess_cost_stats(kind), f.guess_stats(kind)
+        assert cs["off_for_cost"] and gs["learned_solves"] == 3, (cs, gs)
+    f.set_guess_cost(1)
+    for _ in range(2):
+        f.step_forward(adt)
+    for kind in (ia.MG_MAC, ia.MG_FEM):
+        cs = f.guess_cost_stats(kind)
+        assert cs["cost"] > 0 and not cs["off_for_cost"], cs
+    f.close()
