@@ -92,13 +92,20 @@ def run_multibody(m=2048, count=64, steps=200, every=1):
     return out
 
 
-def run_ensemble(count=256, steps=10, threads=16, precision=ia.PREC_FP64):
+def run_ensemble(count=256, steps=10, threads=32, precision=ia.PREC_MIXED, guess=ia.GUESS_QUADRATIC):
+    """256 simple-spin rotors of sizes drawn by seed 20260103 from
+    64/128/256/512, stepped concurrently (one stream per instance). The
+    default is the headline solver (mixed precision, quadratic starts) on
+    32 host threads; precision=PREC_FP64, guess=GUESS_PREVIOUS gives the
+    reference's own solver."""
     from incrmt_amd.ensemble import Ensemble
     specs = wl.rotor_ensemble(count)
-    e = Ensemble(specs, rel_tol=1e-8, precision=precision, threads=threads)
-    e.steps(1)
+    e = Ensemble(specs, rel_tol=1e-8, precision=precision, threads=threads, guess=guess)
+    e.steps(2)  # the quadratic start needs two earlier solutions
     r = e.throughput(steps)
-    r.update(config="batched ensemble", threads=threads, precision="fp64" if precision == ia.PREC_FP64 else "mixed")
+    r.update(config="batched ensemble", threads=threads, precision="fp64" if precision == ia.PREC_FP64 else "mixed",
+             guess={ia.GUESS_PREVIOUS: "previous", ia.GUESS_EXTRAPOLATE: "extrapolate",
+                    ia.GUESS_QUADRATIC: "quadratic", ia.GUESS_LEARNED: "learned"}[guess])
     e.close()
     return r
 
@@ -148,7 +155,8 @@ def main():
     ap.add_argument("--intervals", type=int, default=50)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--every", type=int, default=1)
-    ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=32)
+    ap.add_argument("--fp64", action="store_true", help="ensemble: the FP64 solver with previous-solution starts")
     a = ap.parse_args()
     if a.which == "fem":
         r = run_fem()
@@ -159,7 +167,8 @@ def main():
     elif a.which == "conv-test":
         r = run_conv_test(a.type, tuple(int(x) for x in a.sizes.split(",")), a.intervals)
     else:
-        r = run_ensemble(steps=a.steps or 10, threads=a.threads)
+        r = (run_ensemble(steps=a.steps or 10, threads=a.threads, precision=ia.PREC_FP64, guess=ia.GUESS_PREVIOUS)
+             if a.fp64 else run_ensemble(steps=a.steps or 10, threads=a.threads))
     r["device"] = ia.device_info()[0]
     print(json.dumps(r))
 

@@ -166,12 +166,13 @@ namespace incrmt {
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // Record msg as the calling thread's last error and return code.
 int set_error(int code, const char *msg);
-// Clear device memory and wait for it. A plain hipMemset runs on the null
-// stream and may return before it is done; the library's streams are
-// non-blocking, so their next kernels are not ordered after it.
-inline hipError_t memset_sync(void *p, int v, size_t bytes) {
-    const hipError_t e = hipMemset(p, v, bytes);
-    return e != hipSuccess ? e : hipDeviceSynchronize();
+// Clear device memory on the owner's stream s and wait for that stream
+// only. Nothing here touches the null stream or the whole device: another
+// host thread may be capturing a graph on its own stream meanwhile, and a
+// device-wide synchronisation invalidates that capture.
+inline hipError_t memset_sync(void *p, int v, size_t bytes, hipStream_t s) {
+    const hipError_t e = hipMemsetAsync(p, v, bytes, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
 }
 }
 

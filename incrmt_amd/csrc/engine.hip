@@ -193,7 +193,7 @@ struct FluidEngine {
     int dalloc(double *&p) {
         void *q;
         HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(double)));
-        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(double), s));
         allocs.push_back(q);
         p = (double *)q;
         return 0;
@@ -326,7 +326,7 @@ int FluidEngine::init_tracers() {
     tr.rng = (GlibcRand *)q;
     tr.nout = (int *)((char *)q + ((sizeof(GlibcRand) + 15) / 16) * 16);
     HIP_CHECK(hipMemcpy(tr.rng, &rng, sizeof(GlibcRand), hipMemcpyHostToDevice));
-    HIP_CHECK(memset_sync(tr.nout, 0, sizeof(int)));
+    HIP_CHECK(memset_sync(tr.nout, 0, sizeof(int), s));
     if (nt > 0) {
         HIP_CHECK(hipMalloc(&q, tm.size() * sizeof(double)));
         allocs.push_back(q);
@@ -357,7 +357,7 @@ int FluidEngine::ensure_guess() {
             const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
             void *q;
             HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s));
             allocs.push_back(q);
             gfeat[w] = (double *)q + f.pitch + 1;
         }
@@ -413,7 +413,7 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     ph.xrow = d_xrow;
     HIP_CHECK(hipMalloc((void **)&d_part, 1024 * sizeof(double)));
     allocs.push_back(d_part);
-    HIP_CHECK(memset_sync(d_part, 0, 1024 * sizeof(double)));
+    HIP_CHECK(memset_sync(d_part, 0, 1024 * sizeof(double), s));
     d_pavg = d_part + 1000;
     ph.visc = visc; ph.rhof = rhof; ph.eps = eps;
     ph.w = w_scale * eps; ph.winv = 1 / ph.w; ph.k_rep = w_force * ph.winv;
@@ -427,10 +427,10 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
             if (int rc = dalloc(*p)) return rc;
         void *q;
         HIP_CHECK(hipMalloc(&q, (size_t)g.N * sizeof(int)));
-        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(int)));
+        HIP_CHECK(memset_sync(q, 0, (size_t)g.N * sizeof(int), s));
         allocs.push_back(q);
         o.a.st = (int *)q;
-        if (int rc = ls_alloc(g, o.w)) return rc;
+        if (int rc = ls_alloc(g, o.w, s)) return rc;
         for (int j = -2; j < g.n + 2; j++) {
             double y = g.ay + g.dy * (0.5 + j);
             for (int i = -2; i < g.m + 2; i++) {
@@ -1511,8 +1511,8 @@ int incrmt_mg_create_bc(int kind, int m, int n, int x_prd, int y_prd, int zpr, d
         const size_t len = (size_t)(l.ny + 2) * l.pitch + 64;
         if (hipMalloc((void **)&h->x, len * sizeof(double)) != hipSuccess ||
             hipMalloc((void **)&h->y, len * sizeof(double)) != hipSuccess ||
-            memset_sync(h->x, 0, len * sizeof(double)) != hipSuccess ||
-            memset_sync(h->y, 0, len * sizeof(double)) != hipSuccess)
+            memset_sync(h->x, 0, len * sizeof(double), h->s) != hipSuccess ||
+            memset_sync(h->y, 0, len * sizeof(double), h->s) != hipSuccess)
             rc = fail(INCRMT_ERR_HIP, "allocation failed");
     }
     if (rc) {
@@ -1676,16 +1676,16 @@ struct LsStandalone {
         for (auto p : al) {
             void *q;
             HIP_CHECK(hipMalloc(&q, g.N * sizeof(double)));
-            HIP_CHECK(memset_sync(q, 0, g.N * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, g.N * sizeof(double), s));
             allocs.push_back(q);
             *p = (double *)q;
         }
         void *q;
         HIP_CHECK(hipMalloc(&q, g.N * sizeof(int)));
-        HIP_CHECK(memset_sync(q, 0, g.N * sizeof(int)));
+        HIP_CHECK(memset_sync(q, 0, g.N * sizeof(int), s));
         allocs.push_back(q);
         a.st = (int *)q;
-        return ls_alloc(g, w);
+        return ls_alloc(g, w, s);
     }
     int put(double *dev, const double *host) {
         HIP_CHECK(hipMemcpy2D(dev + g.ls(0, 0), g.P * sizeof(double), host, g.ml * sizeof(double),

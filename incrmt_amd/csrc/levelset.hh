@@ -46,7 +46,7 @@ struct LsWork {
     hipEvent_t ev_bbox = nullptr;
 };
 
-int ls_alloc(const Grid &g, LsWork &w);
+int ls_alloc(const Grid &g, LsWork &w, hipStream_t s);
 void ls_free(LsWork &w);
 
 // Rebuilds and extensions of several solids of one grid run their passes

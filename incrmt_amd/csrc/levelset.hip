@@ -841,26 +841,26 @@ void ls_batch_free(LsBatch &bt) {
     bt = LsBatch{};
 }
 
-int ls_alloc(const Grid &g, LsWork &w) {
+int ls_alloc(const Grid &g, LsWork &w, hipStream_t s) {
     const size_t N = (size_t)g.N;
     double **d[] = {&w.va, &w.vb, &w.ex, &w.ey, &w.fx, &w.fy};
     for (auto p : d) {
         HIP_CHECK(hipMalloc((void **)p, N * sizeof(double)));
-        HIP_CHECK(memset_sync(*p, 0, N * sizeof(double)));
+        HIP_CHECK(memset_sync(*p, 0, N * sizeof(double), s));
     }
     const size_t L = (size_t)g.ml * g.nl + 64;
     HIP_CHECK(hipMalloc((void **)&w.iface, L * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.cand, L * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.pband, L * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.mark, N));
-    HIP_CHECK(memset_sync(w.mark, 0, N));
+    HIP_CHECK(memset_sync(w.mark, 0, N, s));
     HIP_CHECK(hipMalloc((void **)&w.cflag, N));
-    HIP_CHECK(memset_sync(w.cflag, 0, N));
+    HIP_CHECK(memset_sync(w.cflag, 0, N, s));
     w.ntx = (g.ml + kTileX - 1) / kTileX;
     w.nty = (g.nl + kTileY - 1) / kTileY;
     const size_t T = (size_t)w.ntx * w.nty + 64;
     HIP_CHECK(hipMalloc((void **)&w.tflag, T * sizeof(int)));
-    HIP_CHECK(memset_sync(w.tflag, 0, T * sizeof(int)));
+    HIP_CHECK(memset_sync(w.tflag, 0, T * sizeof(int), s));
     HIP_CHECK(hipMalloc((void **)&w.tiles_fim, T * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.tiles_ext, T * sizeof(int)));
     HIP_CHECK(hipMalloc((void **)&w.cnt, (kCntIter + kIterSlots) * sizeof(int)));

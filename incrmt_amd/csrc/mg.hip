@@ -1640,7 +1640,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         auto dalloc = [&](double *&p) -> int {
             void *q;
             HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-            HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s_));
             allocs.push_back(q);
             p = (double *)q + l.pitch + 1;
             return 0;
@@ -1648,7 +1648,7 @@ int Multigrid::init(MgKind kind, int m, int n, double dx, double dy, hipStream_t
         auto falloc = [&](float *&p) -> int {
             void *q;
             HIP_CHECK(hipMalloc(&q, len * sizeof(float)));
-            HIP_CHECK(memset_sync(q, 0, len * sizeof(float)));
+            HIP_CHECK(memset_sync(q, 0, len * sizeof(float), s_));
             allocs.push_back(q);
             p = (float *)q + l.pitch + 1;
             return 0;
@@ -2413,7 +2413,7 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
             for (double **p : {&zq1_, &zq2_}) {
                 void *q;
                 HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-                HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+                HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s_));
                 allocs.push_back(q);
                 *p = (double *)q + f.pitch + 1;
             }
@@ -2459,7 +2459,7 @@ int Multigrid::solve_loop(double tol, int max_cycles, MgStats *st, Cycle cycle) 
                     void *q;
                     const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
                     HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-                    HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+                    HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s_));
                     allocs.push_back(q);
                     zprev_ = (double *)q + f.pitch + 1;
                 }
@@ -2665,7 +2665,7 @@ int Multigrid::guess_buffers() {
     for (double **p : {&zs_, &dz_, &eprev_}) {
         void *q;
         HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
-        HIP_CHECK(memset_sync(q, 0, len * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s_));
         allocs.push_back(q);
         *p = (double *)q + f.pitch + 1;
     }
@@ -2674,7 +2674,7 @@ int Multigrid::guess_buffers() {
     allocs.push_back(q);
     tgt_ = (float *)q;
     HIP_CHECK(hipMalloc(&q, 8 * sizeof(double)));
-    HIP_CHECK(memset_sync(q, 0, 8 * sizeof(double)));
+    HIP_CHECK(memset_sync(q, 0, 8 * sizeof(double), s_));
     allocs.push_back(q);
     d_gs_ = (double *)q;
     return 0;

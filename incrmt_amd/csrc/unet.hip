@@ -470,7 +470,7 @@ UNet::~UNet() {
 
 int UNet::alloc(void **p, size_t bytes) {
     HIP_CHECK(hipMalloc(p, bytes));
-    HIP_CHECK(memset_sync(*p, 0, bytes));
+    HIP_CHECK(memset_sync(*p, 0, bytes, s_));
     allocs_.push_back(*p);
     return 0;
 }

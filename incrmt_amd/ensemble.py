@@ -12,17 +12,19 @@ import math
 import time
 from concurrent.futures import ThreadPoolExecutor
 
-from . import PREC_FP64, simple_spin
+from . import GUESS_PREVIOUS, PREC_FP64, simple_spin
 
 
 class Ensemble:
     """specs: list of (m, omega) — an `ftest simple-spin m` rotor with
     angular-velocity prefactor omega (object.hh:422-467) per instance."""
 
-    def __init__(self, specs, rel_tol=1e-8, precision=PREC_FP64, threads=16, frame=4 * math.pi / 240):
+    def __init__(self, specs, rel_tol=1e-8, precision=PREC_FP64, threads=16, frame=4 * math.pi / 240,
+                 guess=GUESS_PREVIOUS):
         self.specs = list(specs)
         self.threads = threads
-        self.members = [simple_spin(m, omega=om, rel_tol=rel_tol, precision=precision) for m, om in self.specs]
+        self.members = [simple_spin(m, omega=om, rel_tol=rel_tol, precision=precision, guess=guess)
+                        for m, om in self.specs]
         self.adt = [f.timestep_select(frame)[1] for f in self.members]
         # longest jobs first (largest grids), so the pool ends evenly
         self.order = sorted(range(len(self.members)), key=lambda i: -self.specs[i][0])
