@@ -381,6 +381,32 @@ struct LdsAcc {
     __device__ double val(int q) const { return sg * v[q]; }
 };
 
+// The same test on one point's radius-2 cross stencil held in registers
+// (slots: the point, x -1 +1 -2 +2, y -1 +1 -2 +2). The update's reads,
+// taken one by one from LDS, form a chain of dependent round trips; here
+// the statuses are read once per tile and the values once per iteration,
+// all issued together. The offsets the update asks for are compile-time
+// constants after inlining, so the slot lookups fold away.
+struct RegAcc {
+    double v[9];
+    int st[9];
+    int base;
+    double pval, limit, sg;
+    bool pos;
+    __device__ static int slot(int d) {
+        return d == -1 ? 1 : d == 1 ? 2 : d == -2 ? 3 : d == 2 ? 4 : d == -kFW ? 5 : d == kFW ? 6
+             : d == -2 * kFW ? 7 : d == 2 * kFW ? 8 : 0;
+    }
+    __device__ bool in(int q) const {
+        const int k = slot(q - base), s = st[k];
+        if (s < 0) return false;
+        if (pos) return s <= 4 || (v[k] < pval && v[k] < limit);
+        return s >= 3 || (v[k] > pval && v[k] > limit);
+    }
+    __device__ double val(int q) const { return sg * v[slot(q - base)]; }
+};
+__device__ constexpr int kStencil[9] = {0, -1, 1, -2, 2, -kFW, kFW, -2 * kFW, 2 * kFW};
+
 __device__ __forceinline__ void fim_tiles(const Grid &g, const EikGeom &eg, const ObjConst &c, const ObjArrays &a,
                                           const LsWork &w, const double *src, double *dst, int slot, int max_local) {
     __shared__ double sv[2][kFN];
@@ -408,22 +434,29 @@ __device__ __forceinline__ void fim_tiles(const Grid &g, const EikGeom &eg, cons
         const bool mine = I < g.ml && J < g.nl && w.cflag[g.ls(I, J)];
         const bool pos = mine && sst[e] == 7;
         const double limit = pos ? c.phi_up : c.phi_down;
+        // every stencil point of a tile point lies in the tile's 2-point halo
+        RegAcc acc;
+        acc.base = e; acc.limit = limit; acc.sg = pos ? 1.0 : -1.0; acc.pos = pos;
+#pragma unroll
+        for (int k = 0; k < 9; k++) acc.st[k] = sst[e + kStencil[k]];
         int cur = 0;
         for (int it = 0; it < max_local; it++) {
             int changed = 0;
             if (mine) {
-                LdsAcc acc{sv[cur], sst, sv[cur][e], limit, pos ? 1.0 : -1.0, pos};
+#pragma unroll
+                for (int k = 0; k < 9; k++) acc.v[k] = sv[cur][e + kStencil[k]];
+                acc.pval = acc.v[0];
                 const bool any = (I > 0 && acc.in(e - 1)) || (I < g.ml - 1 && acc.in(e + 1)) ||
                                  (J > 0 && acc.in(e - kFW)) || (J < g.nl - 1 && acc.in(e + kFW));
                 double r;
                 if (!any) r = pos ? INFINITY : -INFINITY;
                 else {
-                    Eik<true, LdsAcc> k{acc, eg, g.ml, g.nl, kFW};
+                    Eik<true, RegAcc> k{acc, eg, g.ml, g.nl, kFW};
                     r = k.calc_fast(I, J, e);
                     if (!pos) r = -r;
                 }
                 sv[cur ^ 1][e] = r;
-                changed = __double_as_longlong(r) != __double_as_longlong(sv[cur][e]);
+                changed = __double_as_longlong(r) != __double_as_longlong(acc.v[0]);
             }
             const int ch = __syncthreads_or(changed);
             cur ^= 1;
