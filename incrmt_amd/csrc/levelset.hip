@@ -381,6 +381,25 @@ struct LdsAcc {
     __device__ double val(int q) const { return sg * v[q]; }
 };
 
+// The OR over a workgroup of one predicate per thread, with one barrier:
+// the wavefronts post their votes to one of three LDS slots used in turn,
+// and the slot two rounds ahead is cleared between this round's barrier and
+// the next (its last readers have passed this barrier, its next writers
+// have not passed the next one). The slots are zero before the first round,
+// behind a barrier.
+struct VoteOr {
+    int *slot;
+    int k;
+    __device__ int operator()(int pred) {
+        if (__any(pred) && (threadIdx.x & 63) == 0) slot[k] = 1;
+        __syncthreads();
+        const int r = slot[k];
+        if (threadIdx.x == 0) slot[k == 0 ? 2 : k - 1] = 0;
+        k = k == 2 ? 0 : k + 1;
+        return r;
+    }
+};
+
 // The same test on one point's radius-2 cross stencil held in registers
 // (slots: the point, x -1 +1 -2 +2, y -1 +1 -2 +2). The update's reads,
 // taken one by one from LDS, form a chain of dependent round trips; here
@@ -411,27 +430,42 @@ __device__ __forceinline__ void fim_tiles(const Grid &g, const EikGeom &eg, cons
                                           const LsWork &w, const double *src, double *dst, int slot, int max_local) {
     __shared__ double sv[2][kFN];
     __shared__ int sst[kFN];
+    __shared__ int vslot[3];
     if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
     const int nt = w.cnt[kCntTileFim];
+    if (threadIdx.x < 3) vslot[threadIdx.x] = 0;
+    VoteOr vote{vslot, 0};
     int changed_any = 0;
+    constexpr int kRounds = (kFN + kTileThreads - 1) / kTileThreads;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const int tile = w.tiles_fim[t], tx = tile % w.ntx, ty = tile / w.ntx;
         const int I0 = tx * kTileX - kFH, J0 = ty * kTileY - kFH;
-        for (int e = threadIdx.x; e < kFN; e += blockDim.x) {
-            const int I = I0 + e % kFW, J = J0 + e / kFW;
-            if (I >= 0 && I < g.ml && J >= 0 && J < g.nl) {
+        // the tile and its halo: every global read issued before any wait
+        double rv[kRounds];
+        int rs[kRounds];
+#pragma unroll
+        for (int u = 0; u < kRounds; u++) {
+            const int e = threadIdx.x + u * kTileThreads, I = I0 + e % kFW, J = J0 + e / kFW;
+            rv[u] = 0;
+            rs[u] = -1;
+            if (e < kFN && I >= 0 && I < g.ml && J >= 0 && J < g.nl) {
                 const int q = g.ls(I, J);
-                sv[0][e] = sv[1][e] = src[q];
-                sst[e] = a.st[q];
-            } else {
-                sv[0][e] = sv[1][e] = 0;
-                sst[e] = -1;
+                rv[u] = src[q];
+                rs[u] = a.st[q];
             }
         }
-        __syncthreads();
         const int li = threadIdx.x % kTileX + kFH, lj = threadIdx.x / kTileX + kFH, e = lj * kFW + li;
         const int I = I0 + li, J = J0 + lj;
         const bool mine = I < g.ml && J < g.nl && w.cflag[g.ls(I, J)];
+#pragma unroll
+        for (int u = 0; u < kRounds; u++) {
+            const int f = threadIdx.x + u * kTileThreads;
+            if (f < kFN) {
+                sv[0][f] = sv[1][f] = rv[u];
+                sst[f] = rs[u];
+            }
+        }
+        __syncthreads();
         const bool pos = mine && sst[e] == 7;
         const double limit = pos ? c.phi_up : c.phi_down;
         // every stencil point of a tile point lies in the tile's 2-point halo
@@ -458,7 +492,7 @@ __device__ __forceinline__ void fim_tiles(const Grid &g, const EikGeom &eg, cons
                 sv[cur ^ 1][e] = r;
                 changed = __double_as_longlong(r) != __double_as_longlong(acc.v[0]);
             }
-            const int ch = __syncthreads_or(changed);
+            const int ch = vote(changed);
             cur ^= 1;
             if (!ch) break;
         }
@@ -578,25 +612,44 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                                              const double *srcy, double *dstx, double *dsty, int slot, int max_local) {
     __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
     __shared__ int sst[kEN];
+    __shared__ int vslot[3];
     if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
     const int nt = w.cnt[kCntTileExt], m = g.ml, nn = g.nl;
+    if (threadIdx.x < 3) vslot[threadIdx.x] = 0;
+    VoteOr vote{vslot, 0};
     int changed_any = 0;
+    constexpr int kRounds = (kEN + kTileThreads - 1) / kTileThreads;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const int tile = w.tiles_ext[t], tx = tile % w.ntx, ty = tile / w.ntx;
         const int I0 = tx * kTileX - kEH, J0 = ty * kTileY - kEH;
-        for (int e = threadIdx.x; e < kEN; e += blockDim.x) {
-            const int I = I0 + e % kEW, J = J0 + e / kEW;
-            if (I >= 0 && I < m && J >= 0 && J < nn) {
-                const int q = g.ls(I, J), st = a.st[q];
+        // the tile and its halo: both candidate values of a point are read,
+        // so no read waits on the status, and every read is issued before
+        // any wait
+        double rph[kRounds], rsx[kRounds], rnx[kRounds], rsy[kRounds], rny[kRounds];
+        int rst[kRounds];
+#pragma unroll
+        for (int u = 0; u < kRounds; u++) {
+            const int e = threadIdx.x + u * kTileThreads, I = I0 + e % kEW, J = J0 + e / kEW;
+            rst[u] = -1;
+            rph[u] = rsx[u] = rnx[u] = rsy[u] = rny[u] = 0;
+            if (e < kEN && I >= 0 && I < m && J >= 0 && J < nn) {
+                const int q = g.ls(I, J);
+                rst[u] = a.st[q];
+                rph[u] = a.phi[q];
+                rsx[u] = srcx[q]; rnx[u] = a.nX[q];
+                rsy[u] = srcy[q]; rny[u] = a.nY[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRounds; u++) {
+            const int e = threadIdx.x + u * kTileThreads;
+            if (e < kEN) {
+                const int st = rst[u];
                 sst[e] = st;
-                sphi[e] = a.phi[q];
-                const double vx = st >= 4 ? srcx[q] : a.nX[q], vy = st >= 4 ? srcy[q] : a.nY[q];
+                sphi[e] = rph[u];
+                const double vx = st >= 4 ? rsx[u] : rnx[u], vy = st >= 4 ? rsy[u] : rny[u];
                 sx[0][e] = sx[1][e] = vx;
                 sy[0][e] = sy[1][e] = vy;
-            } else {
-                sst[e] = -1;
-                sphi[e] = 0;
-                sx[0][e] = sx[1][e] = sy[0][e] = sy[1][e] = 0;
             }
         }
         __syncthreads();
@@ -695,7 +748,7 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                               __double_as_longlong(vy) != __double_as_longlong(cy[e]);
                 }
             }
-            const int ch = __syncthreads_or(changed);
+            const int ch = vote(changed);
             cur ^= 1;
             if (!ch) break;
         }
