@@ -606,11 +606,17 @@ __global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
 // 7 while the fit is singular. A workgroup holds its tile and a 7-point halo
 // in LDS and repeats the fit until the tile stops changing; outside points
 // read the previous values (sx,sy), inside points their predictor values.
+// A point's fit reads only neighbours of strictly smaller phi, so the
+// equations form an acyclic system with one solution; the tile therefore
+// updates in place (each fit reads whatever its neighbours hold, some
+// already updated in the same round), and a round in which nothing changes
+// leaves every point satisfying its own equation: that solution, bit for
+// bit, whatever order the wavefronts ran in.
 constexpr int kEH = 7, kEW = kTileX + 2 * kEH, kEHt = kTileY + 2 * kEH, kEN = kEW * kEHt;
 
 __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, const LsWork &w, const double *srcx,
                                              const double *srcy, double *dstx, double *dsty, int slot, int max_local) {
-    __shared__ double sphi[kEN], sx[2][kEN], sy[2][kEN];
+    __shared__ double sphi[kEN], sx[kEN], sy[kEN];
     __shared__ int sst[kEN];
     __shared__ int vslot[3];
     if (slot > 0 && w.cnt[kCntIter + slot - 1] == 0) return;  // fixed point already reached
@@ -648,8 +654,8 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                 sst[e] = st;
                 sphi[e] = rph[u];
                 const double vx = st >= 4 ? rsx[u] : rnx[u], vy = st >= 4 ? rsy[u] : rny[u];
-                sx[0][e] = sx[1][e] = vx;
-                sy[0][e] = sy[1][e] = vy;
+                sx[e] = vx;
+                sy[e] = vy;
             }
         }
         __syncthreads();
@@ -689,11 +695,10 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                 ga = (Sxx * Sy - Sxy * Sx) * det;
             }
         }
-        int cur = 0;
         for (int it = 0; it < max_local; it++) {
             int changed = 0;
             if (mine) {
-                const double *cx = sx[cur], *cy = sy[cur];
+                const double *cx = sx, *cy = sy;
                 double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
                 double a0 = al, b0 = be, g0 = ga;
                 if (!wide) {
@@ -737,24 +742,22 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                     }
                 }
                 if (singular) {
-                    atomicOr(&w.cnt[kCntErr], kErrSingular);
-                    sx[cur ^ 1][e] = cx[e];
-                    sy[cur ^ 1][e] = cy[e];
+                    atomicOr(&w.cnt[kCntErr], kErrSingular);  // the value stays
                 } else {
                     const double vx = p0 * a0 + p1 * b0 + p2 * g0, vy = p3 * a0 + p4 * b0 + p5 * g0;
-                    sx[cur ^ 1][e] = vx;
-                    sy[cur ^ 1][e] = vy;
                     changed = __double_as_longlong(vx) != __double_as_longlong(cx[e]) ||
                               __double_as_longlong(vy) != __double_as_longlong(cy[e]);
+                    if (changed) {  // only this thread writes its point
+                        sx[e] = vx;
+                        sy[e] = vy;
+                    }
                 }
             }
-            const int ch = vote(changed);
-            cur ^= 1;
-            if (!ch) break;
+            if (!vote(changed)) break;
         }
         if (mine) {
             const int q = g.ls(i, j);
-            const double vx = sx[cur][e], vy = sy[cur][e];
+            const double vx = sx[e], vy = sy[e];
             dstx[q] = vx;
             dsty[q] = vy;
             if (__double_as_longlong(vx) != __double_as_longlong(srcx[q]) ||
