@@ -87,6 +87,7 @@ _PROTOS = [
     ("incrmt_fluid_describe", _I, [_V, ctypes.c_char_p, _I]),
     ("incrmt_fluid_set_guess", _I, [_V, _I]),
     ("incrmt_fluid_set_launch", _I, [_V, _I]),
+    ("incrmt_fluid_set_overlap", _I, [_V, _I]),
     ("incrmt_fluid_set_sim_velocity", _I, [_V, _I]),
     ("incrmt_fluid_set_horizontal_flow", _I, [_V, _D]),
     ("incrmt_fluid_time_fine_smoother", _I, [_V, _I, _I, _I, _dp, _ip]),
@@ -324,6 +325,12 @@ class Fluid2D:
         the solvers pick by size: graphs below 512^2 unknowns, direct
         launches above. A call before initialize is kept and applied there."""
         _check(lib().incrmt_fluid_set_launch(self._h, mode))
+
+    def set_overlap(self, on):
+        """Run a lone solid's end-of-step band rebuild on a second stream
+        beside the FEM projection (True, the default) or after it (False);
+        bit-identical."""
+        _check(lib().incrmt_fluid_set_overlap(self._h, int(bool(on))))
 
     def step_forward(self, dt):
         _check(lib().incrmt_fluid_step(self._h, dt))

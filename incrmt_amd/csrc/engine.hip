@@ -133,6 +133,13 @@ struct FluidEngine {
     double *d_xrow = nullptr, *d_pavg = nullptr, *d_part = nullptr;
     Multigrid mac, fem;
     hipStream_t s = nullptr;
+    // The end-of-step band rebuild of a lone solid runs on a second stream
+    // beside the FEM projection and the velocity update (fork after the FEM
+    // set-up, join before the window and density updates); overlap = false
+    // keeps it on s, after the velocity update.
+    bool overlap = true;
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double mg_tol = 1e-8;
     int max_cycles = 100, precision = INCRMT_PREC_FP64;
     incrmt_stats st{};
@@ -188,6 +195,9 @@ struct FluidEngine {
         for (auto &o : obj) ls_free(o.w);
         ls_batch_free(lsb);
         for (void *p : allocs) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (s2) (void)hipStreamDestroy(s2);
         if (s) (void)hipStreamDestroy(s);
     }
     int dalloc(double *&p) {
@@ -263,6 +273,7 @@ struct FluidEngine {
     int init_tracers();
     int step(double dt);
     int pin_all(int *passes);
+    int pin_join(LsRebuild &r, int *passes);
 };
 
 // fluid_2d::init_tracers (fluid_2d.cc:1376-1408) with the reference's
@@ -497,6 +508,11 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (int rc = set_rho()) return rc;
     HIP_CHECK(hipStreamSynchronize(s));
     if (int rc = init_tracers()) return rc;
+    if (!s2) {  // default priority, like s
+        HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     initialized = true;
     return 0;
 }
@@ -520,6 +536,22 @@ int FluidEngine::pin_all(int *passes) {
         if (int rc = update_window(k)) return rc;
         if (passes) *passes = p;
     }
+    if (!const_rho) return set_rho();
+    return 0;
+}
+
+// The join of the overlapped end-of-step rebuild (the rest of what pin_all
+// does for one solid): s waits for everything enqueued on s2, also after an
+// error, then the window and the density follow the new band.
+int FluidEngine::pin_join(LsRebuild &r, int *passes) {
+    Solid &o = obj[0];
+    int p = 0;
+    const int rc = ls_pin_rebuild_end(s2, g, o.c, o.a, o.w, r, &p);
+    HIP_CHECK(hipEventRecord(ev_join, s2));
+    HIP_CHECK(hipStreamWaitEvent(s, ev_join, 0));
+    if (rc) return rc;
+    if (int e = update_window(0)) return e;
+    if (passes) *passes = p;
     if (!const_rho) return set_rho();
     return 0;
 }
@@ -587,26 +619,42 @@ int FluidEngine::step(double dt) {
     fem.t_solve = time;  // t + dt
     if ((rc = proj_begin())) return rc;
     if ((rc = guess_features(fem, 1))) return rc;
+    // From here to update_velocity nothing on s reads the level set, the
+    // band or the windows, and the end-of-step reference map is final in
+    // (nX, nY): a lone solid's end-of-step rebuild starts now on s2.
+    const bool ovl = overlap && s2 && obj.size() == 1;
+    LsRebuild pend;
+    auto drain = [&](int code) {  // an error return while s2 may still run
+        if (ovl) (void)hipStreamSynchronize(s2);
+        return code;
+    };
+    if (ovl) {
+        HIP_CHECK(hipEventRecord(ev_fork, s));
+        HIP_CHECK(hipStreamWaitEvent(s2, ev_fork, 0));
+        Solid &o0 = obj[0];
+        if ((rc = ls_pin_rebuild_begin(s2, g, o0.c, o0.a, o0.w, o0.a.nX, o0.a.nY, pend))) return drain(rc);
+    }
     w0 = std::chrono::steady_clock::now();
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
     wc[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     vcyc[1] += ms.cycles;
     nsolve[1]++;
     if (!rc) rc = proj_end();
-    if (rc == 2) return fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem");
-    if (rc) return rc;
+    if (rc == 2) return drain(fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge in FEM problem"));
+    if (rc) return drain(rc);
     st.fem_cycles = ms.cycles; st.fem_rel_res = ms.rel_res; st.total_fem_cycles += ms.cycles;
     // copy_pressure + update_velocity (fluid_2d.cc:1317-1374)
     // (no average is subtracted with zero pressure on the right edge,
     // fluid_2d.cc:1318: d_pavg stays zero)
     if (!g.zpr) {
         k_pavg_part<<<256, 256, 0, s>>>(fl.nx, fl.ny, g.px, g.py, fl.z, fl.pitch, d_part);
-        HIP_CHECK(hipGetLastError());
+        if (hipError_t e = hipGetLastError()) return drain(hip_fail(e, "k_pavg_part", __FILE__, __LINE__));
         k_pavg_final<<<1, 256, 0, s>>>(d_part, 256, 1. / (double(g.m) * g.n), d_pavg);
-        HIP_CHECK(hipGetLastError());
+        if (hipError_t e = hipGetLastError()) return drain(hip_fail(e, "k_pavg_final", __FILE__, __LINE__));
     }
-    if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return rc;
-    if ((rc = pin_all(&passes))) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
+    if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return drain(rc);
+    rc = ovl ? pin_join(pend, &passes) : pin_all(&passes);
+    if (rc) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
     for (auto &ob : obj)
         if (ob.full_steps > 0) ob.full_steps--;
     if ((rc = launch_set_boundaries(s, g, f, o))) return rc;
@@ -1368,6 +1416,12 @@ int incrmt_fluid_set_horizontal_flow(incrmt_fluid *f, double U) {
     if (!std::isfinite(U)) return fail(INCRMT_ERR_ARG, "bad flow velocity");
     Grid &g = f->e.g;
     g.zpr = 1; g.U = U; g.vel = kSimHorizFlow;
+    return 0;
+}
+
+int incrmt_fluid_set_overlap(incrmt_fluid *f, int on) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    f->e.overlap = on != 0;
     return 0;
 }
 

@@ -5,6 +5,7 @@
 
 #include <vector>
 #include "common.hh"
+#include "eikonal.hh"
 
 namespace incrmt {
 
@@ -75,6 +76,25 @@ int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs
 // levelset.cc:199-335). Returns 0, or an INCRMT_ERR_* code.
 int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
                    int *fim_iters);
+// ls_pin_rebuild in two halves, for a caller that runs it beside other
+// work on another stream s. begin enqueues the set-up and the predicted
+// batch of march passes and returns without waiting; end waits for s, runs
+// any further passes and the finish, as ls_pin_rebuild does. (pin_x,
+// pin_y), when given, stand in for (X, Y) on the interior cells in the pin.
+// Rectangle of level-set points [I0,I1) x [J0,J1) a rebuild processes.
+struct Win {
+    int I0, I1, J0, J1;
+};
+struct LsRebuild {
+    Win wn{};
+    bool whole = false, live = false;
+    int passes = 0, batch = 0;
+    EikGeom eg{};
+};
+int ls_pin_rebuild_begin(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                         const double *pin_x, const double *pin_y, LsRebuild &r);
+int ls_pin_rebuild_end(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, LsRebuild &r,
+                       int *fim_iters);
 // Make the next rebuild cover the whole grid (after phi or the status
 // codes were replaced from outside).
 void ls_reset_window(LsWork &w);

@@ -54,10 +54,6 @@ __device__ inline void wave_flag(bool pred, int *flags, int tile, int bit) {
     }
 }
 
-// Rectangle of level-set points [I0,I1) x [J0,J1) a rebuild processes.
-struct Win {
-    int I0, I1, J0, J1;
-};
 inline dim3 grid_win(const Win &v) { return dim3((v.I1 - v.I0 + BX - 1) / BX, (v.J1 - v.J0 + BY - 1) / BY); }
 constexpr int kListBlocks = 1024, kListThreads = 256;
 constexpr int kTileBlocks = 512;
@@ -232,12 +228,16 @@ __global__ void k_zero_counts(int *cnt, int n) {
 }
 
 // obj_field::pin (obj_field.cc:92-97) over the ghosted grid
-__global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn) {
+// (ix, iy), when given, stand in for (X, Y) on the interior cells: the
+// reference map update_velocity will copy there (X = nX, Y = nY), so a
+// rebuild can pin to the end-of-step map before that copy has run.
+__global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn, const double *ix, const double *iy) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
+    const bool inner = ix && I >= 2 && I < g.m + 2 && J >= 2 && J < g.n + 2;
     const double p = a.phi[ij];
-    if (p < c.pinw && p > -c.pinw) a.phi[ij] = obj_phi(c, a.X[ij], a.Y[ij]);
+    if (p < c.pinw && p > -c.pinw) a.phi[ij] = obj_phi(c, inner ? ix[ij] : a.X[ij], inner ? iy[ij] : a.Y[ij]);
 }
 
 // Sign-change scan (levelset.cc:211-263): a point is on an interface layer
@@ -820,32 +820,43 @@ __global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double
 // makes one host round trip per iteration. *passes_out: passes up to and
 // including the first one that changed nothing. Returns 0, 7 when
 // max_passes pass without convergence, or a HIP error.
+// Its two halves: enqueue one batch of passes (starting at pass `passes`)
+// with the read-back of its counters, and, once the stream has run them,
+// decide from the counters whether the iteration has ended.
+template <class F>
+int iter_enqueue(hipStream_t s, LsWork &w, int predict, int passes, F pass, int &batch) {
+    batch = predict < 2 ? 2 : predict;
+    if (batch > kIterSlots) batch = kIterSlots;
+    k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntIter, kIterSlots);
+    KCHECK();
+    for (int b = 0; b < batch; b++)
+        if (int rc = pass(passes + b, b)) return rc;
+    HIP_CHECK(hipMemcpyAsync(w.hcnt, w.cnt, (kCntIter + kIterSlots) * sizeof(int), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+bool iter_done(LsWork &w, int &predict, int &passes, int batch, int *passes_out) {
+    if (w.hcnt[kCntErr]) {
+        *passes_out = passes + batch;
+        return true;
+    }
+    for (int b = 0; b < batch; b++)
+        if (w.hcnt[kCntIter + b] == 0) {
+            *passes_out = passes + b + 1;
+            predict = passes + b + 1;
+            return true;
+        }
+    passes += batch;
+    predict = passes + 2;
+    return false;
+}
 template <class F>
 int iterate(hipStream_t s, LsWork &w, int &predict, int max_passes, F pass, int *passes_out) {
     int passes = 0;
     while (passes < max_passes) {
-        int batch = predict < 2 ? 2 : predict;
-        if (batch > kIterSlots) batch = kIterSlots;
-        k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntIter, kIterSlots);
-        KCHECK();
-        for (int b = 0; b < batch; b++) {
-            int rc = pass(passes + b, b);
-            if (rc) return rc;
-        }
-        HIP_CHECK(hipMemcpyAsync(w.hcnt, w.cnt, (kCntIter + kIterSlots) * sizeof(int), hipMemcpyDeviceToHost, s));
+        int batch;
+        if (int rc = iter_enqueue(s, w, predict, passes, pass, batch)) return rc;
         HIP_CHECK(hipStreamSynchronize(s));
-        if (w.hcnt[kCntErr]) {
-            *passes_out = passes + batch;
-            return 0;
-        }
-        for (int b = 0; b < batch; b++)
-            if (w.hcnt[kCntIter + b] == 0) {
-                *passes_out = passes + b + 1;
-                predict = passes + b + 1;
-                return 0;
-            }
-        passes += batch;
-        predict = passes + 2;
+        if (iter_done(w, predict, passes, batch, passes_out)) return 0;
     }
     *passes_out = passes;
     return 7;
@@ -1005,7 +1016,7 @@ static Win next_window(const Grid &g, LsWork &w, int R) {
 // passes, and the finish (final band and statuses, the extension's tiles).
 // *live = false: nothing of the solid is left on the grid (nothing to do).
 static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, Win &wn,
-                         bool &whole, bool *live) {
+                         bool &whole, bool *live, const double *pin_x = nullptr, const double *pin_y = nullptr) {
     const dim3 bb(BX, BY);
     // the march cannot accept a point further than limit/h + 2 cells (max
     // norm) from the interface layer: beyond it every path is longer
@@ -1018,7 +1029,7 @@ static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const 
     // the row dilation also covers R rows either side, which the column pass reads
     const Win wr{wn.I0, wn.I1, wn.J0 - R < 0 ? 0 : wn.J0 - R, wn.J1 + R > g.nl ? g.nl : wn.J1 + R};
     if (w.pin_first) {
-        k_pin<<<grid_win(wn), bb, 0, s>>>(g, c, a, wn);
+        k_pin<<<grid_win(wn), bb, 0, s>>>(g, c, a, wn, pin_x, pin_y);
         KCHECK();
     }
     k_zero_counts<<<1, 64, 0, s>>>(w.cnt, kCntIter + kIterSlots);
@@ -1064,6 +1075,17 @@ static int rebuild_finish(hipStream_t s, const Grid &g, const ObjConst &c, const
     return 0;
 }
 
+constexpr int kMaxFimPasses = 4096;
+
+// One march pass of a single solid
+static int fim_pass(hipStream_t s, const Grid &g, const EikGeom &eg, const ObjConst &c, const ObjArrays &a,
+                    const LsWork &w, int p, int slot) {
+    double *bufs[2] = {w.va, w.vb};
+    k_fim_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
+    KCHECK();
+    return 0;
+}
+
 static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
                           int *fim_iters) {
     Win wn;
@@ -1072,13 +1094,9 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
     if (!live) return 0;
     EikGeom eg;
     eg.init(g.dx, g.dy);
-    double *bufs[2] = {w.va, w.vb};
     int passes = 0;
-    int rc = iterate(s, w, w.pred_fim, 4096, [&](int p, int slot) -> int {
-        k_fim_tiles<<<kTileBlocks, kTileThreads, 0, s>>>(g, eg, c, a, w, bufs[p & 1], bufs[(p + 1) & 1], slot, 64);
-        KCHECK();
-        return 0;
-    }, &passes);
+    int rc = iterate(s, w, w.pred_fim, kMaxFimPasses,
+                     [&](int p, int slot) { return fim_pass(s, g, eg, c, a, w, p, slot); }, &passes);
     if (rc) return rc;
     if (w.hcnt[kCntErr] & kErrNewton) return 3;
     if (fim_iters) *fim_iters = passes;
@@ -1143,6 +1161,37 @@ int ls_pin_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArr
                    int *fim_iters) {
     w.pin_first = true;
     return rebuild_common(s, g, c, a, w, fim_iters);
+}
+
+int ls_pin_rebuild_begin(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                         const double *pin_x, const double *pin_y, LsRebuild &r) {
+    w.pin_first = true;
+    r = LsRebuild{};
+    if (int rc = rebuild_setup(s, g, c, a, w, r.wn, r.whole, &r.live, pin_x, pin_y)) return rc;
+    if (!r.live) return 0;
+    r.eg.init(g.dx, g.dy);
+    return iter_enqueue(s, w, w.pred_fim, 0,
+                        [&](int p, int slot) { return fim_pass(s, g, r.eg, c, a, w, p, slot); }, r.batch);
+}
+
+int ls_pin_rebuild_end(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, LsRebuild &r,
+                       int *fim_iters) {
+    if (!r.live) return 0;
+    HIP_CHECK(hipStreamSynchronize(s));
+    int passes = 0;
+    // the rest of iterate from the batch begin enqueued
+    bool done = iter_done(w, w.pred_fim, r.passes, r.batch, &passes);
+    while (!done && r.passes < kMaxFimPasses) {
+        if (int rc = iter_enqueue(s, w, w.pred_fim, r.passes,
+                                  [&](int p, int slot) { return fim_pass(s, g, r.eg, c, a, w, p, slot); }, r.batch))
+            return rc;
+        HIP_CHECK(hipStreamSynchronize(s));
+        done = iter_done(w, w.pred_fim, r.passes, r.batch, &passes);
+    }
+    if (!done) return 7;
+    if (w.hcnt[kCntErr] & kErrNewton) return 3;
+    if (fim_iters) *fim_iters = passes;
+    return rebuild_finish(s, g, c, a, w, r.wn, r.whole, passes);
 }
 
 void ls_reset_window(LsWork &w) {
