@@ -41,6 +41,18 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, par
         f.set_guess_cost(1)
     l, adt = f.timestep_select(4 * math.pi / 240)
     f.steps(adt, warmup)
+    calib = 0
+    if cost_aware:
+        # untimed, like the networks' training: steps until both projections'
+        # cost-aware rule has made its first decision (three learned and one
+        # control solve), so that the timed window is the steady state a long
+        # run spends its time in, not the first probe
+        def decided(k):
+            c = f.guess_cost_stats(k)
+            return c["off_for_cost"] or (c["learned_counted"] >= 3 and c["control_counted"] >= 1)
+        while calib < 16 and not all(decided(k) for k in (ia.MG_MAC, ia.MG_FEM)):
+            f.steps(adt, 1)
+            calib += 1
     f.sync()
     s0 = f.stats()
     t0 = time.perf_counter()
@@ -55,6 +67,8 @@ def gpu_run(m, steps, warmup, precision, timing=False, guess=0, launch=None, par
                mac_rel_res=s1["mac_rel_res"], fem_rel_res=s1["fem_rel_res"],
                reinit_passes=s1["reinit_passes"], extend_passes=s1["extend_passes"],
                smoother_shapes=shapes)
+    if cost_aware:
+        out["policy_calibration_steps"] = calib
     # projection-only time: a second pass of the same steps with HIP events
     # around the two projections (their starts included)
     f.time_projections(True)
