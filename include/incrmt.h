@@ -345,6 +345,14 @@ int incrmt_mg_time_solve(incrmt_mg *h, double rel_tol, int max_cycles, int preci
  * INCRMT_LAUNCH_DIRECT, default by size as for incrmt_fluid_set_launch). No
  * reference counterpart. */
 int incrmt_mg_set_launch(incrmt_mg *h, int mode);
+/* Mixed-precision solves (INCRMT_PREC_MIXED) as conjugate gradients in FP64
+ * preconditioned by one FP32 V-cycle per iteration (on, flexible
+ * Polak-Ribiere form), or as V-cycle refinement (off, the default). The
+ * stopping test is the same relative residual. Walled systems without a
+ * zero-pressure edge only; the others keep the refinement. No reference
+ * counterpart (the reference's TGMG is a multigrid iteration). */
+int incrmt_mg_set_krylov(incrmt_mg *h, int on);
+int incrmt_fluid_set_krylov(incrmt_fluid *f, int on);
 /* No reference counterpart (for checking the smoother): `sweeps` forward
  * colored Gauss-Seidel sweeps of A z = b on the host arrays, with fused
  * launch shape `shape` (0..16; see incrmt_fluid_describe; the FEM form of

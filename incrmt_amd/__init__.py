@@ -108,6 +108,8 @@ _PROTOS = [
     ("incrmt_mg_smooth", _I, [_V, _dp, _dp, _I, _I]),
     ("incrmt_mg_smooth_prec", _I, [_V, _dp, _dp, _I, _I, _I]),
     ("incrmt_mg_set_launch", _I, [_V, _I]),
+    ("incrmt_mg_set_krylov", _I, [_V, _I]),
+    ("incrmt_fluid_set_krylov", _I, [_V, _I]),
     ("incrmt_mg_time_smoother", _I, [_V, _I, _I, _dp, _ip]),
     ("incrmt_mg_time_shape", _I, [_V, _I, _I, _I, _dp, _ip]),
     ("incrmt_mg_destroy", None, [_V]),
@@ -326,6 +328,11 @@ class Fluid2D:
         launches above. A call before initialize is kept and applied there."""
         _check(lib().incrmt_fluid_set_launch(self._h, mode))
 
+    def set_krylov(self, on):
+        """Both projections' mixed-precision solves as FP64 conjugate
+        gradients preconditioned by one FP32 V-cycle per iteration."""
+        _check(lib().incrmt_fluid_set_krylov(self._h, int(bool(on))))
+
     def set_overlap(self, on):
         """Run a lone solid's end-of-step band rebuild on a second stream
         beside the FEM projection (True, the default) or after it (False);
@@ -527,6 +534,12 @@ class MgSolver:
         """LAUNCH_GRAPH or LAUNCH_DIRECT V-cycles (default by size: graphs
         below 512^2 unknowns); bit-identical."""
         _check(lib().incrmt_mg_set_launch(self._h, mode))
+
+    def set_krylov(self, on):
+        """Mixed-precision solves as FP64 conjugate gradients preconditioned
+        by one FP32 V-cycle per iteration (walled systems without a
+        zero-pressure edge), instead of V-cycle refinement."""
+        _check(lib().incrmt_mg_set_krylov(self._h, int(bool(on))))
 
     def apply(self, x):
         xx = np.ascontiguousarray(x, dtype=np.float64).ravel()

@@ -1655,6 +1655,18 @@ int incrmt_mg_time_solve(incrmt_mg *h, double rel_tol, int max_cycles, int preci
     return rc == 2 ? fail(INCRMT_ERR_NOT_CONVERGED, "V-cycle failed to converge") : 0;
 }
 
+int incrmt_mg_set_krylov(incrmt_mg *h, int on) {
+    if (!h) return fail(INCRMT_ERR_ARG, "null handle");
+    h->mg.krylov = on != 0;
+    return 0;
+}
+
+int incrmt_fluid_set_krylov(incrmt_fluid *f, int on) {
+    if (!f) return fail(INCRMT_ERR_ARG, "null handle");
+    f->e.mac.krylov = f->e.fem.krylov = on != 0;
+    return 0;
+}
+
 int incrmt_mg_set_launch(incrmt_mg *h, int mode) {
     if (!h) return fail(INCRMT_ERR_ARG, "null handle");
     if (mode != INCRMT_LAUNCH_DIRECT && mode != INCRMT_LAUNCH_GRAPH) return fail(INCRMT_ERR_ARG, "bad launch mode");

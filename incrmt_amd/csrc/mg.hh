@@ -176,8 +176,12 @@ class Multigrid {
     // Solve with the right-hand side in level 0's b and the warm start in z.
     int solve(double tol, int max_cycles, MgStats *st);
     // Mixed-precision solve: FP64 residual and correction accumulation,
-    // FP32 V-cycles (iterative refinement).
+    // FP32 V-cycles (iterative refinement); with krylov set, conjugate
+    // gradients in FP64 preconditioned by one FP32 V-cycle per iteration
+    // (on walled levels without a zero-pressure edge; elsewhere the
+    // refinement).
     int solve_mixed(double tol, int max_cycles, MgStats *st);
+    bool krylov = false;
     // Launch `sweeps` fine-level smoothing sweeps (for timing the smoother).
     int smooth_fine(int sweeps, bool fp32);
     // `sweeps` forward sweeps on level 0 (FP64, or FP32 on the rounded
@@ -255,8 +259,13 @@ class Multigrid {
     bool want_f32_ = false;          // a mixed solve has run: coefficient updates write FP32 copies too
     int make_f32_coefs();
     int wrap_coefs(size_t l, bool f32 = false);
-    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // FP64 cycle, mixed cycle
-    std::vector<uintptr_t> graph_key_[2];                 // launch state each graph was captured in
+    hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};  // FP64 cycle, mixed cycle, Krylov iteration
+    std::vector<uintptr_t> graph_key_[3];                          // launch state each graph was captured in
+    // Krylov solve: residual, direction, preconditioned residual, A times
+    // the direction (FP64 level-0 arrays) and its device scalars
+    double *kr_ = nullptr, *kp_ = nullptr, *kc_ = nullptr, *kq_ = nullptr, *ks_ = nullptr;
+    int krylov_buffers();
+    int solve_krylov(double tol, int max_cycles, MgStats *st);
     std::vector<uintptr_t> launch_state() const;
     void restore_buffers(const std::vector<uintptr_t> &key);
     int run_cycle(int which, const std::function<int()> &cycle);

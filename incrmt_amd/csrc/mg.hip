@@ -1255,6 +1255,117 @@ __global__ void k_residual_f32(int kind, LevelArgs a, FemC k, const double *z, c
     bf[q] = float(b[q] - off - diag * z[q]);
     put_z(a, zf, i, j, 0.0f);
 }
+// ---- Conjugate gradients preconditioned by the FP32 V-cycle (mixed
+// solves, flexible form). Vectors in FP64 on level 0: r the residual, c the
+// V-cycle's correction for r, p the direction, q = A p. Scalars on the
+// device (ks): the step's r.c, beta, alpha, and a flag for the first
+// direction. Reductions are fixed-order partial sums (kRedBlocks blocks,
+// then k_sum_partials), so a solve is deterministic.
+enum { kKrRho = 0, kKrBeta, kKrAlpha, kKrFirst, kKrSums = 4 };  // sums: 3 doubles at ks + kKrSums
+
+// block sum of three values into part[3 * blockIdx.x + k]
+__device__ void block_sum3(double v0, double v1, double v2, double *part) {
+    __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
+    s0[threadIdx.x] = v0; s1[threadIdx.x] = v1; s2[threadIdx.x] = v2;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0[0];
+        part[3 * blockIdx.x + 1] = s1[0];
+        part[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+// r = b - A z, its FP32 copy for the V-cycle, zf = 0; the next direction is
+// the first
+__global__ void k_kr_init(int kind, LevelArgs a, FemC k, const double *z, const double *b, const double *fw,
+                          const double *fs, const double *kc, double *r, float *bf, float *zf, double *ks) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i == 0 && j == 0) ks[kKrFirst] = 1;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i, c = j * a.cp + i;
+    double diag, off;
+    if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+    else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
+    const double rq = b[q] - off - diag * z[q];
+    r[q] = rq;
+    bf[q] = float(rq);
+    put_z(a, zf, i, j, 0.0f);
+}
+// c <- the V-cycle's correction in FP64; partial sums of r.c (new) and r.c (old)
+__global__ void k_kr_dots(LevelArgs a, const float *zf, const double *r, double *c, double *part) {
+    double rn = 0, ro = 0;
+    const int N = a.nx * a.ny;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        const int q = (p / a.nx) * a.zp + p % a.nx;
+        const double cn = double(zf[q]), rq = r[q];
+        rn += rq * cn;
+        ro += rq * c[q];
+        c[q] = cn;
+    }
+    block_sum3(rn, ro, 0.0, part);
+}
+// beta = r.(c - c_old) / (r.c)_old (Polak-Ribiere, clamped at 0: a restart),
+// 0 for the first direction
+__global__ void k_kr_beta(double *ks) {
+    const double rn = ks[kKrSums], ro = ks[kKrSums + 1], rho = ks[kKrRho];
+    double beta = 0;
+    if (ks[kKrFirst] == 0 && rho != 0) beta = (rn - ro) / rho;
+    if (!(beta > 0) || !isfinite(beta)) beta = 0;
+    ks[kKrBeta] = beta;
+    ks[kKrRho] = rn;
+    ks[kKrFirst] = 0;
+}
+// p <- c + beta p
+__global__ void k_kr_dir(LevelArgs a, const double *c, double *p, const double *ks) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i;
+    put_z(a, p, i, j, c[q] + ks[kKrBeta] * p[q]);
+}
+// q = A p; partial sums of p.q
+__global__ void k_kr_apply(int kind, LevelArgs a, FemC k, const double *p, const double *fw, const double *fs,
+                           const double *kc, double *qv, double *part) {
+    double pq = 0;
+    const int N = a.nx * a.ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        const int i = t % a.nx, j = t / a.nx, q = j * a.zp + i, c = j * a.cp + i;
+        double diag, off;
+        if (kind == kMAC) off = mac_offdiag<double>(p, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
+        else off = fem_offdiag<double>(p, kc, q, c, a.zp, a.cp, k, diag);
+        const double ap = off + diag * p[q];
+        qv[q] = ap;
+        pq += p[q] * ap;
+    }
+    block_sum3(pq, 0.0, 0.0, part);
+}
+// alpha = (r.c) / (p.q), 0 when that breaks down
+__global__ void k_kr_alpha(double *ks) {
+    const double pq = ks[kKrSums];
+    double al = pq != 0 ? ks[kKrRho] / pq : 0;
+    if (!isfinite(al)) al = 0;
+    ks[kKrAlpha] = al;
+}
+// z += alpha p, r -= alpha q; the new residual's FP32 copy, zf = 0
+__global__ void k_kr_update(LevelArgs a, double *z, double *r, const double *p, const double *qv, const double *ks,
+                            float *bf, float *zf) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.nx || j >= a.ny) return;
+    const int q = j * a.zp + i;
+    const double al = ks[kKrAlpha];
+    put_z(a, z, i, j, z[q] + al * p[q]);
+    const double rn = r[q] - al * qv[q];
+    r[q] = rn;
+    bf[q] = float(rn);
+    put_z(a, zf, i, j, 0.0f);
+}
+
 // Starting iterate from the two previous solutions: z <- cb z - cc zp, zp <- z.
 __global__ void k_extrapolate(LevelArgs a, double *z, double *zp, double cb, double cc) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -2749,11 +2860,73 @@ int Multigrid::solve(double tol, int max_cycles, MgStats *st) {
     return solve_loop(tol, max_cycles, st, [&]() { return run_cycle(0, [&]() { return vcycle(0); }); });
 }
 
+int Multigrid::krylov_buffers() {
+    if (kr_) return 0;
+    const MgLevel &f = L[0];
+    const size_t len = (size_t)(f.ny + 2) * f.pitch + 64;
+    for (double **p : {&kr_, &kp_, &kc_, &kq_}) {
+        void *q;
+        HIP_CHECK(hipMalloc(&q, len * sizeof(double)));
+        HIP_CHECK(memset_sync(q, 0, len * sizeof(double), s_));  // finite rings and a finite first p
+        allocs.push_back(q);
+        *p = (double *)q + f.pitch + 1;
+    }
+    void *q;
+    const size_t ns = kKrSums + 3 + 3 * (size_t)kRedBlocks;  // scalars, sums, partials
+    HIP_CHECK(hipMalloc(&q, ns * sizeof(double)));
+    HIP_CHECK(memset_sync(q, 0, ns * sizeof(double), s_));
+    allocs.push_back(q);
+    ks_ = (double *)q;
+    return 0;
+}
+
+// The Krylov form of solve_mixed: one iteration is one FP32 V-cycle on the
+// current residual, then the conjugate-gradient step (the solve loop's
+// "cycle"). The stopping test is the same true relative residual.
+int Multigrid::solve_krylov(double tol, int max_cycles, MgStats *st) {
+    if (int rc = krylov_buffers()) return rc;
+    const FemC k = fem_consts(dx_, dy_);
+    double *part = ks_ + kKrSums + 3;
+    bool started = false;
+    return solve_loop(tol, max_cycles, st, [&]() -> int {
+        const MgLevel &f = L[0];
+        if (!started) {  // the residual of the solve's (possibly learned) start
+            k_kr_init<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s_>>>(kind_, args_of(f), k, f.z, f.b, f.fw, f.fs, f.kc,
+                                                                   kr_, f.bf, f.zf, ks_);
+            KCHECK();
+            started = true;
+        }
+        return run_cycle(2, [&]() -> int {
+            const MgLevel &g = L[0];
+            const LevelArgs a = args_of(g);
+            if (int rc = vcycle_f(0)) return rc;
+            k_kr_dots<<<kRedBlocks, kRedThreads, 0, s_>>>(a, g.zf, kr_, kc_, part);
+            KCHECK();
+            k_sum_partials<<<1, kRedThreads, 0, s_>>>(part, kRedBlocks, ks_ + kKrSums);
+            KCHECK();
+            k_kr_beta<<<1, 1, 0, s_>>>(ks_);
+            KCHECK();
+            k_kr_dir<<<grid2(g.nx, g.ny), dim3(BX, BY), 0, s_>>>(a, kc_, kp_, ks_);
+            KCHECK();
+            k_kr_apply<<<kRedBlocks, kRedThreads, 0, s_>>>(kind_, a, k, kp_, g.fw, g.fs, g.kc, kq_, part);
+            KCHECK();
+            k_sum_partials<<<1, kRedThreads, 0, s_>>>(part, kRedBlocks, ks_ + kKrSums);
+            KCHECK();
+            k_kr_alpha<<<1, 1, 0, s_>>>(ks_);
+            KCHECK();
+            k_kr_update<<<grid2(g.nx, g.ny), dim3(BX, BY), 0, s_>>>(a, g.z, kr_, kp_, kq_, ks_, g.bf, g.zf);
+            KCHECK();
+            return 0;
+        });
+    });
+}
+
 int Multigrid::solve_mixed(double tol, int max_cycles, MgStats *st) {
     const MgLevel &f = L[0];
     want_f32_ = true;
     if (f32_stale_)
         if (int rc = make_f32_coefs()) return rc;
+    if (krylov && !f.px && !f.py && !f.dr && !resampled_) return solve_krylov(tol, max_cycles, st);
     const FemC k = fem_consts(dx_, dy_);
     // FP64 residual -> FP32 right-hand side, FP32 V-cycle, FP64 update
     return solve_loop(tol, max_cycles, st, [&]() -> int {
