@@ -508,11 +508,6 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
     if (int rc = set_rho()) return rc;
     HIP_CHECK(hipStreamSynchronize(s));
     if (int rc = init_tracers()) return rc;
-    if (!s2) {  // default priority, like s
-        HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    }
     initialized = true;
     return 0;
 }
@@ -622,7 +617,15 @@ int FluidEngine::step(double dt) {
     // From here to update_velocity nothing on s reads the level set, the
     // band or the windows, and the end-of-step reference map is final in
     // (nX, nY): a lone solid's end-of-step rebuild starts now on s2.
-    const bool ovl = overlap && s2 && obj.size() == 1;
+    const bool ovl = overlap && obj.size() == 1;
+    if (ovl && !s2) {
+        // made on first use only: a stream per fluid that never overlaps
+        // (an ensemble's instances) would still take a share of the
+        // process's hardware queues from the streams that do the work
+        HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));  // default priority, like s
+        HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     LsRebuild pend;
     auto drain = [&](int code) {  // an error return while s2 may still run
         if (ovl) (void)hipStreamSynchronize(s2);

@@ -25,6 +25,12 @@ class Ensemble:
         self.threads = threads
         self.members = [simple_spin(m, omega=om, rel_tol=rel_tol, precision=precision, guess=guess)
                         for m, om in self.specs]
+        # the instances already run side by side, and a second stream per
+        # instance (made by the first overlapped step) would share the
+        # process's hardware queues with the streams that do the work: the
+        # end-of-step rebuild stays in order
+        for f in self.members:
+            f.set_overlap(False)
         self.adt = [f.timestep_select(frame)[1] for f in self.members]
         # longest jobs first (largest grids), so the pool ends evenly
         self.order = sorted(range(len(self.members)), key=lambda i: -self.specs[i][0])
