@@ -118,6 +118,7 @@ struct FluidEngine {
     ObjConst *d_oc = nullptr;
     ObjArrays *d_oa = nullptr;
     SolidWin *d_win = nullptr;  // per-solid sweep windows (device)
+    const int **d_bbox = nullptr;  // the solids' band boxes (device array of their LsWork::bbox)
     bool windows = true;        // windowed per-solid sweeps and batched band passes (else
                                 // whole-grid sweeps and one solid's passes at a time)
     LsBatch lsb{};              // batched band passes of the solids
@@ -217,6 +218,13 @@ struct FluidEngine {
     }
     int update_window(size_t k, bool full = false) {
         return launch_solid_window(s, g, obj[k].w.bbox, d_win + k, full || !windows || obj[k].full_steps > 0);
+    }
+    // every solid's window in one launch
+    int update_windows() {
+        unsigned long long mask = 0;
+        for (size_t k = 0; k < obj.size(); k++)
+            if (!windows || obj[k].full_steps > 0) mask |= 1ull << k;
+        return launch_solid_windows(s, g, d_bbox, d_win, (int)obj.size(), mask);
     }
 
     // fluid_2d::choose_ev_and_dt + obj_field::set_ex_visc (fluid_2d.cc:152-192, obj_field.cc:35-49)
@@ -472,9 +480,14 @@ int FluidEngine::initialize(double ev_mult, double dt_pad, double dt_ev_pad) {
         HIP_CHECK(hipMalloc((void **)&d_oc, obj.size() * sizeof(ObjConst)));
         HIP_CHECK(hipMalloc((void **)&d_oa, obj.size() * sizeof(ObjArrays)));
         HIP_CHECK(hipMalloc((void **)&d_win, obj.size() * sizeof(SolidWin)));
+        HIP_CHECK(hipMalloc((void **)&d_bbox, obj.size() * sizeof(int *)));
         allocs.push_back(d_oc);
         allocs.push_back(d_oa);
         allocs.push_back(d_win);
+        allocs.push_back((void *)d_bbox);
+        std::vector<const int *> hb;
+        for (auto &o : obj) hb.push_back(o.w.bbox);
+        HIP_CHECK(hipMemcpy((void *)d_bbox, hb.data(), hb.size() * sizeof(int *), hipMemcpyHostToDevice));
         std::vector<ObjConst> hc;
         std::vector<ObjArrays> ha;
         for (auto &o : obj) { hc.push_back(o.c); ha.push_back(o.a); }
@@ -518,8 +531,7 @@ int FluidEngine::pin_all(int *passes) {
             if (int rc = ls_batch_alloc((int)obj.size(), lsb)) return rc;
         int p = 0;
         if (int rc = ls_pin_rebuild_batch(s, g, jobs(), lsb, &p)) return rc;
-        for (size_t k = 0; k < obj.size(); k++)
-            if (int rc = update_window(k)) return rc;
+        if (int rc = update_windows()) return rc;
         if (passes) *passes = p;
         if (!const_rho) return set_rho();
         return 0;

@@ -63,6 +63,13 @@ struct LsBatch {
     void *d_items = nullptr, *h_items = nullptr;
     int *d_cnt = nullptr, *h_cnt = nullptr;
     int cap = 0;
+    // per-solid items of the shared set-up, finish and copy launches: four
+    // regions (set-up, finish, copy-in, copy-out) in pinned memory and on
+    // the device, each rewritten only after its last upload has run
+    void *d_setup = nullptr, *h_setup = nullptr;
+    size_t setup_stride = 0;
+    hipEvent_t ev_setup[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool setup_used[4] = {false, false, false, false};
 };
 int ls_batch_alloc(int cap, LsBatch &bt);
 void ls_batch_free(LsBatch &bt);

@@ -231,7 +231,8 @@ __global__ void k_zero_counts(int *cnt, int n) {
 // (ix, iy), when given, stand in for (X, Y) on the interior cells: the
 // reference map update_velocity will copy there (X = nX, Y = nY), so a
 // rebuild can pin to the end-of-step map before that copy has run.
-__global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn, const double *ix, const double *iy) {
+__device__ __forceinline__ void pin_body(const Grid &g, const ObjConst &c, const ObjArrays &a, const Win &wn,
+                                         const double *ix, const double *iy) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
@@ -239,11 +240,14 @@ __global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn, const double *ix,
     const double p = a.phi[ij];
     if (p < c.pinw && p > -c.pinw) a.phi[ij] = obj_phi(c, inner ? ix[ij] : a.X[ij], inner ? iy[ij] : a.Y[ij]);
 }
+__global__ void k_pin(Grid g, ObjConst c, ObjArrays a, Win wn, const double *ix, const double *iy) {
+    pin_body(g, c, a, wn, ix, iy);
+}
 
 // Sign-change scan (levelset.cc:211-263): a point is on an interface layer
 // when a 4-neighbour has the other sign. Interface points are listed for
 // the Newton solves; the fixed-point buffers start at +-inf elsewhere.
-__global__ void k_scan(Grid g, ObjArrays a, LsWork w, Win wn) {
+__device__ __forceinline__ void scan_body(const Grid &g, const ObjArrays &a, const LsWork &w, const Win &wn) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), P = g.P;
@@ -262,12 +266,13 @@ __global__ void k_scan(Grid g, ObjArrays a, LsWork w, Win wn) {
         w.vb[ij] = inf;
     }
 }
+__global__ void k_scan(Grid g, ObjArrays a, LsWork w, Win wn) { scan_body(g, a, w, wn); }
 
 // Exact closest-point distances of both interface layers, from the pinned
 // phi (levelset.cc:266-291). The results go to the fixed-point buffers; phi
 // itself keeps the pinned values until the band is finalised, since every
 // Newton solve reads the pinned field.
-__global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) {
+__device__ __forceinline__ void newton_body(const Grid &g, const LsGeom &lg, const ObjArrays &a, const LsWork &w) {
     const int n = w.cnt[kCntIface];
     const double *b0 = a.phi + g.ls(0, 0);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -286,11 +291,13 @@ __global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) {
         w.vb[ij] = v;
     }
 }
+__global__ void k_newton(Grid g, LsGeom lg, ObjArrays a, LsWork w) { newton_body(g, lg, a, w); }
 
 // Candidate points: non-interface points within R cells (max norm) of an
 // interface point, R covering the band cut-offs; separable dilation, rows
 // then columns.
-__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
+__device__ __forceinline__ void dilate_rows_body(const Grid &g, const ObjArrays &a, const LsWork &w, int R,
+                                                 const Win &wn) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
@@ -303,7 +310,9 @@ __global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
     }
     w.mark[ij] = hit;
 }
-__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
+__global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R, Win wn) { dilate_rows_body(g, a, w, R, wn); }
+__device__ __forceinline__ void dilate_cols_body(const Grid &g, const ObjArrays &a, const LsWork &w, int R,
+                                                 const Win &wn) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), s = a.st[ij];
@@ -318,6 +327,7 @@ __global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) {
     wave_append(hit, &w.cnt[kCntCand], w.cand, ij);
     wave_flag(hit, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 1);
 }
+__global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) { dilate_cols_body(g, a, w, R, wn); }
 
 __global__ void k_bbox_init(LsWork w) {
     if (threadIdx.x == 0) {
@@ -327,7 +337,7 @@ __global__ void k_bbox_init(LsWork w) {
 }
 
 // Collect the tiles whose flag has `bit` set into a list, clearing the bit.
-__global__ void k_compact_tiles(LsWork w, int bit, int *list, int slot) {
+__device__ __forceinline__ void compact_tiles_body(const LsWork &w, int bit, int *list, int slot) {
     const int n = w.ntx * w.nty;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const int f = w.tflag[t];
@@ -338,6 +348,7 @@ __global__ void k_compact_tiles(LsWork w, int bit, int *list, int slot) {
         }
     }
 }
+__global__ void k_compact_tiles(LsWork w, int bit, int *list, int slot) { compact_tiles_body(w, bit, list, slot); }
 
 // Known-neighbour test of the heap march (heap.hh:59-83) at the fixed point:
 // the other side and the interface layer are always known; a same-side
@@ -532,7 +543,8 @@ __global__ __launch_bounds__(kTileThreads) void k_fim_tiles_batch(Grid g, EikGeo
 // band points are listed for the extension.
 __device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, int I, int J,
                                int (*bb)[BX * BY], int lane);
-__global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, Win wn) {
+__device__ __forceinline__ void finalize_body(const Grid &g, const ObjConst &c, const ObjArrays &a, const LsWork &w,
+                                              const double *v, const Win &wn) {
     __shared__ int bb[4][BX * BY];
     const int t = threadIdx.y * BX + threadIdx.x;
     bb[0][t] = bb[2][t] = 1 << 30;
@@ -556,6 +568,9 @@ __global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const doub
         atomicMin(&w.bbox[2], bb[2][0]);
         atomicMax(&w.bbox[3], bb[3][0]);
     }
+}
+__global__ void k_finalize(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, Win wn) {
+    finalize_body(g, c, a, w, v, wn);
 }
 __device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const double *v, int I, int J,
                                int (*bb)[BX * BY], int lane) {
@@ -593,12 +608,13 @@ __device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const 
     wave_append(pb, &w.cnt[kCntPband], w.pband, ij);
     wave_flag(pb, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 2);
 }
-__global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) {
+__device__ __forceinline__ void commit_status_body(const Grid &g, const ObjArrays &a, const LsWork &w, const Win &wn) {
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
     a.st[ij] = w.mark[ij];
 }
+__global__ void k_commit_status(Grid g, ObjArrays a, LsWork w, Win wn) { commit_status_body(g, a, w, wn); }
 
 // One pass of the extension (traverse.hh:169-231) over the active tiles:
 // outside band points take a weighted least-squares linear fit of (nX,nY)
@@ -801,13 +817,84 @@ __global__ void k_batch_gather(const Item *it, int n, int batch, int *out) {
 
 // Seed both extension buffers with the current nX,nY of the outside band,
 // or copy the converged buffer back into nX,nY.
-__global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double *fy, int to_obj) {
+__device__ __forceinline__ void ext_copy_body(const ObjArrays &a, const LsWork &w, const double *fx, const double *fy,
+                                              int to_obj) {
     const int n = w.cnt[kCntPband];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int ij = w.pband[k];
         if (to_obj) { a.nX[ij] = fx[ij]; a.nY[ij] = fy[ij]; }
         else { w.ex[ij] = w.fx[ij] = a.nX[ij]; w.ey[ij] = w.fy[ij] = a.nY[ij]; }
     }
+}
+__global__ void k_ext_copy(ObjArrays a, LsWork w, const double *fx, const double *fy, int to_obj) {
+    ext_copy_body(a, w, fx, fy, to_obj);
+}
+
+// ---- the set-up and finish of several solids' rebuilds (and the copies
+// around their extensions) in shared launches: workgroup (x, y, z) does
+// what workgroup (x, y) of the single-solid launch does for solid z, whose
+// windows and parameters are item z (grids sized for the largest window;
+// workgroups past a solid's window do nothing). The solids' arrays and
+// counters are their own, so each gets exactly what its own launches give.
+struct SetupItem {
+    ObjConst c;
+    ObjArrays a;
+    LsWork w;
+    Win wn, wr;                   // the rebuild's window, the row dilation's
+    int R;                        // dilation radius
+    int pin;                      // pin before the rebuild
+    const double *pin_x, *pin_y;  // (nX, nY) standing in for (X, Y) inside, or null
+    const double *v;              // finish: the fixed-point buffer of the last pass
+    const double *fx, *fy;        // extension copy-out: the buffers of the last pass
+};
+__global__ void k_pin_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    if (x.pin) pin_body(g, x.c, x.a, x.wn, x.pin_x, x.pin_y);
+}
+__global__ void k_counts_b(const SetupItem *it, int from, int n) {
+    int *cnt = it[blockIdx.z].w.cnt + from;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) cnt[k] = 0;
+}
+__global__ void k_scan_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    scan_body(g, x.a, x.w, x.wn);
+}
+__global__ void k_newton_b(Grid g, LsGeom lg, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    newton_body(g, lg, x.a, x.w);
+}
+__global__ void k_dilate_rows_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    dilate_rows_body(g, x.a, x.w, x.R < g.ml ? x.R : g.ml, x.wr);
+}
+__global__ void k_dilate_cols_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    dilate_cols_body(g, x.a, x.w, x.R < g.nl ? x.R : g.nl, x.wn);
+}
+__global__ void k_compact_tiles_b(const SetupItem *it, int ext) {
+    const LsWork &w = it[blockIdx.z].w;
+    if (ext) compact_tiles_body(w, 2, w.tiles_ext, kCntTileExt);
+    else compact_tiles_body(w, 1, w.tiles_fim, kCntTileFim);
+}
+__global__ void k_bbox_init_b(const SetupItem *it) {
+    const LsWork &w = it[blockIdx.z].w;
+    if (threadIdx.x == 0) {
+        w.cnt[kCntPband] = 0;
+        w.bbox[0] = w.bbox[2] = 1 << 30;
+        w.bbox[1] = w.bbox[3] = -1;
+    }
+}
+__global__ void k_finalize_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    finalize_body(g, x.c, x.a, x.w, x.v, x.wn);
+}
+__global__ void k_commit_status_b(Grid g, const SetupItem *it) {
+    const SetupItem &x = it[blockIdx.z];
+    commit_status_body(g, x.a, x.w, x.wn);
+}
+__global__ void k_ext_copy_b(const SetupItem *it, int to_obj) {
+    const SetupItem &x = it[blockIdx.z];
+    ext_copy_body(x.a, x.w, x.fx, x.fy, to_obj);
 }
 
 #define KCHECK() HIP_CHECK(hipGetLastError())
@@ -929,6 +1016,10 @@ int ls_batch_alloc(int cap, LsBatch &bt) {
     HIP_CHECK(hipHostMalloc(&bt.h_items, bytes, hipHostMallocDefault));
     HIP_CHECK(hipMalloc((void **)&bt.d_cnt, (size_t)cap * (kIterSlots + 1) * sizeof(int)));
     HIP_CHECK(hipHostMalloc((void **)&bt.h_cnt, (size_t)cap * (kIterSlots + 1) * sizeof(int), hipHostMallocDefault));
+    bt.setup_stride = ((size_t)cap * sizeof(SetupItem) + 255) / 256 * 256;
+    HIP_CHECK(hipMalloc(&bt.d_setup, 4 * bt.setup_stride));
+    HIP_CHECK(hipHostMalloc(&bt.h_setup, 4 * bt.setup_stride, hipHostMallocDefault));
+    for (hipEvent_t &e : bt.ev_setup) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     bt.cap = cap;
     return 0;
 }
@@ -938,6 +1029,10 @@ void ls_batch_free(LsBatch &bt) {
     if (bt.h_items) (void)hipHostFree(bt.h_items);
     if (bt.d_cnt) (void)hipFree(bt.d_cnt);
     if (bt.h_cnt) (void)hipHostFree(bt.h_cnt);
+    if (bt.d_setup) (void)hipFree(bt.d_setup);
+    if (bt.h_setup) (void)hipHostFree(bt.h_setup);
+    for (hipEvent_t e : bt.ev_setup)
+        if (e) (void)hipEventDestroy(e);
     bt = LsBatch{};
 }
 
@@ -1015,19 +1110,28 @@ static Win next_window(const Grid &g, LsWork &w, int R) {
 // closest points, dilation to the march region, active tiles), the march
 // passes, and the finish (final band and statuses, the extension's tiles).
 // *live = false: nothing of the solid is left on the grid (nothing to do).
-static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, Win &wn,
-                         bool &whole, bool *live, const double *pin_x = nullptr, const double *pin_y = nullptr) {
-    const dim3 bb(BX, BY);
+// The host side of a rebuild's set-up: its window wn, the row dilation's
+// window wr, the dilation radius R; false when the window is empty.
+static bool rebuild_window(const Grid &g, const ObjConst &c, LsWork &w, Win &wn, Win &wr, bool &whole, int &R) {
     // the march cannot accept a point further than limit/h + 2 cells (max
     // norm) from the interface layer: beyond it every path is longer
     const double reach = fmax(-c.phi_down, c.phi_up) / fmin(g.dx, g.dy) + 3;
-    const int R = reach > g.ml + g.nl ? g.ml + g.nl : (int)reach;
+    R = reach > g.ml + g.nl ? g.ml + g.nl : (int)reach;
     whole = !w.have_win;
     wn = next_window(g, w, R);
-    *live = !(wn.I1 <= wn.I0 || wn.J1 <= wn.J0);
-    if (!*live) return 0;
+    if (wn.I1 <= wn.I0 || wn.J1 <= wn.J0) return false;
     // the row dilation also covers R rows either side, which the column pass reads
-    const Win wr{wn.I0, wn.I1, wn.J0 - R < 0 ? 0 : wn.J0 - R, wn.J1 + R > g.nl ? g.nl : wn.J1 + R};
+    wr = Win{wn.I0, wn.I1, wn.J0 - R < 0 ? 0 : wn.J0 - R, wn.J1 + R > g.nl ? g.nl : wn.J1 + R};
+    return true;
+}
+
+static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, Win &wn,
+                         bool &whole, bool *live, const double *pin_x = nullptr, const double *pin_y = nullptr) {
+    const dim3 bb(BX, BY);
+    Win wr;
+    int R;
+    *live = rebuild_window(g, c, w, wn, wr, whole, R);
+    if (!*live) return 0;
     if (w.pin_first) {
         k_pin<<<grid_win(wn), bb, 0, s>>>(g, c, a, wn, pin_x, pin_y);
         KCHECK();
@@ -1047,16 +1151,9 @@ static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const 
     return 0;
 }
 
-static int rebuild_finish(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
-                          const Win &wn, bool whole, int passes) {
-    const dim3 bb(BX, BY);
-    double *bufs[2] = {w.va, w.vb};
-    k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
-    KCHECK();
-    k_bbox_init<<<1, 64, 0, s>>>(w);
-    KCHECK();
-    k_finalize<<<grid_win(wn), bb, 0, s>>>(g, c, a, w, bufs[passes & 1], wn);
-    KCHECK();
+// The host side of a rebuild's finish (after the finalisation): the band
+// box's read-back and the window the next rebuild grows from.
+static int finish_window(hipStream_t s, LsWork &w, const Win &wn, bool whole) {
     if (w.ev_bbox) {  // a work set without the event keeps no window (one-off rebuilds)
         HIP_CHECK(hipMemcpyAsync(w.hbbox, w.bbox, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(w.ev_bbox, s));
@@ -1068,6 +1165,20 @@ static int rebuild_finish(hipStream_t s, const Grid &g, const ObjConst &c, const
     } else {
         w.win[0] = wn.I0; w.win[1] = wn.I1; w.win[2] = wn.J0; w.win[3] = wn.J1;
     }
+    return 0;
+}
+
+static int rebuild_finish(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w,
+                          const Win &wn, bool whole, int passes) {
+    const dim3 bb(BX, BY);
+    double *bufs[2] = {w.va, w.vb};
+    k_zero_counts<<<1, 64, 0, s>>>(w.cnt + kCntPband, 1);
+    KCHECK();
+    k_bbox_init<<<1, 64, 0, s>>>(w);
+    KCHECK();
+    k_finalize<<<grid_win(wn), bb, 0, s>>>(g, c, a, w, bufs[passes & 1], wn);
+    KCHECK();
+    if (int rc = finish_window(s, w, wn, whole)) return rc;
     k_compact_tiles<<<64, 256, 0, s>>>(w, 2, w.tiles_ext, kCntTileExt);
     KCHECK();
     k_commit_status<<<grid_win(wn), bb, 0, s>>>(g, a, w, wn);
@@ -1107,46 +1218,118 @@ static int rebuild_common(hipStream_t s, const Grid &g, const ObjConst &c, const
 // at least 64
 static int batch_blocks(int n) { return std::max(64, kTileBlocks / std::max(1, (n + 3) / 4)); }
 
+// Upload the items of one region (0 set-up, 1 finish, 2 copy-in, 3
+// copy-out) after its last upload has run; the device copy of the items.
+static const SetupItem *upload_items(hipStream_t s, LsBatch &bt, int region, const std::vector<SetupItem> &v,
+                                     int *rc) {
+    *rc = 0;
+    if (bt.setup_used[region]) {
+        const hipError_t e = hipEventSynchronize(bt.ev_setup[region]);
+        if (e != hipSuccess) {
+            *rc = hip_fail(e, "hipEventSynchronize", __FILE__, __LINE__);
+            return nullptr;
+        }
+    }
+    SetupItem *h = (SetupItem *)((char *)bt.h_setup + region * bt.setup_stride);
+    SetupItem *d = (SetupItem *)((char *)bt.d_setup + region * bt.setup_stride);
+    std::copy(v.begin(), v.end(), h);
+    hipError_t e = hipMemcpyAsync(d, h, v.size() * sizeof(SetupItem), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(bt.ev_setup[region], s);
+    if (e != hipSuccess) {
+        *rc = hip_fail(e, "upload of batch items", __FILE__, __LINE__);
+        return nullptr;
+    }
+    bt.setup_used[region] = true;
+    return d;
+}
+// a grid covering the largest of the items' windows (rows: their row windows)
+static dim3 grid_items(const std::vector<SetupItem> &v, bool rows) {
+    unsigned gx = 1, gy = 1;
+    for (const SetupItem &x : v) {
+        const dim3 d = grid_win(rows ? x.wr : x.wn);
+        gx = std::max(gx, d.x);
+        gy = std::max(gy, d.y);
+    }
+    return dim3(gx, gy, (unsigned)v.size());
+}
+static int list_blocks(size_t n) { return std::max(16, kListBlocks / (int)std::max<size_t>(1, n)); }
+
 int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
                          int *fim_iters) {
     const size_t n = jobs.size();
     if (n > (size_t)bt.cap) return 7;
     std::vector<Win> wn(n);
-    std::vector<char> whole(n), live(n);
+    std::vector<char> whole(n);
     std::vector<FimItem> items;
+    std::vector<SetupItem> su;
     std::vector<int *> pred;
     std::vector<size_t> who;
     for (size_t k = 0; k < n; k++) {
         LsWork &w = *jobs[k].w;
         w.pin_first = true;
-        bool wh, lv;
-        if (int rc = rebuild_setup(s, g, *jobs[k].c, *jobs[k].a, w, wn[k], wh, &lv)) return rc;
+        Win wr;
+        int R;
+        bool wh;
+        const bool lv = rebuild_window(g, *jobs[k].c, w, wn[k], wr, wh, R);
         whole[k] = wh;
-        live[k] = lv;
         if (!lv) continue;
+        su.push_back(SetupItem{*jobs[k].c, *jobs[k].a, w, wn[k], wr, R, 1, nullptr, nullptr, nullptr, nullptr, nullptr});
         items.push_back(FimItem{*jobs[k].c, *jobs[k].a, w, {w.va, w.vb}});
         pred.push_back(&w.pred_fim);
         who.push_back(k);
     }
     if (items.empty()) return 0;
+    // the set-up of every live solid (the single-solid rebuild_setup's
+    // launches, one per kernel for all of them)
+    int rc = 0;
+    const SetupItem *d = upload_items(s, bt, 0, su, &rc);
+    if (rc) return rc;
+    const dim3 bb(BX, BY), gw = grid_items(su, false), gr = grid_items(su, true);
+    const unsigned m = (unsigned)su.size();
+    k_pin_b<<<gw, bb, 0, s>>>(g, d);
+    KCHECK();
+    k_counts_b<<<dim3(1, 1, m), 64, 0, s>>>(d, 0, kCntIter + kIterSlots);
+    KCHECK();
+    k_scan_b<<<gw, bb, 0, s>>>(g, d);
+    KCHECK();
+    k_newton_b<<<dim3(list_blocks(m), 1, m), kListThreads, 0, s>>>(g, geom_of(g), d);
+    KCHECK();
+    k_dilate_rows_b<<<gr, bb, 0, s>>>(g, d);
+    KCHECK();
+    k_dilate_cols_b<<<gw, bb, 0, s>>>(g, d);
+    KCHECK();
+    k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 0);
+    KCHECK();
     EikGeom eg;
     eg.init(g.dx, g.dy);
     const int nb = batch_blocks((int)items.size());
     std::vector<int> passes;
-    int rc = iterate_batch(s, bt, items, pred, 4096, [&](int p, int slot, const FimItem *d, int m) -> int {
-        k_fim_tiles_batch<<<dim3(nb, m), kTileThreads, 0, s>>>(g, eg, d, p, slot, 64);
+    rc = iterate_batch(s, bt, items, pred, kMaxFimPasses, [&](int p, int slot, const FimItem *dd, int mm) -> int {
+        k_fim_tiles_batch<<<dim3(nb, mm), kTileThreads, 0, s>>>(g, eg, dd, p, slot, 64);
         KCHECK();
         return 0;
     }, passes);
     if (rc) return rc;
     int most = 0;
     for (size_t t = 0; t < who.size(); t++) {
-        const size_t k = who[t];
-        LsWork &w = *jobs[k].w;
+        LsWork &w = *jobs[who[t]].w;
         if (w.hcnt[kCntErr] & kErrNewton) return 3;
         most = std::max(most, passes[t]);
-        if ((rc = rebuild_finish(s, g, *jobs[k].c, *jobs[k].a, w, wn[k], whole[k], passes[t]))) return rc;
+        su[t].v = passes[t] & 1 ? w.vb : w.va;
     }
+    // the finish of every live solid (rebuild_finish's launches, shared)
+    d = upload_items(s, bt, 1, su, &rc);
+    if (rc) return rc;
+    k_bbox_init_b<<<dim3(1, 1, m), 64, 0, s>>>(d);
+    KCHECK();
+    k_finalize_b<<<gw, bb, 0, s>>>(g, d);
+    KCHECK();
+    for (size_t t = 0; t < who.size(); t++)
+        if ((rc = finish_window(s, *jobs[who[t]].w, wn[who[t]], whole[who[t]]))) return rc;
+    k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 1);
+    KCHECK();
+    k_commit_status_b<<<gw, bb, 0, s>>>(g, d);
+    KCHECK();
     if (fim_iters) *fim_iters = most;
     return 0;
 }
@@ -1222,20 +1405,25 @@ int ls_extend(hipStream_t s, const Grid &g, const ObjArrays &a, LsWork &w, int *
 int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt, int *ext_iters) {
     const size_t n = jobs.size();
     if (n > (size_t)bt.cap) return 7;
+    if (!n) return 0;
     std::vector<ExtItem> items;
+    std::vector<SetupItem> cp;
     std::vector<int *> pred;
     for (size_t k = 0; k < n; k++) {
         LsWork &w = *jobs[k].w;
-        k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(*jobs[k].a, w, w.ex, w.ey, 0);
-        KCHECK();
+        cp.push_back(SetupItem{*jobs[k].c, *jobs[k].a, w, {}, {}, 0, 0, nullptr, nullptr, nullptr, w.ex, w.ey});
         items.push_back(ExtItem{*jobs[k].a, w, {w.ex, w.fx}, {w.ey, w.fy}});
         pred.push_back(&w.pred_ext);
     }
-    if (items.empty()) return 0;
+    int rc = 0;
+    const SetupItem *d = upload_items(s, bt, 2, cp, &rc);
+    if (rc) return rc;
+    k_ext_copy_b<<<dim3(list_blocks(n), 1, (unsigned)n), kListThreads, 0, s>>>(d, 0);
+    KCHECK();
     const int nb = batch_blocks((int)items.size());
     std::vector<int> passes;
-    int rc = iterate_batch(s, bt, items, pred, 4096, [&](int p, int slot, const ExtItem *d, int m) -> int {
-        k_extend_tiles_batch<<<dim3(nb, m), kTileThreads, 0, s>>>(g, d, p, slot, 128);
+    rc = iterate_batch(s, bt, items, pred, 4096, [&](int p, int slot, const ExtItem *dd, int m) -> int {
+        k_extend_tiles_batch<<<dim3(nb, m), kTileThreads, 0, s>>>(g, dd, p, slot, 128);
         KCHECK();
         return 0;
     }, passes);
@@ -1246,9 +1434,13 @@ int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs
         if (w.hcnt[kCntErr] & kErrSingular) return 5;
         most = std::max(most, passes[k]);
         const int q = passes[k];
-        k_ext_copy<<<kListBlocks, kListThreads, 0, s>>>(*jobs[k].a, w, q & 1 ? w.fx : w.ex, q & 1 ? w.fy : w.ey, 1);
-        KCHECK();
+        cp[k].fx = q & 1 ? w.fx : w.ex;
+        cp[k].fy = q & 1 ? w.fy : w.ey;
     }
+    d = upload_items(s, bt, 3, cp, &rc);
+    if (rc) return rc;
+    k_ext_copy_b<<<dim3(list_blocks(n), 1, (unsigned)n), kListThreads, 0, s>>>(d, 1);
+    KCHECK();
     if (ext_iters) *ext_iters = most;
     return 0;
 }

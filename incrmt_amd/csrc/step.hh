@@ -82,6 +82,10 @@ int launch_init_velocity(hipStream_t s, const Grid &g, const Fields &f, Solids o
 // solid k's window from the band box `bbox` (level-set coordinates, as the
 // rebuild leaves it on the device), or the whole grid (full)
 int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full);
+// every solid's window (n <= 64 solids; bbox: device array of the solids'
+// band boxes; bit k of full: solid k takes the whole grid)
+int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, SolidWin *win, int n,
+                         unsigned long long full);
 
 }  // namespace incrmt
 

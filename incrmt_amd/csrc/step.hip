@@ -805,8 +805,7 @@ int launch_tang_half(hipStream_t s, const Grid &g, const Fields &f, Solids o, co
 }
 
 // ---- a solid's sweep window from its band's box (see SolidWin)
-__global__ void k_solid_window(Grid g, const int *bbox, SolidWin *win, int full) {
-    if (threadIdx.x) return;
+__device__ void solid_window(const Grid &g, const int *bbox, SolidWin *win, bool full) {
     const SolidWin all{-2, g.m + 2, -2, g.n + 2};
     if (full || g.px || g.py || bbox[1] < 0) {
         *win = all;
@@ -817,8 +816,23 @@ __global__ void k_solid_window(Grid g, const int *bbox, SolidWin *win, int full)
     const int j0 = bbox[2] - 2 - kWinMargin, j1 = bbox[3] - 2 + 1 + kWinMargin;
     *win = (i0 < 0 || j0 < 0 || i1 > g.m || j1 > g.n) ? all : SolidWin{i0, i1, j0, j1};
 }
+__global__ void k_solid_window(Grid g, const int *bbox, SolidWin *win, int full) {
+    if (threadIdx.x == 0) solid_window(g, bbox, win, full != 0);
+}
+// every solid's window in one launch (workgroup k: solid k)
+__global__ void k_solid_windows(Grid g, const int *const *bbox, SolidWin *win, unsigned long long full) {
+    const int k = blockIdx.x;
+    if (threadIdx.x == 0) solid_window(g, bbox[k], win + k, (full >> k) & 1);
+}
 int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full) {
     k_solid_window<<<1, 64, 0, s>>>(g, bbox, win, full);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, SolidWin *win, int n,
+                         unsigned long long full) {
+    if (n <= 0) return 0;
+    k_solid_windows<<<n, 64, 0, s>>>(g, bbox, win, full);
     LAUNCH_CHECK();
     return 0;
 }
