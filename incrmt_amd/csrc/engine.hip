@@ -217,14 +217,15 @@ struct FluidEngine {
         return j;
     }
     int update_window(size_t k, bool full = false) {
-        return launch_solid_window(s, g, obj[k].w.bbox, d_win + k, full || !windows || obj[k].full_steps > 0);
+        return launch_solid_window(s, g, obj[k].w.bbox, obj[k].a.st, d_win + k,
+                                   full || !windows || obj[k].full_steps > 0);
     }
     // every solid's window in one launch
     int update_windows() {
         unsigned long long mask = 0;
         for (size_t k = 0; k < obj.size(); k++)
             if (!windows || obj[k].full_steps > 0) mask |= 1ull << k;
-        return launch_solid_windows(s, g, d_bbox, d_win, (int)obj.size(), mask);
+        return launch_solid_windows(s, g, d_bbox, d_oa, d_win, (int)obj.size(), mask);
     }
 
     // fluid_2d::choose_ev_and_dt + obj_field::set_ex_visc (fluid_2d.cc:152-192, obj_field.cc:35-49)

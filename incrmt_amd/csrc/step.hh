@@ -37,10 +37,12 @@ struct Phys {
 // A solid's sweep window: the cells [i0, i1) x [j0, j1) (ghost cells are
 // -2, -1 and m, m+1) outside which the solid leaves nothing to compute.
 // Written on the device after every band rebuild (launch_solid_window):
-// the band's bounding box (statuses 1..6) widened by kWinMargin cells when
-// that stays clear of the grid's edges, else the whole ghosted grid (a band
-// touching an edge, a periodic grid, no band). Inside a closed interface the
-// solid's interior lies in that box, so outside the window every point of
+// the band's bounding box (statuses 1..6) widened by kWinMargin cells; where
+// that crosses the grid's edges it is clipped to the ghosted grid when the
+// solid lies on the side of its interface the box encloses, else it is the
+// whole ghosted grid (two opposite edges crossed, the solid on the far side,
+// a periodic grid, no band). Inside a closed interface the solid's interior
+// lies in that box, so outside the window every point of
 // the solid is far outside it (status 7, phi at the positive cut-off, above
 // the transition half-width): no transition function, wall force,
 // stress, collision, reference-map update or density term of the solid is
@@ -81,11 +83,13 @@ int launch_set_boundaries_solid(hipStream_t s, const Grid &g, Solids o);
 int launch_init_velocity(hipStream_t s, const Grid &g, const Fields &f, Solids o, double rhof);
 // solid k's window from the band box `bbox` (level-set coordinates, as the
 // rebuild leaves it on the device), or the whole grid (full)
-int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full);
+// (st: the solid's status codes, which tell on which side of an interface
+// cut by the grid's edge the solid lies)
+int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, const int *st, SolidWin *win, bool full);
 // every solid's window (n <= 64 solids; bbox: device array of the solids'
-// band boxes; bit k of full: solid k takes the whole grid)
-int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, SolidWin *win, int n,
-                         unsigned long long full);
+// band boxes; oa: their fields; bit k of full: solid k takes the whole grid)
+int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, const ObjArrays *oa, SolidWin *win,
+                         int n, unsigned long long full);
 
 }  // namespace incrmt
 

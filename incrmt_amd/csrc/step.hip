@@ -805,7 +805,13 @@ int launch_tang_half(hipStream_t s, const Grid &g, const Fields &f, Solids o, co
 }
 
 // ---- a solid's sweep window from its band's box (see SolidWin)
-__device__ void solid_window(const Grid &g, const int *bbox, SolidWin *win, bool full) {
+// A box widened past one edge (or two adjacent ones) still holds the whole
+// solid when the solid's interior is the side of the interface the box
+// encloses: outside the box there is no interface, and with no two opposite
+// edges crossed the rest of the grid is connected, so one status sampled
+// just past a side that crosses no edge tells (7, far outside: the window
+// is the box clipped to the ghosted grid; 0, inside: the whole grid).
+__device__ void solid_window(const Grid &g, const int *bbox, const int *st, SolidWin *win, bool full) {
     const SolidWin all{-2, g.m + 2, -2, g.n + 2};
     if (full || g.px || g.py || bbox[1] < 0) {
         *win = all;
@@ -814,25 +820,42 @@ __device__ void solid_window(const Grid &g, const int *bbox, SolidWin *win, bool
     // level-set coordinates are cell coordinates + 2
     const int i0 = bbox[0] - 2 - kWinMargin, i1 = bbox[1] - 2 + 1 + kWinMargin;
     const int j0 = bbox[2] - 2 - kWinMargin, j1 = bbox[3] - 2 + 1 + kWinMargin;
-    *win = (i0 < 0 || j0 < 0 || i1 > g.m || j1 > g.n) ? all : SolidWin{i0, i1, j0, j1};
+    const bool l = i0 < 0, r = i1 > g.m, d = j0 < 0, u = j1 > g.n;
+    if (!l && !r && !d && !u) {
+        *win = SolidWin{i0, i1, j0, j1};
+        return;
+    }
+    if ((l && r) || (d && u)) {
+        *win = all;
+        return;
+    }
+    // a level-set point just past the box's right or left side (one of the
+    // two crosses no edge)
+    const int I = !r ? bbox[1] + 1 : bbox[0] - 1, J = bbox[2];
+    if (st[g.ls(I, J)] != 7) {
+        *win = all;
+        return;
+    }
+    *win = SolidWin{l ? -2 : i0, r ? g.m + 2 : i1, d ? -2 : j0, u ? g.n + 2 : j1};
 }
-__global__ void k_solid_window(Grid g, const int *bbox, SolidWin *win, int full) {
-    if (threadIdx.x == 0) solid_window(g, bbox, win, full != 0);
+__global__ void k_solid_window(Grid g, const int *bbox, const int *st, SolidWin *win, int full) {
+    if (threadIdx.x == 0) solid_window(g, bbox, st, win, full != 0);
 }
 // every solid's window in one launch (workgroup k: solid k)
-__global__ void k_solid_windows(Grid g, const int *const *bbox, SolidWin *win, unsigned long long full) {
+__global__ void k_solid_windows(Grid g, const int *const *bbox, const ObjArrays *oa, SolidWin *win,
+                                unsigned long long full) {
     const int k = blockIdx.x;
-    if (threadIdx.x == 0) solid_window(g, bbox[k], win + k, (full >> k) & 1);
+    if (threadIdx.x == 0) solid_window(g, bbox[k], oa[k].st, win + k, (full >> k) & 1);
 }
-int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, SolidWin *win, bool full) {
-    k_solid_window<<<1, 64, 0, s>>>(g, bbox, win, full);
+int launch_solid_window(hipStream_t s, const Grid &g, const int *bbox, const int *st, SolidWin *win, bool full) {
+    k_solid_window<<<1, 64, 0, s>>>(g, bbox, st, win, full);
     LAUNCH_CHECK();
     return 0;
 }
-int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, SolidWin *win, int n,
-                         unsigned long long full) {
+int launch_solid_windows(hipStream_t s, const Grid &g, const int *const *bbox, const ObjArrays *oa, SolidWin *win,
+                         int n, unsigned long long full) {
     if (n <= 0) return 0;
-    k_solid_windows<<<n, 64, 0, s>>>(g, bbox, win, full);
+    k_solid_windows<<<n, 64, 0, s>>>(g, bbox, oa, win, full);
     LAUNCH_CHECK();
     return 0;
 }

@@ -69,13 +69,39 @@ def _circles_fluid(spec, windows):
     return f
 
 
+def test_wall_cut_windows_bit_identical():
+    """Windows of solids whose band box crosses the grid's edge: circles cut
+    by one wall and by a corner, one just clear of a wall, one cut by two
+    opposite walls (whole grid) and one in the middle; the windowed step
+    equals the whole-grid step bit for bit over 8 steps."""
+    spec = spec_circles(128, 1)
+    spec["objects"] = [(ol.CIRCLE, [cx, cy, r, 1.0, 0, 0], 1.0, 2.0, 1.0) for cx, cy, r in [
+        (-0.95, -0.3, 0.2),   # cut by the left wall
+        (-1.1, 1.1, 0.45),    # cut by the left and top walls
+        (-0.2, -0.8, 0.16),   # clear of the bottom wall by 3 cells
+        (1.6, 0.0, 1.2),      # cut by the top and bottom walls
+        (-0.3, 0.25, 0.15),   # in the middle
+    ]]
+    a, b = _circles_fluid(spec, True), _circles_fluid(spec, False)
+    for k in range(1, 9):
+        a.step_forward(a.dt_reg)
+        b.step_forward(b.dt_reg)
+    for nm in ("u", "v", "p"):
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+    for o in range(len(spec["objects"])):
+        for nm in ("X", "Y", "phi", "s"):
+            assert np.array_equal(a.get(nm, o), b.get(nm, o)), (nm, o)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("m,count", [(128, 8), (256, 12)])
 def test_solid_windows_bit_identical(m, count):
     """Per-solid sweeps over each solid's band box widened by a few cells
     (the default) give every field of the whole-grid sweeps bit for bit:
     velocities, pressure, and every solid's reference map, level set and
-    band status, over 8 steps (circles near a wall take the whole-grid
-    window, the others a box)."""
+    band status, over 8 steps (circles near a wall take a box clipped to
+    the grid, the others a box)."""
     spec = spec_circles(m, count)
     a, b = _circles_fluid(spec, True), _circles_fluid(spec, False)
     for k in range(1, 9):
