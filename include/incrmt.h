@@ -211,10 +211,11 @@ int incrmt_guess_policy_run(const double *rel, int n, double tol, int *decision)
 #define INCRMT_LAUNCH_DIRECT 0
 #define INCRMT_LAUNCH_GRAPH 1
 int incrmt_fluid_set_launch(incrmt_fluid *f, int mode);
-/* Whether a lone solid's end-of-step band rebuild (the second pin of
- * fluid_2d::step_forward, fluid_2d.cc:359) runs on a second stream beside
- * the FEM projection and the velocity update (on, the default) or after
- * them on the step's stream. It pins to the end-of-step reference map
+/* Whether the end-of-step band rebuild (the second pin of
+ * fluid_2d::step_forward, fluid_2d.cc:359; a lone solid's, or the shared
+ * one of several solids with windowed sweeps) runs on a second stream
+ * beside the FEM projection and the velocity update (on, the default) or
+ * after them on the step's stream. It pins to the end-of-step reference map
  * (nX, nY), which update_velocity copies into (X, Y). Results are
  * bit-identical. No reference counterpart (the reference's step is serial
  * host code). */

@@ -334,9 +334,9 @@ class Fluid2D:
         _check(lib().incrmt_fluid_set_krylov(self._h, int(bool(on))))
 
     def set_overlap(self, on):
-        """Run a lone solid's end-of-step band rebuild on a second stream
-        beside the FEM projection (True, the default) or after it (False);
-        bit-identical."""
+        """Run the end-of-step band rebuild (a lone solid's, or the shared
+        one of several solids) on a second stream beside the FEM projection
+        (True, the default) or after it (False); bit-identical."""
         _check(lib().incrmt_fluid_set_overlap(self._h, int(bool(on))))
 
     def step_forward(self, dt):

@@ -282,7 +282,7 @@ struct FluidEngine {
     int init_tracers();
     int step(double dt);
     int pin_all(int *passes);
-    int pin_join(LsRebuild &r, int *passes);
+    int pin_join(LsRebuild &r, LsBatchPending *bp, int *passes);
 };
 
 // fluid_2d::init_tracers (fluid_2d.cc:1376-1408) with the reference's
@@ -551,14 +551,19 @@ int FluidEngine::pin_all(int *passes) {
 // The join of the overlapped end-of-step rebuild (the rest of what pin_all
 // does for one solid): s waits for everything enqueued on s2, also after an
 // error, then the window and the density follow the new band.
-int FluidEngine::pin_join(LsRebuild &r, int *passes) {
-    Solid &o = obj[0];
+int FluidEngine::pin_join(LsRebuild &r, LsBatchPending *bp, int *passes) {
     int p = 0;
-    const int rc = ls_pin_rebuild_end(s2, g, o.c, o.a, o.w, r, &p);
+    int rc;
+    if (bp) {
+        rc = ls_pin_rebuild_batch_end(s2, g, jobs(), lsb, bp, &p);
+    } else {
+        Solid &o = obj[0];
+        rc = ls_pin_rebuild_end(s2, g, o.c, o.a, o.w, r, &p);
+    }
     HIP_CHECK(hipEventRecord(ev_join, s2));
     HIP_CHECK(hipStreamWaitEvent(s, ev_join, 0));
     if (rc) return rc;
-    if (int e = update_window(0)) return e;
+    if (int e = bp ? update_windows() : update_window(0)) return e;
     if (passes) *passes = p;
     if (!const_rho) return set_rho();
     return 0;
@@ -629,8 +634,9 @@ int FluidEngine::step(double dt) {
     if ((rc = guess_features(fem, 1))) return rc;
     // From here to update_velocity nothing on s reads the level set, the
     // band or the windows, and the end-of-step reference map is final in
-    // (nX, nY): a lone solid's end-of-step rebuild starts now on s2.
-    const bool ovl = overlap && obj.size() == 1;
+    // (nX, nY): the end-of-step rebuild (a lone solid's, or the batched one
+    // of several) starts now on s2.
+    const bool ovl = overlap && (obj.size() == 1 || (obj.size() > 1 && windows));
     if (ovl && !s2) {
         // made on first use only: a stream per fluid that never overlaps
         // (an ensemble's instances) would still take a share of the
@@ -640,15 +646,22 @@ int FluidEngine::step(double dt) {
         HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     LsRebuild pend;
+    LsBatchPending *bpend = nullptr;
     auto drain = [&](int code) {  // an error return while s2 may still run
         if (ovl) (void)hipStreamSynchronize(s2);
+        ls_batch_pending_free(bpend);
         return code;
     };
     if (ovl) {
         HIP_CHECK(hipEventRecord(ev_fork, s));
         HIP_CHECK(hipStreamWaitEvent(s2, ev_fork, 0));
-        Solid &o0 = obj[0];
-        if ((rc = ls_pin_rebuild_begin(s2, g, o0.c, o0.a, o0.w, o0.a.nX, o0.a.nY, pend))) return drain(rc);
+        if (obj.size() == 1) {
+            Solid &o0 = obj[0];
+            if ((rc = ls_pin_rebuild_begin(s2, g, o0.c, o0.a, o0.w, o0.a.nX, o0.a.nY, pend))) return drain(rc);
+        } else {
+            if (!lsb.cap && (rc = ls_batch_alloc((int)obj.size(), lsb))) return drain(rc);
+            if ((rc = ls_pin_rebuild_batch_begin(s2, g, jobs(), lsb, true, &bpend))) return drain(rc);
+        }
     }
     w0 = std::chrono::steady_clock::now();
     rc = precision == INCRMT_PREC_MIXED ? fem.solve_mixed(mg_tol, max_cycles, &ms) : fem.solve(mg_tol, max_cycles, &ms);
@@ -669,7 +682,7 @@ int FluidEngine::step(double dt) {
         if (hipError_t e = hipGetLastError()) return drain(hip_fail(e, "k_pavg_final", __FILE__, __LINE__));
     }
     if ((rc = launch_pressure_velocity(s, g, f, o, fl.z, fl.pitch, d_pavg, dt))) return drain(rc);
-    rc = ovl ? pin_join(pend, &passes) : pin_all(&passes);
+    rc = ovl ? pin_join(pend, bpend, &passes) : pin_all(&passes);
     if (rc) return rc == 3 ? fail(INCRMT_ERR_NEWTON, "Newton iteration failed.") : rc;
     for (auto &ob : obj)
         if (ob.full_steps > 0) ob.full_steps--;

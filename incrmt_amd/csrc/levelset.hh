@@ -76,6 +76,18 @@ void ls_batch_free(LsBatch &bt);
 // ls_pin_rebuild of every job; *fim_iters: the most passes any took
 int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
                          int *fim_iters);
+// ls_pin_rebuild_batch in two halves (as ls_pin_rebuild_begin/_end):
+// begin enqueues the set-up and the predicted batch of march passes on s
+// and returns the pending state; end waits for s, runs any further passes
+// and the finish, and frees the state. pin_new: pin to (nX, nY) on the
+// interior cells instead of (X, Y). After an error between the halves,
+// synchronise s and free the state with ls_batch_pending_free.
+struct LsBatchPending;
+int ls_pin_rebuild_batch_begin(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                               bool pin_new, LsBatchPending **out);
+int ls_pin_rebuild_batch_end(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                             LsBatchPending *p, int *fim_iters);
+void ls_batch_pending_free(LsBatchPending *p);
 // ls_extend of every job; *ext_iters: the most passes any took
 int ls_extend_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt, int *ext_iters);
 

@@ -954,58 +954,97 @@ int iterate(hipStream_t s, LsWork &w, int &predict, int max_passes, F pass, int 
 // one gather of their counters per batch. Each solid's passes, results,
 // pass count and prediction are those of iterate on it alone; its error
 // flags go to its host mirror (hcnt[kCntErr]).
+// Its state between the halves (enqueue one batch of passes with the
+// gather and read-back of the counters; once the stream has run them,
+// decide per solid), so that a caller can overlap the first batch with other
+// work.
+template <class Item>
+struct BatchIter {
+    std::vector<Item> items;
+    std::vector<int *> predict;
+    std::vector<size_t> idx;  // solids still going
+    std::vector<int> passes_out;
+    int passes = 0, batch = 0;
+    void start(std::vector<Item> it, std::vector<int *> pr) {
+        items = std::move(it);
+        predict = std::move(pr);
+        idx.resize(items.size());
+        for (size_t k = 0; k < idx.size(); k++) idx[k] = k;
+        passes_out.assign(items.size(), 0);
+        passes = batch = 0;
+    }
+};
+template <class Item, class F>
+int batch_enqueue(hipStream_t s, LsBatch &bt, BatchIter<Item> &it, F launch) {
+    Item *d_items = (Item *)bt.d_items, *h_items = (Item *)bt.h_items;
+    const int n = (int)it.idx.size();
+    int batch = 2;
+    for (size_t k : it.idx) batch = std::max(batch, *it.predict[k] < 2 ? 2 : *it.predict[k]);
+    if (batch > kIterSlots) batch = kIterSlots;
+    it.batch = batch;
+    for (int k = 0; k < n; k++) h_items[k] = it.items[it.idx[k]];
+    HIP_CHECK(hipMemcpyAsync(d_items, h_items, n * sizeof(Item), hipMemcpyHostToDevice, s));
+    k_batch_zero<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n);
+    KCHECK();
+    for (int b = 0; b < batch; b++)
+        if (int rc = launch(it.passes + b, b, d_items, n)) return rc;
+    k_batch_gather<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n, batch, bt.d_cnt);
+    KCHECK();
+    HIP_CHECK(hipMemcpyAsync(bt.h_cnt, bt.d_cnt, (size_t)n * (batch + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+template <class Item>
+void batch_check(LsBatch &bt, BatchIter<Item> &it) {
+    const int n = (int)it.idx.size(), batch = it.batch;
+    std::vector<size_t> rest;
+    for (int k = 0; k < n; k++) {
+        const int *c = bt.h_cnt + (size_t)k * (batch + 1);
+        const size_t id = it.idx[k];
+        it.items[id].w.hcnt[kCntErr] = c[0];
+        if (c[0]) {
+            it.passes_out[id] = it.passes + batch;
+            continue;
+        }
+        int done = -1;
+        for (int b = 0; b < batch && done < 0; b++)
+            if (c[1 + b] == 0) done = b;
+        if (done >= 0) {
+            it.passes_out[id] = it.passes + done + 1;
+            *it.predict[id] = it.passes + done + 1;
+        } else {
+            rest.push_back(id);
+        }
+    }
+    it.passes += batch;
+    for (size_t id : rest) *it.predict[id] = it.passes + 2;
+    it.idx.swap(rest);
+}
+// the loop from the state left by batch_enqueue's first batch, or from the start
+template <class Item, class F>
+int batch_finish(hipStream_t s, LsBatch &bt, BatchIter<Item> &it, int max_passes, F launch, bool pending) {
+    if (pending) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        batch_check(bt, it);
+    }
+    while (!it.idx.empty()) {
+        if (it.passes >= max_passes) {
+            for (size_t k : it.idx) it.passes_out[k] = it.passes;
+            return 7;
+        }
+        if (int rc = batch_enqueue(s, bt, it, launch)) return rc;
+        HIP_CHECK(hipStreamSynchronize(s));
+        batch_check(bt, it);
+    }
+    return 0;
+}
 template <class Item, class F>
 int iterate_batch(hipStream_t s, LsBatch &bt, std::vector<Item> items, std::vector<int *> predict, int max_passes,
                   F launch, std::vector<int> &passes_out) {
-    const size_t n0 = items.size();
-    std::vector<size_t> idx(n0);
-    for (size_t k = 0; k < n0; k++) idx[k] = k;
-    passes_out.assign(n0, 0);
-    Item *d_items = (Item *)bt.d_items, *h_items = (Item *)bt.h_items;
-    int passes = 0;
-    while (!idx.empty()) {
-        if (passes >= max_passes) {
-            for (size_t k : idx) passes_out[k] = passes;
-            return 7;
-        }
-        const int n = (int)idx.size();
-        int batch = 2;
-        for (size_t k : idx) batch = std::max(batch, *predict[k] < 2 ? 2 : *predict[k]);
-        if (batch > kIterSlots) batch = kIterSlots;
-        for (int k = 0; k < n; k++) h_items[k] = items[idx[k]];
-        HIP_CHECK(hipMemcpyAsync(d_items, h_items, n * sizeof(Item), hipMemcpyHostToDevice, s));
-        k_batch_zero<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n);
-        KCHECK();
-        for (int b = 0; b < batch; b++)
-            if (int rc = launch(passes + b, b, d_items, n)) return rc;
-        k_batch_gather<Item><<<n < 256 ? n : 256, 64, 0, s>>>(d_items, n, batch, bt.d_cnt);
-        KCHECK();
-        HIP_CHECK(hipMemcpyAsync(bt.h_cnt, bt.d_cnt, (size_t)n * (batch + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<size_t> rest;
-        for (int k = 0; k < n; k++) {
-            const int *c = bt.h_cnt + (size_t)k * (batch + 1);
-            const size_t id = idx[k];
-            items[id].w.hcnt[kCntErr] = c[0];
-            if (c[0]) {
-                passes_out[id] = passes + batch;
-                continue;
-            }
-            int done = -1;
-            for (int b = 0; b < batch && done < 0; b++)
-                if (c[1 + b] == 0) done = b;
-            if (done >= 0) {
-                passes_out[id] = passes + done + 1;
-                *predict[id] = passes + done + 1;
-            } else {
-                rest.push_back(id);
-            }
-        }
-        passes += batch;
-        for (size_t id : rest) *predict[id] = passes + 2;
-        idx.swap(rest);
-    }
-    return 0;
+    BatchIter<Item> it;
+    it.start(std::move(items), std::move(predict));
+    const int rc = batch_finish(s, bt, it, max_passes, launch, false);
+    passes_out = it.passes_out;
+    return rc;
 }
 
 }  // namespace
@@ -1254,38 +1293,54 @@ static dim3 grid_items(const std::vector<SetupItem> &v, bool rows) {
 }
 static int list_blocks(size_t n) { return std::max(16, kListBlocks / (int)std::max<size_t>(1, n)); }
 
-int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
-                         int *fim_iters) {
+// A batched rebuild between its halves.
+struct LsBatchPending {
+    std::vector<Win> wn;
+    std::vector<char> whole;
+    std::vector<SetupItem> su;
+    std::vector<size_t> who;
+    BatchIter<FimItem> it;
+    EikGeom eg;
+    bool live = false;
+};
+void ls_batch_pending_free(LsBatchPending *p) { delete p; }
+
+int ls_pin_rebuild_batch_begin(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                               bool pin_new, LsBatchPending **out) {
+    *out = nullptr;
     const size_t n = jobs.size();
     if (n > (size_t)bt.cap) return 7;
-    std::vector<Win> wn(n);
-    std::vector<char> whole(n);
+    LsBatchPending *p = new LsBatchPending;
+    *out = p;
+    p->wn.resize(n);
+    p->whole.resize(n);
     std::vector<FimItem> items;
-    std::vector<SetupItem> su;
     std::vector<int *> pred;
-    std::vector<size_t> who;
     for (size_t k = 0; k < n; k++) {
         LsWork &w = *jobs[k].w;
         w.pin_first = true;
         Win wr;
         int R;
         bool wh;
-        const bool lv = rebuild_window(g, *jobs[k].c, w, wn[k], wr, wh, R);
-        whole[k] = wh;
+        const bool lv = rebuild_window(g, *jobs[k].c, w, p->wn[k], wr, wh, R);
+        p->whole[k] = wh;
         if (!lv) continue;
-        su.push_back(SetupItem{*jobs[k].c, *jobs[k].a, w, wn[k], wr, R, 1, nullptr, nullptr, nullptr, nullptr, nullptr});
-        items.push_back(FimItem{*jobs[k].c, *jobs[k].a, w, {w.va, w.vb}});
+        const ObjArrays &a = *jobs[k].a;
+        p->su.push_back(SetupItem{*jobs[k].c, a, w, p->wn[k], wr, R, 1, pin_new ? a.nX : nullptr,
+                                  pin_new ? a.nY : nullptr, nullptr, nullptr, nullptr});
+        items.push_back(FimItem{*jobs[k].c, a, w, {w.va, w.vb}});
         pred.push_back(&w.pred_fim);
-        who.push_back(k);
+        p->who.push_back(k);
     }
     if (items.empty()) return 0;
+    p->live = true;
     // the set-up of every live solid (the single-solid rebuild_setup's
     // launches, one per kernel for all of them)
     int rc = 0;
-    const SetupItem *d = upload_items(s, bt, 0, su, &rc);
+    const SetupItem *d = upload_items(s, bt, 0, p->su, &rc);
     if (rc) return rc;
-    const dim3 bb(BX, BY), gw = grid_items(su, false), gr = grid_items(su, true);
-    const unsigned m = (unsigned)su.size();
+    const dim3 bb(BX, BY), gw = grid_items(p->su, false), gr = grid_items(p->su, true);
+    const unsigned m = (unsigned)p->su.size();
     k_pin_b<<<gw, bb, 0, s>>>(g, d);
     KCHECK();
     k_counts_b<<<dim3(1, 1, m), 64, 0, s>>>(d, 0, kCntIter + kIterSlots);
@@ -1300,38 +1355,71 @@ int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> 
     KCHECK();
     k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 0);
     KCHECK();
-    EikGeom eg;
-    eg.init(g.dx, g.dy);
-    const int nb = batch_blocks((int)items.size());
-    std::vector<int> passes;
-    rc = iterate_batch(s, bt, items, pred, kMaxFimPasses, [&](int p, int slot, const FimItem *dd, int mm) -> int {
-        k_fim_tiles_batch<<<dim3(nb, mm), kTileThreads, 0, s>>>(g, eg, dd, p, slot, 64);
+    // the predicted batch of march passes, its counters read back later
+    p->eg.init(g.dx, g.dy);
+    p->it.start(std::move(items), std::move(pred));
+    const int nb = batch_blocks((int)p->it.items.size());
+    const EikGeom eg = p->eg;
+    return batch_enqueue(s, bt, p->it, [&](int q, int slot, const FimItem *dd, int mm) -> int {
+        k_fim_tiles_batch<<<dim3(nb, mm), kTileThreads, 0, s>>>(g, eg, dd, q, slot, 64);
         KCHECK();
         return 0;
-    }, passes);
-    if (rc) return rc;
-    int most = 0;
-    for (size_t t = 0; t < who.size(); t++) {
-        LsWork &w = *jobs[who[t]].w;
-        if (w.hcnt[kCntErr] & kErrNewton) return 3;
-        most = std::max(most, passes[t]);
-        su[t].v = passes[t] & 1 ? w.vb : w.va;
+    });
+}
+
+int ls_pin_rebuild_batch_end(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                             LsBatchPending *p, int *fim_iters) {
+    if (!p->live) {
+        delete p;
+        return 0;
     }
-    // the finish of every live solid (rebuild_finish's launches, shared)
-    d = upload_items(s, bt, 1, su, &rc);
-    if (rc) return rc;
-    k_bbox_init_b<<<dim3(1, 1, m), 64, 0, s>>>(d);
-    KCHECK();
-    k_finalize_b<<<gw, bb, 0, s>>>(g, d);
-    KCHECK();
-    for (size_t t = 0; t < who.size(); t++)
-        if ((rc = finish_window(s, *jobs[who[t]].w, wn[who[t]], whole[who[t]]))) return rc;
-    k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 1);
-    KCHECK();
-    k_commit_status_b<<<gw, bb, 0, s>>>(g, d);
-    KCHECK();
-    if (fim_iters) *fim_iters = most;
-    return 0;
+    const int nb = batch_blocks((int)p->it.items.size());
+    const EikGeom eg = p->eg;
+    int rc = batch_finish(s, bt, p->it, kMaxFimPasses, [&](int q, int slot, const FimItem *dd, int mm) -> int {
+        k_fim_tiles_batch<<<dim3(nb, mm), kTileThreads, 0, s>>>(g, eg, dd, q, slot, 64);
+        KCHECK();
+        return 0;
+    }, true);
+    if (!rc) rc = [&]() -> int {
+        int most = 0;
+        for (size_t t = 0; t < p->who.size(); t++) {
+            LsWork &w = *jobs[p->who[t]].w;
+            if (w.hcnt[kCntErr] & kErrNewton) return 3;
+            most = std::max(most, p->it.passes_out[t]);
+            p->su[t].v = p->it.passes_out[t] & 1 ? w.vb : w.va;
+        }
+        // the finish of every live solid (rebuild_finish's launches, shared)
+        int e = 0;
+        const SetupItem *d = upload_items(s, bt, 1, p->su, &e);
+        if (e) return e;
+        const dim3 bb(BX, BY), gw = grid_items(p->su, false);
+        const unsigned m = (unsigned)p->su.size();
+        k_bbox_init_b<<<dim3(1, 1, m), 64, 0, s>>>(d);
+        KCHECK();
+        k_finalize_b<<<gw, bb, 0, s>>>(g, d);
+        KCHECK();
+        for (size_t t = 0; t < p->who.size(); t++)
+            if ((e = finish_window(s, *jobs[p->who[t]].w, p->wn[p->who[t]], p->whole[p->who[t]]))) return e;
+        k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 1);
+        KCHECK();
+        k_commit_status_b<<<gw, bb, 0, s>>>(g, d);
+        KCHECK();
+        if (fim_iters) *fim_iters = most;
+        return 0;
+    }();
+    delete p;
+    return rc;
+}
+
+int ls_pin_rebuild_batch(hipStream_t s, const Grid &g, const std::vector<LsJob> &jobs, LsBatch &bt,
+                         int *fim_iters) {
+    LsBatchPending *p = nullptr;
+    if (int rc = ls_pin_rebuild_batch_begin(s, g, jobs, bt, false, &p)) {
+        if (p) (void)hipStreamSynchronize(s);
+        delete p;
+        return rc;
+    }
+    return ls_pin_rebuild_batch_end(s, g, jobs, bt, p, fim_iters);
 }
 
 int ls_rebuild(hipStream_t s, const Grid &g, const ObjConst &c, const ObjArrays &a, LsWork &w, int *fim_iters) {
