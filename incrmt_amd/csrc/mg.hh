@@ -317,6 +317,7 @@ class Multigrid {
     size_t ev_used_ = 0;
     int record_begin();
     int record_end();
+    hipEvent_t tev_[2] = {nullptr, nullptr};  // the timing pair of the next timed smoother call
     int collect_events();
 };
 

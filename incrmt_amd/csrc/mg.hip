@@ -14,6 +14,7 @@
 #include <cmath>
 #include <limits>
 #include <type_traits>
+#include <hip/hip_ext.h>
 #include "mg.hh"
 #include "unet.hh"
 
@@ -1971,6 +1972,8 @@ int Multigrid::set_irho_host(const double *irho) {
     return coarsen_coefs();
 }
 
+// The next pair of timing events, recorded by the timed smoother launches
+// themselves (their first dispatch's start, their last dispatch's end).
 int Multigrid::record_begin() {
     if (!time_smoother) return 0;
     if (ev_used_ == ev_.size()) {
@@ -1979,12 +1982,12 @@ int Multigrid::record_begin() {
         HIP_CHECK(hipEventCreate(&b));
         ev_.push_back({a, b});
     }
-    HIP_CHECK(hipEventRecord(ev_[ev_used_].first, s_));
+    tev_[0] = ev_[ev_used_].first;
+    tev_[1] = ev_[ev_used_].second;
     return 0;
 }
 int Multigrid::record_end() {
     if (!time_smoother) return 0;
-    HIP_CHECK(hipEventRecord(ev_[ev_used_].second, s_));
     ev_used_++;
     return 0;
 }
@@ -2033,15 +2036,27 @@ inline int smooth_tiles(int n, int S, int H, bool periodic) {
     return g;
 }
 
+// tev: when given, the call's first launch records tev[0] at its start and
+// its last launch tev[1] at its end (events of the dispatches themselves,
+// hipExtLaunchKernelGGL, so the span is the kernels' own time)
 template <class T, int KIND, int H, int SX, int SY>
 static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2, int sweeps,
-                        bool reverse, hipStream_t s, int *nlaunch, ProArgs pa) {
+                        bool reverse, hipStream_t s, int *nlaunch, ProArgs pa, const hipEvent_t *tev) {
     const dim3 grid(smooth_tiles(a.nx, SX, H, a.px), smooth_tiles(a.ny, SY, H, a.py)), block(smooth_threads<SX, SY>());
     const int ncol = KIND == kMAC ? 2 : 4, per = H / ncol;  // sweeps per launch
     for (int sw = 0; sw < sweeps; sw += per) {
         const int nsw = sweeps - sw < per ? sweeps - sw : per;
         const ProArgs p = sw == 0 ? pa : ProArgs{nullptr, 0, 0, 0};  // the correction enters once
-        if (a.px || a.py || a.dr)
+        const bool edges = a.px || a.py || a.dr;
+        if (tev) {
+            const hipEvent_t e0 = sw == 0 ? tev[0] : nullptr, e1 = sw + per >= sweeps ? tev[1] : nullptr;
+            if (edges)
+                hipExtLaunchKernelGGL(k_smooth_fused<T, KIND, H, SX, SY, true>, grid, block, 0, s, e0, e1, 0, a, k, *z, *z2,
+                                      b, c1, c2, nsw * ncol, reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
+            else
+                hipExtLaunchKernelGGL(k_smooth_fused<T, KIND, H, SX, SY, false>, grid, block, 0, s, e0, e1, 0, a, k, *z,
+                                      *z2, b, c1, c2, nsw * ncol, reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
+        } else if (edges)
             k_smooth_fused<T, KIND, H, SX, SY, true><<<grid, block, 0, s>>>(a, k, *z, *z2, b, c1, c2, nsw * ncol,
                                                                             reverse ? ncol - 1 : 0, reverse ? -1 : 1, p);
         else
@@ -2058,11 +2073,11 @@ static int launch_shape(const LevelArgs &a, FemC k, T **z, T **z2, const T *b, c
 
 template <class T, int KIND>
 static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1, const T *c2,
-                       int sweeps, bool reverse, hipStream_t s, int *nl, ProArgs pa) {
+                       int sweeps, bool reverse, hipStream_t s, int *nl, ProArgs pa, const hipEvent_t *tev) {
 #define SHAPE(id, H, SX, SY)                                                                          \
     case id:                                                                                          \
         if constexpr (smooth_lds(KIND, H, SX, SY, sizeof(T)) <= kLdsBudget)                          \
-            return launch_shape<T, KIND, H, SX, SY>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa); \
+            return launch_shape<T, KIND, H, SX, SY>(a, k, z, z2, b, c1, c2, sweeps, reverse, s, nl, pa, tev); \
         else                                                                                          \
             return INCRMT_ERR_ARG_SHAPE;
     switch (shape) {
@@ -2099,10 +2114,10 @@ static int launch_kind(int shape, const LevelArgs &a, FemC k, T **z, T **z2, con
 template <class T>
 static int launch_smooth(MgKind kind, int shape, const LevelArgs &a, FemC k, T **z, T **z2, const T *b, const T *c1,
                          const T *c2, int sweeps, bool reverse, hipStream_t s, int *nlaunch,
-                         ProArgs pa = ProArgs{nullptr, 0, 0, 0}) {
+                         ProArgs pa = ProArgs{nullptr, 0, 0, 0}, const hipEvent_t *tev = nullptr) {
     *nlaunch = 0;
-    if (kind == kMAC) return launch_kind<T, kMAC>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa);
-    return launch_kind<T, kFEM>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa);
+    if (kind == kMAC) return launch_kind<T, kMAC>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa, tev);
+    return launch_kind<T, kFEM>(shape, a, k, z, z2, b, c1, c2, sweeps, reverse, s, nlaunch, pa, tev);
 }
 
 // Launch shapes measured on gfx950 (MI355X) for the walled levels of the
@@ -2286,10 +2301,11 @@ int Multigrid::smooth(size_t l, int sweeps, bool reverse, bool correct) {
         pa = ProArgs{L[l + 1].z, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py, L[l + 1].dr};
     MgLevel &w = L[l];
     int rc = launch_smooth<double>(kind_, v.shape64, args_of(v), fem_consts(v.hx, v.hy), &w.z, &w.z2, v.b,
-                                   kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl, pa);
+                                   kind_ == kMAC ? v.fw : v.kc, v.fs, sweeps, reverse, s_, &nl, pa,
+                                   timed && time_smoother ? tev_ : nullptr);
     if (rc) return rc;
     if (timed && time_smoother) {
-        if (record_end()) return 6;
+        if (nl > 0 && record_end()) return 6;  // (no launch recorded no event)
         // z, b, coefficients read and z written per sweep; a fused coarse
         // correction adds the coarse level's read (a quarter point per point)
         smoother_bytes += (kind_ == kMAC ? 40.0 : 32.0) * v.nx * v.ny * sweeps + (correct ? 2.0 * v.nx * v.ny : 0.0);
@@ -2309,10 +2325,11 @@ int Multigrid::smooth_f(size_t l, int sweeps, bool reverse, bool correct) {
         pa = ProArgs{L[l + 1].zf, L[l + 1].pitch, L[l + 1].nx, L[l + 1].ny, L[l + 1].px, L[l + 1].py, L[l + 1].dr};
     MgLevel &w = L[l];
     int rc = launch_smooth<float>(kind_, v.shape32, args_of(v), fem_consts(v.hx, v.hy), &w.zf, &w.zf2, v.bf,
-                                  kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl, pa);
+                                  kind_ == kMAC ? v.fwf : v.kcf, v.fsf, sweeps, reverse, s_, &nl, pa,
+                                  timed && time_smoother ? tev_ : nullptr);
     if (rc) return rc;
     if (timed && time_smoother) {
-        if (record_end()) return 6;
+        if (nl > 0 && record_end()) return 6;  // (no launch recorded no event)
         smoother_bytes += (kind_ == kMAC ? 20.0 : 16.0) * v.nx * v.ny * sweeps + (correct ? 1.0 * v.nx * v.ny : 0.0);
         smoother_launches += nl;
     }
