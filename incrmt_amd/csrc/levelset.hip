@@ -52,7 +52,7 @@ __device__ inline void wave_flag(bool pred, int *flags, int tile, int bit) {
         if (lane == l) atomicOr(&flags[t], bit);
         m &= ~same;
     }
-}
+}}
 
 inline dim3 grid_win(const Win &v) { return dim3((v.I1 - v.I0 + BX - 1) / BX, (v.J1 - v.J0 + BY - 1) / BY); }
 constexpr int kListBlocks = 1024, kListThreads = 256;
@@ -301,12 +301,14 @@ __device__ __forceinline__ void dilate_rows_body(const Grid &g, const ObjArrays 
     const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J);
+    // every read of the range issued without waiting on the one before (the
+    // OR over it is the same with or without stopping at the first hit)
+    const int d0 = I - R < 0 ? -I : -R, d1 = I + R >= g.ml ? g.ml - 1 - I : R;
     bool hit = false;
-    for (int d = -R; d <= R && !hit; d++) {
-        const int I2 = I + d;
-        if (I2 < 0 || I2 >= g.ml) continue;
+#pragma unroll 4
+    for (int d = d0; d <= d1; d++) {
         const int s = a.st[ij + d];
-        hit = s == 3 || s == 4;
+        hit |= s == 3 || s == 4;
     }
     w.mark[ij] = hit;
 }
@@ -317,12 +319,11 @@ __device__ __forceinline__ void dilate_cols_body(const Grid &g, const ObjArrays 
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), s = a.st[ij];
     bool hit = false;
-    if (s != 3 && s != 4)
-        for (int d = -R; d <= R && !hit; d++) {
-            const int J2 = J + d;
-            if (J2 < 0 || J2 >= g.nl) continue;
-            hit = w.mark[ij + d * g.P] != 0;
-        }
+    if (s != 3 && s != 4) {
+        const int d0 = J - R < 0 ? -J : -R, d1 = J + R >= g.nl ? g.nl - 1 - J : R;
+#pragma unroll 4
+        for (int d = d0; d <= d1; d++) hit |= w.mark[ij + d * g.P] != 0;
+    }
     w.cflag[ij] = hit;
     wave_append(hit, &w.cnt[kCntCand], w.cand, ij);
     wave_flag(hit, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 1);
