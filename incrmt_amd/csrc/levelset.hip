@@ -52,7 +52,7 @@ __device__ inline void wave_flag(bool pred, int *flags, int tile, int bit) {
         if (lane == l) atomicOr(&flags[t], bit);
         m &= ~same;
     }
-}}
+}
 
 inline dim3 grid_win(const Win &v) { return dim3((v.I1 - v.I0 + BX - 1) / BX, (v.J1 - v.J0 + BY - 1) / BY); }
 constexpr int kListBlocks = 1024, kListThreads = 256;
