@@ -15,6 +15,26 @@ constexpr int BX = 64, BY = 4;
 
 inline dim3 grid2(int nx, int ny) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY); }
 
+// The solids whose windows meet this wavefront's cells: a wavefront is one
+// row j of a BX x BY workgroup (BX = 64 lanes), the cells [i0, i0 + 64) in
+// window coordinates; lane k tests solid k (n <= kMaxObj = 64), one ballot.
+// Every lane of the workgroup must call it (before any early return). A
+// loop over the set bits in increasing order visits the same solids, in the
+// same order, as a loop over all solids that skips those outside their
+// windows, since a solid missing from the mask is outside its window at
+// every cell of the row; the per-cell window test stays.
+static_assert(BX == 64, "one wavefront per row of the workgroup");
+__device__ inline unsigned long long row_solids(const Solids &o, int i0, int j) {
+    const int k = threadIdx.x;
+    bool hit = false;
+    if (k < o.n) {
+        const SolidWin w = o.win[k];
+        hit = j >= w.j0 && j < w.j1 && w.i0 < i0 + BX && w.i1 > i0;
+    }
+    return __ballot(hit);
+}
+__device__ inline int low_bit(unsigned long long m) { return __ffsll((long long)m) - 1; }
+
 // ---- monotonicity-limited fourth-order slope (fluid_2d.cc:604-634)
 __device__ inline double delta_lim(double f0, double f1, double f2) {
     double drm = f1 - f0, drp = f2 - f1;
@@ -271,8 +291,9 @@ __device__ inline void wall_force(const Grid &g, const Phys &ph, double x, doubl
     }
 }
 
-__device__ inline void acceleration(const Grid &g, const Fields &f, const Solids &o, const Phys &ph, int i, int j,
-                                    int ij, double &accx, double &accy, double x, double y, bool pressure) {
+__device__ inline void acceleration(const Grid &g, const Fields &f, const Solids &o, const Phys &ph,
+                                    unsigned long long near, int i, int j, int ij, double &accx, double &accy,
+                                    double x, double y, bool pressure) {
     const int P = g.P;
     double accx_ = 0, accy_ = 0;
     accx = g.xsp * (f.s11[ij + 1] - f.s11[ij]) + g.ysp * (f.s12[ij + P] - f.s12[ij]);
@@ -282,7 +303,8 @@ __device__ inline void acceleration(const Grid &g, const Fields &f, const Solids
         accx -= 0.5 * g.xsp * (p[ij + P + 1] + p[ij + 1] - p[ij + P] - p[ij]);
         accy -= 0.5 * g.ysp * (p[ij + P + 1] - p[ij + 1] + p[ij + P] - p[ij]);
     }
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], i, j)) continue;
         const ObjConst &c = o.oc[k];
         const double phiv = o.oa[k].phi[ij];
@@ -316,6 +338,7 @@ __device__ inline void acceleration(const Grid &g, const Fields &f, const Solids
 // reading the A edges and writing the B edges keeps neighbours race-free.
 __global__ void k_tang_half(Grid g, Fields f, Solids o, Phys ph, double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j), P = g.P;
     double *const *A = f.eA;
@@ -327,7 +350,7 @@ __global__ void k_tang_half(Grid g, Fields f, Solids o, Phys ph, double dt) {
     const double c3 = v_adv * (A[VU][ij] - A[VU][ij - P]);
     const double y = g.ay + g.dy * (0.5 + j), x = g.ax + g.dx * (0.5 + i);
     double accx, accy;
-    acceleration(g, f, o, ph, i, j, ij, accx, accy, x, y, true);
+    acceleration(g, f, o, ph, near, i, j, ij, accx, accy, x, y, true);
     const double u = f.u[ij], v = f.v[ij], ux = f.ux[ij], vx = f.vx[ij], uy = f.uy[ij], vy = f.vy[ij];
     double *const *B = f.eB;
     B[UL][ij] = u + 0.5 * (-g.dx * ux + dt * (-u * ux - c2 + accx));
@@ -379,6 +402,7 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
                                       const double *fw, const double *fs, int cp, bool const_rho,
                                       double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j), P = g.P, q = j * zp + i;
     double *ul = f.eB[UL], *vd = f.eB[VD];
@@ -404,7 +428,8 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
     const double ulr = corr_l(i + 1, j), vdu = corr_d(i, j + 1);
     (void)q;
     const double hx = 0.5 * dt * g.xsp, hy = 0.5 * dt * g.ysp;
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         if (a.st[ij] >= 4) continue;
@@ -422,9 +447,11 @@ __global__ void k_mac_correct_predict(Grid g, Fields f, Solids o, const double *
 // ---- update_reference_map's last loop (fluid_2d.cc:1265-1272)
 __global__ void k_half_timestep(Grid g, Solids o) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j);
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         if (a.st[ij] < 6) {
@@ -437,10 +464,12 @@ __global__ void k_half_timestep(Grid g, Solids o) {
 // ---- fluid_2d::set_rho over the ghosted grid (fluid_2d.cc:1124-1137)
 __global__ void k_set_rho(Grid g, Fields f, Solids o, double rhof) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX - 2, J - 2);
     if (I >= g.ml || J >= g.nl) return;
     const int ij = g.ls(I, J);
     double rho = 0, sfrac = 0;
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], I - 2, J - 2)) continue;
         double tf = trans_func(o.oc[k], o.oa[k].phi[ij]);
         sfrac += tf;
@@ -455,12 +484,13 @@ __global__ void k_set_rho(Grid g, Fields f, Solids o, double rhof) {
 // cell (i, j) is in it, and with it, for any edge where the solid is
 // present, the neighbour across the edge)
 template <bool LEFT>
-__device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids &o, const Phys &ph, int i, int j,
-                                   int ij, double &a, double &b) {
+__device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids &o, const Phys &ph,
+                                   unsigned long long near, int i, int j, int ij, double &a, double &b) {
     const int P = g.P, d = LEFT ? 1 : P;
     double s1s = 0, s2s = 0, sfrac = 0;
     int ncol = 0;
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], i, j)) continue;
         const ObjConst &c = o.oc[k];
         const ObjArrays &A = o.oa[k];
@@ -542,12 +572,14 @@ __device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids 
             }
         };
         const double small_number = 100. * 2.220446049250313e-16;
-        for (int k1 = 0; k1 < o.n; k1++) {
+        for (unsigned long long m1 = near; m1; m1 &= m1 - 1) {
+            const int k1 = low_bit(m1);
             if (!in_win(o.win[k1], i, j)) continue;
             if (0.5 * (o.oa[k1].phi[ij] + o.oa[k1].phi[ij - d]) > o.oc[k1].eps) continue;
             double p1, p1x, p1y;
             edge(k1, p1, p1x, p1y);
-            for (int k2 = k1 + 1; k2 < o.n; k2++) {
+            for (unsigned long long m2 = m1 & (m1 - 1); m2; m2 &= m2 - 1) {
+                const int k2 = low_bit(m2);
                 if (!in_win(o.win[k2], i, j)) continue;
                 if (0.5 * (o.oa[k2].phi[ij] + o.oa[k2].phi[ij - d]) > o.oc[k2].eps) continue;
                 double p2, p2x, p2y;
@@ -569,12 +601,13 @@ __device__ inline void edge_stress(const Grid &g, const Fields &f, const Solids 
 
 __global__ void k_stress(Grid g, Fields f, Solids o, Phys ph) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i > g.m || j > g.n) return;
     const int ij = g.at(i, j);
     double a, b;
-    edge_stress<true>(g, f, o, ph, i, j, ij, a, b);
+    edge_stress<true>(g, f, o, ph, near, i, j, ij, a, b);
     f.s11[ij] = a; f.s21[ij] = b;
-    edge_stress<false>(g, f, o, ph, i, j, ij, a, b);
+    edge_stress<false>(g, f, o, ph, near, i, j, ij, a, b);
     f.s12[ij] = a; f.s22[ij] = b;
 }
 
@@ -583,11 +616,12 @@ __global__ void k_stress(Grid g, Fields f, Solids o, Phys ph) {
 // once on the host) so that the wall and anchor terms see the same x.
 __global__ void k_ustar(Grid g, Fields f, Solids o, Phys ph, double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i >= g.m || j >= g.n) return;
     const int ij = g.at(i, j), P = g.P;
     const double y = g.ay + g.dy * (.5 + j), x = ph.xrow[i];
     double accx, accy;
-    acceleration(g, f, o, ph, i, j, ij, accx, accy, x, y, false);
+    acceleration(g, f, o, ph, near, i, j, ij, accx, accy, x, y, false);
     const double hx = 0.5 * dt * g.xsp, hy = 0.5 * dt * g.ysp;
     const double *ul = f.ulc, *vl = f.eB[VL], *ud = f.eB[UD], *vd = f.vdc;
     f.c0[ij] = f.u[ij] - hx * (ul[ij + 1] + ul[ij]) * (ul[ij + 1] - ul[ij]) - hy * (vd[ij + P] + vd[ij]) * (ud[ij + P] - ud[ij]) + dt * accx;
@@ -631,6 +665,7 @@ __global__ void k_fem_src(Grid g, Fields f, double dt, double *b, int bp) {
 __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z, int zp,
                                     const double *pavg_p, double dt) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const unsigned long long near = row_solids(o, blockIdx.x * BX, j);
     if (i > g.m || j > g.n) return;
     const int ij = g.at(i, j);
     const double pavg = *pavg_p;
@@ -643,7 +678,8 @@ __global__ void k_pressure_velocity(Grid g, Fields f, Solids o, const double *z,
     f.u[ij] = u_star - 0.5 * dt * rhoinv * g.xsp * (p11 + p10 - p01 - p00);
     f.v[ij] = v_star - 0.5 * dt * rhoinv * g.ysp * (p11 - p10 + p01 - p00);
     // (outside a solid's window X = nX already: neither changes there)
-    for (int k = 0; k < o.n; k++) {
+    for (unsigned long long mk = near; mk; mk &= mk - 1) {
+        const int k = low_bit(mk);
         if (!in_win(o.win[k], i, j)) continue;
         const ObjArrays &a = o.oa[k];
         a.X[ij] = a.nX[ij];
