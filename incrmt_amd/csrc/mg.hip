@@ -1199,14 +1199,23 @@ __global__ void k_norms(int kind, LevelArgs a, FemC k, const double *z, const do
                         const double *fs, const double *kc, double *part) {
     __shared__ double s0[kRedThreads], s1[kRedThreads], s2[kRedThreads];
     double bb = 0, rr = 0, bs = 0;
-    const int N = a.nx * a.ny;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
-        const int i = p % a.nx, j = p / a.nx, q = j * a.zp + i, c = j * a.cp + i;
+    const int N = a.nx * a.ny, S = gridDim.x * blockDim.x, di = S % a.nx, dj = S / a.nx;
+    // point p = j nx + i, stepped by S without a division per point; the
+    // unrolled iterations' loads are in flight together (the sums keep
+    // their order)
+    const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = p0 % a.nx, j = p0 / a.nx;
+#pragma unroll 4
+    for (int p = p0; p < N; p += S) {
+        const int q = j * a.zp + i, c = j * a.cp + i;
         double diag, off;
         if (kind == kMAC) off = mac_offdiag<double>(z, fw, fs, q, c, a.zp, a.cp, a.kx, a.ky, diag);
         else off = fem_offdiag<double>(z, kc, q, c, a.zp, a.cp, k, diag);
         const double bq = b[q], r = bq - off - diag * z[q];
         bb += bq * bq; rr += r * r; bs += bq;
+        i += di;
+        j += dj;
+        if (i >= a.nx) { i -= a.nx; j++; }
     }
     s0[threadIdx.x] = bb; s1[threadIdx.x] = rr; s2[threadIdx.x] = bs;
     __syncthreads();
