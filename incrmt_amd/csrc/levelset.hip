@@ -719,12 +719,36 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                 double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
                 double a0 = al, b0 = be, g0 = ga;
                 if (!wide) {
-                    for (unsigned long long mm = mask; mm; mm &= mm - 1) {
-                        const int bit = __builtin_ctzll(mm), dj = bit / 7 - 3, di = bit % 7 - 3;
-                        const int q = e + dj * kEW + di;
-                        const double wgt = pp - sphi[q], fa = cx[q], fb = cy[q], xx = di, yy = dj;
-                        p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
-                        p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                    // the neighbours four at a time: their twelve LDS reads
+                    // issued together, then accumulated in mask order (a
+                    // slot past the last neighbour reads the point itself
+                    // and adds nothing)
+                    for (unsigned long long mm = mask; mm;) {
+                        int qs[4], dis[4], djs[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            ok[u] = mm != 0;
+                            const int bit = ok[u] ? __builtin_ctzll(mm) : 24;
+                            mm &= mm - 1;
+                            djs[u] = bit / 7 - 3;
+                            dis[u] = bit % 7 - 3;
+                            qs[u] = e + djs[u] * kEW + dis[u];
+                        }
+                        double ws[4], fas[4], fbs[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            ws[u] = pp - sphi[qs[u]];
+                            fas[u] = cx[qs[u]];
+                            fbs[u] = cy[qs[u]];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (ok[u]) {
+                                const double wgt = ws[u], fa = fas[u], fb = fbs[u], xx = dis[u], yy = djs[u];
+                                p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                                p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                            }
                     }
                 } else {
                     double S, Sx, Sy, Sxx, Sxy, Syy, det;
