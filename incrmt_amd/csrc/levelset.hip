@@ -688,6 +688,7 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
         unsigned long long mask = 0;
         double al = 0, be = 0, ga = 0;
         bool wide = false, singular = false;
+        int rw = 3;  // the fit's radius
         if (mine) {
             const int r = 3, ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
                       jhi = j + r >= nn ? nn - 1 - j : r;
@@ -704,8 +705,34 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                     mask |= 1ull << ((dj + 3) * 7 + di + 3);
                 }
             double det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
-            if (fabs(det) < 1e-10) wide = true;  // the radius grows (traverse.hh:211-222)
-            else {
+            if (fabs(det) < 1e-10) {
+                // the radius grows (traverse.hh:211-222) while the fit is
+                // singular, to 7 at most; like the weights, the radius it
+                // stops at and its coefficients depend on phi and the
+                // status only, so they are found once here
+                wide = true;
+                int rr = 4;
+                do {
+                    const int ilo = i <= rr ? -i : -rr, ihi = i + rr >= m ? m - 1 - i : rr,
+                              jlo = j <= rr ? -j : -rr, jhi = j + rr >= nn ? nn - 1 - j : rr;
+                    S = Sx = Sy = Sxx = Sxy = Syy = 0;
+                    for (int dj = jlo; dj <= jhi; dj++)
+                        for (int di = ilo; di <= ihi; di++) {
+                            const int q = e + dj * kEW + di, s = sst[q];
+                            if (s < 2 || s > 5) continue;
+                            double wgt = sphi[q];
+                            if (!(wgt < pp)) continue;
+                            wgt = pp - wgt;
+                            S += wgt; Sx += di * wgt; Sy += dj * wgt;
+                            Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
+                        }
+                    det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
+                    rr++;
+                } while (fabs(det) < 1e-10 && rr <= 7);
+                rw = rr - 1;
+                if (fabs(det) < 1e-10) singular = true;
+            }
+            if (!singular) {
                 det = 1 / det;
                 al = (Sxy * Sxy - Sxx * Syy) * det;
                 be = (Sx * Syy - Sxy * Sy) * det;
@@ -750,37 +777,22 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                                 p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
                             }
                     }
-                } else {
-                    double S, Sx, Sy, Sxx, Sxy, Syy, det;
-                    int r = 4;
-                    do {
-                        const int ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
-                                  jhi = j + r >= nn ? nn - 1 - j : r;
-                        S = Sx = Sy = Sxx = Sxy = Syy = 0;
-                        p0 = p1 = p2 = p3 = p4 = p5 = 0;
-                        for (int dj = jlo; dj <= jhi; dj++)
-                            for (int di = ilo; di <= ihi; di++) {
-                                const int q = e + dj * kEW + di, s = sst[q];
-                                if (s < 2 || s > 5) continue;
-                                double wgt = sphi[q];
-                                if (!(wgt < pp)) continue;
-                                wgt = pp - wgt;
-                                S += wgt; Sx += di * wgt; Sy += dj * wgt;
-                                Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
-                                const double fa = cx[q], fb = cy[q], xx = di, yy = dj;
-                                p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
-                                p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
-                            }
-                        det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
-                        r++;
-                    } while (fabs(det) < 1e-10 && r <= 7);
-                    if (fabs(det) < 1e-10) singular = true;
-                    else {
-                        det = 1 / det;
-                        a0 = (Sxy * Sxy - Sxx * Syy) * det;
-                        b0 = (Sx * Syy - Sxy * Sy) * det;
-                        g0 = (Sxx * Sy - Sxy * Sx) * det;
-                    }
+                } else if (!singular) {
+                    // the value sums over the window of radius rw, in the
+                    // order the moments were formed
+                    const int ilo = i <= rw ? -i : -rw, ihi = i + rw >= m ? m - 1 - i : rw,
+                              jlo = j <= rw ? -j : -rw, jhi = j + rw >= nn ? nn - 1 - j : rw;
+                    for (int dj = jlo; dj <= jhi; dj++)
+                        for (int di = ilo; di <= ihi; di++) {
+                            const int q = e + dj * kEW + di, s = sst[q];
+                            if (s < 2 || s > 5) continue;
+                            double wgt = sphi[q];
+                            if (!(wgt < pp)) continue;
+                            wgt = pp - wgt;
+                            const double fa = cx[q], fb = cy[q], xx = di, yy = dj;
+                            p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
+                            p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
+                        }
                 }
                 if (singular) {
                     atomicOr(&w.cnt[kCntErr], kErrSingular);  // the value stays
