@@ -746,33 +746,26 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
                 double p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
                 double a0 = al, b0 = be, g0 = ga;
                 if (!wide) {
-                    // the neighbours four at a time: their twelve LDS reads
-                    // issued together, then accumulated in mask order (a
-                    // slot past the last neighbour reads the point itself
-                    // and adds nothing)
-                    for (unsigned long long mm = mask; mm;) {
-                        int qs[4], dis[4], djs[4];
-                        bool ok[4];
+                    // the 7x7 window unrolled in mask-bit order, a row at a
+                    // time: the row's 21 LDS reads are issued together
+                    // (every window point lies in the tile's halo), then
+                    // its neighbours are summed in order. Offsets and
+                    // coordinates are constants, so no mask bit is decoded
+                    // at run time.
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            ok[u] = mm != 0;
-                            const int bit = ok[u] ? __builtin_ctzll(mm) : 24;
-                            mm &= mm - 1;
-                            djs[u] = bit / 7 - 3;
-                            dis[u] = bit % 7 - 3;
-                            qs[u] = e + djs[u] * kEW + dis[u];
-                        }
-                        double ws[4], fas[4], fbs[4];
+                    for (int dj = -3; dj <= 3; dj++) {
+                        double rp[7], ra[7], rb[7];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            ws[u] = pp - sphi[qs[u]];
-                            fas[u] = cx[qs[u]];
-                            fbs[u] = cy[qs[u]];
+                        for (int di = -3; di <= 3; di++) {
+                            const int q = e + dj * kEW + di;
+                            rp[di + 3] = sphi[q];
+                            ra[di + 3] = cx[q];
+                            rb[di + 3] = cy[q];
                         }
 #pragma unroll
-                        for (int u = 0; u < 4; u++)
-                            if (ok[u]) {
-                                const double wgt = ws[u], fa = fas[u], fb = fbs[u], xx = dis[u], yy = djs[u];
+                        for (int di = -3; di <= 3; di++)
+                            if ((mask >> ((dj + 3) * 7 + di + 3)) & 1) {
+                                const double wgt = pp - rp[di + 3], fa = ra[di + 3], fb = rb[di + 3], xx = di, yy = dj;
                                 p0 += wgt * fa; p1 += wgt * xx * fa; p2 += wgt * yy * fa;
                                 p3 += wgt * fb; p4 += wgt * xx * fb; p5 += wgt * yy * fb;
                             }
