@@ -690,20 +690,32 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
         bool wide = false, singular = false;
         int rw = 3;  // the fit's radius
         if (mine) {
-            const int r = 3, ilo = i <= r ? -i : -r, ihi = i + r >= m ? m - 1 - i : r, jlo = j <= r ? -j : -r,
-                      jhi = j + r >= nn ? nn - 1 - j : r;
+            // the radius-3 window, unrolled a row at a time like the
+            // iterations below; window points outside the grid hold status
+            // -1 in the tile, which the status test rejects, so the window
+            // is the clipped one of traverse.hh:180-203
             double S = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0, Syy = 0;
-            for (int dj = jlo; dj <= jhi; dj++)
-                for (int di = ilo; di <= ihi; di++) {
-                    const int q = e + dj * kEW + di, s = sst[q];
+#pragma unroll
+            for (int dj = -3; dj <= 3; dj++) {
+                int rs[7];
+                double rp[7];
+#pragma unroll
+                for (int di = -3; di <= 3; di++) {
+                    rs[di + 3] = sst[e + dj * kEW + di];
+                    rp[di + 3] = sphi[e + dj * kEW + di];
+                }
+#pragma unroll
+                for (int di = -3; di <= 3; di++) {
+                    const int s = rs[di + 3];
                     if (s < 2 || s > 5) continue;
-                    double wgt = sphi[q];
+                    double wgt = rp[di + 3];
                     if (!(wgt < pp)) continue;
                     wgt = pp - wgt;
                     S += wgt; Sx += di * wgt; Sy += dj * wgt;
                     Sxx += di * di * wgt; Sxy += di * dj * wgt; Syy += dj * dj * wgt;
                     mask |= 1ull << ((dj + 3) * 7 + di + 3);
                 }
+            }
             double det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
             if (fabs(det) < 1e-10) {
                 // the radius grows (traverse.hh:211-222) while the fit is
