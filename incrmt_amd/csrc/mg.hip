@@ -864,83 +864,103 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         if (w) wave_sync();
         else __syncthreads();
     };
-    auto offd = [&](int l, int i, int j, T &diag) -> T {
-        const TailLev &v = A.lv[l];
-        const int q = j * v.zp + i;
-        if (A.kind == kMAC) return mac_offdiag<T>(arr(0, l), arr(2, l), arr(3, l), q, q, v.zp, v.zp, T(v.kx), T(v.ky), diag);
-        return fem_offdiag<T>(arr(0, l), arr(2, l), q, q, v.zp, v.zp, A.k, diag);
+    // A level's parameters and LDS arrays, read from the kernel arguments
+    // once per phase: a level index known only at run time keeps the
+    // compiler from hoisting those reads out of the conditional point
+    // updates, where each would wait on a scalar load.
+    struct Lv {
+        TailLev v;
+        T *z, *b, *c1, *c2;
+        LevelArgs la;
+    };
+    auto lvl = [&](int l) -> Lv {
+        Lv L;
+        L.v = A.lv[l];
+        L.z = arr(0, l); L.b = arr(1, l); L.c1 = arr(2, l); L.c2 = arr(3, l);
+        L.la = LevelArgs{L.v.nx, L.v.ny, L.v.zp, L.v.zp, L.v.kx, L.v.ky, px, py};
+        return L;
+    };
+    const int kind = A.kind;
+    const FemC kk = A.k;
+    auto offd = [&](const Lv &L, int i, int j, T &diag) -> T {
+        const int q = j * L.v.zp + i;
+        if (kind == kMAC) return mac_offdiag<T>(L.z, L.c1, L.c2, q, q, L.v.zp, L.v.zp, T(L.v.kx), T(L.v.ky), diag);
+        return fem_offdiag<T>(L.z, L.c1, q, q, L.v.zp, L.v.zp, kk, diag);
     };
     // colour phases enumerate only their own points: the k-th point of a
     // colour is in colour row r = k / ha, column a = k - r ha
     auto smooth = [&](int l, int sweeps, bool reverse, bool w) {
-        const TailLev &v = A.lv[l];
-        const int ncol = A.kind == kMAC ? 2 : 4, ha = (v.nx + 1) >> 1;
-        const int N = ha * (A.kind == kMAC ? v.ny : (v.ny + 1) >> 1);
+        const Lv L = lvl(l);
+        const TailLev &v = L.v;
+        const int ncol = kind == kMAC ? 2 : 4, ha = (v.nx + 1) >> 1;
+        const int N = ha * (kind == kMAC ? v.ny : (v.ny + 1) >> 1);
         const int t0 = w ? lane : tid, st = w ? 64 : nt;
         for (int sw = 0; sw < sweeps; sw++)
             for (int cc = 0; cc < ncol; cc++) {
                 const int col = reverse ? ncol - 1 - cc : cc;
                 for (int k = t0; k < N; k += st) {
                     const int r = (int)(((unsigned)k * v.mag) >> 20), a = k - r * ha;
-                    const int j = A.kind == kMAC ? r : 2 * r + (col >> 1);
-                    const int i = A.kind == kMAC ? 2 * a + ((col + j) & 1) : 2 * a + (col & 1);
+                    const int j = kind == kMAC ? r : 2 * r + (col >> 1);
+                    const int i = kind == kMAC ? 2 * a + ((col + j) & 1) : 2 * a + (col & 1);
                     if (i >= v.nx || j >= v.ny) continue;
                     T diag;
-                    const T off = offd(l, i, j, diag);
-                    if (diag != T(0)) tput(lds(l), arr(0, l), i, j, (arr(1, l)[j * v.zp + i] - off) / diag);
+                    const T off = offd(L, i, j, diag);
+                    if (diag != T(0)) tput(L.la, L.z, i, j, (L.b[j * v.zp + i] - off) / diag);
                 }
                 sync(w);
             }
     };
-    auto residual = [&](int l, int i, int j) -> T {
-        const TailLev &v = A.lv[l];
+    auto residual = [&](const Lv &L, int i, int j) -> T {
         T diag;
-        const T off = offd(l, i, j, diag);
-        return arr(1, l)[j * v.zp + i] - off - diag * arr(0, l)[j * v.zp + i];
+        const T off = offd(L, i, j, diag);
+        return L.b[j * L.v.zp + i] - off - diag * L.z[j * L.v.zp + i];
     };
     // pre-smooth level l, restrict its residual to l+1's right-hand side and
     // clear l+1's iterate
     auto down = [&](int l, bool w) {
         smooth(l, A.nu1, false, w);
-        const TailLev &f = A.lv[l], &c = A.lv[l + 1];
+        const Lv F = lvl(l), C = lvl(l + 1);
+        const TailLev &f = F.v, &c = C.v;
         const int t0 = w ? lane : tid, st = w ? 64 : nt;
-        if (A.kind == kMAC) {
+        if (kind == kMAC) {
             for (int p = t0; p < c.nx * c.ny; p += st) {
                 const int I = p % c.nx, J = p / c.nx;
                 T s = 0;
                 for (int dj = 0; dj < 2; dj++)
-                    for (int di = 0; di < 2; di++) s += residual(l, 2 * I + di, 2 * J + dj);
-                arr(1, l + 1)[J * c.zp + I] = T(0.25) * s;
-                tput(lds(l + 1), arr(0, l + 1), I, J, T(0));
+                    for (int di = 0; di < 2; di++) s += residual(F, 2 * I + di, 2 * J + dj);
+                C.b[J * c.zp + I] = T(0.25) * s;
+                tput(C.la, C.z, I, J, T(0));
             }
         } else {
             for (int p = t0; p < f.nx * f.ny; p += st) {
                 const int i = p % f.nx, j = p / f.nx;
-                tput(lds(l), arr(3, l), i, j, residual(l, i, j));
+                tput(F.la, F.c2, i, j, residual(F, i, j));
             }
             sync(w);
-            const T *r = arr(3, l);
+            const T *r = F.c2;
             const int pz = f.zp;
             for (int p = t0; p < c.nx * c.ny; p += st) {
                 const int I = p % c.nx, J = p / c.nx, q = 2 * J * pz + 2 * I;
-                arr(1, l + 1)[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
-                                              + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
-                tput(lds(l + 1), arr(0, l + 1), I, J, T(0));
+                C.b[J * c.zp + I] = r[q] + T(0.5) * (r[q - 1] + r[q + 1] + r[q - pz] + r[q + pz])
+                                    + T(0.25) * (r[q - pz - 1] + r[q - pz + 1] + r[q + pz - 1] + r[q + pz + 1]);
+                tput(C.la, C.z, I, J, T(0));
             }
         }
         sync(w);
     };
     // add level l+1's correction to level l and post-smooth level l
     auto up = [&](int l, bool w) {
-        const TailLev &f = A.lv[l], &c = A.lv[l + 1];
-        const T *e = arr(0, l + 1);
+        const Lv F = lvl(l), C = lvl(l + 1);
+        const TailLev &f = F.v, &c = C.v;
+        const T *e = C.z;
         const int p2 = c.zp, t0 = w ? lane : tid, st = w ? 64 : nt;
+        const bool dr = A.dr;
         for (int p = t0; p < f.nx * f.ny; p += st) {
             const int i = p % f.nx, j = p / f.nx, I = i >> 1, J = j >> 1;
             T add;
-            if (A.kind == kMAC) {
+            if (kind == kMAC) {
                 int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
-                const bool flip = A.dr && I2 >= c.nx;
+                const bool flip = dr && I2 >= c.nx;
                 if (I2 < 0 || I2 >= c.nx) I2 = px ? wrap_idx(I2, c.nx) : I;
                 if (J2 < 0 || J2 >= c.ny) J2 = py ? wrap_idx(J2, c.ny) : J;
                 T e2 = e[J * p2 + I2], e22 = e[J2 * p2 + I2];
@@ -953,7 +973,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 else if (!(i & 1)) add = T(0.5) * (e[q] + e[q + p2]);
                 else add = T(0.25) * (e[q] + e[q + 1] + e[q + p2] + e[q + p2 + 1]);
             }
-            tput(lds(l), arr(0, l), i, j, arr(0, l)[j * f.zp + i] + add);
+            tput(F.la, F.z, i, j, F.z[j * f.zp + i] + add);
         }
         sync(w);
         smooth(l, A.nu2, true, w);
@@ -966,7 +986,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     // without pivoting (row r of dm is point r+o). False (nothing written)
     // when a pivot is zero, not finite or of the other sign.
     auto direct = [&](int l) -> bool {
-        const TailLev &v = A.lv[l];
+        const Lv L = lvl(l);
+        const TailLev &v = L.v;
         const int o = A.sing ? 1 : 0, N = v.nx * v.ny, n = N - o;
         if (lane < n) {
             const int p = lane + o, i = p % v.nx, j = p / v.nx, q = j * v.zp + i;
@@ -980,8 +1001,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 const int pn = jj * v.nx + ii;
                 if (pn >= o) row[pn - o] += a;
             };
-            if (A.kind == kMAC) {
-                const T *fw = arr(2, l), *fs = arr(3, l), kx = T(v.kx), ky = T(v.ky);
+            if (kind == kMAC) {
+                const T *fw = L.c1, *fs = L.c2, kx = T(v.kx), ky = T(v.ky);
                 const T wl = fw[q], wr = fw[q + 1], sd = fs[q], su = fs[q + v.zp];
                 put(0, 0, -(kx * (wl + wr) + ky * (sd + su)));
                 put(-1, 0, kx * wl);
@@ -989,24 +1010,24 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
                 put(0, -1, ky * sd);
                 put(0, 1, ky * su);
             } else {
-                const T *kc = arr(2, l);
+                const T *kc = L.c1;
                 const T kdl = kc[q - v.zp - 1], kdr = kc[q - v.zp], kul = kc[q - 1], kur = kc[q];
-                put(0, 0, T(A.k.fm) * (kdl + kdr + kul + kur));
-                put(-1, -1, T(A.k.fc) * kdl);
-                put(1, -1, T(A.k.fc) * kdr);
-                put(-1, 1, T(A.k.fc) * kul);
-                put(1, 1, T(A.k.fc) * kur);
-                put(0, -1, T(A.k.hey) * (kdl + kdr));
-                put(0, 1, T(A.k.hey) * (kul + kur));
-                put(-1, 0, T(A.k.hex) * (kdl + kul));
-                put(1, 0, T(A.k.hex) * (kdr + kur));
+                put(0, 0, T(kk.fm) * (kdl + kdr + kul + kur));
+                put(-1, -1, T(kk.fc) * kdl);
+                put(1, -1, T(kk.fc) * kdr);
+                put(-1, 1, T(kk.fc) * kul);
+                put(1, 1, T(kk.fc) * kur);
+                put(0, -1, T(kk.hey) * (kdl + kdr));
+                put(0, 1, T(kk.hey) * (kul + kur));
+                put(-1, 0, T(kk.hex) * (kdl + kul));
+                put(1, 0, T(kk.hex) * (kdr + kur));
             }
         }
         // lane r keeps row r's right-hand side in a register
         T rhs = 0;
         if (lane < n) {
             const int p = lane + o;
-            rhs = arr(1, l)[(p / v.nx) * v.zp + p % v.nx];
+            rhs = L.b[(p / v.nx) * v.zp + p % v.nx];
         }
         wave_sync();
         bool ok = true;
@@ -1038,11 +1059,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
         }
         if (lane < n) {
             const int p = lane + o;
-            tput(lds(l), arr(0, l), p % v.nx, p / v.nx, x);
+            tput(L.la, L.z, p % v.nx, p / v.nx, x);
         }
         // the pinned point 0 (a warm-started single-level tail holds an
         // iterate there)
-        if (o && lane == 0) tput(lds(l), arr(0, l), 0, 0, T(0));
+        if (o && lane == 0) tput(L.la, L.z, 0, 0, T(0));
         wave_sync();
         return true;
     };
@@ -1051,20 +1072,21 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
     // definite level skips both projections)
     auto coarsest = [&](bool w) {
         const int l = A.nlev - 1;
-        const TailLev &v = A.lv[l];
+        const Lv L = lvl(l);
+        const TailLev &v = L.v;
         const int N = v.nx * v.ny, t0 = w ? lane : tid, st = w ? 64 : nt;
         auto total = [&](double a) -> double { return w ? tail_wave_sum<T>(a) : tail_block_sum<T>(a, red); };
         double sb = 0;
-        for (int p = t0; p < N; p += st) sb += double(arr(1, l)[(p / v.nx) * v.zp + p % v.nx]);
+        for (int p = t0; p < N; p += st) sb += double(L.b[(p / v.nx) * v.zp + p % v.nx]);
         const T mb = A.sing ? T(total(sb) / N) : T(0);
-        for (int p = t0; p < N; p += st) arr(1, l)[(p / v.nx) * v.zp + p % v.nx] -= mb;
+        for (int p = t0; p < N; p += st) L.b[(p / v.nx) * v.zp + p % v.nx] -= mb;
         sync(w);
         const bool solved = w && N <= kTailDirect && N > 1 && direct(l);
         if (!solved) {
             auto r2 = [&]() -> double {
                 double a = 0;
                 for (int p = t0; p < N; p += st) {
-                    const double r = double(residual(l, p % v.nx, p / v.nx));
+                    const double r = double(residual(L, p % v.nx, p / v.nx));
                     a += r * r;
                 }
                 return total(a);
@@ -1076,12 +1098,12 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs A) {
             }
         }
         double sz = 0;
-        for (int p = t0; p < N; p += st) sz += double(arr(0, l)[(p / v.nx) * v.zp + p % v.nx]);
+        for (int p = t0; p < N; p += st) sz += double(L.z[(p / v.nx) * v.zp + p % v.nx]);
         const T mz = A.sing ? T(total(sz) / N) : T(0);
         // (total() synchronises: every lane has read z before it changes)
         for (int p = t0; p < N; p += st) {
             const int i = p % v.nx, j = p / v.nx;
-            tput(lds(l), arr(0, l), i, j, arr(0, l)[j * v.zp + i] - mz);
+            tput(L.la, L.z, i, j, L.z[j * v.zp + i] - mz);
         }
         sync(w);
     };
