@@ -718,7 +718,7 @@ __device__ __forceinline__ void extend_tiles(const Grid &g, const ObjArrays &a, 
             }
             double det = S * (Sxy * Sxy - Sxx * Syy) + Sx * Sx * Syy - 2 * Sx * Sxy * Sy + Sy * Sy * Sxx;
             if (fabs(det) < 1e-10) {
-                // the radius grows (traverse.hh:211-222) while the fit is
+                // the radius grows (traverse.hh:179-207) while the fit is
                 // singular, to 7 at most; like the weights, the radius it
                 // stops at and its coefficients depend on phi and the
                 // status only, so they are found once here
