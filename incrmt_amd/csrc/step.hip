@@ -78,8 +78,18 @@ __global__ void k_normal_derivs(Grid g, Fields f, Solids o, double dt) {
 
 // Per-solid sweeps over the solid's window only: workgroup (x, z) walks the
 // BX x BY tiles x, x + gridDim.x, ... of solid z's window (clipped to the
-// cells [0, m) x [0, n)).
-constexpr int kWinBlocks = 96;
+// cells [0, m) x [0, n)). The windows live on the device, so the launch is
+// sized by the solid count: about 4096 workgroups in all (a lone solid's
+// window of a large grid gets one or two tiles per workgroup, many solids
+// at least 96 workgroups each); a workgroup past its window's tiles does
+// nothing. Never more than the grid's own tile count.
+constexpr int kWinBlocks = 96, kWinTotal = 4096;
+inline int win_blocks(const Grid &g, int n) {
+    int b = n > 0 && kWinTotal / n > kWinBlocks ? kWinTotal / n : kWinBlocks;
+    const int tiles = ((g.m + BX - 1) / BX) * ((g.n + BY - 1) / BY);
+    if (b > tiles) b = tiles;
+    return b < 1 ? 1 : b;
+}
 template <class F>
 __device__ inline void window_cells(const Grid &g, const SolidWin &w0, F f) {
     const int i0 = max(w0.i0, 0), i1 = min(w0.i1, g.m), j0 = max(w0.j0, 0), j1 = min(w0.j1, g.n);
@@ -815,14 +825,14 @@ int launch_normal_derivs(hipStream_t s, const Grid &g, const Fields &f, Solids o
     k_normal_derivs<<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o, dt);
     LAUNCH_CHECK();
     if (o.n) {
-        k_refmap_slopes<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, o);
+        k_refmap_slopes<<<dim3(win_blocks(g, o.n), 1, o.n), dim3(BX, BY), 0, s>>>(g, o);
         LAUNCH_CHECK();
     }
     return 0;
 }
 int launch_godunov(hipStream_t s, const Grid &g, const Fields &f, Solids o, bool tang) {
     if (!tang && o.n) {
-        k_refmap_upwind<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o);
+        k_refmap_upwind<<<dim3(win_blocks(g, o.n), 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o);
         LAUNCH_CHECK();
     }
     if (tang) k_godunov<true><<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o);
@@ -834,7 +844,7 @@ int launch_tang_half(hipStream_t s, const Grid &g, const Fields &f, Solids o, co
     k_tang_half<<<grid2(g.m, g.n), dim3(BX, BY), 0, s>>>(g, f, o, ph, dt);
     LAUNCH_CHECK();
     if (o.n) {
-        k_refmap_half<<<dim3(kWinBlocks, 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o, dt);
+        k_refmap_half<<<dim3(win_blocks(g, o.n), 1, o.n), dim3(BX, BY), 0, s>>>(g, f, o, dt);
         LAUNCH_CHECK();
     }
     return 0;
