@@ -313,20 +313,68 @@ __device__ __forceinline__ void dilate_rows_body(const Grid &g, const ObjArrays 
     w.mark[ij] = hit;
 }
 __global__ void k_dilate_rows(Grid g, ObjArrays a, LsWork w, int R, Win wn) { dilate_rows_body(g, a, w, R, wn); }
+// The column pass: a thread takes kColSeg consecutive rows of one column and
+// keeps the count of marked points in the column window [J - R, J + R]
+// (clipped to the grid) as it slides down, one point in and one out per row,
+// instead of reading the 2R + 1 marks of every point.
+constexpr int kColSeg = 8;
+static_assert(kTileY >= kColSeg, "a thread's rows span at most two tile rows");
+inline dim3 grid_win_cols(const Win &v) {
+    return dim3((v.I1 - v.I0 + BX - 1) / BX, (v.J1 - v.J0 + BY * kColSeg - 1) / (BY * kColSeg));
+}
 __device__ __forceinline__ void dilate_cols_body(const Grid &g, const ObjArrays &a, const LsWork &w, int R,
                                                  const Win &wn) {
-    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x, J = wn.J0 + blockIdx.y * BY + threadIdx.y;
-    if (I >= wn.I1 || J >= wn.J1) return;
-    const int ij = g.ls(I, J), s = a.st[ij];
-    bool hit = false;
-    if (s != 3 && s != 4) {
-        const int d0 = J - R < 0 ? -J : -R, d1 = J + R >= g.nl ? g.nl - 1 - J : R;
+    const int I = wn.I0 + blockIdx.x * BX + threadIdx.x;
+    const int J0 = wn.J0 + (blockIdx.y * BY + threadIdx.y) * kColSeg;
+    const bool col = I < wn.I1 && J0 < wn.J1;
+    const unsigned char *mk = w.mark + (col ? g.ls(I, 0) : 0);
+    int cnt = 0;
+    if (col) {
+        const int lo = J0 - R < 0 ? 0 : J0 - R, hi = J0 + R >= g.nl ? g.nl - 1 : J0 + R;
 #pragma unroll 4
-        for (int d = d0; d <= d1; d++) hit |= w.mark[ij + d * g.P] != 0;
+        for (int J = lo; J <= hi; J++) cnt += mk[J * g.P] != 0;
     }
-    w.cflag[ij] = hit;
-    wave_append(hit, &w.cnt[kCntCand], w.cand, ij);
-    wave_flag(hit, w.tflag, (J / kTileY) * w.ntx + I / kTileX, 1);
+    unsigned hm = 0;  // bit k: row J0 + k is a candidate
+#pragma unroll
+    for (int k = 0; k < kColSeg; k++) {
+        const int J = J0 + k;
+        const bool in = col && J < wn.J1;
+        if (in && k > 0) {
+            // the window moves from [J-1-R, J-1+R] to [J-R, J+R]
+            if (J + R < g.nl) cnt += mk[(J + R) * g.P] != 0;
+            if (J - 1 - R >= 0) cnt -= mk[(J - 1 - R) * g.P] != 0;
+        }
+        if (in) {
+            const int ij = g.ls(I, J), s = a.st[ij];
+            const bool hit = s != 3 && s != 4 && cnt > 0;
+            w.cflag[ij] = hit;
+            hm |= (unsigned)hit << k;
+        }
+    }
+    // the candidates of the whole wavefront appended with one atomic (an
+    // inclusive prefix count across the lanes), and the tile flags set once
+    // per wavefront and tile row: a thread's rows span at most two tile rows
+    const int lane = __lane_id(), nh = __popc(hm);
+    int incl = nh;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(&w.cnt[kCntCand], total);
+    base = __shfl(base, 63, 64);
+    int pos = base + incl - nh;
+#pragma unroll
+    for (int k = 0; k < kColSeg; k++)
+        if ((hm >> k) & 1) w.cand[pos++] = g.ls(I, J0 + k);
+    const int ta = J0 / kTileY, tb = (J0 + kColSeg - 1) / kTileY;
+    unsigned ma = 0;  // rows of tile row ta
+#pragma unroll
+    for (int k = 0; k < kColSeg; k++)
+        if ((J0 + k) / kTileY == ta) ma |= 1u << k;
+    wave_flag((hm & ma) != 0, w.tflag, ta * w.ntx + I / kTileX, 1);
+    if (tb != ta) wave_flag((hm & ~ma) != 0, w.tflag, tb * w.ntx + I / kTileX, 1);
 }
 __global__ void k_dilate_cols(Grid g, ObjArrays a, LsWork w, int R, Win wn) { dilate_cols_body(g, a, w, R, wn); }
 
@@ -1225,7 +1273,7 @@ static int rebuild_setup(hipStream_t s, const Grid &g, const ObjConst &c, const 
     KCHECK();
     k_dilate_rows<<<grid_win(wr), bb, 0, s>>>(g, a, w, R < g.ml ? R : g.ml, wr);
     KCHECK();
-    k_dilate_cols<<<grid_win(wn), bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl, wn);
+    k_dilate_cols<<<grid_win_cols(wn), bb, 0, s>>>(g, a, w, R < g.nl ? R : g.nl, wn);
     KCHECK();
     k_compact_tiles<<<64, 256, 0, s>>>(w, 1, w.tiles_fim, kCntTileFim);
     KCHECK();
@@ -1324,10 +1372,10 @@ static const SetupItem *upload_items(hipStream_t s, LsBatch &bt, int region, con
     return d;
 }
 // a grid covering the largest of the items' windows (rows: their row windows)
-static dim3 grid_items(const std::vector<SetupItem> &v, bool rows) {
+static dim3 grid_items(const std::vector<SetupItem> &v, bool rows, bool cols = false) {
     unsigned gx = 1, gy = 1;
     for (const SetupItem &x : v) {
-        const dim3 d = grid_win(rows ? x.wr : x.wn);
+        const dim3 d = cols ? grid_win_cols(x.wn) : grid_win(rows ? x.wr : x.wn);
         gx = std::max(gx, d.x);
         gy = std::max(gy, d.y);
     }
@@ -1381,7 +1429,8 @@ int ls_pin_rebuild_batch_begin(hipStream_t s, const Grid &g, const std::vector<L
     int rc = 0;
     const SetupItem *d = upload_items(s, bt, 0, p->su, &rc);
     if (rc) return rc;
-    const dim3 bb(BX, BY), gw = grid_items(p->su, false), gr = grid_items(p->su, true);
+    const dim3 bb(BX, BY), gw = grid_items(p->su, false), gr = grid_items(p->su, true),
+               gc = grid_items(p->su, false, true);
     const unsigned m = (unsigned)p->su.size();
     k_pin_b<<<gw, bb, 0, s>>>(g, d);
     KCHECK();
@@ -1393,7 +1442,7 @@ int ls_pin_rebuild_batch_begin(hipStream_t s, const Grid &g, const std::vector<L
     KCHECK();
     k_dilate_rows_b<<<gr, bb, 0, s>>>(g, d);
     KCHECK();
-    k_dilate_cols_b<<<gw, bb, 0, s>>>(g, d);
+    k_dilate_cols_b<<<gc, bb, 0, s>>>(g, d);
     KCHECK();
     k_compact_tiles_b<<<dim3(64, 1, m), 256, 0, s>>>(d, 0);
     KCHECK();
