@@ -252,12 +252,12 @@ __device__ __forceinline__ void scan_body(const Grid &g, const ObjArrays &a, con
     if (I >= wn.I1 || J >= wn.J1) return;
     const int ij = g.ls(I, J), P = g.P;
     const double *phi = a.phi;
-    const bool pos = phi[ij] > 0;
-    bool iface = false;
-    if (I > 0 && (phi[ij - 1] > 0) != pos) iface = true;
-    if (I < g.ml - 1 && (phi[ij + 1] > 0) != pos) iface = true;
-    if (J > 0 && (phi[ij - P] > 0) != pos) iface = true;
-    if (J < g.nl - 1 && (phi[ij + P] > 0) != pos) iface = true;
+    // the four neighbours read together (a neighbour off the grid reads the
+    // point itself, which never differs in sign), then tested
+    const double p0 = phi[ij], pl = phi[I > 0 ? ij - 1 : ij], pr = phi[I < g.ml - 1 ? ij + 1 : ij],
+                 pd = phi[J > 0 ? ij - P : ij], pu = phi[J < g.nl - 1 ? ij + P : ij];
+    const bool pos = p0 > 0;
+    const bool iface = ((pl > 0) != pos) | ((pr > 0) != pos) | ((pd > 0) != pos) | ((pu > 0) != pos);
     a.st[ij] = pos ? (iface ? 4 : 7) : (iface ? 3 : 0);
     wave_append(iface, &w.cnt[kCntIface], w.iface, ij);
     if (!iface) {
@@ -625,26 +625,33 @@ __device__ void finalize_point(Grid g, ObjConst c, ObjArrays a, LsWork w, const 
                                int (*bb)[BX * BY], int lane) {
     const int ij = g.ls(I, J), P = g.P, s = a.st[ij];
     const double lim = c.phi_up, dlim = c.phi_down;
-    auto acc_pos = [&](int q) { const int t = a.st[q]; return t == 4 || (t == 7 && v[q] < lim); };
-    auto acc_neg = [&](int q) { const int t = a.st[q]; return t == 3 || (t == 0 && v[q] > dlim); };
+    // a point outside the band tests its four neighbours: their statuses
+    // and values are read together (a neighbour off the grid reads the point
+    // itself and is masked out)
+    const bool hl = I > 0, hr = I < g.ml - 1, hd = J > 0, hu = J < g.nl - 1;
+    const int ql = hl ? ij - 1 : ij, qr = hr ? ij + 1 : ij, qd = hd ? ij - P : ij, qu = hu ? ij + P : ij;
+    auto any_nb = [&](auto acc) {
+        const int tl = a.st[ql], tr = a.st[qr], td = a.st[qd], tu = a.st[qu];
+        const double vl = v[ql], vr = v[qr], vd = v[qd], vu = v[qu];
+        return (hl && acc(tl, vl)) | (hr && acc(tr, vr)) | (hd && acc(td, vd)) | (hu && acc(tu, vu));
+    };
+    auto acc_pos = [&](int t, double x) { return t == 4 || (t == 7 && x < lim); };
+    auto acc_neg = [&](int t, double x) { return t == 3 || (t == 0 && x > dlim); };
+    const double v0 = v[ij];
     int ns = s;
     double phi;
     if (s == 3 || s == 4) {
-        phi = v[ij];
+        phi = v0;
     } else if (s == 7) {
-        if (v[ij] < lim) { ns = 5; phi = v[ij]; }
+        if (v0 < lim) { ns = 5; phi = v0; }
         else {
-            const bool tr = (I > 0 && acc_pos(ij - 1)) || (I < g.ml - 1 && acc_pos(ij + 1)) ||
-                            (J > 0 && acc_pos(ij - P)) || (J < g.nl - 1 && acc_pos(ij + P));
-            ns = tr ? 6 : 7;
+            ns = any_nb(acc_pos) ? 6 : 7;
             phi = lim;
         }
     } else {
-        if (v[ij] > dlim) { ns = 2; phi = v[ij]; }
+        if (v0 > dlim) { ns = 2; phi = v0; }
         else {
-            const bool tr = (I > 0 && acc_neg(ij - 1)) || (I < g.ml - 1 && acc_neg(ij + 1)) ||
-                            (J > 0 && acc_neg(ij - P)) || (J < g.nl - 1 && acc_neg(ij + P));
-            ns = tr ? 1 : 0;
+            ns = any_nb(acc_neg) ? 1 : 0;
             phi = dlim;
         }
     }
